@@ -1,0 +1,147 @@
+/*
+ * wakis_fit.h - C ABI of the MI355X FIT time-stepping engine (libwakis_fit.so).
+ *
+ * Plain C: device pointers are void*, streams are void* (a hipStream_t, or
+ * NULL for the default stream). Every entry point returns 0 on success and a
+ * negative WF_E* code on failure; wf_last_error() gives the message of the
+ * last failure on the calling thread. No entry point allocates device memory
+ * except wf_create (small metric vectors); none synchronises the stream except
+ * where stated.
+ *
+ * Field layout (identical to the reference Field, wakis/field.py:19-24, 57-91):
+ * a flat array of 3N values, x|y|z component blocks, each block x-fastest,
+ * n = i + Nx*j + Nx*Ny*k. All coefficient arrays use the same layout.
+ *
+ * Reference interfaces each entry point replaces are cited per function
+ * (paths relative to the Wakis v0.7.0 tree).
+ */
+#ifndef WAKIS_FIT_H
+#define WAKIS_FIT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WF_OK 0
+#define WF_EINVAL (-1)   /* bad argument (maps to ValueError) */
+#define WF_ERANGE (-2)   /* index/size out of range (maps to IndexError) */
+#define WF_EHIP (-3)     /* HIP runtime error (maps to RuntimeError) */
+#define WF_ESTATE (-4)   /* call not valid in this state (maps to RuntimeError) */
+
+#define WF_F64 0
+#define WF_F32 1
+
+/* Face bit numbering used by the mask words: xlo, ylo, zlo, xhi, yhi, zhi. */
+#define WF_FACE_XLO 0
+#define WF_FACE_YLO 1
+#define WF_FACE_ZLO 2
+#define WF_FACE_XHI 3
+#define WF_FACE_YHI 4
+#define WF_FACE_ZHI 5
+
+/* J-term modes of the E update (solverFIT3D.py:492-499, "- ieps*J"). */
+#define WF_J_NONE 0  /* J identically zero: E += dt*(M2 H)                        */
+#define WF_J_LINE 1  /* J nonzero only on the beam line (ixs, iys, :, z)          */
+#define WF_J_FULL 2  /* J read everywhere                                         */
+
+typedef struct wf_handle wf_handle;
+
+/* Grid and boundary description.
+ * Replaces the operator assembly of SolverFIT3D.__init__
+ * (solverFIT3D.py:203-227, 312-322) and BCsMixin._apply_bc_to_C
+ * (boundaries.py:14-188): instead of the 3N x 3N CSR matrices the engine keeps
+ * 1-D metric vectors and evaluates the curl stencil with the same entries
+ * +-fl(a_r*L_c) / +-fl(b_r*tL_c) and the same per-row summation order. */
+typedef struct wf_grid {
+  int32_t Nx, Ny, Nz;
+  int32_t dtype;           /* WF_F64 or WF_F32: storage type of fields/coefficients   */
+  int32_t pec_zero_faces;  /* bit f: face f zeroes tangential E columns (PEC/pml)    */
+  int32_t pmc_zero_faces;  /* bit f: face f zeroes tangential H rows (0 if no PMC)   */
+  int32_t m1_descending;   /* 1: H-row sums run in descending column order          */
+  const double *Lx, *Ly, *Lz;    /* host, primal edge lengths (Nx, Ny, Nz values)    */
+  const double *tLx, *tLy, *tLz; /* host, dual edge lengths after periodic/ABC edits */
+} wf_grid;
+
+/* One leapfrog batch. Replaces SolverFIT3D._one_step (solverFIT3D.py:482-503)
+ * called nsteps times, plus Beam.update's J line write (sources.py:108-110) and
+ * wakesolve's per-step Ez window save (routines.py:419-425, 440).
+ * E/H/J are updated in place. */
+typedef struct wf_step_args {
+  void *E, *H, *J;        /* device 3N fields                                        */
+  const void *a;          /* device 3N: fl(iA*imu), PMC row mask folded in            */
+  const void *b;          /* device 3N: fl(ieps*itA), PEC row mask folded in          */
+  const void *ieps;       /* device 3N (read when jmode != WF_J_NONE)                 */
+  const void *sigma;      /* device 3N (read when conductive)                         */
+  double dt;
+  int32_t nsteps;
+  int32_t jmode;          /* WF_J_*                                                  */
+  int32_t conductive;     /* 1: J <- sigma*E after each E update                     */
+  int32_t ixs, iys;       /* beam line (for WF_J_LINE and the beam table)            */
+  const void *beam_table; /* device [rows][Nz] in the field dtype, or NULL           */
+  int64_t beam_row0;      /* table row used by the first step of this batch          */
+  int64_t beam_rows;      /* rows in the table                                       */
+  void *capture;          /* device [nlines][cap_samples][Nz] field dtype, or NULL   */
+  int32_t cap_ix0, cap_iy0, cap_nx, cap_ny; /* Ez window origin and extent            */
+  int64_t cap_samples;    /* samples per line in the buffer                          */
+  int64_t cap_sample0;    /* sample written by the first step of this batch          */
+  int32_t path;           /* 0: exact general two-kernel path; 1: fused fast path    */
+  int32_t compute_f64;    /* 1 with WF_F32 storage: dt products and updates in FP64
+                             (numpy promotion when dt is a float64 scalar)           */
+} wf_step_args;
+
+int wf_version(void);
+const char *wf_last_error(void);
+
+/* Replaces SolverFIT3D.__init__'s operator set-up (solverFIT3D.py:203-227). */
+int wf_create(const wf_grid *g, wf_handle **out);
+int wf_destroy(wf_handle *h);
+
+/* First-step ghost zeroing, SolverFIT3D._set_ghosts_to_0 (solverFIT3D.py:820-840). */
+int wf_zero_ghosts(wf_handle *h, void *E, void *H, void *stream);
+
+/* nsteps of SolverFIT3D._one_step (solverFIT3D.py:482-503). */
+int wf_step(wf_handle *h, const wf_step_args *a, void *stream);
+
+/* Copy the current Ez window into sample `sample` of the capture buffer:
+ * wakesolve's save_to_h5(E[xx,yy,zz,'z']) (routines.py:419-425). */
+int wf_capture(wf_handle *h, const void *E, void *capture, int32_t ix0, int32_t iy0,
+               int32_t nx, int32_t ny, int64_t samples, int64_t sample, void *stream);
+
+/* CPU-semantics ABC face update of emsolve (boundaries.py:313-410,
+ * routines.py:216-220): for each low face bit set in `low_faces`, in x, y, z
+ * order, plane 0 <- plane 1 for all components of E then H. */
+int wf_abc_low(wf_handle *h, void *E, void *H, int32_t low_faces, void *stream);
+
+/* Longitudinal wake potential of every captured line with the reference's
+ * line-to-line carry (WakeSolver.calc_long_WP_3d, wakeSolver.py:422-554;
+ * calc_long_WP, :310-420 for one line). Output WP3d[nlines][ns] (FP64).
+ * it(n,k) = trunc(((z[k]+s[n])/v - zmin_v - t0 + ti)/dt), kept if it < nt;
+ * each line ends with WP /= qe12 where qe12 = q*1e12 as the caller computes it.
+ * capture is [nlines][cap_samples][Nz] in `dtype`; k runs over z[0..nz) which
+ * map to capture z index k + k0; dz is indexed from 0 (as the reference). */
+int wf_wake_wp(const void *capture, int32_t dtype, int32_t nlines, int64_t cap_samples,
+               int32_t Nz, int32_t k0, int32_t nz, const double *z, const double *dz,
+               const double *s, int64_t ns, double v, double zmin_v, double t0,
+               double ti, double dt, int64_t nt, double qe12, double *WP3d,
+               void *stream);
+
+/* Transverse wake potential (WakeSolver.calc_trans_WP, wakeSolver.py:556-629):
+ * int_WP[l][n] = numpy-sum(WP3d[l][0:n]) * ds with numpy's chunked pairwise
+ * summation, WPx = -(int[2,1]-int[0,1])/dxsum, WPy = -(int[1,2]-int[1,0])/dysum
+ * on a 3x3 window (lines i-major). */
+int wf_wake_trans(const double *WP3d, int64_t ns, double ds, double dxsum, double dysum,
+                  double *WPx, double *WPy, void *stream);
+
+/* Zero-padded DFT bins 0..nbins-1 of real signals (numpy.fft.fft(x, n=N)[mask],
+ * wakeSolver.py:691-702, 751-766): X[sig][f] = sum_n x[sig][n] exp(-2 pi i f n/N),
+ * twiddle phase reduced exactly as (f*n mod N), compensated FP64 sums.
+ * x is [nsig][len], out is [nsig][nbins][2] (re, im). */
+int wf_dft(const double *x, int32_t nsig, int64_t len, int64_t N, int64_t nbins,
+           double *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WAKIS_FIT_H */

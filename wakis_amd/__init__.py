@@ -1,0 +1,19 @@
+"""wakis_amd: MI355X-native FIT time-stepping engine, drop-in for Wakis' hot path.
+
+Exports the reference's public names for the FIT leapfrog and wakefield path
+(wakis/__init__.py:6-24): Field, GridFIT3D, SolverFIT3D, WakeSolver, plus the
+Beam/PlaneWave sources and analytic solids. The compute path is the HIP
+library libwakis_fit.so (include/wakis_fit.h); there is no CPU fallback.
+"""
+
+from . import solids, sources
+from ._version import __version__
+from .field import Field
+from .gridFIT3D import GridFIT3D
+from .materials import material_lib
+from .solverFIT3D import SolverFIT3D
+from .sources import Beam, PlaneWave
+from .wakeSolver import WakeSolver
+
+__all__ = ["Field", "GridFIT3D", "SolverFIT3D", "WakeSolver", "Beam", "PlaneWave",
+           "material_lib", "solids", "sources", "__version__"]

@@ -1,0 +1,294 @@
+// libwakis_fit: C ABI of the MI355X FIT engine (declared in include/wakis_fit.h).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/wakis_fit.h"
+#include "fit_common.h"
+#include "step_general.h"
+#include "wake.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define WF_HIP(call)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(WF_EHIP, "%s failed: %s", #call, hipGetErrorString(e_));         \
+  } while (0)
+
+#define WF_LAUNCH_CHECK()                                                          \
+  do {                                                                             \
+    hipError_t e_ = hipGetLastError();                                             \
+    if (e_ != hipSuccess) return fail(WF_EHIP, "kernel launch: %s", hipGetErrorString(e_)); \
+  } while (0)
+
+inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+struct wf_handle {
+  wf::GridDev g;
+  int dtype;
+  int device;
+  void *metric_buf;  // device: 6 vectors in the storage type
+  wf::MetricDev<double> md;
+  wf::MetricDev<float> mf;
+};
+
+extern "C" {
+
+int wf_version(void) { return 1; }
+
+const char *wf_last_error(void) { return g_err.c_str(); }
+
+int wf_create(const wf_grid *gd, wf_handle **out) {
+  if (!gd || !out) return fail(WF_EINVAL, "wf_create: null argument");
+  *out = nullptr;
+  if (gd->Nx < 2 || gd->Ny < 2 || gd->Nz < 2)
+    return fail(WF_EINVAL, "wf_create: need at least 2 cells per axis (got %d,%d,%d)", gd->Nx,
+                gd->Ny, gd->Nz);
+  if (gd->dtype != WF_F64 && gd->dtype != WF_F32)
+    return fail(WF_EINVAL, "wf_create: dtype must be WF_F64 or WF_F32");
+  if (!gd->Lx || !gd->Ly || !gd->Lz || !gd->tLx || !gd->tLy || !gd->tLz)
+    return fail(WF_EINVAL, "wf_create: null metric vector");
+  const int64_t N = (int64_t)gd->Nx * gd->Ny * gd->Nz;
+  wf_handle *h = new wf_handle();
+  h->g.Nx = gd->Nx; h->g.Ny = gd->Ny; h->g.Nz = gd->Nz;
+  h->g.N = N; h->g.sy = gd->Nx; h->g.sz = (int64_t)gd->Nx * gd->Ny;
+  h->g.pecz = gd->pec_zero_faces & 63;
+  h->g.pmcz = gd->pmc_zero_faces & 63;
+  h->g.m1desc = gd->m1_descending ? 1 : 0;
+  h->dtype = gd->dtype;
+  hipGetDevice(&h->device);
+  const int n[6] = {gd->Nx, gd->Ny, gd->Nz, gd->Nx, gd->Ny, gd->Nz};
+  const double *src[6] = {gd->Lx, gd->Ly, gd->Lz, gd->tLx, gd->tLy, gd->tLz};
+  const size_t esz = gd->dtype == WF_F64 ? 8 : 4;
+  const size_t tot = (size_t)(2 * (gd->Nx + gd->Ny + gd->Nz));
+  std::vector<unsigned char> host(tot * esz);
+  size_t off[6], o = 0;
+  for (int v = 0; v < 6; ++v) {
+    off[v] = o;
+    for (int i = 0; i < n[v]; ++i) {
+      if (esz == 8) ((double *)host.data())[o + i] = src[v][i];
+      else ((float *)host.data())[o + i] = (float)src[v][i];  // numpy astype: RNE
+    }
+    o += n[v];
+  }
+  hipError_t e = hipMalloc(&h->metric_buf, tot * esz);
+  if (e != hipSuccess) {
+    delete h;
+    return fail(WF_EHIP, "wf_create: hipMalloc: %s", hipGetErrorString(e));
+  }
+  e = hipMemcpy(h->metric_buf, host.data(), tot * esz, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(h->metric_buf);
+    delete h;
+    return fail(WF_EHIP, "wf_create: hipMemcpy: %s", hipGetErrorString(e));
+  }
+  if (esz == 8) {
+    const double *b = (const double *)h->metric_buf;
+    h->md = {b + off[0], b + off[1], b + off[2], b + off[3], b + off[4], b + off[5]};
+  } else {
+    const float *b = (const float *)h->metric_buf;
+    h->mf = {b + off[0], b + off[1], b + off[2], b + off[3], b + off[4], b + off[5]};
+  }
+  *out = h;
+  return WF_OK;
+}
+
+int wf_destroy(wf_handle *h) {
+  if (!h) return WF_OK;
+  if (h->metric_buf) hipFree(h->metric_buf);
+  delete h;
+  return WF_OK;
+}
+
+int wf_zero_ghosts(wf_handle *h, void *E, void *H, void *stream) {
+  if (!h || !E || !H) return fail(WF_EINVAL, "wf_zero_ghosts: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  if (h->dtype == WF_F64)
+    hipLaunchKernelGGL(wf::k_zero_ghosts<double>, dim3(nblocks(h->g.N, 256)), dim3(256), 0, st,
+                       h->g, (double *)E, (double *)H);
+  else
+    hipLaunchKernelGGL(wf::k_zero_ghosts<float>, dim3(nblocks(h->g.N, 256)), dim3(256), 0, st,
+                       h->g, (float *)E, (float *)H);
+  WF_LAUNCH_CHECK();
+  return WF_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+template <typename S, typename Cm>
+int step_general(wf_handle *h, const wf_step_args *a, const wf::MetricDev<S> &m,
+                 hipStream_t st) {
+  const wf::GridDev &g = h->g;
+  const unsigned nb = nblocks(g.N, 256);
+  wf::EStepAux x;
+  x.jmode = a->jmode; x.ixs = a->ixs; x.iys = a->iys;
+  x.cap_ix0 = a->cap_ix0; x.cap_iy0 = a->cap_iy0; x.cap_nx = a->cap_nx; x.cap_ny = a->cap_ny;
+  x.cap_samples = a->cap_samples;
+  const Cm dt = (Cm)a->dt;
+  for (int s = 0; s < a->nsteps; ++s) {
+    hipLaunchKernelGGL((wf::k_h_general<S, Cm>), dim3(nb), dim3(256), 0, st, g, m, (S *)a->H,
+                       (const S *)a->E, (const S *)a->a, dt);
+    WF_LAUNCH_CHECK();
+    const S *row = a->beam_table ? (const S *)a->beam_table + (a->beam_row0 + s) * g.Nz : nullptr;
+    x.cap_sample = a->cap_sample0 + s;
+    S *cap = (S *)a->capture;
+#define WF_E_LAUNCH(JM, CO)                                                                  \
+  hipLaunchKernelGGL((wf::k_e_general<S, Cm, JM, CO>), dim3(nb), dim3(256), 0, st, g, m,    \
+                     (S *)a->E, (const S *)a->H, (S *)a->J, (const S *)a->b,                 \
+                     (const S *)a->ieps, (const S *)a->sigma, dt, x, row, cap)
+    if (a->conductive) WF_E_LAUNCH(2, true);
+    else if (a->jmode == WF_J_FULL) WF_E_LAUNCH(2, false);
+    else if (a->jmode == WF_J_LINE) WF_E_LAUNCH(1, false);
+    else WF_E_LAUNCH(0, false);
+#undef WF_E_LAUNCH
+    WF_LAUNCH_CHECK();
+  }
+  return WF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wf_step(wf_handle *h, const wf_step_args *a, void *stream) {
+  if (!h || !a) return fail(WF_EINVAL, "wf_step: null argument");
+  if (!a->E || !a->H || !a->a || !a->b) return fail(WF_EINVAL, "wf_step: null field/coefficient");
+  if (a->nsteps < 0) return fail(WF_EINVAL, "wf_step: nsteps < 0");
+  if (a->jmode < WF_J_NONE || a->jmode > WF_J_FULL) return fail(WF_EINVAL, "wf_step: bad jmode");
+  if (a->conductive && (!a->sigma || !a->J || a->jmode != WF_J_FULL))
+    return fail(WF_EINVAL, "wf_step: conductive step needs J, sigma and jmode WF_J_FULL");
+  if (a->jmode != WF_J_NONE && !a->ieps) return fail(WF_EINVAL, "wf_step: J term needs ieps");
+  if (a->jmode != WF_J_NONE && !a->J && !a->beam_table)
+    return fail(WF_EINVAL, "wf_step: J term needs J or a beam table");
+  if (a->jmode == WF_J_FULL && !a->J) return fail(WF_EINVAL, "wf_step: WF_J_FULL needs J");
+  if ((a->jmode == WF_J_LINE || a->beam_table) &&
+      (a->ixs < 0 || a->ixs >= h->g.Nx || a->iys < 0 || a->iys >= h->g.Ny))
+    return fail(WF_ERANGE, "wf_step: beam line (%d,%d) outside the grid", a->ixs, a->iys);
+  if (a->beam_table && (a->beam_row0 < 0 || a->beam_row0 + a->nsteps > a->beam_rows))
+    return fail(WF_ERANGE, "wf_step: beam rows [%lld,%lld) outside table of %lld rows",
+                (long long)a->beam_row0, (long long)(a->beam_row0 + a->nsteps),
+                (long long)a->beam_rows);
+  if (a->beam_table && !a->J) return fail(WF_EINVAL, "wf_step: beam table needs J");
+  if (a->capture) {
+    if (a->cap_nx < 1 || a->cap_ny < 1 || a->cap_ix0 < 0 || a->cap_iy0 < 0 ||
+        a->cap_ix0 + a->cap_nx > h->g.Nx || a->cap_iy0 + a->cap_ny > h->g.Ny)
+      return fail(WF_ERANGE, "wf_step: capture window outside the grid");
+    if (a->cap_sample0 < 0 || a->cap_samples < 1)
+      return fail(WF_ERANGE, "wf_step: bad capture sample range");
+  }
+  if (a->path != 0) return fail(WF_EINVAL, "wf_step: path %d not available", a->path);
+  hipStream_t st = (hipStream_t)stream;
+  if (h->dtype == WF_F64) return step_general<double, double>(h, a, h->md, st);
+  if (a->compute_f64) return step_general<float, double>(h, a, h->mf, st);
+  return step_general<float, float>(h, a, h->mf, st);
+}
+
+int wf_capture(wf_handle *h, const void *E, void *capture, int32_t ix0, int32_t iy0, int32_t nx,
+               int32_t ny, int64_t samples, int64_t sample, void *stream) {
+  if (!h || !E || !capture) return fail(WF_EINVAL, "wf_capture: null argument");
+  if (nx < 1 || ny < 1 || ix0 < 0 || iy0 < 0 || ix0 + nx > h->g.Nx || iy0 + ny > h->g.Ny)
+    return fail(WF_ERANGE, "wf_capture: window outside the grid");
+  if (sample < 0 || sample >= samples) return fail(WF_ERANGE, "wf_capture: sample out of range");
+  const int64_t tot = (int64_t)nx * ny * h->g.Nz;
+  hipStream_t st = (hipStream_t)stream;
+  if (h->dtype == WF_F64)
+    hipLaunchKernelGGL(wf::k_capture<double>, dim3(nblocks(tot, 256)), dim3(256), 0, st, h->g,
+                       (const double *)E, (double *)capture, ix0, iy0, nx, ny, samples, sample);
+  else
+    hipLaunchKernelGGL(wf::k_capture<float>, dim3(nblocks(tot, 256)), dim3(256), 0, st, h->g,
+                       (const float *)E, (float *)capture, ix0, iy0, nx, ny, samples, sample);
+  WF_LAUNCH_CHECK();
+  return WF_OK;
+}
+
+int wf_abc_low(wf_handle *h, void *E, void *H, int32_t low_faces, void *stream) {
+  if (!h || !E || !H) return fail(WF_EINVAL, "wf_abc_low: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  for (int axis = 0; axis < 3; ++axis) {
+    if (!((low_faces >> axis) & 1)) continue;
+    const int64_t na = axis == 0 ? h->g.Ny : h->g.Nx;
+    const int64_t nb = axis == 2 ? h->g.Ny : h->g.Nz;
+    if (h->dtype == WF_F64)
+      hipLaunchKernelGGL(wf::k_abc_face<double>, dim3(nblocks(na * nb, 256)), dim3(256), 0, st,
+                         h->g, axis, (double *)E, (double *)H);
+    else
+      hipLaunchKernelGGL(wf::k_abc_face<float>, dim3(nblocks(na * nb, 256)), dim3(256), 0, st,
+                         h->g, axis, (float *)E, (float *)H);
+    WF_LAUNCH_CHECK();
+  }
+  return WF_OK;
+}
+
+int wf_wake_wp(const void *capture, int32_t dtype, int32_t nlines, int64_t cap_samples,
+               int32_t Nz, int32_t k0, int32_t nz, const double *z, const double *dz,
+               const double *s, int64_t ns, double v, double zmin_v, double t0, double ti,
+               double dt, int64_t nt, double qe12, double *WP3d, void *stream) {
+  if (!capture || !z || !dz || !s || !WP3d) return fail(WF_EINVAL, "wf_wake_wp: null argument");
+  if (nlines < 1 || Nz < 1 || k0 < 0 || nz < 0 || k0 + nz > Nz || ns < 0 || cap_samples < 1)
+    return fail(WF_ERANGE, "wf_wake_wp: inconsistent sizes");
+  if (nt > cap_samples) return fail(WF_ERANGE, "wf_wake_wp: nt > captured samples");
+  if (ns == 0) return WF_OK;
+  hipStream_t st = (hipStream_t)stream;
+  // the reference's Ezt has len(t) columns; python negative indices wrap there
+  const int64_t ncols = nt;
+  if (dtype == WF_F64)
+    hipLaunchKernelGGL(wf::k_wake_wp<double>, dim3(nblocks(ns, 256)), dim3(256), 0, st,
+                       (const double *)capture, nlines, cap_samples, Nz, k0, nz, z, dz, s, ns, v,
+                       zmin_v, t0, ti, dt, nt, ncols, qe12, WP3d);
+  else if (dtype == WF_F32)
+    hipLaunchKernelGGL(wf::k_wake_wp<float>, dim3(nblocks(ns, 256)), dim3(256), 0, st,
+                       (const float *)capture, nlines, cap_samples, Nz, k0, nz, z, dz, s, ns, v,
+                       zmin_v, t0, ti, dt, nt, ncols, qe12, WP3d);
+  else
+    return fail(WF_EINVAL, "wf_wake_wp: bad dtype");
+  WF_LAUNCH_CHECK();
+  return WF_OK;
+}
+
+int wf_wake_trans(const double *WP3d, int64_t ns, double ds, double dxsum, double dysum,
+                  double *WPx, double *WPy, void *stream) {
+  if (!WP3d || !WPx || !WPy) return fail(WF_EINVAL, "wf_wake_trans: null argument");
+  if (ns < 0) return fail(WF_ERANGE, "wf_wake_trans: ns < 0");
+  if (ns == 0) return WF_OK;
+  hipLaunchKernelGGL(wf::k_wake_trans, dim3(nblocks(ns, 256)), dim3(256), 0, (hipStream_t)stream,
+                     WP3d, ns, ds, dxsum, dysum, WPx, WPy);
+  WF_LAUNCH_CHECK();
+  return WF_OK;
+}
+
+int wf_dft(const double *x, int32_t nsig, int64_t len, int64_t N, int64_t nbins, double *out,
+           void *stream) {
+  if (!x || !out) return fail(WF_EINVAL, "wf_dft: null argument");
+  if (nsig < 1 || len < 0 || N < 1 || nbins < 0 || nbins > N || len > N)
+    return fail(WF_ERANGE, "wf_dft: inconsistent sizes");
+  if (nbins == 0) return WF_OK;
+  if (nbins > 2147483647LL || nsig > 65535) return fail(WF_ERANGE, "wf_dft: too many bins");
+  hipLaunchKernelGGL(wf::k_dft, dim3((unsigned)nbins, (unsigned)nsig), dim3(256), 0,
+                     (hipStream_t)stream, x, len, N, nbins, out);
+  WF_LAUNCH_CHECK();
+  return WF_OK;
+}
+
+}  // extern "C"

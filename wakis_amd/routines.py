@@ -1,0 +1,127 @@
+"""Time loops ``emsolve`` and ``wakesolve`` (wakis/routines.py:17-482).
+
+``wakesolve`` keeps the whole step loop on the device: the beam current is
+uploaded once as an (Nt, Nz) table, the step kernel writes the beam line and
+captures the Ez window after every E update, and the wake potential/impedance
+are computed from that capture by device kernels. There is no per-step host
+traffic unless a callback or field monitor asks for it.
+"""
+
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from .field import torch_dtype
+
+
+class RoutinesMixin:
+    def emsolve(self, Nt, source=None, callback=None, save=False, fields=["E"],
+                components=["Abs"], save_every=1, subdomain=None, plot=False, plot_every=1,
+                use_etd=False, plot3d=False, **kwargs):
+        """Time loop with host sources and the CPU-semantics ABC (routines.py:17-229)."""
+        if save or plot or plot3d:
+            raise NotImplementedError("emsolve: HDF5 saving and plotting are not part of "
+                                      "this engine")
+        self.Nt = Nt
+        if source is not None:
+            self.source = source
+        low_abc = sum(1 << a for a in range(3) if self.bc_low[a].lower() == "abc")
+        for n in range(Nt):
+            if source is not None:
+                source.update(self, n * self.dt)
+            self.one_step()
+            if self.activate_abc and low_abc:
+                _lib.check(_lib.lib().wf_abc_low(self._handle, _lib.ptr(self.E.array),
+                                                 _lib.ptr(self.H.array), low_abc,
+                                                 _lib.stream_ptr()), "wf_abc_low")
+            if callback is not None:
+                callback(self, n * self.dt)
+
+    def wakesolve(self, wakelength, wake=None, callback=None, compute_plane="both", plot=False,
+                  plot_func=None, plot_from=None, plot_every=1, plot_until=None, save_J=False,
+                  add_space=None, use_field_monitor=False, field_monitor=None, use_edt=None,
+                  chunk=None, **kwargs):
+        """Wakefield run and WakeSolver post-processing (routines.py:231-482)."""
+        from .sources import Beam
+
+        if wake is not None:
+            self.wake = wake
+        if self.wake is None:
+            raise AttributeError("Wake solver information not passed to the solver instantiation")
+        if add_space is not None:
+            self.wake.skip_cells = add_space
+        if plot or save_J:
+            raise NotImplementedError("wakesolve: plotting and Jz.h5 saving are not part of "
+                                      "this engine")
+
+        self.xtest, self.ytest = self.wake.xtest, self.wake.ytest
+        self.ixt = np.abs(self.x - self.xtest).argmin()
+        self.iyt = np.abs(self.y - self.ytest).argmin()
+        if compute_plane.lower() == "longitudinal":
+            ix0, iy0, nx, ny = self.ixt, self.iyt, 1, 1
+        else:
+            ix0, iy0, nx, ny = self.ixt - 1, self.iyt - 1, 3, 3
+            if ix0 < 0 or iy0 < 0 or ix0 + 3 > self.Nx or iy0 + 3 > self.Ny:
+                raise IndexError("wakesolve: the 3x3 test window leaves the grid")
+
+        self.wake.wakelength = wakelength
+        self.ti = self.wake.ti
+        self.v = self.wake.v
+        z = self.z
+        tmax = (wakelength + self.ti * self.v + (z.max() - z.min())) / self.v
+        Nt = int(tmax / self.dt)
+        self.tmax, self.Nt = tmax, Nt
+
+        beam = Beam(q=self.wake.q, sigmaz=self.wake.sigmaz, beta=self.wake.beta,
+                    xsource=self.wake.xsource, ysource=self.wake.ysource)
+        dev, tdt = self.device, torch_dtype(self.dtype)
+        table = torch.as_tensor(beam.table(self, Nt), device=dev).to(tdt).contiguous()
+        self._wake_beam = beam
+        nlines = nx * ny
+        cap = torch.zeros((nlines, max(Nt, 1), self.Nz), dtype=tdt, device=dev)
+        t_axis = np.arange(0, Nt * self.dt, self.dt)
+
+        print("Running electromagnetic time-domain simulation...")
+        t0 = time.time()
+        lib = _lib.lib()
+        st = _lib.stream_ptr()
+        if Nt > 0:  # sample 0: E before the first step (routines.py:437-440)
+            _lib.check(lib.wf_capture(self._handle, _lib.ptr(self.E.array), _lib.ptr(cap),
+                                      ix0, iy0, nx, ny, Nt, 0, st), "wf_capture")
+        if self.step_0:
+            self._set_ghosts_to_0()
+            self.step_0 = False
+        per_step = callback is not None or (use_field_monitor and field_monitor is not None)
+        step = 1 if per_step else (Nt if chunk is None else int(chunk))
+        n = 0
+        while n < Nt:
+            k = min(step, Nt - n)
+            # the beam table rows are what Beam.update writes: J stays "beam line only"
+            self._beam_line = (int(beam.ixs), int(beam.iys)) \
+                if self.J.version == self._J_clean_version else self._beam_line
+            a = self._step_args(k)
+            a.ixs, a.iys = int(beam.ixs), int(beam.iys)
+            if a.jmode == _lib.WF_J_NONE:
+                a.jmode = _lib.WF_J_LINE
+            a.beam_table, a.beam_row0, a.beam_rows = _lib.ptr(table), n, Nt
+            a.capture = _lib.ptr(cap)
+            a.cap_ix0, a.cap_iy0, a.cap_nx, a.cap_ny = ix0, iy0, nx, ny
+            a.cap_samples, a.cap_sample0 = Nt, n + 1
+            self._run(a)
+            self.nstep += k
+            if per_step:
+                if use_field_monitor and field_monitor is not None:
+                    field_monitor.update(self.E, self.dt)
+                if callback is not None:
+                    callback(self, n * self.dt)
+            n += k
+        torch.cuda.synchronize(dev)
+        self.simulationTime = time.time() - t0
+
+        self.wake._set_capture(cap=cap, t=t_axis, xf=self.x[ix0:ix0 + nx],
+                               yf=self.y[iy0:iy0 + ny], zf=z, dx=self.grid.dx,
+                               dy=self.grid.dy, dz=self.grid.dz, dtype=self.dtype,
+                               nsamples=Nt)
+        self.wake.solve(compute_plane=compute_plane)

@@ -1,0 +1,423 @@
+"""SolverFIT3D on the MI355X engine.
+
+Mirrors ``wakis.solverFIT3D.SolverFIT3D`` (solverFIT3D.py:40-354, 416-503,
+820-1029) and ``wakis.boundaries.BCsMixin`` (boundaries.py:14-311): same
+constructor, same exposed Fields (E, H, J, ieps, imu, sigma), same dt and
+boundary semantics, ``one_step`` as the seam (solverFIT3D.py:142). The
+3N x 3N CSR operators are never built: the curl runs as a HIP stencil on
+coefficient arrays folded once from the same diagonals,
+
+    a = fl(iA*imu) * (PMC row mask)      (rows of M1 = iDa*iDmu*C*Ds)
+    b = fl(ieps*itA) * (PEC row mask)    (rows of M2 = iDeps*itDa*C^T*tDs)
+
+with the edge lengths L, tL kept as per-axis vectors and the PEC/PMC column
+masks evaluated from the face rule inside the kernel.
+"""
+
+import ctypes
+import time
+
+import numpy as np
+import torch
+from scipy.constants import c as c_light
+from scipy.constants import epsilon_0 as eps_0
+from scipy.constants import mu_0
+
+from . import _lib
+from .field import Field, torch_dtype
+from .materials import material_lib
+from .routines import RoutinesMixin
+
+PEC_KEYS = ("electric", "pec", "pml")
+
+
+def _face_zero_bits(bc_low, bc_high, test):
+    bits = 0
+    for a in range(3):
+        if test(bc_low[a]):
+            bits |= 1 << a
+        if test(bc_high[a]):
+            bits |= 1 << (3 + a)
+    return bits
+
+
+def face_mask_kji(comp, zbits, Nx, Ny, Nz, device):
+    """Face mask of one component as a (Nz,Ny,Nx) bool tensor.
+
+    Same rule as the kernel's face_mask: for component d the two faces of the
+    other axes are assigned in x, y, z order and the last assignment wins
+    (boundaries.py:90-103).
+    """
+    i = torch.arange(Nx, device=device).view(1, 1, Nx)
+    j = torch.arange(Ny, device=device).view(1, Ny, 1)
+    k = torch.arange(Nz, device=device).view(Nz, 1, 1)
+    f = torch.full((Nz, Ny, Nx), -1, dtype=torch.int8, device=device)
+    order = {0: [(j, 0, 1), (j, Ny - 1, 4), (k, 0, 2), (k, Nz - 1, 5)],
+             1: [(i, 0, 0), (i, Nx - 1, 3), (k, 0, 2), (k, Nz - 1, 5)],
+             2: [(i, 0, 0), (i, Nx - 1, 3), (j, 0, 1), (j, Ny - 1, 4)]}[comp]
+    for idx, val, face in order:
+        f = torch.where(idx == val, torch.tensor(face, dtype=torch.int8, device=device), f)
+    zero = torch.zeros_like(f, dtype=torch.bool)
+    for face in range(6):
+        if (zbits >> face) & 1:
+            zero |= f == face
+    return ~zero
+
+
+class SolverFIT3D(RoutinesMixin):
+    def __init__(self, grid, wake=None, cfln=0.5, dt=None,
+                 bc_low=["Periodic", "Periodic", "Periodic"],
+                 bc_high=["Periodic", "Periodic", "Periodic"],
+                 use_stl=False, use_conductors=False, use_gpu=True, use_mpi=False,
+                 use_sibc=True, fmax=1e9, dtype=np.float64, n_pml=10, bg=[1.0, 1.0],
+                 verbose=1, path="general"):
+        t0 = time.time()
+        if use_mpi:
+            raise NotImplementedError("wakis_amd runs the full domain on one GPU (use_mpi=False)")
+        if use_conductors:
+            raise NotImplementedError("legacy conductors are not supported; use solids")
+        self.verbose = verbose
+        self.step_0 = True
+        self.nstep = 0
+        self.use_stl, self.use_gpu, self.use_mpi = use_stl, True, False
+        self.use_sibc, self.fmax = use_sibc, fmax
+        self.activate_abc = False
+        self.activate_pml = False
+        self.use_conductivity = False
+        self.path = path
+        lib = _lib.lib()  # fail loudly before allocating anything if the engine is missing
+
+        self.grid = grid
+        self.background = bg
+        self.Nx, self.Ny, self.Nz = grid.Nx, grid.Ny, grid.Nz
+        self.N = N = self.Nx * self.Ny * self.Nz
+        self.dims = (self.Nx, self.Ny, self.Nz)
+        self.dx, self.dy, self.dz = grid.dx, grid.dy, grid.dz
+        self.x = grid.x[:-1] + self.dx / 2
+        self.y = grid.y[:-1] + self.dy / 2
+        self.z = grid.z[:-1] + self.dz / 2
+
+        self.wake = wake
+        if wake is not None and fmax == 1e9:
+            self.fmax = wake.fmax
+
+        self.dtype = np.dtype(dtype).type
+        self.E = Field(self.Nx, self.Ny, self.Nz, dtype=self.dtype)
+        self.H = Field(self.Nx, self.Ny, self.Nz, dtype=self.dtype)
+        self.J = Field(self.Nx, self.Ny, self.Nz, dtype=self.dtype)
+        self.device = self.E.device
+
+        # boundaries (boundaries.py:14-188)
+        self.bc_low, self.bc_high = list(bc_low), list(bc_high)
+        self._setup_bcs()
+
+        # materials (solverFIT3D.py:237-273, 416-480) on the host
+        if type(bg) is str:
+            bg = material_lib[bg.lower()]
+        if len(bg) == 3:
+            self.eps_bg, self.mu_bg, self.sigma_bg = bg[0] * eps_0, bg[1] * mu_0, bg[2]
+            self.use_conductivity = True
+        else:
+            self.eps_bg, self.mu_bg, self.sigma_bg = bg[0] * eps_0, bg[1] * mu_0, 0.0
+        ieps = np.ones(3 * N, dtype=self.dtype) * (1.0 / self.eps_bg)
+        imu = np.ones(3 * N, dtype=self.dtype) * (1.0 / self.mu_bg)
+        sigma = np.ones(3 * N, dtype=self.dtype) * self.sigma_bg
+        self._h = {"ieps": ieps, "imu": imu, "sigma": sigma}
+        if self.use_stl:
+            self._apply_stl_materials()
+        if self.activate_pml:
+            self.n_pml = n_pml
+            self._initialize_PML()
+
+        # timestep (solverFIT3D.py:283-310)
+        self.cfln = cfln
+        if dt is None:
+            dt = cfln / (c_light * np.sqrt(1 / np.min(self.grid.dx) ** 2
+                                           + 1 / np.min(self.grid.dy) ** 2
+                                           + 1 / np.min(self.grid.dz) ** 2))
+        self.dt = self.dtype(dt)
+        if self.use_conductivity:
+            h = self._h
+            m = np.logical_and(h["sigma"] != 0, h["ieps"] != 0)
+            self.tau = (1 / h["ieps"][m]) / h["sigma"][m]
+            if self.tau.size and self.dt > self.tau.min():
+                self.dt = self.tau.min()
+
+        # material Fields on the device
+        self.ieps = Field(self.Nx, self.Ny, self.Nz, dtype=self.dtype)
+        self.imu = Field(self.Nx, self.Ny, self.Nz, dtype=self.dtype)
+        self.sigma = Field(self.Nx, self.Ny, self.Nz, dtype=self.dtype)
+        self.ieps.fromarray(self._h["ieps"])
+        self.imu.fromarray(self._h["imu"])
+        self.sigma.fromarray(self._h["sigma"])
+        del self._h
+
+        self._create_handle(lib)
+        self.update_tensors("all", _initial=True)
+        self._beam_line = None
+        self._J_clean_version = self.J.version
+        self.solverInitializationTime = time.time() - t0
+        if verbose:
+            print(f"Total solver initialization time: {self.solverInitializationTime} s")
+
+    # ------------------------------------------------------------------ BCs
+    def _setup_bcs(self):
+        lo, hi = self.bc_low, self.bc_high
+        self.pec_active = any(b.lower() in PEC_KEYS for b in lo) or any(
+            b.lower() in PEC_KEYS for b in hi)
+        self.pmc_active = any(b.lower() in ("magnetic", "pmc") for b in lo) or any(
+            b.lower() in ("magnetic", "pmc") for b in hi)
+        self.pec_zero = _face_zero_bits(lo, hi, lambda b: b.lower() in PEC_KEYS) \
+            if self.pec_active else 0
+        self.pmc_zero = 0
+        if self.pmc_active:
+            self.pmc_zero = self.pec_zero | _face_zero_bits(
+                lo, hi, lambda b: b.lower() == "magnetic" or b == "pmc")
+        # scipy's csr_matmat reverses a row's column order once per diagonal
+        # product; the M1 chain ends descending iff exactly one mask was applied
+        self.m1_descending = int(self.pec_active) ^ int(self.pmc_active)
+        # metric copies (boundaries.py:35-70, 152-181), in the reference's order
+        self._metric_edits = []
+        if any(b.lower() == "periodic" for b in lo):
+            for a in range(3):
+                if lo[a].lower() == "periodic" and hi[a].lower() == "periodic":
+                    self._metric_edits.append(a)
+        if any(b.lower() == "abc" for b in lo) or any(b.lower() == "abc" for b in hi):
+            for a in range(3):
+                if hi[a].lower() == "abc":
+                    self._metric_edits.append(a)
+            self.activate_abc = True
+        if any(b.lower() == "pml" for b in lo) or any(b.lower() == "pml" for b in hi):
+            self.activate_pml = True
+            self.use_conductivity = True
+        g = self.grid
+        self._L1 = [np.array(g.dx, dtype=np.float64), np.array(g.dy, dtype=np.float64),
+                    np.array(g.dz, dtype=np.float64)]
+        self._tL1_base = [np.array(g.tdx, dtype=np.float64), np.array(g.tdy, dtype=np.float64),
+                          np.array(g.tdz, dtype=np.float64)]
+        self._tL1 = [v.copy() for v in self._tL1_base]
+        for a in self._metric_edits:
+            self._tL1[a][-1] = self._L1[a][0]
+
+    def _create_handle(self, lib):
+        self._vec_keep = [np.ascontiguousarray(v, dtype=np.float64)
+                          for v in self._L1 + self._tL1]
+        P = ctypes.POINTER(ctypes.c_double)
+        ptrs = [v.ctypes.data_as(P) for v in self._vec_keep]
+        g = _lib.WfGrid(self.Nx, self.Ny, self.Nz,
+                        _lib.WF_F64 if self.dtype == np.float64 else _lib.WF_F32,
+                        self.pec_zero, self.pmc_zero, self.m1_descending, *ptrs)
+        h = ctypes.c_void_p()
+        _lib.check(lib.wf_create(ctypes.byref(g), ctypes.byref(h)), "wf_create")
+        self._handle = h
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().wf_destroy(h)
+            except Exception:
+                pass
+            self._handle = None
+
+    # ------------------------------------------------------------- materials
+    def _comp_h(self, arr, d):
+        c = {"x": 0, "y": 1, "z": 2}[d]
+        return np.reshape(arr[c * self.N:(c + 1) * self.N], self.dims, order="F")
+
+    def _apply_stl_materials(self):
+        """solverFIT3D.py:416-480 with analytic cell masks (no SIBC)."""
+        h = self._h
+        for key in self.grid.stl_solids.keys():
+            mask = np.asarray(self.grid.grid[key]).astype(int)
+            mat = self.grid.stl_materials[key]
+            if isinstance(mat, str):
+                mat = material_lib[mat.lower()]
+            eps, mu = mat[0] * eps_0, mat[1] * mu_0
+
+            def scaled_add(arr, scale3d):
+                tmp = np.zeros_like(arr)
+                for d in ("x", "y", "z"):
+                    tmp[{"x": 0, "y": 1, "z": 2}[d] * self.N:][:self.N] = np.reshape(
+                        self._comp_h(arr, d) * scale3d, self.N, order="F")
+                return arr + tmp
+
+            def mat_add(arr, m3d):
+                out = np.empty_like(arr)
+                for d in ("x", "y", "z"):
+                    out[{"x": 0, "y": 1, "z": 2}[d] * self.N:][:self.N] = np.reshape(
+                        self._comp_h(arr, d) + m3d, self.N, order="F")
+                return out
+
+            if len(mat) == 3:
+                sig = mat[2]
+                dn = np.sqrt(2) * min(self.dx.min(), self.dy.min(), self.dz.min())
+                sigma_max = 10 / (np.pi * self.fmax * mu * dn ** 2)
+                if self.use_sibc and sig > sigma_max:
+                    raise NotImplementedError(
+                        f"solid {key!r}: sigma={sig} S/m exceeds {sigma_max:.3g} S/m and "
+                        "the reference would switch to its pyvista-based SIBC surface; "
+                        "pass use_sibc=False to keep the volume conductivity")
+                h["sigma"] = scaled_add(h["sigma"], -1.0 * mask)
+                h["sigma"] = mat_add(h["sigma"], mask * sig)
+                self.use_conductivity = True
+            elif self.sigma_bg > 0.0:
+                h["sigma"] = scaled_add(h["sigma"], -1.0 * mask)
+            h["ieps"] = scaled_add(h["ieps"], -1.0 * mask)
+            h["imu"] = scaled_add(h["imu"], -1.0 * mask)
+            h["ieps"] = mat_add(h["ieps"], mask * 1.0 / eps)
+            h["imu"] = mat_add(h["imu"], mask * 1.0 / mu)
+
+    def _initialize_PML(self):
+        """Graded PML conductivity (boundaries.py:190-311): sigma only, no aux field."""
+        h = self._h
+        n = self.n_pml
+        lo, hi = 5.0e-3, 10.0
+        self.pml_lo, self.pml_hi, self.pml_func, self.pml_eps_r = lo, hi, np.geomspace, 1.0
+        for side in ("low", "high"):
+            for axis in range(3):
+                bc = self.bc_low[axis] if side == "low" else self.bc_high[axis]
+                if bc.lower() != "pml":
+                    continue
+                prof = np.zeros(self.dims[axis])
+                if side == "low":
+                    prof[0:n] = np.geomspace(hi, lo, n)
+                else:
+                    prof[-n:] = np.geomspace(lo, hi, n)
+                for d in ("x", "y", "z"):
+                    centre = [self.Nx // 2, self.Ny // 2, self.Nz // 2]
+                    centre[axis] = (n + 1) if side == "low" else -(n + 1)
+                    ieps0 = self._comp_h(h["ieps"], d)[tuple(centre)]
+                    sig0 = self._comp_h(h["sigma"], d)[tuple(centre)]
+                    mult = 1 if sig0 < 1 else sig0
+                    for m in range(n):
+                        idx = m if side == "low" else -(m + 1)
+                        sl = [slice(None)] * 3
+                        sl[axis] = idx
+                        self._comp_h(h["ieps"], d)[tuple(sl)] = ieps0
+                        self._comp_h(h["sigma"], d)[tuple(sl)] = sig0 + mult * prof[idx]
+
+    # --------------------------------------------------------- coefficients
+    def update_tensors(self, tensor="all", _initial=False):
+        """Re-fold the stencil coefficients from ieps/imu (solverFIT3D.py:356-414).
+
+        Unlike the reference, this stays valid after the first step.
+        """
+        if self.verbose and not _initial:
+            print(f'Re-computing tensor "{tensor}"...')
+        Nx, Ny, Nz, N = self.Nx, self.Ny, self.Nz, self.N
+        dev, tdt = self.device, torch_dtype(self.dtype)
+        f64 = torch.float64
+        L = [torch.as_tensor(v, dtype=f64, device=dev) for v in self._L1]
+        tL = [torch.as_tensor(v, dtype=f64, device=dev) for v in self._tL1_base]
+        shape = [(1, 1, Nx), (1, Ny, 1), (Nz, 1, 1)]
+        L = [L[a].view(shape[a]) for a in range(3)]
+        tL = [tL[a].view(shape[a]) for a in range(3)]
+        if not hasattr(self, "_a"):
+            self._a = torch.empty(3 * N, dtype=tdt, device=dev)
+            self._b = torch.empty(3 * N, dtype=tdt, device=dev)
+        others = {0: (1, 2), 1: (0, 2), 2: (0, 1)}
+        for c in range(3):
+            u, w = others[c]
+            iA = torch.broadcast_to(1.0 / (L[u] * L[w]), (Nz, Ny, Nx))
+            if tensor in ("all", "imu"):
+                a_c = iA.to(tdt) * self.imu._block(c).view(Nz, Ny, Nx)
+                if self.pmc_zero:
+                    a_c = a_c * face_mask_kji(c, self.pmc_zero, Nx, Ny, Nz, dev).to(tdt)
+                self._a[c * N:(c + 1) * N].copy_(a_c.reshape(-1))
+            if tensor in ("all", "ieps"):
+                aux = torch.broadcast_to(tL[u] * tL[w], (Nz, Ny, Nx))
+                itA = torch.where(aux != 0, 1.0 / torch.where(aux != 0, aux, 1.0),
+                                  torch.zeros((), dtype=f64, device=dev)).clone()
+                for e in self._metric_edits:  # itA[-1 along e] = iA[0 along e]
+                    if e == c:
+                        continue
+                    ax = 2 - e  # (Nz,Ny,Nx) axis of grid axis e
+                    itA.select(ax, itA.shape[ax] - 1).copy_(iA.select(ax, 0))
+                b_c = self.ieps._block(c).view(Nz, Ny, Nx) * itA.to(tdt)
+                if self.pec_zero:
+                    b_c = b_c * face_mask_kji(c, self.pec_zero, Nx, Ny, Nz, dev).to(tdt)
+                self._b[c * N:(c + 1) * N].copy_(b_c.reshape(-1))
+        torch.cuda.synchronize(dev)
+        if not _initial:
+            self.step_0 = False  # as the reference (solverFIT3D.py:414): no ghost zeroing after
+
+    # ------------------------------------------------------------ stepping
+    def _set_ghosts_to_0(self):
+        _lib.check(_lib.lib().wf_zero_ghosts(self._handle, _lib.ptr(self.E.array),
+                                             _lib.ptr(self.H.array), _lib.stream_ptr()),
+                   "wf_zero_ghosts")
+
+    def _compute_f64(self):
+        return (self.dtype == np.float32
+                and np.result_type(self.dt, np.float32) == np.dtype(np.float64))
+
+    def _jmode(self):
+        if self.use_conductivity or self.J.version != self._J_clean_version:
+            return _lib.WF_J_FULL
+        if self._beam_line is not None:
+            return _lib.WF_J_LINE
+        return _lib.WF_J_NONE
+
+    def _write_beam_line(self, ixs, iys, values):
+        """J[ixs, iys, :, 'z'] = values without marking J as user-written."""
+        line = (int(ixs), int(iys))
+        v = self.J.version
+        self.J[ixs, iys, :, "z"] = values
+        if self._beam_line is None or self._beam_line == line:
+            self.J.version = v  # J stays zero outside this one line
+            if v == self._J_clean_version:
+                self._beam_line = line
+
+    def _step_args(self, nsteps):
+        jm = self._jmode()
+        a = _lib.WfStepArgs()
+        a.E, a.H, a.J = _lib.ptr(self.E.array), _lib.ptr(self.H.array), _lib.ptr(self.J.array)
+        a.a, a.b = _lib.ptr(self._a), _lib.ptr(self._b)
+        a.ieps, a.sigma = _lib.ptr(self.ieps.array), _lib.ptr(self.sigma.array)
+        a.dt = float(self.dt)
+        a.nsteps = nsteps
+        a.jmode = jm
+        a.conductive = 1 if self.use_conductivity else 0
+        bl = self._beam_line if self._beam_line is not None else (0, 0)
+        a.ixs, a.iys = bl
+        a.path = 0
+        a.compute_f64 = 1 if self._compute_f64() else 0
+        return a
+
+    def _run(self, args):
+        _lib.check(_lib.lib().wf_step(self._handle, ctypes.byref(args), _lib.stream_ptr()),
+                   "wf_step")
+
+    def one_step(self):
+        """One leapfrog step (solverFIT3D.py:482-503)."""
+        if self.step_0:
+            self._set_ghosts_to_0()
+            self.step_0 = False
+        self._run(self._step_args(1))
+        self.nstep += 1
+
+    def _one_step(self):
+        self.one_step()
+
+    # ------------------------------------------------------- state helpers
+    def reset_fields(self):
+        """solverFIT3D.py:995-1004."""
+        for F in (self.E, self.H, self.J):
+            F.array.zero_()
+        self._J_clean_version = self.J.version
+        self._beam_line = None
+
+    def save_state(self, filename="solver_state.npz", close=True):
+        """Dynamic state E, H, J (solverFIT3D.py:886-934) as an .npz file."""
+        np.savez(filename, H=self.H.to_numpy(), E=self.E.to_numpy(), J=self.J.to_numpy())
+
+    def load_state(self, filename="solver_state.npz"):
+        """solverFIT3D.py:936-980 (single device)."""
+        with np.load(filename, allow_pickle=False) as st:
+            self.E.fromarray(st["E"])
+            self.H.fromarray(st["H"])
+            self.J.fromarray(st["J"])
+
+    def synchronize(self):
+        torch.cuda.synchronize(self.device)

@@ -1,0 +1,243 @@
+"""WakeSolver: wake potential and impedance from the device Ez capture.
+
+Mirrors ``wakis.wakeSolver.WakeSolver`` (wakeSolver.py:22-254, 310-870):
+same constructor, same ``solve`` dispatch and result attributes (s, WP, WP_3d,
+WPx, WPy, f, Z, Zx, Zy, lambdas, lambdaf). The Ez samples come from the
+device capture written during ``wakesolve`` instead of ``Ez.h5``; the line
+integrals (``wf_wake_wp``), the transverse prefix sums (``wf_wake_trans``)
+and the DFT bins (``wf_dft``) run as HIP kernels in FP64.
+"""
+
+import ctypes
+import os
+import time
+
+import numpy as np
+import torch
+from scipy.constants import c as c_light
+
+from . import _lib
+
+
+class WakeSolver:
+    def __init__(self, wakelength=None, q=1e-9, sigmaz=1e-3, beta=1.0, xsource=0.0,
+                 ysource=0.0, xtest=0.0, ytest=0.0, chargedist=None, ti=None, fmax=None,
+                 compute_plane="both", skip_cells=0, add_space=None, Ez_file=None, save=True,
+                 results_folder="results/", verbose=1, counter_moving=False):
+        self.verbose = verbose
+        self.q, self.sigmaz, self.beta = q, sigmaz, beta
+        self.v = self.beta * c_light
+        self.xsource, self.ysource = xsource, ysource
+        self.xtest, self.ytest = xtest, ytest
+        self.chargedist = chargedist
+        self.fmax = self.v / (3.0 * self.sigmaz) if fmax is None else fmax
+        self.skip_cells = skip_cells if add_space is None else add_space
+        self.compute_plane = compute_plane
+        self.counter_moving = counter_moving
+        if ti is not None:
+            self.ti = ti
+        else:  # wakeSolver.py:164-172 (CST convention for beta <= 1)
+            self.ti = 8.548921333333334 * self.sigmaz / (np.sqrt(self.beta) * self.v)
+        self.Ez_file = Ez_file
+        self.t = None
+        self.xf = self.yf = self.zf = None
+        self.wakelength = wakelength
+        self.s = self.lambdas = self.WP = self.WP_3d = None
+        self.n_transverse_cells = 1
+        self.WPx = self.WPy = self.f = self.Z = self.Zx = self.Zy = self.lambdaf = None
+        self.save = save
+        self.folder = results_folder if results_folder.endswith("/") else results_folder + "/"
+        if self.save:
+            os.makedirs(self.folder, exist_ok=True)
+        self._cap = None
+
+    def log(self, msg):
+        if self.verbose:
+            print(msg)
+
+    # capture handed over by SolverFIT3D.wakesolve
+    def _set_capture(self, cap, t, xf, yf, zf, dx, dy, dz, dtype, nsamples):
+        self._cap = cap
+        self._cap_dtype = np.dtype(dtype)
+        self._cap_samples = nsamples
+        self.t, self.xf, self.yf, self.zf = t, xf, yf, zf
+        self.dx, self.dy, self.dz = dx, dy, dz
+
+    def solve(self, compute_plane=None, **kwargs):
+        if compute_plane is None:
+            compute_plane = self.compute_plane
+        for key, val in kwargs.items():
+            setattr(self, key, val)
+        t0 = time.time()
+        if compute_plane.lower() in ["both", "transverse"]:
+            self.calc_long_WP_3d()
+            self.calc_trans_WP()
+            self.calc_long_Z()
+            self.calc_trans_Z()
+        elif compute_plane.lower() == "longitudinal":
+            self.calc_long_WP()
+            self.calc_long_Z()
+        self.log("Calculation terminated in %ds" % (time.time() - t0))
+
+    # ---------------------------------------------------------------- WP
+    def _s_axis(self):
+        nt = len(self.t)
+        dt = self.t[2] - self.t[1]
+        zmax, zmin = np.max(self.zf), np.min(self.zf)
+        if self.wakelength is not None:
+            wakelength = self.wakelength
+        else:
+            wakelength = nt * dt * self.v - (zmax - zmin) - self.ti * self.v
+            self.wakelength = wakelength
+        return np.arange(-self.ti * self.v, wakelength, dt * self.v), dt, zmin
+
+    def _device_wp(self, nlines):
+        if self._cap is None:
+            raise RuntimeError("no device Ez capture: run SolverFIT3D.wakesolve first "
+                               "(Ez.h5 input needs h5py, not part of this engine)")
+        if self.counter_moving:
+            raise NotImplementedError("counter_moving wake integration is not implemented")
+        s, dt, zmin = self._s_axis()
+        zz = slice(self.skip_cells, -self.skip_cells) if self.skip_cells != 0 else np.s_[:]
+        z = np.ascontiguousarray(self.zf[zz], dtype=np.float64)
+        nz = len(z)
+        k0 = self.skip_cells if self.skip_cells != 0 else 0
+        Nz = self._cap.shape[2]
+        nt = min(len(self.t), self._cap_samples)
+        dev = self._cap.device
+        zt = torch.as_tensor(z, device=dev)
+        dzt = torch.as_tensor(np.ascontiguousarray(self.dz[:max(nz, 1)], dtype=np.float64),
+                              device=dev)
+        st_ = torch.as_tensor(np.ascontiguousarray(s, dtype=np.float64), device=dev)
+        WP3d = torch.zeros((nlines, len(s)), dtype=torch.float64, device=dev)
+        dtype_code = _lib.WF_F64 if self._cap_dtype == np.float64 else _lib.WF_F32
+        _lib.check(_lib.lib().wf_wake_wp(
+            _lib.ptr(self._cap), dtype_code, nlines, self._cap.shape[1], Nz, k0, nz,
+            _lib.ptr(zt), _lib.ptr(dzt), _lib.ptr(st_), len(s), float(self.v),
+            float(zmin / self.v), float(self.t[0]), float(self.ti), float(dt), nt,
+            float(self.q * 1e12), _lib.ptr(WP3d), _lib.stream_ptr()), "wf_wake_wp")
+        return s, WP3d
+
+    def calc_long_WP_3d(self, **kwargs):
+        """wakeSolver.py:422-554 (3x3 lines, carry between lines)."""
+        for key, val in kwargs.items():
+            setattr(self, key, val)
+        i0 = self.n_transverse_cells
+        s, WP3d = self._device_wp((2 * i0 + 1) ** 2)
+        self._WP3d_dev = WP3d
+        self.s = s
+        self.WP_3d = WP3d.cpu().numpy().reshape(2 * i0 + 1, 2 * i0 + 1, len(s))
+        self.WP = self.WP_3d[i0, i0, :].copy()
+        self._save_txt("WP.txt", self.s, self.WP, "WP [V/pC]")
+
+    def calc_long_WP(self, Ezt=None, **kwargs):
+        """wakeSolver.py:310-420, one captured line (compute_plane='longitudinal')."""
+        for key, val in kwargs.items():
+            setattr(self, key, val)
+        if Ezt is not None:
+            raise NotImplementedError("calc_long_WP(Ezt=...) host input is not supported")
+        if self._cap.shape[0] != 1:
+            raise ValueError("calc_long_WP needs a single captured line")
+        if self.skip_cells != 0:  # wakeSolver.py:390-392 assigns the full line: shape error
+            raise ValueError("could not broadcast the captured line into Ezt with skip_cells")
+        s, WP3d = self._device_wp(1)
+        self.s = s
+        self.WP = WP3d[0].cpu().numpy()
+        self._save_txt("WP.txt", self.s, self.WP, "WP [V/pC]")
+
+    def calc_trans_WP(self, **kwargs):
+        """wakeSolver.py:556-629 on the device (numpy-exact prefix sums)."""
+        for key, val in kwargs.items():
+            setattr(self, key, val)
+        ds = self.s[2] - self.s[1]
+        dxf, dyf = np.diff(self.xf), np.diff(self.yf)
+        i0 = self.n_transverse_cells
+        ns = len(self.s)
+        WP3d = getattr(self, "_WP3d_dev", None)
+        if WP3d is None:
+            WP3d = torch.as_tensor(np.ascontiguousarray(self.WP_3d.reshape(-1, ns)),
+                                   device=torch.device("cuda"))
+        WPx = torch.zeros(ns, dtype=torch.float64, device=WP3d.device)
+        WPy = torch.zeros_like(WPx)
+        _lib.check(_lib.lib().wf_wake_trans(
+            _lib.ptr(WP3d), ns, float(ds), float(dxf[i0 - 1] + dxf[i0]),
+            float(dyf[i0 - 1] + dyf[i0]), _lib.ptr(WPx), _lib.ptr(WPy), _lib.stream_ptr()),
+            "wf_wake_trans")
+        self.WPx, self.WPy = WPx.cpu().numpy(), WPy.cpu().numpy()
+        self._save_txt("WPx.txt", self.s, self.WPx, "WP [V/pC]")
+        self._save_txt("WPy.txt", self.s, self.WPy, "WP [V/pC]")
+
+    # ----------------------------------------------------------------- Z
+    def calc_lambdas_analytic(self, **kwargs):
+        """wakeSolver.py:838-870."""
+        for key, val in kwargs.items():
+            setattr(self, key, val)
+        self.lambdas = (1 / (self.sigmaz * np.sqrt(2 * np.pi))
+                        * np.exp(-(self.s ** 2) / (2 * self.sigmaz ** 2)))
+
+    def calc_lambdas(self, **kwargs):
+        """wakeSolver.py:783-836 (interpolated user charge distribution)."""
+        for key, val in kwargs.items():
+            setattr(self, key, val)
+        if isinstance(self.chargedist, dict):
+            keys = list(self.chargedist.keys())
+            z, chargedist = self.chargedist[keys[0]], self.chargedist[keys[1]]
+        else:
+            chargedist = self.chargedist
+            z = self.zf if self.zf is not None and len(self.zf) == len(chargedist) else None
+            if z is None:
+                raise ValueError("Dimension error: check input dimensions")
+        self.lambdas = np.interp(self.s, z, chargedist / self.q)
+
+    def _dft_bins(self, signals, samples, fmax):
+        ds = np.mean(self.s[1:] - self.s[:-1])
+        N = int((self.v / ds) // fmax * samples)
+        ffft = np.fft.fftfreq(N, ds / self.v)
+        mask = np.logical_and(ffft >= 0, ffft < fmax)
+        bins = np.nonzero(mask)[0]
+        nb = len(bins)
+        if nb and not np.array_equal(bins, np.arange(nb)):
+            raise RuntimeError("unexpected DFT bin layout")
+        L = min(len(self.s), N)
+        x = np.ascontiguousarray(np.stack([np.asarray(sig, dtype=np.float64)[:L]
+                                           for sig in signals]))
+        xt = torch.as_tensor(x, device=torch.device("cuda"))
+        out = torch.zeros((len(signals), max(nb, 1), 2), dtype=torch.float64, device=xt.device)
+        _lib.check(_lib.lib().wf_dft(_lib.ptr(xt), len(signals), L, N, nb, _lib.ptr(out),
+                                     _lib.stream_ptr()), "wf_dft")
+        o = out.cpu().numpy()[:, :nb]
+        return ffft[mask], ds, o[..., 0] + 1j * o[..., 1]
+
+    def calc_long_Z(self, samples=1001, fmax=None, **kwargs):
+        """wakeSolver.py:631-719 with the kept DFT bins evaluated on the device."""
+        for key, val in kwargs.items():
+            setattr(self, key, val)
+        if self.lambdas is None and self.chargedist is not None:
+            self.calc_lambdas()
+        elif self.lambdas is None or len(self.lambdas) != len(self.s):
+            self.calc_lambdas_analytic()
+        fmax = self.fmax if fmax is None else fmax
+        f, ds, X = self._dft_bins([self.lambdas * self.v, self.WP * 1e12], samples, fmax)
+        lambdaf = X[0] * ds
+        WPf = X[1] * ds
+        self.f = f
+        self.Z = -WPf / lambdaf
+        self.lambdaf = lambdaf
+        self._save_txt("Z.txt", self.f, self.Z, "Z [Ohm]")
+
+    def calc_trans_Z(self, samples=1001, fmax=None):
+        """wakeSolver.py:721-781."""
+        fmax = self.fmax if fmax is None else fmax
+        f, ds, X = self._dft_bins([self.lambdas * self.v, self.WPx * 1e12, self.WPy * 1e12],
+                                  samples, fmax)
+        lambdaf = X[0] * ds
+        self.Zx = 1j * (X[1] * ds) / lambdaf
+        self.Zy = 1j * (X[2] * ds) / lambdaf
+        self.fx = self.fy = f
+        self._save_txt("Zx.txt", self.fx, self.Zx, "Zx [Ohm]")
+        self._save_txt("Zy.txt", self.fy, self.Zy, "Zy [Ohm]")
+
+    def _save_txt(self, name, a, b, label):
+        if self.save:
+            np.savetxt(self.folder + name, np.c_[a, b],
+                       header="   s [m]" + " " * 20 + label + "\n" + "-" * 48)
