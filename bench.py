@@ -67,6 +67,7 @@ def time_gpu(Nx, Ny, Nz, dtype, steps, warmup, path):
 
     run(0, warmup)
     torch.cuda.synchronize(dev)
+    used_path = s.last_path
     stream = torch.cuda.current_stream(dev)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
@@ -79,7 +80,7 @@ def time_gpu(Nx, Ny, Nz, dtype, steps, warmup, path):
     ms = e0.elapsed_time(e1)
     finite = bool(torch.isfinite(s.E.array).all().item())
     N = s.N
-    out = dict(N=N, steps=steps, ms_total=ms, ms_per_step=ms / steps, wall_s=wall,
+    out = dict(N=N, steps=steps, ms_total=ms, ms_per_step=ms / steps, wall_s=wall, path=used_path,
                gcell_s=N * steps / (ms * 1e-3) / 1e9, finite=finite,
                bytes_per_cell=VALUES_PER_CELL * np.dtype(dtype).itemsize)
     out["achieved_GBs"] = N * out["bytes_per_cell"] / (out["ms_per_step"] * 1e-3) / 1e9
@@ -113,7 +114,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--grid", default="256,256,512")
-    ap.add_argument("--path", default="general")
+    ap.add_argument("--path", default="auto")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-grid", default="128,128,256")
@@ -154,7 +155,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"wakefield leapfrog, PEC cavity 50mm + pipe r=5mm, "
                                f"{Nx}x{Ny}x{Nz} cells, beam on axis, PEC x/y ABC z",
-                   "cells": r32["N"], "path": args.path},
+                   "cells": r32["N"], "path": r32["path"]},
         "roofline": {"bound": "hbm", "achieved": round(r32["achieved_GBs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r32["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": None,

@@ -89,6 +89,9 @@ typedef struct wf_step_args {
   int32_t path;           /* 0: exact general two-kernel path; 1: fused fast path    */
   int32_t compute_f64;    /* 1 with WF_F32 storage: dt products and updates in FP64
                              (numpy promotion when dt is a float64 scalar)           */
+  void *E2, *H2;          /* fused path: second E/H buffers (ping-pong), 3N each    */
+  int32_t kz;             /* fused path: z-planes per workgroup (0 = 32)             */
+  int32_t result_swapped; /* out: 1 if the fields of the last step are in E2/H2      */
 } wf_step_args;
 
 int wf_version(void);
@@ -101,8 +104,14 @@ int wf_destroy(wf_handle *h);
 /* First-step ghost zeroing, SolverFIT3D._set_ghosts_to_0 (solverFIT3D.py:820-840). */
 int wf_zero_ghosts(wf_handle *h, void *E, void *H, void *stream);
 
-/* nsteps of SolverFIT3D._one_step (solverFIT3D.py:482-503). */
-int wf_step(wf_handle *h, const wf_step_args *a, void *stream);
+/* nsteps of SolverFIT3D._one_step (solverFIT3D.py:482-503).
+ * path 0 updates E/H/J in place. path 1 (fused) alternates E/H -> E2/H2 ->
+ * E/H ...; a->result_swapped tells where the last step's fields are. The
+ * fused path drops the flat-index wrap couplings (i=Nx-1 to the next row,
+ * j=Ny-1 to the next plane): callers use it only when those terms are zero.
+ * With `conductive` it computes J_old = fl(sigma*E_old) (J must hold sigma*E
+ * except on the beam line) and writes J = fl(sigma*E_new). */
+int wf_step(wf_handle *h, wf_step_args *a, void *stream);
 
 /* Copy the current Ez window into sample `sample` of the capture buffer:
  * wakesolve's save_to_h5(E[xx,yy,zz,'z']) (routines.py:419-425). */

@@ -1,0 +1,78 @@
+"""GPU: the fused single-pass kernel against the oracle (bit-identical)."""
+
+import numpy as np
+import pytest
+
+from oracle import cases
+from oracle import fit_oracle as fo
+from tests import gpu_cases as gc
+from tests.test_stencil_rules import BCS
+
+pytestmark = pytest.mark.gpu
+
+PEC_ALL = (["pec", "pec", "pec"], ["pec", "pec", "pec"])
+
+
+@pytest.mark.parametrize("dims", [(12, 10, 14), (70, 13, 37), (64, 8, 33)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("conductive", [False, True])
+@pytest.mark.parametrize("kz", [32, 5, 1])
+def test_fused_matches_oracle(dims, dtype, conductive, kz):
+    print("fused", dims, np.dtype(dtype).name, conductive, kz, flush=True)
+    case = gc.random_case(PEC_ALL, dtype, conductive, seed=sum(dims) + kz, dims=dims)
+    # the fused step has no general J read: J starts at zero (conductive: J == sigma*E
+    # is re-established by one exact step)
+    case["fields"][2] = np.zeros_like(case["fields"][2])
+    so = gc.oracle_solver(case)
+    sa = gc.amd_solver(case, path="fused")
+    sa.fused_kz = kz
+    if conductive:
+        so.J[:] = 0
+        so.one_step()          # reference step with J = 0
+        sa.path = "general"    # the first step (J not sigma*E yet) runs exactly
+        sa.one_step()
+        sa.path = "fused"
+    for _ in range(5):
+        so.one_step()
+        sa.one_step()
+        assert sa.last_path == "fused"
+    for name in ("E", "H", "J"):
+        got, ref = getattr(sa, name).to_numpy(), getattr(so, name)
+        assert np.array_equal(got, ref), f"{name}: rel-L2 {gc.rel_l2(got, ref):.3e}"
+
+
+def test_fused_falls_back_when_wraps_live():
+    case = gc.random_case(BCS[2], np.float64, False, seed=3)
+    sa = gc.amd_solver(case)
+    so = gc.oracle_solver(case)
+    for _ in range(3):
+        so.one_step()
+        sa.one_step()
+    assert sa.last_path == "general"
+    assert np.array_equal(sa.E.to_numpy(), so.E) and np.array_equal(sa.H.to_numpy(), so.H)
+    with pytest.raises(RuntimeError):
+        sb = gc.amd_solver(case, path="fused")
+        sb.one_step()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_fused_beam_loop_matches_oracle(dtype):
+    """Beam.update + one_step user loop on the cavity/pipe (beam-line J mode)."""
+    import wakis_amd as wa
+    x, y, z, mask = cases.cavity_pipe_geometry(24, 24, 40)
+    bc_lo, bc_hi = ["pec", "pec", "abc"], ["pec", "pec", "abc"]
+    so = fo.OracleSolver(x, y, z, bc_lo, bc_hi, bg="pec", solids=[(mask, "vacuum")],
+                         dtype=dtype)
+    grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"cav": mask},
+                        stl_materials={"cav": "vacuum"}, verbose=0)
+    sa = wa.SolverFIT3D(grid, bc_low=bc_lo, bc_high=bc_hi, use_stl=True, bg="pec",
+                        dtype=dtype, verbose=0, path="fused")
+    bo, ba = fo.OracleBeam(q=1e-9, sigmaz=5e-3), wa.Beam(q=1e-9, sigmaz=5e-3)
+    for n in range(80):
+        bo.update(so, n * so.dt)
+        ba.update(sa, n * sa.dt)
+        so.one_step()
+        sa.one_step()
+    assert sa.last_path == "fused"
+    for name in ("E", "H", "J"):
+        assert np.array_equal(getattr(sa, name).to_numpy(), getattr(so, name)), name

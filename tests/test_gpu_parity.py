@@ -30,7 +30,7 @@ def test_step_matches_oracle(bci, dtype, conductive):
     _say("case", bci, np.dtype(dtype).name, conductive)
     case = gc.random_case(BCS[bci], dtype, conductive, seed=11 + bci)
     so = gc.oracle_solver(case)
-    sa = gc.amd_solver(case)
+    sa = gc.amd_solver(case, path="general")
     assert sa.dt == so.dt and type(sa.dt) is type(so.dt)
     for _ in range(5):
         so.one_step()
@@ -79,6 +79,7 @@ def _run_both_wakes(Nx, Ny, Nz, dtype, wakelength, skip, plane="both"):
                         bc_high=["pec", "pec", "abc"], use_stl=True, bg="pec", dtype=dtype,
                         verbose=0)
     sa.wakesolve(wakelength=wakelength, compute_plane=plane)
+    assert sa.last_path == "fused", "the cavity/pipe wakefield run should use the fused kernel"
     return so, sa, ref, wake
 
 

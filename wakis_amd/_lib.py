@@ -35,6 +35,7 @@ class WfStepArgs(ctypes.Structure):
         ("cap_ix0", c_int32), ("cap_iy0", c_int32), ("cap_nx", c_int32), ("cap_ny", c_int32),
         ("cap_samples", c_int64), ("cap_sample0", c_int64),
         ("path", c_int32), ("compute_f64", c_int32),
+        ("E2", c_void_p), ("H2", c_void_p), ("kz", c_int32), ("result_swapped", c_int32),
     ]
 
 

@@ -10,6 +10,7 @@
 #include "../../include/wakis_fit.h"
 #include "fit_common.h"
 #include "step_general.h"
+#include "step_fused.h"
 #include "wake.h"
 
 namespace {
@@ -168,11 +169,51 @@ int step_general(wf_handle *h, const wf_step_args *a, const wf::MetricDev<S> &m,
   return WF_OK;
 }
 
+constexpr int FTX = 64, FTY = 4;
+
+template <typename S, typename Cm>
+int step_fused(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStream_t st) {
+  const wf::GridDev &g = h->g;
+  wf::FusedAux x;
+  x.ixs = (a->jmode == WF_J_LINE || a->beam_table || a->conductive) ? a->ixs : -1;
+  x.iys = (x.ixs >= 0) ? a->iys : -1;
+  x.cap_ix0 = a->cap_ix0; x.cap_iy0 = a->cap_iy0; x.cap_nx = a->cap_nx; x.cap_ny = a->cap_ny;
+  x.cap_samples = a->cap_samples;
+  x.KZ = a->kz > 0 ? a->kz : 32;
+  x.ntx = (g.Nx + FTX - 1) / FTX;
+  x.nty = (g.Ny + FTY - 1) / FTY;
+  x.ntz = (g.Nz + x.KZ - 1) / x.KZ;
+  const int64_t nb = (int64_t)x.ntx * x.nty * x.ntz;
+  if (nb > 2147483647LL) return fail(WF_ERANGE, "wf_step: grid too large for the fused path");
+  const Cm dt = (Cm)a->dt;
+  S *Ebuf[2] = {(S *)a->E, (S *)a->E2};
+  S *Hbuf[2] = {(S *)a->H, (S *)a->H2};
+  int cur = 0;
+  for (int s = 0; s < a->nsteps; ++s) {
+    const S *row = a->beam_table ? (const S *)a->beam_table + (a->beam_row0 + s) * g.Nz : nullptr;
+    x.cap_sample = a->cap_sample0 + s;
+    S *cap = (S *)a->capture;
+#define WF_F_LAUNCH(JM, CO)                                                                  \
+  hipLaunchKernelGGL((wf::k_fused<S, Cm, JM, CO, FTX, FTY>), dim3((unsigned)nb),            \
+                     dim3(FTX * FTY), 0, st, g, m, Ebuf[cur], Hbuf[cur], Ebuf[1 - cur],     \
+                     Hbuf[1 - cur], (S *)a->J, (const S *)a->a, (const S *)a->b,           \
+                     (const S *)a->ieps, (const S *)a->sigma, dt, x, row, cap)
+    if (a->conductive) WF_F_LAUNCH(0, true);
+    else if (a->jmode == WF_J_LINE) WF_F_LAUNCH(1, false);
+    else WF_F_LAUNCH(0, false);
+#undef WF_F_LAUNCH
+    WF_LAUNCH_CHECK();
+    cur = 1 - cur;
+  }
+  a->result_swapped = cur;
+  return WF_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
-int wf_step(wf_handle *h, const wf_step_args *a, void *stream) {
+int wf_step(wf_handle *h, wf_step_args *a, void *stream) {
   if (!h || !a) return fail(WF_EINVAL, "wf_step: null argument");
   if (!a->E || !a->H || !a->a || !a->b) return fail(WF_EINVAL, "wf_step: null field/coefficient");
   if (a->nsteps < 0) return fail(WF_EINVAL, "wf_step: nsteps < 0");
@@ -198,8 +239,21 @@ int wf_step(wf_handle *h, const wf_step_args *a, void *stream) {
     if (a->cap_sample0 < 0 || a->cap_samples < 1)
       return fail(WF_ERANGE, "wf_step: bad capture sample range");
   }
-  if (a->path != 0) return fail(WF_EINVAL, "wf_step: path %d not available", a->path);
+  a->result_swapped = 0;
   hipStream_t st = (hipStream_t)stream;
+  if (a->path == 1) {
+    if (!a->E2 || !a->H2 || a->E2 == a->E || a->H2 == a->H || a->E2 == a->H2)
+      return fail(WF_EINVAL, "wf_step: fused path needs distinct E2/H2 buffers");
+    if (a->jmode == WF_J_FULL && !a->conductive)
+      return fail(WF_EINVAL, "wf_step: fused path has no general J read (use path 0)");
+    if (a->conductive && (!a->J || !a->ieps || !a->sigma))
+      return fail(WF_EINVAL, "wf_step: fused conductive step needs J, ieps and sigma");
+    if (a->kz < 0) return fail(WF_EINVAL, "wf_step: kz < 0");
+    if (h->dtype == WF_F64) return step_fused<double, double>(h, a, h->md, st);
+    if (a->compute_f64) return step_fused<float, double>(h, a, h->mf, st);
+    return step_fused<float, float>(h, a, h->mf, st);
+  }
+  if (a->path != 0) return fail(WF_EINVAL, "wf_step: path %d not available", a->path);
   if (h->dtype == WF_F64) return step_general<double, double>(h, a, h->md, st);
   if (a->compute_f64) return step_general<float, double>(h, a, h->mf, st);
   return step_general<float, float>(h, a, h->mf, st);

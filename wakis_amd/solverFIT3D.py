@@ -70,7 +70,7 @@ class SolverFIT3D(RoutinesMixin):
                  bc_high=["Periodic", "Periodic", "Periodic"],
                  use_stl=False, use_conductors=False, use_gpu=True, use_mpi=False,
                  use_sibc=True, fmax=1e9, dtype=np.float64, n_pml=10, bg=[1.0, 1.0],
-                 verbose=1, path="general"):
+                 verbose=1, path="auto"):
         t0 = time.time()
         if use_mpi:
             raise NotImplementedError("wakis_amd runs the full domain on one GPU (use_mpi=False)")
@@ -152,10 +152,18 @@ class SolverFIT3D(RoutinesMixin):
         self.sigma.fromarray(self._h["sigma"])
         del self._h
 
+        if path not in ("auto", "general", "fused"):
+            raise ValueError("path must be 'auto', 'general' or 'fused'")
         self._create_handle(lib)
+        self._fold_count = 0
+        self._wrap_key = None
+        self._E2 = self._H2 = None
+        self.fused_kz = 32  # z-planes per workgroup of the fused kernel
         self.update_tensors("all", _initial=True)
         self._beam_line = None
         self._J_clean_version = self.J.version
+        self._sigmaE_versions = (self.E.version, self.J.version)  # J == sigma*E (all zero)
+        self.last_path = None
         self.solverInitializationTime = time.time() - t0
         if verbose:
             print(f"Total solver initialization time: {self.solverInitializationTime} s")
@@ -339,6 +347,7 @@ class SolverFIT3D(RoutinesMixin):
                     b_c = b_c * face_mask_kji(c, self.pec_zero, Nx, Ny, Nz, dev).to(tdt)
                 self._b[c * N:(c + 1) * N].copy_(b_c.reshape(-1))
         torch.cuda.synchronize(dev)
+        self._fold_count = getattr(self, "_fold_count", 0) + 1
         if not _initial:
             self.step_0 = False  # as the reference (solverFIT3D.py:414): no ghost zeroing after
 
@@ -379,15 +388,152 @@ class SolverFIT3D(RoutinesMixin):
         a.nsteps = nsteps
         a.jmode = jm
         a.conductive = 1 if self.use_conductivity else 0
-        bl = self._beam_line if self._beam_line is not None else (0, 0)
+        bl = self._beam_line if self._beam_line is not None else (-1, -1)
         a.ixs, a.iys = bl
         a.path = 0
         a.compute_f64 = 1 if self._compute_f64() else 0
         return a
 
+    # -------------------------------------------------------- fast path
+    def _sigmaE_holds(self):
+        """J == fl(sigma*E) everywhere except the registered beam line."""
+        return (self.E.version, self.J.version) == self._sigmaE_versions
+
+    def _choose_fused(self, args):
+        if self.path == "general":
+            return False
+        why = None
+        if self.use_conductivity:
+            if not self._sigmaE_holds():
+                why = "J is not sigma*E (user write since the last conductive step)"
+        elif args.jmode == _lib.WF_J_FULL:
+            why = "J was written outside the beam line"
+        if why is None and not self._wraps_dead():
+            why = "flat-index wrap terms are live for this boundary/material set-up"
+        if why is not None:
+            if self.path == "fused":
+                raise RuntimeError(f"fused path not applicable: {why}")
+            return False
+        return True
+
+    def _wraps_dead(self):
+        """True if every flat-index wrap term of M1/M2 contributes exactly zero.
+
+        The reference operator couples i=Nx-1 to (0, j+1, k) and j=Ny-1 to
+        (i, 0, k+1) (and the transposes). A term is dead if its weight is zero
+        (row coefficient or column mask) or its column is frozen at zero:
+        E with b=0, E=0 and no live source (ieps=0 or J=0, not on the beam
+        line); H with a=0 and H=0. Frozen columns never change, so the result
+        holds until E/H/J are written from outside or coefficients re-folded.
+        """
+        key = (self.E.version, self.H.version, self.J.version, self._fold_count,
+               self._beam_line)
+        if key == self._wrap_key:
+            return self._wrap_ok
+        Nx, Ny, Nz, N = self.Nx, self.Ny, self.Nz, self.N
+        dev = self.device
+        ar = torch.arange
+        a, b = self._a, self._b
+        E, H, J, ie = self.E.array, self.H.array, self.J.array, self.ieps.array
+        bl = self._beam_line
+
+        def lin(i, j, k):
+            return i + Nx * j + Nx * Ny * k
+
+        def fmask(zbits, comp, i, j, k):
+            if zbits == 0:
+                return torch.ones_like(i, dtype=torch.bool)
+            f = torch.full_like(i, -1)
+            order = {0: [(j, 0, 1), (j, Ny - 1, 4), (k, 0, 2), (k, Nz - 1, 5)],
+                     1: [(i, 0, 0), (i, Nx - 1, 3), (k, 0, 2), (k, Nz - 1, 5)],
+                     2: [(i, 0, 0), (i, Nx - 1, 3), (j, 0, 1), (j, Ny - 1, 4)]}[comp]
+            for idx, val, face in order:
+                f = torch.where(idx == val, torch.full_like(f, face), f)
+            zero = torch.zeros_like(i, dtype=torch.bool)
+            for face in range(6):
+                if (zbits >> face) & 1:
+                    zero |= f == face
+            return ~zero
+
+        def frozen_E(c, i, j, k):
+            t = c * N + lin(i, j, k)
+            fz = (b[t] == 0) & (E[t] == 0) & ((ie[t] == 0) | (J[t] == 0))
+            if bl is not None and c == 2:
+                fz &= ~((i == bl[0]) & (j == bl[1]))
+            return fz
+
+        def frozen_H(c, i, j, k):
+            t = c * N + lin(i, j, k)
+            return (a[t] == 0) & (H[t] == 0)
+
+        bad = torch.zeros((), dtype=torch.bool, device=dev)
+        # (1) H_y, H_z rows at i=Nx-1 use E_z, E_y at (0, j+1, k) / (0, 0, k+1)
+        jj, kk = torch.meshgrid(ar(Ny, device=dev), ar(Nz, device=dev), indexing="ij")
+        jj, kk = jj.reshape(-1), kk.reshape(-1)
+        valid = ~((jj == Ny - 1) & (kk == Nz - 1))
+        tj = torch.where(jj < Ny - 1, jj + 1, torch.zeros_like(jj))
+        tk = torch.where(jj < Ny - 1, kk, kk + 1).clamp(max=Nz - 1)
+        ti = torch.zeros_like(jj)
+        rn = lin(torch.full_like(jj, Nx - 1), jj, kk)
+        for rc, cc in ((1, 2), (2, 1)):
+            w = valid & (a[rc * N + rn] != 0) & fmask(self.pec_zero, cc, ti, tj, tk)
+            bad |= (w & ~frozen_E(cc, ti, tj, tk)).any()
+        # (2) H_x, H_z rows at j=Ny-1 use E_z, E_x at (i, 0, k+1)
+        ii, kk = torch.meshgrid(ar(Nx, device=dev), ar(Nz - 1, device=dev), indexing="ij")
+        ii, kk = ii.reshape(-1), kk.reshape(-1)
+        rn = lin(ii, torch.full_like(ii, Ny - 1), kk)
+        tj, tk = torch.zeros_like(ii), kk + 1
+        for rc, cc in ((0, 2), (2, 0)):
+            w = (a[rc * N + rn] != 0) & fmask(self.pec_zero, cc, ii, tj, tk)
+            bad |= (w & ~frozen_E(cc, ii, tj, tk)).any()
+        # (3) E_y, E_z rows at i=0 use H_z, H_y at (Nx-1, j-1, k) / (Nx-1, Ny-1, k-1)
+        jj, kk = torch.meshgrid(ar(Ny, device=dev), ar(Nz, device=dev), indexing="ij")
+        jj, kk = jj.reshape(-1), kk.reshape(-1)
+        valid = ~((jj == 0) & (kk == 0))
+        sj = torch.where(jj > 0, jj - 1, torch.full_like(jj, Ny - 1))
+        sk = torch.where(jj > 0, kk, kk - 1).clamp(min=0)
+        si = torch.full_like(jj, Nx - 1)
+        rn = lin(torch.zeros_like(jj), jj, kk)
+        tLz = torch.as_tensor(self._tL1[2], device=dev)
+        tLy = torch.as_tensor(self._tL1[1], device=dev)
+        tLx = torch.as_tensor(self._tL1[0], device=dev)
+        for rc, cc, tl in ((1, 2, tLz[sk]), (2, 1, tLy[sj])):
+            w = valid & (b[rc * N + rn] != 0) & (tl != 0) & fmask(self.pmc_zero, cc, si, sj, sk)
+            bad |= (w & ~frozen_H(cc, si, sj, sk)).any()
+        # (4) E_x, E_z rows at j=0 use H_z, H_x at (i, Ny-1, k-1)
+        ii, kk = torch.meshgrid(ar(Nx, device=dev), ar(1, Nz, device=dev), indexing="ij")
+        ii, kk = ii.reshape(-1), kk.reshape(-1)
+        rn = lin(ii, torch.zeros_like(ii), kk)
+        sj, sk = torch.full_like(ii, Ny - 1), kk - 1
+        for rc, cc, tl in ((0, 2, tLz[sk]), (2, 0, tLx[ii])):
+            w = (b[rc * N + rn] != 0) & (tl != 0) & fmask(self.pmc_zero, cc, ii, sj, sk)
+            bad |= (w & ~frozen_H(cc, ii, sj, sk)).any()
+        ok = not bool(bad.item())
+        self._wrap_key, self._wrap_ok = key, ok
+        return ok
+
     def _run(self, args):
+        """Run one wf_step batch on the fast or the exact path."""
+        fused = self._choose_fused(args)
+        if fused:
+            if self._E2 is None:
+                self._E2 = torch.empty_like(self.E.array)
+                self._H2 = torch.empty_like(self.H.array)
+            args.path = 1
+            args.E2, args.H2 = _lib.ptr(self._E2), _lib.ptr(self._H2)
+            args.kz = int(self.fused_kz)
+            if args.jmode == _lib.WF_J_FULL and not args.conductive:
+                raise RuntimeError("internal: fused path without a J mode")
+            if args.conductive:
+                args.jmode = _lib.WF_J_FULL
         _lib.check(_lib.lib().wf_step(self._handle, ctypes.byref(args), _lib.stream_ptr()),
                    "wf_step")
+        self.last_path = "fused" if fused else "general"
+        if fused and args.result_swapped:
+            self.E.array, self._E2 = self._E2, self.E.array
+            self.H.array, self._H2 = self._H2, self.H.array
+        if self.use_conductivity:
+            self._sigmaE_versions = (self.E.version, self.J.version)
 
     def one_step(self):
         """One leapfrog step (solverFIT3D.py:482-503)."""
