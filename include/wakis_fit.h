@@ -90,8 +90,22 @@ typedef struct wf_step_args {
   int32_t compute_f64;    /* 1 with WF_F32 storage: dt products and updates in FP64
                              (numpy promotion when dt is a float64 scalar)           */
   void *E2, *H2;          /* fused path: second E/H buffers (ping-pong), 3N each    */
-  int32_t kz;             /* fused path: z-planes per workgroup (0 = 32)             */
+  int32_t kz;             /* fused path: z-planes per workgroup (0 = automatic)      */
   int32_t result_swapped; /* out: 1 if the fields of the last step are in E2/H2      */
+  /* Fused path, optional compressed coefficients: instead of reading a/b
+   * (and ieps/sigma) per cell, read a uint8 material code per cell and
+   * recompute a = fl(iA*imu), b = fl(ieps*itA) from a material table and
+   * 2-D metric tables (must reproduce the a/b arrays exactly). */
+  const void *code;       /* device uint8[N]                                          */
+  const void *mat;        /* device [nmat][9]: imu_xyz, ieps_xyz, sigma_xyz (dtype)   */
+  const void *metric2d;   /* device, dtype: iA_x[Ny*Nz] (j+Ny*k), iA_y[Nx*Nz] (i+Nx*k),
+                             iA_z[Nx*Ny] (i+Nx*j), then itA_x, itA_y, itA_z likewise
+                             (itA after the periodic/ABC copies)                     */
+  int32_t nmat;           /* 1..256                                                   */
+  int32_t variant;        /* fused kernel choice: 0 auto (x-vectorised when Nx is a
+                             multiple of 16/sizeof(dtype): 8 rows/workgroup FP32, 2
+                             FP64), 1 scalar, 2 vectorised 8 rows (FP32; 4 for FP64),
+                             3 vectorised 2 rows, 4 vectorised 4 rows             */
 } wf_step_args;
 
 int wf_version(void);

@@ -16,9 +16,11 @@ PEC_ALL = (["pec", "pec", "pec"], ["pec", "pec", "pec"])
 @pytest.mark.parametrize("dims", [(12, 10, 14), (70, 13, 37), (64, 8, 33)])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("conductive", [False, True])
-@pytest.mark.parametrize("kz", [32, 5, 1])
-def test_fused_matches_oracle(dims, dtype, conductive, kz):
-    print("fused", dims, np.dtype(dtype).name, conductive, kz, flush=True)
+@pytest.mark.parametrize("kz", [0, 5, 1])
+@pytest.mark.parametrize("use_code", [True, False])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+def test_fused_matches_oracle(dims, dtype, conductive, kz, use_code, variant):
+    print("fused", dims, np.dtype(dtype).name, conductive, kz, use_code, variant, flush=True)
     case = gc.random_case(PEC_ALL, dtype, conductive, seed=sum(dims) + kz, dims=dims)
     # the fused step has no general J read: J starts at zero (conductive: J == sigma*E
     # is re-established by one exact step)
@@ -26,6 +28,8 @@ def test_fused_matches_oracle(dims, dtype, conductive, kz):
     so = gc.oracle_solver(case)
     sa = gc.amd_solver(case, path="fused")
     sa.fused_kz = kz
+    sa.use_material_code = use_code
+    sa.fused_variant = variant
     if conductive:
         so.J[:] = 0
         so.one_step()          # reference step with J = 0
@@ -36,6 +40,7 @@ def test_fused_matches_oracle(dims, dtype, conductive, kz):
         so.one_step()
         sa.one_step()
         assert sa.last_path == "fused"
+    assert (sa._code is not None) == use_code
     for name in ("E", "H", "J"):
         got, ref = getattr(sa, name).to_numpy(), getattr(so, name)
         assert np.array_equal(got, ref), f"{name}: rel-L2 {gc.rel_l2(got, ref):.3e}"
@@ -56,17 +61,26 @@ def test_fused_falls_back_when_wraps_live():
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_fused_beam_loop_matches_oracle(dtype):
-    """Beam.update + one_step user loop on the cavity/pipe (beam-line J mode)."""
+@pytest.mark.parametrize("zbc", ["abc", "pml"])
+@pytest.mark.parametrize("use_code", [True, False])
+@pytest.mark.parametrize("variant", [0, 1])
+def test_fused_beam_loop_matches_oracle(dtype, zbc, use_code, variant):
+    """Beam.update + one_step user loop on the cavity/pipe (beam-line J mode;
+    with PML on z the conductive fused step), with and without the material
+    code table."""
     import wakis_amd as wa
+    print("beam loop", np.dtype(dtype).name, zbc, use_code, flush=True)
     x, y, z, mask = cases.cavity_pipe_geometry(24, 24, 40)
-    bc_lo, bc_hi = ["pec", "pec", "abc"], ["pec", "pec", "abc"]
+    bc_lo, bc_hi = ["pec", "pec", zbc], ["pec", "pec", zbc]
     so = fo.OracleSolver(x, y, z, bc_lo, bc_hi, bg="pec", solids=[(mask, "vacuum")],
-                         dtype=dtype)
+                         dtype=dtype, n_pml=4)
     grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"cav": mask},
                         stl_materials={"cav": "vacuum"}, verbose=0)
     sa = wa.SolverFIT3D(grid, bc_low=bc_lo, bc_high=bc_hi, use_stl=True, bg="pec",
-                        dtype=dtype, verbose=0, path="fused")
+                        dtype=dtype, verbose=0, path="fused", n_pml=4)
+    sa.use_material_code = use_code
+    sa.fused_variant = variant
+    assert sa.dt == so.dt
     bo, ba = fo.OracleBeam(q=1e-9, sigmaz=5e-3), wa.Beam(q=1e-9, sigmaz=5e-3)
     for n in range(80):
         bo.update(so, n * so.dt)
@@ -74,5 +88,6 @@ def test_fused_beam_loop_matches_oracle(dtype):
         so.one_step()
         sa.one_step()
     assert sa.last_path == "fused"
+    assert (sa._code is not None) == use_code
     for name in ("E", "H", "J"):
         assert np.array_equal(getattr(sa, name).to_numpy(), getattr(so, name)), name

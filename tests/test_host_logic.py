@@ -113,3 +113,28 @@ def test_wrap_proof_is_sound(host_only, kind, bci, seed):
         assert same, "proof says the wrap terms are dead but dropping them changes the run"
     if kind == "cavity":
         assert dead, "the cavity/pipe set-up should be provably wrap-free"
+
+
+@pytest.mark.parametrize("conductive", [False, True])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_material_table_reproduces_coefficients(host_only, conductive, dtype):
+    x, y, z, mask = cases.cavity_pipe_geometry(20, 20, 32)
+    zb = "pml" if conductive else "abc"
+    case = dict(x=x, y=y, z=z, bc=(["pec", "pec", zb], ["pec", "pec", zb]),
+                bg="pec", solids=[("cav", mask, "vacuum")], dtype=dtype, n_pml=4,
+                fields=[np.zeros(3 * mask.size)] * 3)
+    sa = gc.amd_solver(case)
+    assert sa.use_conductivity == conductive
+    assert sa._material_table(), "compressed coefficients rejected"
+    assert sa._nmat <= 256 and sa._code.dtype == torch.uint8
+
+
+def test_material_table_on_random_grid(host_only):
+    """Non-uniform grid, three materials: still exact (metric tables are 2-D)."""
+    case = gc.random_case((["pec", "pec", "pec"], ["pec", "pec", "pec"]), np.float64, True,
+                          seed=4)
+    sa = gc.amd_solver(case)
+    assert sa._material_table()
+    sa.ieps.array[0] += 1.0  # edit without update_tensors -> arrays again
+    sa.ieps.version += 1
+    assert not sa._material_table()

@@ -11,6 +11,7 @@
 #include "fit_common.h"
 #include "step_general.h"
 #include "step_fused.h"
+#include "step_fused4.h"
 #include "wake.h"
 
 namespace {
@@ -171,9 +172,92 @@ int step_general(wf_handle *h, const wf_step_args *a, const wf::MetricDev<S> &m,
 
 constexpr int FTX = 64, FTY = 4;
 
+template <typename S, int TY>
+int step_fused4(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStream_t st) {
+  const wf::GridDev &g = h->g;
+  wf::FusedAux x;
+  x.ixs = (a->jmode == WF_J_LINE || a->beam_table || a->conductive) ? a->ixs : -1;
+  x.iys = (x.ixs >= 0) ? a->iys : -1;
+  x.cap_ix0 = a->cap_ix0; x.cap_iy0 = a->cap_iy0; x.cap_nx = a->cap_nx; x.cap_ny = a->cap_ny;
+  x.cap_samples = a->cap_samples;
+  constexpr int Wc = 64 * (16 / (int)sizeof(S));
+  x.ntx = (g.Nx + Wc - 1) / Wc;
+  x.nty = (g.Ny + TY - 1) / TY;
+  if (a->kz > 0) {
+    x.KZ = a->kz;
+  } else {  // about 256 workgroups: long z marches, one wave of tiles over the CUs
+    const int64_t nxy = (int64_t)x.ntx * x.nty;
+    int64_t kz = (g.Nz * nxy + 255) / 256;
+    x.KZ = (int)(kz < 8 ? 8 : (kz > 64 ? 64 : kz));
+  }
+  x.ntz = (g.Nz + x.KZ - 1) / x.KZ;
+  const int64_t nb = (int64_t)x.ntx * x.nty * x.ntz;
+  if (nb > 2147483647LL) return fail(WF_ERANGE, "wf_step: grid too large for the fused path");
+  const S dt = (S)a->dt;
+  S *Ebuf[2] = {(S *)a->E, (S *)a->E2};
+  S *Hbuf[2] = {(S *)a->H, (S *)a->H2};
+  wf::Coefs<S> cf;
+  cf.a = (const S *)a->a; cf.b = (const S *)a->b;
+  cf.ieps = (const S *)a->ieps; cf.sigma = (const S *)a->sigma;
+  cf.code = (const uint8_t *)a->code; cf.mat = (const S *)a->mat; cf.nmat = a->nmat;
+  const bool lut = a->code && a->mat && a->nmat > 0 && a->metric2d;
+  if (lut) {
+    const S *t = (const S *)a->metric2d;
+    const int64_t yz = (int64_t)g.Ny * g.Nz, xz = (int64_t)g.Nx * g.Nz, xy = (int64_t)g.Nx * g.Ny;
+    cf.iAx = t; cf.iAy = t + yz; cf.iAz = t + yz + xz;
+    cf.itAx = t + yz + xz + xy; cf.itAy = cf.itAx + yz; cf.itAz = cf.itAy + xz;
+  }
+  const size_t lds = lut ? (size_t)a->nmat * 9 * sizeof(S) : 0;
+  int cur = 0;
+  for (int s = 0; s < a->nsteps; ++s) {
+    const S *row = a->beam_table ? (const S *)a->beam_table + (a->beam_row0 + s) * g.Nz : nullptr;
+    x.cap_sample = a->cap_sample0 + s;
+    S *cap = (S *)a->capture;
+#define WF_G_LAUNCH3(I, JM, CO, LU, DE)                                                       \
+  hipLaunchKernelGGL((wf::k_fused4<S, JM, CO, LU, DE, TY>), dim3((unsigned)nb),               \
+                     dim3(64 * (TY + 1)), lds, st, g, m, Ebuf[cur], Hbuf[cur], Ebuf[1 - cur], \
+                     Hbuf[1 - cur], (S *)a->J, cf, dt, x, row, cap)
+#define WF_G_LAUNCH2(I, JM, CO, LU)                                                          \
+  if (g.m1desc) WF_G_LAUNCH3(I, JM, CO, LU, true);                                           \
+  else WF_G_LAUNCH3(I, JM, CO, LU, false)
+#define WF_G_LAUNCH(JM, CO, LU)                                                              \
+  do {                                                                                       \
+    WF_G_LAUNCH2(uint32_t, JM, CO, LU);                                                      \
+  } while (0)
+    if (lut) {
+      if (a->conductive) WF_G_LAUNCH(0, true, true);
+      else if (a->jmode == WF_J_LINE) WF_G_LAUNCH(1, false, true);
+      else WF_G_LAUNCH(0, false, true);
+    } else {
+      if (a->conductive) WF_G_LAUNCH(0, true, false);
+      else if (a->jmode == WF_J_LINE) WF_G_LAUNCH(1, false, false);
+      else WF_G_LAUNCH(0, false, false);
+    }
+#undef WF_G_LAUNCH
+#undef WF_G_LAUNCH2
+#undef WF_G_LAUNCH3
+    WF_LAUNCH_CHECK();
+    cur = 1 - cur;
+  }
+  a->result_swapped = cur;
+  return WF_OK;
+}
+
 template <typename S, typename Cm>
 int step_fused(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStream_t st) {
   const wf::GridDev &g = h->g;
+  // x-vectorised kernel: Nx a multiple of the lane width (16 bytes) and every
+  // 3N array within 4 GiB
+  const bool idx32v = (uint64_t)3 * (uint64_t)g.N * sizeof(S) < (1ull << 32);
+  if (g.Nx % (int)(16 / sizeof(S)) == 0 && idx32v && a->variant != 1) {
+    if constexpr (sizeof(S) == 4) {
+      if (a->variant == 2 || a->variant == 0) return step_fused4<S, 8>(h, a, m, st);
+    } else {
+      if (a->variant == 0) return step_fused4<S, 2>(h, a, m, st);
+    }
+    if (a->variant == 3) return step_fused4<S, 2>(h, a, m, st);
+    return step_fused4<S, 4>(h, a, m, st);  // variant 4 (and 2 for FP64)
+  }
   wf::FusedAux x;
   x.ixs = (a->jmode == WF_J_LINE || a->beam_table || a->conductive) ? a->ixs : -1;
   x.iys = (x.ixs >= 0) ? a->iys : -1;
@@ -188,20 +272,48 @@ int step_fused(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStre
   const Cm dt = (Cm)a->dt;
   S *Ebuf[2] = {(S *)a->E, (S *)a->E2};
   S *Hbuf[2] = {(S *)a->H, (S *)a->H2};
+  wf::Coefs<S> cf;
+  cf.a = (const S *)a->a; cf.b = (const S *)a->b;
+  cf.ieps = (const S *)a->ieps; cf.sigma = (const S *)a->sigma;
+  cf.code = (const uint8_t *)a->code; cf.mat = (const S *)a->mat; cf.nmat = a->nmat;
+  const bool lut = a->code && a->mat && a->nmat > 0 && a->metric2d;
+  if (lut) {
+    const S *t = (const S *)a->metric2d;
+    const int64_t yz = (int64_t)g.Ny * g.Nz, xz = (int64_t)g.Nx * g.Nz, xy = (int64_t)g.Nx * g.Ny;
+    cf.iAx = t; cf.iAy = t + yz; cf.iAz = t + yz + xz;
+    cf.itAx = t + yz + xz + xy; cf.itAy = cf.itAx + yz; cf.itAz = cf.itAy + xz;
+  }
+  const size_t lds = lut ? (size_t)a->nmat * 9 * sizeof(S) : 0;
+  // 32-bit byte offsets when every 3N array fits in 4 GiB
+  const bool idx32 = (uint64_t)3 * (uint64_t)g.N * sizeof(S) < (1ull << 32);
   int cur = 0;
   for (int s = 0; s < a->nsteps; ++s) {
     const S *row = a->beam_table ? (const S *)a->beam_table + (a->beam_row0 + s) * g.Nz : nullptr;
     x.cap_sample = a->cap_sample0 + s;
     S *cap = (S *)a->capture;
-#define WF_F_LAUNCH(JM, CO)                                                                  \
-  hipLaunchKernelGGL((wf::k_fused<S, Cm, JM, CO, FTX, FTY>), dim3((unsigned)nb),            \
-                     dim3(FTX * FTY), 0, st, g, m, Ebuf[cur], Hbuf[cur], Ebuf[1 - cur],     \
-                     Hbuf[1 - cur], (S *)a->J, (const S *)a->a, (const S *)a->b,           \
-                     (const S *)a->ieps, (const S *)a->sigma, dt, x, row, cap)
-    if (a->conductive) WF_F_LAUNCH(0, true);
-    else if (a->jmode == WF_J_LINE) WF_F_LAUNCH(1, false);
-    else WF_F_LAUNCH(0, false);
+#define WF_F_LAUNCH3(I, JM, CO, LU, DE)                                                       \
+  hipLaunchKernelGGL((wf::k_fused<S, Cm, I, JM, CO, LU, DE, FTX, FTY>), dim3((unsigned)nb),  \
+                     dim3(FTX * FTY), lds, st, g, m, Ebuf[cur], Hbuf[cur], Ebuf[1 - cur],     \
+                     Hbuf[1 - cur], (S *)a->J, cf, dt, x, row, cap)
+#define WF_F_LAUNCH2(I, JM, CO, LU)                                                          \
+  if (g.m1desc) WF_F_LAUNCH3(I, JM, CO, LU, true);                                           \
+  else WF_F_LAUNCH3(I, JM, CO, LU, false)
+#define WF_F_LAUNCH(JM, CO, LU)                                                              \
+  do {                                                                                       \
+    WF_F_LAUNCH2(int64_t, JM, CO, LU);                                                       \
+  } while (0)
+    if (lut) {
+      if (a->conductive) WF_F_LAUNCH(0, true, true);
+      else if (a->jmode == WF_J_LINE) WF_F_LAUNCH(1, false, true);
+      else WF_F_LAUNCH(0, false, true);
+    } else {
+      if (a->conductive) WF_F_LAUNCH(0, true, false);
+      else if (a->jmode == WF_J_LINE) WF_F_LAUNCH(1, false, false);
+      else WF_F_LAUNCH(0, false, false);
+    }
 #undef WF_F_LAUNCH
+#undef WF_F_LAUNCH2
+#undef WF_F_LAUNCH3
     WF_LAUNCH_CHECK();
     cur = 1 - cur;
   }
@@ -249,8 +361,11 @@ int wf_step(wf_handle *h, wf_step_args *a, void *stream) {
     if (a->conductive && (!a->J || !a->ieps || !a->sigma))
       return fail(WF_EINVAL, "wf_step: fused conductive step needs J, ieps and sigma");
     if (a->kz < 0) return fail(WF_EINVAL, "wf_step: kz < 0");
+    if (a->code && (!a->mat || !a->metric2d || a->nmat < 1 || a->nmat > 256))
+      return fail(WF_EINVAL, "wf_step: material code needs a table of 1..256 materials "
+                  "and the 2-D metric tables");
+    if (a->compute_f64) return fail(WF_EINVAL, "wf_step: FP64 compute on FP32 storage uses path 0");
     if (h->dtype == WF_F64) return step_fused<double, double>(h, a, h->md, st);
-    if (a->compute_f64) return step_fused<float, double>(h, a, h->mf, st);
     return step_fused<float, float>(h, a, h->mf, st);
   }
   if (a->path != 0) return fail(WF_EINVAL, "wf_step: path %d not available", a->path);

@@ -59,55 +59,99 @@ __device__ __forceinline__ bool mz(int zb, int i, int j, const GridDev &g) {
   return xpart(zb, i, g.Nx);
 }
 
-// One M1 row: H - sum(fl(fl(dt*fl(a*L_p))*E_p)) over the 4 stencil terms in
-// ascending column order (t0..t3), or descending when desc. Terms 0,1 share
-// edge length L0, terms 2,3 share L1. Returns the new H.
-template <typename S, typename Cm>
+// One M1 row without masks: H - sum fl(fl(dt*fl(a*L_p))*E_p) over the four
+// stencil terms in CSR column order (ascending, or descending when DESC).
+// Terms 0,1 share edge length L0, terms 2,3 share L1. Masked or absent
+// columns come in as E_p = 0 (their product is an exact zero).
+template <typename S, typename Cm, bool DESC>
 __device__ __forceinline__ S h_row(S h, S ar, Cm dt, S L0, S L1, S e0, S e1, S e2, S e3,
-                                   bool u0, bool u1, bool u2, bool u3, int s0, int s1, int s2,
-                                   int s3, bool desc) {
+                                   int s0, int s2) {
+  // s0: sign of term 0 (term 1 has the opposite sign); s2 likewise for terms 2,3
   const Cm w0 = dt * (Cm)(S)(ar * L0);
   const Cm w1 = dt * (Cm)(S)(ar * L1);
   const Cm p0 = w0 * (Cm)e0, p1 = w0 * (Cm)e1, p2 = w1 * (Cm)e2, p3 = w1 * (Cm)e3;
   Cm acc = Cm(0);
-  if (!desc) {
-    if (u0) acc = s0 > 0 ? acc + p0 : acc - p0;
-    if (u1) acc = s1 > 0 ? acc + p1 : acc - p1;
-    if (u2) acc = s2 > 0 ? acc + p2 : acc - p2;
-    if (u3) acc = s3 > 0 ? acc + p3 : acc - p3;
+  if (!DESC) {
+    acc = s0 > 0 ? acc + p0 : acc - p0;
+    acc = s0 > 0 ? acc - p1 : acc + p1;
+    acc = s2 > 0 ? acc + p2 : acc - p2;
+    acc = s2 > 0 ? acc - p3 : acc + p3;
   } else {
-    if (u3) acc = s3 > 0 ? acc + p3 : acc - p3;
-    if (u2) acc = s2 > 0 ? acc + p2 : acc - p2;
-    if (u1) acc = s1 > 0 ? acc + p1 : acc - p1;
-    if (u0) acc = s0 > 0 ? acc + p0 : acc - p0;
+    acc = s2 > 0 ? acc - p3 : acc + p3;
+    acc = s2 > 0 ? acc + p2 : acc - p2;
+    acc = s0 > 0 ? acc - p1 : acc + p1;
+    acc = s0 > 0 ? acc + p0 : acc - p0;
   }
   return (S)((Cm)h - acc);
 }
 
-// One M2 row sum (storage precision, ascending order).
+// One M2 row sum (storage precision, ascending order), masked columns as 0.
 template <typename S>
-__device__ __forceinline__ S e_sum(S br, S tL0, S tL1, S h0, S h1, S h2, S h3, bool u0,
-                                   bool u1, bool u2, bool u3, int s0, int s1, int s2, int s3) {
+__device__ __forceinline__ S e_sum(S br, S tL0, S tL1, S h0, S h1, S h2, S h3, int s0, int s2) {
   const S m0 = br * tL0, m1 = br * tL1;
   const S p0 = m0 * h0, p1 = m0 * h1, p2 = m1 * h2, p3 = m1 * h3;
   S acc = S(0);
-  if (u0) acc = s0 > 0 ? acc + p0 : acc - p0;
-  if (u1) acc = s1 > 0 ? acc + p1 : acc - p1;
-  if (u2) acc = s2 > 0 ? acc + p2 : acc - p2;
-  if (u3) acc = s3 > 0 ? acc + p3 : acc - p3;
+  acc = s0 > 0 ? acc + p0 : acc - p0;
+  acc = s0 > 0 ? acc - p1 : acc + p1;
+  acc = s2 > 0 ? acc + p2 : acc - p2;
+  acc = s2 > 0 ? acc - p3 : acc + p3;
   return acc;
 }
 
-template <typename S, typename Cm, int JM, bool COND, int TX, int TY>
+// Coefficient source: either the full 3N arrays a/b (and ieps/sigma), or a
+// per-cell uint8 material code with a table of per-component material values
+// (imu_xyz, ieps_xyz, sigma_xyz) staged in LDS plus 2-D metric tables
+// iA_c/itA_c over the two transverse axes of component c. The kernel then
+// recomputes a = fl(iA*imu) and b = fl(ieps*itA) exactly as the host fold.
+template <typename S>
+struct Coefs {
+  const S *a, *b, *ieps, *sigma;
+  const uint8_t *code;
+  const S *mat;   // [nmat][9]
+  int nmat;
+  const S *iAx, *iAy, *iAz, *itAx, *itAy, *itAz;  // (j,k), (i,k), (i,j) tables
+};
+
+// Global access with a 32-bit byte offset (global_load ... v_off, s[base]) when
+// the whole array fits in 4 GiB, otherwise 64-bit indexing.
+template <typename S, typename I>
+__device__ __forceinline__ S ldg(const S *__restrict__ p, I idx) {
+  if constexpr (sizeof(I) == 4)
+    return *reinterpret_cast<const S *>(reinterpret_cast<const char *>(p) +
+                                        (uint32_t)(idx * (uint32_t)sizeof(S)));
+  else
+    return p[idx];
+}
+template <typename S, typename I>
+__device__ __forceinline__ void stg(S *__restrict__ p, I idx, S v) {
+  if constexpr (sizeof(I) == 4)
+    *reinterpret_cast<S *>(reinterpret_cast<char *>(p) + (uint32_t)(idx * (uint32_t)sizeof(S))) = v;
+  else
+    p[idx] = v;
+}
+
+// mask value of a component-x/y face rule at plane k given the in-plane part
+__device__ __forceinline__ bool zsel(int zk, bool inplane) {
+  return (zk >= 0) ? (zk != 0) : inplane;
+}
+
+template <typename S, typename Cm, typename I, int JM, bool COND, bool LUT, bool DESC, int TX,
+          int TY>
 __global__ void __launch_bounds__(TX *TY)
 k_fused(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restrict__ Hin,
-        S *__restrict__ Eout, S *__restrict__ Hout, S *__restrict__ J, const S *__restrict__ a,
-        const S *__restrict__ b, const S *__restrict__ ieps, const S *__restrict__ sigma, Cm dt,
+        S *__restrict__ Eout, S *__restrict__ Hout, S *__restrict__ J, Coefs<S> cf, Cm dt,
         FusedAux x, const S *__restrict__ beam_row, S *__restrict__ cap) {
   constexpr int NT = TX * TY;
   constexpr int NRING = 2 * (TX + 2) + 2 * TY;
-  __shared__ S sE[3][TY + 2][TX + 2];
-  __shared__ S sH[3][TY + 1][TX + 1];
+  constexpr int NM = 9;
+  // E staged with PEC column masks applied; H staged with PMC column masks
+  __shared__ S sE[1][3][TY + 2][TX + 2];
+  __shared__ S sH[1][3][TY + 1][TX + 1];
+  extern __shared__ unsigned char s_dyn[];
+  S *sMat = reinterpret_cast<S *>(s_dyn);
+  if (LUT) {
+    for (int t = threadIdx.x; t < cf.nmat * NM; t += NT) sMat[t] = cf.mat[t];
+  }
 
   // XCD-aware tile order: consecutive dispatch slots of one XCD take
   // consecutive tiles, so x/y halo neighbours share that XCD's L2.
@@ -126,11 +170,12 @@ k_fused(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restric
   const int tx = tid % TX, ty = tid / TX;
   const int i = i0 + tx, j = j0 + ty;
   const bool act = (i < g.Nx) && (j < g.Ny);
-  const int64_t N = g.N, sy = g.sy, sz = g.sz;
-  const int64_t col = (int64_t)i + sy * j;  // in-plane offset of the own column
+  const I N = (I)g.N, sy = (I)g.sy, sz = (I)g.sz;
+  const I col = act ? (I)i + sy * (I)j : (I)0;
+  const int pz = g.pecz, hz = g.pmcz;
 
   // halo ring cell of this thread (threads < NRING): position in sE
-  int rjj = -1, rii = -1;
+  int rjj = 0, rii = 0;
   if (tid < NRING) {
     if (tid < TX + 2) { rjj = 0; rii = tid; }
     else if (tid < 2 * (TX + 2)) { rjj = TY + 1; rii = tid - (TX + 2); }
@@ -139,196 +184,288 @@ k_fused(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restric
   }
   const int ri = i0 - 1 + rii, rj = j0 - 1 + rjj;
   const bool ring_in = (tid < NRING) && ri >= 0 && ri < g.Nx && rj >= 0 && rj < g.Ny;
-  const int64_t rcol = ring_in ? (int64_t)ri + sy * rj : 0;
+  const I rcol = ring_in ? (I)ri + sy * (I)rj : (I)0;
+  // in-plane mask parts of the ring cell: x-comp (faces y), y-comp (faces x), z-comp
+  const bool rmx = ring_in && ypart(pz, rj, g.Ny), rmy = ring_in && xpart(pz, ri, g.Nx);
+  const bool rmz = ring_in && mz(pz, ri, rj, g);
 
   // halo H cells: bottom row (ty==0) and left column (tx==0)
   const int jb = j0 - 1, ib = i0 - 1;
   const bool hb = (ty == 0) && jb >= 0 && i < g.Nx;
   const bool hl = (tx == 0) && ib >= 0 && j < g.Ny;
-  const int64_t bcol = (int64_t)i + sy * jb, lcol = (int64_t)ib + sy * j;
+  const I bcol = hb ? (I)i + sy * (I)jb : (I)0;
+  const I lcol = hl ? (I)ib + sy * (I)j : (I)0;
 
-  // per-column constants
-  const int pz = g.pecz, hz = g.pmcz;
-  const bool desc = g.m1desc != 0;
-  S Lx = 0, Ly = 0, Lz = 0, tLx = 0, tLy = 0;
+  S Lx = 0, Ly = 0, tLx = 0, tLy = 0;
   if (i < g.Nx) { Lx = m.Lx[i]; tLx = m.tLx[i]; }
   if (j < g.Ny) { Ly = m.Ly[j]; tLy = m.tLy[j]; }
-
-  const S Lx_l = (ib >= 0) ? m.Lx[ib] : S(0);   // left halo column
-  const S Ly_b = (jb >= 0) ? m.Ly[jb] : S(0);   // bottom halo row
-  const bool in_x1 = (i + 1 < g.Nx), in_y1 = (j + 1 < g.Ny);
-  const bool in_xm = (i >= 1), in_ym = (j >= 1);
+  const S Lx_l = hl ? m.Lx[ib] : S(0);
+  const S Ly_b = hb ? m.Ly[jb] : S(0);
   const bool online = act && (i == x.ixs) && (j == x.iys);
   const int li = i - x.cap_ix0, lj = j - x.cap_iy0;
   const bool incap = cap && act && li >= 0 && li < x.cap_nx && lj >= 0 && lj < x.cap_ny &&
                      x.cap_sample < x.cap_samples;
   const int64_t capbase = incap ? (((int64_t)li * x.cap_ny + lj) * x.cap_samples +
                                    x.cap_sample) * g.Nz : 0;
+  // in-plane mask parts of the own cell (PEC: E columns and b rows)
+  const bool mI_y = act && xpart(pz, i, g.Nx), mJ_x = act && ypart(pz, j, g.Ny);
+  const bool mZ = act && mz(pz, i, j, g);
+  // halo-row/column cells' own E_y(k+1) / E_x(k+1) masks use the same in-plane parts
+  const bool bmI_y = hb && xpart(pz, i, g.Nx), lmJ_x = hl && ypart(pz, j, g.Ny);
+  // code mode: k-independent metric entries
+  S iAz_ij = 0, itAz_ij = 0, iAz_b = 0, iAz_l = 0;
+  if (LUT) {
+    if (act) { iAz_ij = cf.iAz[(int64_t)i + (int64_t)g.Nx * j]; itAz_ij = cf.itAz[(int64_t)i + (int64_t)g.Nx * j]; }
+    if (hb) iAz_b = cf.iAz[(int64_t)i + (int64_t)g.Nx * jb];
+    if (hl) iAz_l = cf.iAz[(int64_t)ib + (int64_t)g.Nx * j];
+  }
+
+  // per-plane inputs (one plane ahead)
+  struct In {
+    S Ho[3], c0[3], c1[3], iep[3], sg[3], ring[3];
+    S bH[2], bA, bE1, lH[2], lA, lE1, bA2, lA2;
+    int code, bcode, lcode;
+  };
+  auto load = [&](int kk, In &P) {
+    const bool body = kk >= k0;
+    const I nk = col + sz * (I)kk;
+    if (act) {
+      P.Ho[0] = ldg(Hin, nk); P.Ho[1] = ldg(Hin, N + nk); P.Ho[2] = ldg(Hin, 2 * N + nk);
+      if (LUT) {
+        P.code = (int)cf.code[nk];
+        P.c0[0] = cf.iAx[(int64_t)j + (int64_t)g.Ny * kk];
+        P.c0[1] = cf.iAy[(int64_t)i + (int64_t)g.Nx * kk];
+        if (body) {
+          P.c1[0] = cf.itAx[(int64_t)j + (int64_t)g.Ny * kk];
+          P.c1[1] = cf.itAy[(int64_t)i + (int64_t)g.Nx * kk];
+        }
+      } else {
+        P.c0[0] = ldg(cf.a, nk); P.c0[1] = ldg(cf.a, N + nk); P.c0[2] = ldg(cf.a, 2 * N + nk);
+        if (body) {
+          P.c1[0] = ldg(cf.b, nk); P.c1[1] = ldg(cf.b, N + nk); P.c1[2] = ldg(cf.b, 2 * N + nk);
+          if (COND) {
+            P.iep[0] = ldg(cf.ieps, nk); P.iep[1] = ldg(cf.ieps, N + nk);
+            P.iep[2] = ldg(cf.ieps, 2 * N + nk);
+            P.sg[0] = ldg(cf.sigma, nk); P.sg[1] = ldg(cf.sigma, N + nk);
+            P.sg[2] = ldg(cf.sigma, 2 * N + nk);
+          } else if (JM == 1 && online) {
+            P.iep[2] = ldg(cf.ieps, 2 * N + nk);
+          }
+        }
+      }
+    }
+    if (ring_in) {
+      const I n = rcol + sz * (I)kk;
+      P.ring[0] = ldg(Ein, n); P.ring[1] = ldg(Ein, N + n); P.ring[2] = ldg(Ein, 2 * N + n);
+    } else {
+      P.ring[0] = P.ring[1] = P.ring[2] = S(0);
+    }
+    const bool kp1 = kk + 1 < g.Nz;
+    if (body && hb) {
+      const I n = bcol + sz * (I)kk;
+      P.bH[0] = ldg(Hin, n); P.bH[1] = ldg(Hin, 2 * N + n);
+      if (LUT) {
+        P.bcode = (int)cf.code[n];
+        P.bA = cf.iAx[(int64_t)jb + (int64_t)g.Ny * kk];
+      } else {
+        P.bA = ldg(cf.a, n); P.bA2 = ldg(cf.a, 2 * N + n);
+      }
+      P.bE1 = kp1 ? ldg(Ein, N + n + sz) : S(0);
+    }
+    if (body && hl) {
+      const I n = lcol + sz * (I)kk;
+      P.lH[0] = ldg(Hin, N + n); P.lH[1] = ldg(Hin, 2 * N + n);
+      if (LUT) {
+        P.lcode = (int)cf.code[n];
+        P.lA = cf.iAy[(int64_t)ib + (int64_t)g.Nx * kk];
+      } else {
+        P.lA = ldg(cf.a, N + n); P.lA2 = ldg(cf.a, 2 * N + n);
+      }
+      P.lE1 = kp1 ? ldg(Ein, n + sz) : S(0);
+    }
+  };
 
   const int kstart = (k0 > 0) ? k0 - 1 : k0;
   S Ek[3] = {0, 0, 0}, Ek1[3] = {0, 0, 0};
-  S ring[3] = {0, 0, 0};
   if (act) {
-    const int64_t n = col + sz * kstart;
-    Ek[0] = Ein[n]; Ek[1] = Ein[N + n]; Ek[2] = Ein[2 * N + n];
+    const I n = col + sz * (I)kstart;
+    Ek[0] = ldg(Ein, n); Ek[1] = ldg(Ein, N + n); Ek[2] = ldg(Ein, 2 * N + n);
     if (kstart + 1 < g.Nz) {
-      Ek1[0] = Ein[n + sz]; Ek1[1] = Ein[N + n + sz]; Ek1[2] = Ein[2 * N + n + sz];
+      Ek1[0] = ldg(Ein, n + sz); Ek1[1] = ldg(Ein, N + n + sz); Ek1[2] = ldg(Ein, 2 * N + n + sz);
     }
   }
-  if (ring_in) {
-    const int64_t n = rcol + sz * kstart;
-    ring[0] = Ein[n]; ring[1] = Ein[N + n]; ring[2] = Ein[2 * N + n];
-  }
-  S Hpx = 0, Hpy = 0;  // H_new at plane k-1, own column
+  In P0, P1;
+  load(kstart, P0);
+  if (LUT) __syncthreads();  // material table staged
+  S Hpx = 0, Hpy = 0;        // H_new at plane k-1 (PMC-masked), own column
 
-  for (int k = kstart; k < kend; ++k) {
+  auto plane = [&](const int k, const int buf, const In &cur, In &nxt) {
     const bool body = (k >= k0);  // k0-1 is the warm-up plane (H only)
-    const int64_t nk = col + sz * k;
-    // stage E_old(k)
-    if (act) {
-      sE[0][ty + 1][tx + 1] = Ek[0];
-      sE[1][ty + 1][tx + 1] = Ek[1];
-      sE[2][ty + 1][tx + 1] = Ek[2];
-    }
-    if (tid < NRING) {
-      sE[0][rjj][rii] = ring[0];
-      sE[1][rjj][rii] = ring[1];
-      sE[2][rjj][rii] = ring[2];
-    }
-    // loads for this plane (own) and prefetch of the next plane
-    S Ho[3] = {0, 0, 0}, ao[3] = {0, 0, 0}, bo[3] = {0, 0, 0}, En2[3] = {0, 0, 0};
-    S iep[3] = {0, 0, 0}, sg[3] = {0, 0, 0};
+    const I nk = col + sz * (I)k;
     const bool kp1 = (k + 1 < g.Nz), kp2 = (k + 2 < g.Nz);
-    if (act) {
-      Ho[0] = Hin[nk]; Ho[1] = Hin[N + nk]; Ho[2] = Hin[2 * N + nk];
-      ao[0] = a[nk]; ao[1] = a[N + nk]; ao[2] = a[2 * N + nk];
-      if (body) {
-        bo[0] = b[nk]; bo[1] = b[N + nk]; bo[2] = b[2 * N + nk];
-        if (COND) {
-          iep[0] = ieps[nk]; iep[1] = ieps[N + nk]; iep[2] = ieps[2 * N + nk];
-          sg[0] = sigma[nk]; sg[1] = sigma[N + nk]; sg[2] = sigma[2 * N + nk];
-        } else if (JM == 1 && online) {
-          iep[2] = ieps[2 * N + nk];
-        }
-      }
-      if (kp2) {
-        const int64_t n2 = nk + 2 * sz;
-        En2[0] = Ein[n2]; En2[1] = Ein[N + n2]; En2[2] = Ein[2 * N + n2];
-      }
+    if (k + 1 < kend) load(k + 1, nxt);
+    S En2[3] = {0, 0, 0};
+    if (act && kp2) {
+      const I n2 = nk + 2 * sz;
+      En2[0] = ldg(Ein, n2); En2[1] = ldg(Ein, N + n2); En2[2] = ldg(Ein, 2 * N + n2);
     }
-    S ringn[3] = {0, 0, 0};
-    if (ring_in && kp1) {
-      const int64_t n1 = rcol + sz * (k + 1);
-      ringn[0] = Ein[n1]; ringn[1] = Ein[N + n1]; ringn[2] = Ein[2 * N + n1];
+    // PEC column masks of E at planes k and k+1 (z faces override in-plane parts)
+    const int zk = zface(pz, k, g.Nz), zk1 = kp1 ? zface(pz, k + 1, g.Nz) : -1;
+    const bool mEx = zsel(zk, mJ_x), mEy = zsel(zk, mI_y);
+    const bool mEx1 = kp1 && zsel(zk1, mJ_x), mEy1 = kp1 && zsel(zk1, mI_y);
+    sE[buf][0][ty + 1][tx + 1] = mEx ? Ek[0] : S(0);
+    sE[buf][1][ty + 1][tx + 1] = mEy ? Ek[1] : S(0);
+    sE[buf][2][ty + 1][tx + 1] = mZ ? Ek[2] : S(0);
+    if (tid < NRING) {
+      sE[buf][0][rjj][rii] = (ring_in && zsel(zk, rmx)) ? cur.ring[0] : S(0);
+      sE[buf][1][rjj][rii] = (ring_in && zsel(zk, rmy)) ? cur.ring[1] : S(0);
+      sE[buf][2][rjj][rii] = rmz ? cur.ring[2] : S(0);
     }
-    // halo-H inputs
-    S bHx = 0, bHz = 0, bax = 0, baz = 0, bEy1 = 0;
-    S lHy = 0, lHz = 0, lay = 0, laz = 0, lEx1 = 0;
-    if (body && hb) {
-      const int64_t n = bcol + sz * k;
-      bHx = Hin[n]; bHz = Hin[2 * N + n]; bax = a[n]; baz = a[2 * N + n];
-      if (kp1) bEy1 = Ein[N + n + sz];
-    }
-    if (body && hl) {
-      const int64_t n = lcol + sz * k;
-      lHy = Hin[N + n]; lHz = Hin[2 * N + n]; lay = a[N + n]; laz = a[2 * N + n];
-      if (kp1) lEx1 = Ein[n + sz];
-    }
-    if (k < g.Nz) Lz = m.Lz[k];
+    const S Lz = m.Lz[k];
     __syncthreads();
 
-    // ---- H_new(k), own column
     S Hn[3] = {0, 0, 0};
+    S bo[3] = {0, 0, 0}, iep[3] = {0, 0, 0}, sg[3] = {0, 0, 0};
     if (act) {
-      const int zk = zface(pz, k, g.Nz), zk1 = kp1 ? zface(pz, k + 1, g.Nz) : -1;
-      const bool myk = (zk >= 0) ? zk : xpart(pz, i, g.Nx);
-      const bool myk1 = (zk1 >= 0) ? zk1 : xpart(pz, i, g.Nx);
-      const bool mxk = (zk >= 0) ? zk : ypart(pz, j, g.Ny);
-      const bool mxk1 = (zk1 >= 0) ? zk1 : ypart(pz, j, g.Ny);
-      const bool myk_i1 = in_x1 && ((zk >= 0) ? zk : xpart(pz, i + 1, g.Nx));
-      const bool mxk_j1 = in_y1 && ((zk >= 0) ? zk : ypart(pz, j + 1, g.Ny));
-      const S exk = sE[0][ty + 1][tx + 1], eyk = sE[1][ty + 1][tx + 1], ezk = sE[2][ty + 1][tx + 1];
+      S ao[3];
+      if (LUT) {
+        const S *t = sMat + NM * cur.code;
+        ao[0] = (S)(cur.c0[0] * t[0]);
+        ao[1] = (S)(cur.c0[1] * t[1]);
+        ao[2] = (S)(iAz_ij * t[2]);
+        if (hz) {  // PMC row mask
+          const int zh = zface(hz, k, g.Nz);
+          if (!zsel(zh, ypart(hz, j, g.Ny))) ao[0] = S(0);
+          if (!zsel(zh, xpart(hz, i, g.Nx))) ao[1] = S(0);
+          if (!mz(hz, i, j, g)) ao[2] = S(0);
+        }
+        if (body) {  // PEC row mask
+          bo[0] = mEx ? (S)(t[3] * cur.c1[0]) : S(0);
+          bo[1] = mEy ? (S)(t[4] * cur.c1[1]) : S(0);
+          bo[2] = mZ ? (S)(t[5] * itAz_ij) : S(0);
+          if (COND) {
+            iep[0] = t[3]; iep[1] = t[4]; iep[2] = t[5];
+            sg[0] = t[6]; sg[1] = t[7]; sg[2] = t[8];
+          } else if (JM == 1) {
+            iep[2] = t[5];
+          }
+        }
+      } else {
+        ao[0] = cur.c0[0]; ao[1] = cur.c0[1]; ao[2] = cur.c0[2];
+        bo[0] = cur.c1[0]; bo[1] = cur.c1[1]; bo[2] = cur.c1[2];
+        if (COND) {
+          iep[0] = cur.iep[0]; iep[1] = cur.iep[1]; iep[2] = cur.iep[2];
+          sg[0] = cur.sg[0]; sg[1] = cur.sg[1]; sg[2] = cur.sg[2];
+        } else if (JM == 1) {
+          iep[2] = cur.iep[2];
+        }
+      }
+      const S exk = sE[buf][0][ty + 1][tx + 1], eyk = sE[buf][1][ty + 1][tx + 1];
+      const S ezk = sE[buf][2][ty + 1][tx + 1];
+      const S ex1 = mEx1 ? Ek1[0] : S(0), ey1 = mEy1 ? Ek1[1] : S(0);
       // H_x: Ey(k)+, Ey(k+1)-, Ez(k)-, Ez(j+1)+
-      Hn[0] = h_row<S, Cm>(Ho[0], ao[0], dt, Ly, Lz, eyk, Ek1[1], ezk, sE[2][ty + 2][tx + 1],
-                           myk, kp1 && myk1, mz(pz, i, j, g), in_y1 && mz(pz, i, j + 1, g),
-                           +1, -1, -1, +1, desc);
+      Hn[0] = h_row<S, Cm, DESC>(cur.Ho[0], ao[0], dt, Ly, Lz, eyk, ey1, ezk,
+                                 sE[buf][2][ty + 2][tx + 1], +1, -1);
       // H_y: Ex(k)-, Ex(k+1)+, Ez(k)+, Ez(i+1)-
-      Hn[1] = h_row<S, Cm>(Ho[1], ao[1], dt, Lx, Lz, exk, Ek1[0], ezk, sE[2][ty + 1][tx + 2],
-                           mxk, kp1 && mxk1, mz(pz, i, j, g), in_x1 && mz(pz, i + 1, j, g),
-                           -1, +1, +1, -1, desc);
+      Hn[1] = h_row<S, Cm, DESC>(cur.Ho[1], ao[1], dt, Lx, Lz, exk, ex1, ezk,
+                                 sE[buf][2][ty + 1][tx + 2], -1, +1);
       // H_z: Ex(k)+, Ex(j+1)-, Ey(k)-, Ey(i+1)+
-      Hn[2] = h_row<S, Cm>(Ho[2], ao[2], dt, Lx, Ly, exk, sE[0][ty + 2][tx + 1], eyk,
-                           sE[1][ty + 1][tx + 2], mxk, mxk_j1, myk, myk_i1, +1, -1, -1, +1,
-                           desc);
-      sH[0][ty + 1][tx + 1] = Hn[0];
-      sH[1][ty + 1][tx + 1] = Hn[1];
-      sH[2][ty + 1][tx + 1] = Hn[2];
+      Hn[2] = h_row<S, Cm, DESC>(cur.Ho[2], ao[2], dt, Lx, Ly, exk, sE[buf][0][ty + 2][tx + 1],
+                                 eyk, sE[buf][1][ty + 1][tx + 2], +1, -1);
     }
-    // ---- H_new(k) on the low halo row (Hx, Hz) and column (Hy, Hz)
-    if (body && hb) {
-      const int zk = zface(pz, k, g.Nz), zk1 = kp1 ? zface(pz, k + 1, g.Nz) : -1;
-      const bool myk = (zk >= 0) ? zk : xpart(pz, i, g.Nx);
-      const bool myk1 = (zk1 >= 0) ? zk1 : xpart(pz, i, g.Nx);
-      const bool mxk = (zk >= 0) ? zk : ypart(pz, jb, g.Ny);
-      const bool mxk_j1 = (zk >= 0) ? zk : ypart(pz, j0, g.Ny);
-      const bool myk_i1 = in_x1 && ((zk >= 0) ? zk : xpart(pz, i + 1, g.Nx));
-      const S exb = sE[0][0][tx + 1], eyb = sE[1][0][tx + 1], ezb = sE[2][0][tx + 1];
-      sH[0][0][tx + 1] = h_row<S, Cm>(bHx, bax, dt, Ly_b, Lz, eyb, bEy1, ezb, sE[2][1][tx + 1],
-                                      myk, kp1 && myk1, mz(pz, i, jb, g), mz(pz, i, j0, g),
-                                      +1, -1, -1, +1, desc);
-      sH[2][0][tx + 1] = h_row<S, Cm>(bHz, baz, dt, Lx, Ly_b, exb, sE[0][1][tx + 1], eyb,
-                                      sE[1][0][tx + 2], mxk, mxk_j1, myk, myk_i1, +1, -1, -1,
-                                      +1, desc);
+    // staged H (PMC column masks; absent cells are zero)
+    {
+      S h0 = Hn[0], h1 = Hn[1], h2 = Hn[2];
+      if (hz && act) {
+        const int zh = zface(hz, k, g.Nz);
+        if (!zsel(zh, ypart(hz, j, g.Ny))) h0 = S(0);
+        if (!zsel(zh, xpart(hz, i, g.Nx))) h1 = S(0);
+        if (!mz(hz, i, j, g)) h2 = S(0);
+      }
+      sH[buf][0][ty + 1][tx + 1] = h0;
+      sH[buf][1][ty + 1][tx + 1] = h1;
+      sH[buf][2][ty + 1][tx + 1] = h2;
     }
-    if (body && hl) {
-      const int zk = zface(pz, k, g.Nz), zk1 = kp1 ? zface(pz, k + 1, g.Nz) : -1;
-      const bool mxk = (zk >= 0) ? zk : ypart(pz, j, g.Ny);
-      const bool mxk1 = (zk1 >= 0) ? zk1 : ypart(pz, j, g.Ny);
-      const bool myk = (zk >= 0) ? zk : xpart(pz, ib, g.Nx);
-      const bool myk_i1 = (zk >= 0) ? zk : xpart(pz, i0, g.Nx);
-      const bool mxk_j1 = in_y1 && ((zk >= 0) ? zk : ypart(pz, j + 1, g.Ny));
-      const S exl = sE[0][ty + 1][0], eyl = sE[1][ty + 1][0], ezl = sE[2][ty + 1][0];
-      sH[1][ty + 1][0] = h_row<S, Cm>(lHy, lay, dt, Lx_l, Lz, exl, lEx1, ezl, sE[2][ty + 1][1],
-                                      mxk, kp1 && mxk1, mz(pz, ib, j, g), mz(pz, i0, j, g),
-                                      -1, +1, +1, -1, desc);
-      sH[2][ty + 1][0] = h_row<S, Cm>(lHz, laz, dt, Lx_l, Ly, exl, sE[0][ty + 2][0], eyl,
-                                      sE[1][ty + 1][1], mxk, mxk_j1, myk, myk_i1, +1, -1, -1,
-                                      +1, desc);
+    // ---- H_new(k) on the low halo row (Hx, Hz) and column (Hy, Hz); zero when absent
+    if (ty == 0) {
+      S hx = 0, hzv = 0;
+      if (body && hb) {
+        S bax, baz;
+        if (LUT) {
+          const S *t = sMat + NM * cur.bcode;
+          bax = (S)(cur.bA * t[0]);
+          baz = (S)(iAz_b * t[2]);
+          if (hz) {
+            const int zh = zface(hz, k, g.Nz);
+            if (!zsel(zh, ypart(hz, jb, g.Ny))) bax = S(0);
+            if (!mz(hz, i, jb, g)) baz = S(0);
+          }
+        } else {
+          bax = cur.bA; baz = cur.bA2;
+        }
+        const S ey1 = (kp1 && zsel(zk1, bmI_y)) ? cur.bE1 : S(0);
+        hx = h_row<S, Cm, DESC>(cur.bH[0], bax, dt, Ly_b, Lz, sE[buf][1][0][tx + 1], ey1,
+                                sE[buf][2][0][tx + 1], sE[buf][2][1][tx + 1], +1, -1);
+        hzv = h_row<S, Cm, DESC>(cur.bH[1], baz, dt, Lx, Ly_b, sE[buf][0][0][tx + 1],
+                                 sE[buf][0][1][tx + 1], sE[buf][1][0][tx + 1],
+                                 sE[buf][1][0][tx + 2], +1, -1);
+        if (hz) {
+          const int zh = zface(hz, k, g.Nz);
+          if (!zsel(zh, ypart(hz, jb, g.Ny))) hx = S(0);
+          if (!mz(hz, i, jb, g)) hzv = S(0);
+        }
+      }
+      sH[buf][0][0][tx + 1] = hx;
+      sH[buf][2][0][tx + 1] = hzv;
+    }
+    if (tx == 0) {
+      S hy = 0, hzv = 0;
+      if (body && hl) {
+        S lay, laz;
+        if (LUT) {
+          const S *t = sMat + NM * cur.lcode;
+          lay = (S)(cur.lA * t[1]);
+          laz = (S)(iAz_l * t[2]);
+          if (hz) {
+            const int zh = zface(hz, k, g.Nz);
+            if (!zsel(zh, xpart(hz, ib, g.Nx))) lay = S(0);
+            if (!mz(hz, ib, j, g)) laz = S(0);
+          }
+        } else {
+          lay = cur.lA; laz = cur.lA2;
+        }
+        const S ex1 = (kp1 && zsel(zk1, lmJ_x)) ? cur.lE1 : S(0);
+        hy = h_row<S, Cm, DESC>(cur.lH[0], lay, dt, Lx_l, Lz, sE[buf][0][ty + 1][0], ex1,
+                                sE[buf][2][ty + 1][0], sE[buf][2][ty + 1][1], -1, +1);
+        hzv = h_row<S, Cm, DESC>(cur.lH[1], laz, dt, Lx_l, Ly, sE[buf][0][ty + 1][0],
+                                 sE[buf][0][ty + 2][0], sE[buf][1][ty + 1][0],
+                                 sE[buf][1][ty + 1][1], +1, -1);
+        if (hz) {
+          const int zh = zface(hz, k, g.Nz);
+          if (!zsel(zh, xpart(hz, ib, g.Nx))) hy = S(0);
+          if (!mz(hz, ib, j, g)) hzv = S(0);
+        }
+      }
+      sH[buf][1][ty + 1][0] = hy;
+      sH[buf][2][ty + 1][0] = hzv;
     }
     __syncthreads();
 
     // ---- E_new(k), own column
+    const S hxm = sH[buf][0][ty + 1][tx + 1], hym = sH[buf][1][ty + 1][tx + 1];
     if (body && act) {
-      const bool km = (k >= 1);
-      // PMC column masks on H (all true when there is no PMC face)
-      bool hxk = true, hxkm = true, hyk = true, hykm = true, hzk = true, hzjm = true,
-           hzim = true, hxjm = true, hyim = true;
-      if (hz) {
-        const int zk = zface(hz, k, g.Nz), zkm = km ? zface(hz, k - 1, g.Nz) : -1;
-        hxk = (zk >= 0) ? zk : ypart(hz, j, g.Ny);
-        hxkm = (zkm >= 0) ? zkm : ypart(hz, j, g.Ny);
-        hyk = (zk >= 0) ? zk : xpart(hz, i, g.Nx);
-        hykm = (zkm >= 0) ? zkm : xpart(hz, i, g.Nx);
-        hzk = mz(hz, i, j, g);
-        hzjm = in_ym && mz(hz, i, j - 1, g);
-        hzim = in_xm && mz(hz, i - 1, j, g);
-        hxjm = in_ym && ((zk >= 0) ? zk : ypart(hz, j - 1, g.Ny));
-        hyim = in_xm && ((zk >= 0) ? zk : xpart(hz, i - 1, g.Nx));
-      }
-      const S hxc = sH[0][ty + 1][tx + 1], hyc = sH[1][ty + 1][tx + 1], hzc = sH[2][ty + 1][tx + 1];
+      const S hzm = sH[buf][2][ty + 1][tx + 1];
       const S tLz = m.tLz[k];
       // E_x: Hy(k-1)+, Hy(k)-, Hz(j-1)-, Hz(k)+
-      const S sx = e_sum<S>(bo[0], tLy, tLz, Hpy, hyc, sH[2][ty][tx + 1], hzc, km && hykm, hyk,
-                            in_ym && hzjm, hzk, +1, -1, -1, +1);
+      const S sx = e_sum<S>(bo[0], tLy, tLz, Hpy, hym, sH[buf][2][ty][tx + 1], hzm, +1, -1);
       // E_y: Hx(k-1)-, Hx(k)+, Hz(i-1)+, Hz(k)-
-      const S sy_ = e_sum<S>(bo[1], tLx, tLz, Hpx, hxc, sH[2][ty + 1][tx], hzc, km && hxkm, hxk,
-                             in_xm && hzim, hzk, -1, +1, +1, -1);
+      const S sy_ = e_sum<S>(bo[1], tLx, tLz, Hpx, hxm, sH[buf][2][ty + 1][tx], hzm, -1, +1);
       // E_z: Hx(j-1)+, Hx(k)-, Hy(i-1)-, Hy(k)+
-      const S sz_ = e_sum<S>(bo[2], tLx, tLy, sH[0][ty][tx + 1], hxc, sH[1][ty + 1][tx], hyc,
-                             in_ym && hxjm, hxk, in_xm && hyim, hyk, +1, -1, -1, +1);
+      const S sz_ = e_sum<S>(bo[2], tLx, tLy, sH[buf][0][ty][tx + 1], hxm,
+                             sH[buf][1][ty + 1][tx], hym, +1, -1);
       S d[3] = {sx, sy_, sz_};
       S jline = 0;
-      if (online && (JM == 1 || COND)) jline = beam_row ? beam_row[k] : J[2 * N + nk];
+      if (online && (JM == 1 || COND)) jline = beam_row ? beam_row[k] : ldg(J, 2 * N + nk);
       if (COND) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -341,24 +478,30 @@ k_fused(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restric
       S En[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) En[c] = (S)((Cm)Ek[c] + dt * (Cm)d[c]);
-      Eout[nk] = En[0]; Eout[N + nk] = En[1]; Eout[2 * N + nk] = En[2];
-      Hout[nk] = Hn[0]; Hout[N + nk] = Hn[1]; Hout[2 * N + nk] = Hn[2];
+      stg(Eout, nk, En[0]); stg(Eout, N + nk, En[1]); stg(Eout, 2 * N + nk, En[2]);
+      stg(Hout, nk, Hn[0]); stg(Hout, N + nk, Hn[1]); stg(Hout, 2 * N + nk, Hn[2]);
       if (COND) {
-        if (J) { J[nk] = sg[0] * En[0]; J[N + nk] = sg[1] * En[1]; J[2 * N + nk] = sg[2] * En[2]; }
+        stg(J, nk, (S)(sg[0] * En[0]));
+        stg(J, N + nk, (S)(sg[1] * En[1]));
+        stg(J, 2 * N + nk, (S)(sg[2] * En[2]));
       } else if (online && beam_row) {
-        J[2 * N + nk] = jline;
+        stg(J, 2 * N + nk, jline);
       }
       if (incap) cap[capbase + k] = En[2];
     }
-    Hpx = Hn[0];
-    Hpy = Hn[1];
+    Hpx = hxm;
+    Hpy = hym;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       Ek[c] = Ek1[c];
       Ek1[c] = En2[c];
-      ring[c] = ringn[c];
     }
-    __syncthreads();
+  };
+
+  // unrolled by two so the plane inputs ping-pong between P0 and P1
+  for (int k = kstart; k < kend; k += 2) {
+    plane(k, 0, P0, P1);
+    if (k + 1 < kend) plane(k + 1, 0, P1, P0);
   }
 }
 

@@ -1,0 +1,463 @@
+// Fused single-pass leapfrog step, x-vectorised (fast path for Nx % 4 == 0).
+//
+// Same per-row arithmetic as k_fused (step_fused.h), organised for CDNA issue
+// rate: each lane owns 4 consecutive x cells (16-byte loads/stores, one
+// 32-bit load for four material codes), each wave owns one row segment of
+// 256 cells, and a workgroup holds TY own rows plus one extra wave that
+// recomputes H on the row below the tile (j0-1) and stages the row above
+// (j0+TY). x neighbours come from the lane's own registers or from an aligned
+// LDS row (slot 3 = i0-1, slots 4..259 = cells, slot 260 = i0+256); the
+// workgroup marches up its chunk of z planes with E two planes and the other
+// per-plane inputs one plane ahead. PEC/PMC column masks and absent
+// neighbours are applied once, when a value is staged (as 0).
+#pragma once
+#include "fit_common.h"
+#include "step_fused.h"
+
+namespace wf {
+
+template <typename S>
+struct alignas(16) Q4 {
+  S v[4];
+};
+
+template <typename S, typename I>
+__device__ __forceinline__ const S *at(const S *p, I idx) {
+  if constexpr (sizeof(I) == 4)
+    return reinterpret_cast<const S *>(reinterpret_cast<const char *>(p) +
+                                       (uint32_t)(idx * (uint32_t)sizeof(S)));
+  else
+    return p + idx;
+}
+template <typename S, typename I>
+__device__ __forceinline__ S *atw(S *p, I idx) {
+  if constexpr (sizeof(I) == 4)
+    return reinterpret_cast<S *>(reinterpret_cast<char *>(p) + (uint32_t)(idx * (uint32_t)sizeof(S)));
+  else
+    return p + idx;
+}
+// V consecutive values (16 bytes) starting at a multiple-of-V index
+template <int V, typename S, typename I>
+__device__ __forceinline__ void ldv(const S *__restrict__ p, I idx, S (&o)[V]) {
+  static_assert(V * sizeof(S) == 16, "16-byte vectors");
+  const S *q = at(p, idx);
+  if constexpr (sizeof(S) == 4) {
+    const float4 t = *reinterpret_cast<const float4 *>(q);
+    o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+  } else {
+    const double2 t = *reinterpret_cast<const double2 *>(q);
+    o[0] = t.x; o[1] = t.y;
+  }
+}
+template <int V, typename S, typename I>
+__device__ __forceinline__ void stv(S *__restrict__ p, I idx, const S (&v)[V]) {
+  S *q = atw(p, idx);
+  if constexpr (sizeof(S) == 4) {
+    *reinterpret_cast<float4 *>(q) = make_float4(v[0], v[1], v[2], v[3]);
+  } else {
+    *reinterpret_cast<double2 *>(q) = make_double2(v[0], v[1]);
+  }
+}
+
+template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY>
+__global__ void __launch_bounds__(64 * (TY + 1))
+k_fused4(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restrict__ Hin,
+         S *__restrict__ Eout, S *__restrict__ Hout, S *__restrict__ J, Coefs<S> cf, S dt,
+         FusedAux x, const S *__restrict__ beam_row, S *__restrict__ cap) {
+  using I = uint32_t;
+  constexpr int V = 16 / sizeof(S);            // cells per lane
+  constexpr int W = 64 * V;                    // cells per row segment
+  constexpr int OFF = V, RW = W + 2 * V;       // cell xi at LDS slot xi + OFF
+  constexpr int NM = 9;
+  // one buffer each: a plane's E staging happens after the barrier that ends
+  // every read of the previous plane's E, and its H writes after the barrier
+  // that ends every read of the previous plane's H
+  __shared__ S sE[3][TY + 2][RW];  // rows: 0 = j0-1, 1..TY own, TY+1 = j0+TY
+  __shared__ S sH[3][TY + 1][RW];  // rows: 0 = j0-1, 1..TY own (PMC-masked)
+  extern __shared__ unsigned char s_dyn[];
+  S *sMat = reinterpret_cast<S *>(s_dyn);
+  if (LUT) {
+    for (int t = threadIdx.x; t < cf.nmat * NM; t += 64 * (TY + 1)) sMat[t] = cf.mat[t];
+  }
+
+  const int nb = x.ntx * x.nty * x.ntz;
+  const int blin = blockIdx.x;
+  const int xcd = blin & 7, slot = blin >> 3;
+  const int q8 = nb >> 3, r8 = nb & 7;
+  const int tile = xcd * q8 + (xcd < r8 ? xcd : r8) + slot;
+  const int tz = tile / (x.ntx * x.nty);
+  const int rem = tile - tz * x.ntx * x.nty;
+  const int tyb = rem / x.ntx, txb = rem - tyb * x.ntx;
+  const int i0 = txb * W, j0 = tyb * TY, k0 = tz * x.KZ;
+  const int kend = min(k0 + x.KZ, g.Nz);
+
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = (w == 0) ? j0 - 1 : j0 + w - 1;  // row of this wave
+  const bool rowok = (j >= 0) && (j < g.Ny);
+  const int xi = V * lane;
+  const int ic = i0 + xi;
+  const bool lact = rowok && (ic < g.Nx);          // Nx % V == 0: all V cells or none
+  const bool own = (w >= 1) && lact;
+  const I N = (I)g.N, sz = (I)g.sz;
+  const I col = lact ? (I)ic + (I)g.sy * (I)j : (I)0;
+  const int pz = g.pecz, hz = g.pmcz;
+  const int jt = j0 + TY;
+  const bool top = (w == 0) && (jt < g.Ny) && (ic < g.Nx);
+  const I tcol = top ? (I)ic + (I)g.sy * (I)jt : (I)0;
+  const bool hasL = rowok && (lane == 0) && (i0 > 0);
+  const bool hasR = rowok && (lane == 63) && (i0 + W < g.Nx);
+  const bool topR = (w == 0) && (lane == 63) && (jt < g.Ny) && (i0 + W < g.Nx);
+  const bool topL = (w == 0) && (lane == 0) && (jt < g.Ny) && (i0 > 0);
+
+  const bool mXj = rowok && ypart(pz, j, g.Ny);
+  const bool mXt = ypart(pz, jt, g.Ny);
+  bool mYi[V], mZi[V], mZt[V], mYt[V];
+#pragma unroll
+  for (int q = 0; q < V; ++q) {
+    mYi[q] = lact && xpart(pz, ic + q, g.Nx);
+    mZi[q] = lact && mz(pz, ic + q, j, g);
+    mZt[q] = top && mz(pz, ic + q, jt, g);
+    mYt[q] = top && xpart(pz, ic + q, g.Nx);
+  }
+  const bool mYL = xpart(pz, i0 - 1, g.Nx), mZL = rowok && (i0 > 0) && mz(pz, i0 - 1, j, g);
+  const bool mYR = xpart(pz, i0 + W, g.Nx), mZR = rowok && (i0 + W < g.Nx) && mz(pz, i0 + W, j, g);
+  const bool mZtR = topR && mz(pz, i0 + W, jt, g);
+
+  S Lx[V], tLx[V];
+#pragma unroll
+  for (int q = 0; q < V; ++q) {
+    Lx[q] = lact ? m.Lx[ic + q] : S(0);
+    tLx[q] = lact ? m.tLx[ic + q] : S(0);
+  }
+  const S Ly = rowok ? m.Ly[j] : S(0), tLy = rowok ? m.tLy[j] : S(0);
+  const S LxL = (i0 > 0) ? m.Lx[i0 - 1] : S(0);
+  const int qbeam = (own && j == x.iys && x.ixs >= ic && x.ixs < ic + V) ? x.ixs - ic : -1;
+  int capq0 = V, capq1 = 0;
+  int64_t capbase = 0;
+  if (cap && own && x.cap_sample < x.cap_samples && j >= x.cap_iy0 && j < x.cap_iy0 + x.cap_ny) {
+    const int lo = max(ic, x.cap_ix0), hi = min(ic + V, x.cap_ix0 + x.cap_nx);
+    if (lo < hi) {
+      capq0 = lo - ic; capq1 = hi - ic;
+      capbase = ((int64_t)(j - x.cap_iy0) * x.cap_samples + x.cap_sample) * g.Nz;
+    }
+  }
+  S iAz[V], itAz[V];
+  S iAzL = 0;
+#pragma unroll
+  for (int q = 0; q < V; ++q) { iAz[q] = S(0); itAz[q] = S(0); }
+  if (LUT && lact) {
+    const I o = (I)ic + (I)g.Nx * (I)j;
+    ldv<V>(cf.iAz, o, iAz);
+    ldv<V>(cf.itAz, o, itAz);
+  }
+  if (LUT && hasL) iAzL = cf.iAz[(int64_t)(i0 - 1) + (int64_t)g.Nx * j];
+
+  const int kstart = (k0 > 0) ? k0 - 1 : k0;
+  S Ek[3][V], Ek1[3][V];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int q = 0; q < V; ++q) { Ek[c][q] = S(0); Ek1[c][q] = S(0); }
+  if (lact) {
+    const I n = col + sz * (I)kstart;
+    ldv<V>(Ein, n, Ek[0]); ldv<V>(Ein, N + n, Ek[1]); ldv<V>(Ein, 2 * N + n, Ek[2]);
+    if (kstart + 1 < g.Nz) {
+      ldv<V>(Ein, n + sz, Ek1[0]); ldv<V>(Ein, N + n + sz, Ek1[1]); ldv<V>(Ein, 2 * N + n + sz, Ek1[2]);
+    }
+  }
+  if (LUT) __syncthreads();
+  S Hpx[V], Hpy[V];  // PMC-masked H_new at k-1
+#pragma unroll
+  for (int q = 0; q < V; ++q) { Hpx[q] = S(0); Hpy[q] = S(0); }
+
+  for (int k = kstart; k < kend; ++k) {
+    const bool body = (k >= k0);
+    const I nk = col + sz * (I)k;
+    const bool kp1 = (k + 1 < g.Nz), kp2 = (k + 2 < g.Nz);
+    // ---- loads of this plane (issued before the staging barrier) and E(k+2)
+    S Ho[3][V], En2[3][V];
+    S c0y[V], c1y[V], iax = 0, itax = 0;
+    uint32_t code = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < V; ++q) { Ho[c][q] = S(0); En2[c][q] = S(0); }
+#pragma unroll
+    for (int q = 0; q < V; ++q) { c0y[q] = S(0); c1y[q] = S(0); }
+    S ao[3][V], bo[3][V], ie[3][V], sg[3][V];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < V; ++q) { ao[c][q] = S(0); bo[c][q] = S(0); ie[c][q] = S(0); sg[c][q] = S(0); }
+    if (lact && (w >= 1 || body)) {
+      ldv<V>(Hin, nk, Ho[0]); ldv<V>(Hin, N + nk, Ho[1]); ldv<V>(Hin, 2 * N + nk, Ho[2]);
+      if (LUT) {
+        if constexpr (V == 4) code = *reinterpret_cast<const uint32_t *>(cf.code + nk);
+        else code = *reinterpret_cast<const uint16_t *>(cf.code + nk);
+        iax = cf.iAx[(int64_t)j + (int64_t)g.Ny * k];
+        ldv<V>(cf.iAy, (I)ic + (I)g.Nx * (I)k, c0y);
+        if (w >= 1 && body) {
+          itax = cf.itAx[(int64_t)j + (int64_t)g.Ny * k];
+          ldv<V>(cf.itAy, (I)ic + (I)g.Nx * (I)k, c1y);
+        }
+      } else {
+        ldv<V>(cf.a, nk, ao[0]); ldv<V>(cf.a, N + nk, ao[1]); ldv<V>(cf.a, 2 * N + nk, ao[2]);
+        if (w >= 1 && body) {
+          ldv<V>(cf.b, nk, bo[0]); ldv<V>(cf.b, N + nk, bo[1]); ldv<V>(cf.b, 2 * N + nk, bo[2]);
+          if (COND) {
+            ldv<V>(cf.ieps, nk, ie[0]); ldv<V>(cf.ieps, N + nk, ie[1]); ldv<V>(cf.ieps, 2 * N + nk, ie[2]);
+            ldv<V>(cf.sigma, nk, sg[0]); ldv<V>(cf.sigma, N + nk, sg[1]); ldv<V>(cf.sigma, 2 * N + nk, sg[2]);
+          } else if (JM == 1 && qbeam >= 0) {
+            ldv<V>(cf.ieps, 2 * N + nk, ie[2]);
+          }
+        }
+      }
+    }
+    if (lact && kp2) {
+      const I n2 = nk + 2 * sz;
+      ldv<V>(Ein, n2, En2[0]); ldv<V>(Ein, N + n2, En2[1]); ldv<V>(Ein, 2 * N + n2, En2[2]);
+    }
+    S Tp[3][V];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < V; ++q) Tp[c][q] = S(0);
+    if (top) {
+      const I nt = tcol + sz * (I)k;
+      ldv<V>(Ein, nt, Tp[0]); ldv<V>(Ein, N + nt, Tp[1]); ldv<V>(Ein, 2 * N + nt, Tp[2]);
+    }
+    S hxL[3] = {0, 0, 0}, hxR[3] = {0, 0, 0}, htL = 0, htR[3] = {0, 0, 0};
+    S LH0 = 0, LH1 = 0, LA0 = 0, LA1 = 0, LE1 = 0;
+    int Lcode = 0;
+    if (hasL) {
+      const I n = col - 1 + sz * (I)k;
+      hxL[0] = *at(Ein, n); hxL[1] = *at(Ein, N + n); hxL[2] = *at(Ein, 2 * N + n);
+      if (w >= 1 && body) {
+        LH0 = *at(Hin, N + n); LH1 = *at(Hin, 2 * N + n);
+        if (LUT) {
+          Lcode = cf.code[n];
+          LA0 = cf.iAy[(int64_t)(i0 - 1) + (int64_t)g.Nx * k];
+        } else {
+          LA0 = *at(cf.a, N + n); LA1 = *at(cf.a, 2 * N + n);
+        }
+        LE1 = kp1 ? *at(Ein, n + sz) : S(0);
+      }
+    }
+    if (hasR) {
+      const I n = col + V + sz * (I)k;
+      hxR[0] = *at(Ein, n); hxR[1] = *at(Ein, N + n); hxR[2] = *at(Ein, 2 * N + n);
+    }
+    if (topL) htL = *at(Ein, tcol - 1 + sz * (I)k);
+    if (topR) {
+      const I n = tcol + V + sz * (I)k;
+      htR[0] = *at(Ein, n); htR[1] = *at(Ein, N + n); htR[2] = *at(Ein, 2 * N + n);
+    }
+
+    // ---- stage masked E(k)
+    const int zk = zface(pz, k, g.Nz), zk1 = kp1 ? zface(pz, k + 1, g.Nz) : -1;
+    const bool mEx = zsel(zk, mXj), mEx1 = kp1 && zsel(zk1, mXj);
+    bool mEy[V], mEy1[V];
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+      mEy[q] = lact && zsel(zk, mYi[q]);
+      mEy1[q] = lact && kp1 && zsel(zk1, mYi[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+      sE[0][w][OFF + xi + q] = mEx ? Ek[0][q] : S(0);
+      sE[1][w][OFF + xi + q] = mEy[q] ? Ek[1][q] : S(0);
+      sE[2][w][OFF + xi + q] = mZi[q] ? Ek[2][q] : S(0);
+    }
+    if (lane == 0) {
+      sE[0][w][OFF - 1] = (hasL && mEx) ? hxL[0] : S(0);
+      sE[1][w][OFF - 1] = (hasL && zsel(zk, mYL)) ? hxL[1] : S(0);
+      sE[2][w][OFF - 1] = mZL ? hxL[2] : S(0);
+    }
+    if (lane == 63) {
+      sE[0][w][OFF + W] = (hasR && mEx) ? hxR[0] : S(0);
+      sE[1][w][OFF + W] = (hasR && zsel(zk, mYR)) ? hxR[1] : S(0);
+      sE[2][w][OFF + W] = mZR ? hxR[2] : S(0);
+    }
+    if (w == 0) {
+      const bool mEt = (jt < g.Ny) && zsel(zk, mXt);
+#pragma unroll
+      for (int q = 0; q < V; ++q) {
+        sE[0][TY + 1][OFF + xi + q] = (top && mEt) ? Tp[0][q] : S(0);
+        sE[1][TY + 1][OFF + xi + q] = (top && zsel(zk, mYt[q])) ? Tp[1][q] : S(0);
+        sE[2][TY + 1][OFF + xi + q] = mZt[q] ? Tp[2][q] : S(0);
+      }
+      if (lane == 0) sE[0][TY + 1][OFF - 1] = (topL && mEt) ? htL : S(0);
+      if (lane == 63) {
+        sE[0][TY + 1][OFF + W] = (topR && mEt) ? htR[0] : S(0);
+        sE[1][TY + 1][OFF + W] = (topR && zsel(zk, mYR)) ? htR[1] : S(0);
+        sE[2][TY + 1][OFF + W] = mZtR ? htR[2] : S(0);
+      }
+    }
+    const S Lz = m.Lz[k];
+    __syncthreads();
+
+    // ---- H_new(k) of this row (own rows always, the halo row on body planes)
+    S Hn[3][V];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < V; ++q) Hn[c][q] = S(0);
+    if (lact && (w >= 1 || body)) {
+      if (LUT) {
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+          const S *t = sMat + NM * (int)((code >> (8 * q)) & 0xffu);
+          ao[0][q] = (S)(iax * t[0]);
+          ao[1][q] = (S)(c0y[q] * t[1]);
+          ao[2][q] = (S)(iAz[q] * t[2]);
+          if (w >= 1 && body) {
+            bo[0][q] = mEx ? (S)(t[3] * itax) : S(0);
+            bo[1][q] = mEy[q] ? (S)(t[4] * c1y[q]) : S(0);
+            bo[2][q] = mZi[q] ? (S)(t[5] * itAz[q]) : S(0);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { ie[c][q] = t[3 + c]; sg[c][q] = t[6 + c]; }
+          }
+        }
+      }
+      if (hz) {
+        const int zh = zface(hz, k, g.Nz);
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+          if (!zsel(zh, ypart(hz, j, g.Ny))) ao[0][q] = S(0);
+          if (!zsel(zh, xpart(hz, ic + q, g.Nx))) ao[1][q] = S(0);
+          if (!mz(hz, ic + q, j, g)) ao[2][q] = S(0);
+        }
+      }
+      S EX[V + 1], EY[V + 1], EZ[V + 1];
+#pragma unroll
+      for (int q = 0; q < V + 1; ++q) {
+        EX[q] = sE[0][w][OFF + xi + q]; EY[q] = sE[1][w][OFF + xi + q]; EZ[q] = sE[2][w][OFF + xi + q];
+      }
+#pragma unroll
+      for (int q = 0; q < V; ++q) {
+        const S ex1 = mEx1 ? Ek1[0][q] : S(0), ey1 = mEy1[q] ? Ek1[1][q] : S(0);
+        Hn[0][q] = h_row<S, S, DESC>(Ho[0][q], ao[0][q], dt, Ly, Lz, EY[q], ey1, EZ[q],
+                                     sE[2][w + 1][OFF + xi + q], +1, -1);
+        Hn[1][q] = h_row<S, S, DESC>(Ho[1][q], ao[1][q], dt, Lx[q], Lz, EX[q], ex1, EZ[q],
+                                     EZ[q + 1], -1, +1);
+        Hn[2][q] = h_row<S, S, DESC>(Ho[2][q], ao[2][q], dt, Lx[q], Ly, EX[q],
+                                     sE[0][w + 1][OFF + xi + q], EY[q], EY[q + 1], +1, -1);
+      }
+    }
+    {
+      S hm[3][V];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int q = 0; q < V; ++q) hm[c][q] = Hn[c][q];
+      if (hz && lact) {
+        const int zh = zface(hz, k, g.Nz);
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+          if (!zsel(zh, ypart(hz, j, g.Ny))) hm[0][q] = S(0);
+          if (!zsel(zh, xpart(hz, ic + q, g.Nx))) hm[1][q] = S(0);
+          if (!mz(hz, ic + q, j, g)) hm[2][q] = S(0);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int q = 0; q < V; ++q) sH[c][w][OFF + xi + q] = hm[c][q];
+      if (lane == 0) {
+        S hy = 0, hzv = 0;
+        if (hasL && w >= 1 && body) {
+          S lay, laz;
+          if (LUT) {
+            const S *t = sMat + NM * Lcode;
+            lay = (S)(LA0 * t[1]);
+            laz = (S)(iAzL * t[2]);
+          } else {
+            lay = LA0; laz = LA1;
+          }
+          if (hz) {
+            const int zh = zface(hz, k, g.Nz);
+            if (!zsel(zh, xpart(hz, i0 - 1, g.Nx))) lay = S(0);
+            if (!mz(hz, i0 - 1, j, g)) laz = S(0);
+          }
+          const S exl = sE[0][w][OFF - 1], eyl = sE[1][w][OFF - 1], ezl = sE[2][w][OFF - 1];
+          const S ex1 = mEx1 ? LE1 : S(0);
+          hy = h_row<S, S, DESC>(LH0, lay, dt, LxL, Lz, exl, ex1, ezl, sE[2][w][OFF], -1, +1);
+          hzv = h_row<S, S, DESC>(LH1, laz, dt, LxL, Ly, exl, sE[0][w + 1][OFF - 1], eyl,
+                                  sE[1][w][OFF], +1, -1);
+          if (hz) {
+            const int zh = zface(hz, k, g.Nz);
+            if (!zsel(zh, xpart(hz, i0 - 1, g.Nx))) hy = S(0);
+            if (!mz(hz, i0 - 1, j, g)) hzv = S(0);
+          }
+        }
+        sH[0][w][OFF - 1] = S(0);
+        sH[1][w][OFF - 1] = hy;
+        sH[2][w][OFF - 1] = hzv;
+      }
+    }
+    __syncthreads();
+
+    // ---- E_new(k) for own rows
+    if (own && body) {
+      const S tLz = m.tLz[k];
+      S HX[V + 1], HY[V + 1], HZ[V + 1];  // [0] = -x neighbour
+#pragma unroll
+      for (int q = 0; q < V + 1; ++q) {
+        HX[q] = sH[0][w][OFF + xi + q - 1];
+        HY[q] = sH[1][w][OFF + xi + q - 1];
+        HZ[q] = sH[2][w][OFF + xi + q - 1];
+      }
+      S En[3][V];
+#pragma unroll
+      for (int q = 0; q < V; ++q) {
+        S d0 = e_sum<S>(bo[0][q], tLy, tLz, Hpy[q], HY[q + 1], sH[2][w - 1][OFF + xi + q],
+                        HZ[q + 1], +1, -1);
+        S d1 = e_sum<S>(bo[1][q], tLx[q], tLz, Hpx[q], HX[q + 1], HZ[q], HZ[q + 1], -1, +1);
+        S d2 = e_sum<S>(bo[2][q], tLx[q], tLy, sH[0][w - 1][OFF + xi + q], HX[q + 1], HY[q],
+                        HY[q + 1], +1, -1);
+        const bool online = (q == qbeam);
+        S jline = 0;
+        if (online && (JM == 1 || COND)) jline = beam_row ? beam_row[k] : *at(J, 2 * N + nk + q);
+        if (COND) {
+          d0 = d0 - ie[0][q] * (S)(sg[0][q] * Ek[0][q]);
+          d1 = d1 - ie[1][q] * (S)(sg[1][q] * Ek[1][q]);
+          d2 = d2 - ie[2][q] * (online ? jline : (S)(sg[2][q] * Ek[2][q]));
+        } else if (JM == 1 && online) {
+          d2 = d2 - ie[2][q] * jline;
+        }
+        En[0][q] = (S)(Ek[0][q] + dt * d0);
+        En[1][q] = (S)(Ek[1][q] + dt * d1);
+        En[2][q] = (S)(Ek[2][q] + dt * d2);
+        if (!COND && online && beam_row) *atw(J, 2 * N + nk + q) = jline;
+        if (q >= capq0 && q < capq1)
+          cap[capbase + (int64_t)(q + ic - x.cap_ix0) * x.cap_ny * x.cap_samples * g.Nz + k] =
+              En[2][q];
+      }
+      stv<V>(Eout, nk, En[0]); stv<V>(Eout, N + nk, En[1]); stv<V>(Eout, 2 * N + nk, En[2]);
+      stv<V>(Hout, nk, Hn[0]); stv<V>(Hout, N + nk, Hn[1]); stv<V>(Hout, 2 * N + nk, Hn[2]);
+      if (COND) {
+        S Jn[3][V];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int q = 0; q < V; ++q) Jn[c][q] = (S)(sg[c][q] * En[c][q]);
+        stv<V>(J, nk, Jn[0]); stv<V>(J, N + nk, Jn[1]); stv<V>(J, 2 * N + nk, Jn[2]);
+      }
+#pragma unroll
+      for (int q = 0; q < V; ++q) { Hpx[q] = HX[q + 1]; Hpy[q] = HY[q + 1]; }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < V; ++q) { Ek[c][q] = Ek1[c][q]; Ek1[c][q] = En2[c][q]; }
+    if (w >= 1 && !body) {  // warm-up plane: H_new(k0-1) seeds the z-1 terms
+#pragma unroll
+      for (int q = 0; q < V; ++q) {
+        Hpx[q] = sH[0][w][OFF + xi + q];
+        Hpy[q] = sH[1][w][OFF + xi + q];
+      }
+    }
+  }
+}
+
+}  // namespace wf

@@ -158,7 +158,12 @@ class SolverFIT3D(RoutinesMixin):
         self._fold_count = 0
         self._wrap_key = None
         self._E2 = self._H2 = None
-        self.fused_kz = 32  # z-planes per workgroup of the fused kernel
+        self.fused_kz = 0  # z-planes per workgroup of the fused kernel (0: automatic)
+        self.fused_variant = 0  # 0 auto, 1 scalar, 2/3/4 vectorised with 8/2/4 rows
+        self.use_material_code = True  # fused path: uint8 code + coefficient table
+        self._lut_key = None
+        self._code = self._mat = self._metric2d = None
+        self._nmat = 0
         self.update_tensors("all", _initial=True)
         self._beam_line = None
         self._J_clean_version = self.J.version
@@ -325,9 +330,13 @@ class SolverFIT3D(RoutinesMixin):
             self._a = torch.empty(3 * N, dtype=tdt, device=dev)
             self._b = torch.empty(3 * N, dtype=tdt, device=dev)
         others = {0: (1, 2), 1: (0, 2), 2: (0, 1)}
+        self._iA2d, self._itA2d = [None] * 3, [None] * 3
+        plane = {0: (slice(None), slice(None), 0), 1: (slice(None), 0, slice(None)),
+                 2: (0, slice(None), slice(None))}
         for c in range(3):
             u, w = others[c]
             iA = torch.broadcast_to(1.0 / (L[u] * L[w]), (Nz, Ny, Nx))
+            self._iA2d[c] = iA[plane[c]].to(tdt).contiguous().reshape(-1)
             if tensor in ("all", "imu"):
                 a_c = iA.to(tdt) * self.imu._block(c).view(Nz, Ny, Nx)
                 if self.pmc_zero:
@@ -342,12 +351,14 @@ class SolverFIT3D(RoutinesMixin):
                         continue
                     ax = 2 - e  # (Nz,Ny,Nx) axis of grid axis e
                     itA.select(ax, itA.shape[ax] - 1).copy_(iA.select(ax, 0))
+                self._itA2d[c] = itA[plane[c]].to(tdt).contiguous().reshape(-1)
                 b_c = self.ieps._block(c).view(Nz, Ny, Nx) * itA.to(tdt)
                 if self.pec_zero:
                     b_c = b_c * face_mask_kji(c, self.pec_zero, Nx, Ny, Nz, dev).to(tdt)
                 self._b[c * N:(c + 1) * N].copy_(b_c.reshape(-1))
         torch.cuda.synchronize(dev)
         self._fold_count = getattr(self, "_fold_count", 0) + 1
+        self._fold_versions = (self.ieps.version, self.imu.version)
         if not _initial:
             self.step_0 = False  # as the reference (solverFIT3D.py:414): no ghost zeroing after
 
@@ -408,12 +419,72 @@ class SolverFIT3D(RoutinesMixin):
                 why = "J is not sigma*E (user write since the last conductive step)"
         elif args.jmode == _lib.WF_J_FULL:
             why = "J was written outside the beam line"
+        if why is None and args.compute_f64:
+            why = "FP64 arithmetic on FP32 storage runs on the exact general path"
         if why is None and not self._wraps_dead():
             why = "flat-index wrap terms are live for this boundary/material set-up"
         if why is not None:
             if self.path == "fused":
                 raise RuntimeError(f"fused path not applicable: {why}")
             return False
+        return True
+
+    def _material_table(self):
+        """Compressed coefficients for the fused kernel, if the set-up allows.
+
+        Per cell a uint8 material code into a table of (imu, ieps, sigma) per
+        component (<= 256 materials), plus 2-D metric tables iA_c, itA_c over
+        the transverse axes of each component. The kernel recomputes
+        a = fl(iA*imu) and b = fl(ieps*itA) with the row masks; this method
+        checks that the recomputation reproduces the folded arrays exactly
+        before enabling it. Valid until coefficients are re-folded or
+        ieps/imu/sigma change.
+        """
+        key = (self._fold_count, self.ieps.version, self.imu.version, self.sigma.version)
+        if key == self._lut_key:
+            return self._code is not None
+        self._lut_key = key
+        self._code = self._mat = self._metric2d = None
+        self._nmat = 0
+        if (self.ieps.version, self.imu.version) != self._fold_versions:
+            return False  # materials edited without update_tensors: use the arrays
+        N, (Nx, Ny, Nz), dev = self.N, self.dims, self.device
+        cols = ([self.imu.array[c * N:(c + 1) * N] for c in range(3)]
+                + [self.ieps.array[c * N:(c + 1) * N] for c in range(3)]
+                + [self.sigma.array[c * N:(c + 1) * N] for c in range(3)])
+        code = torch.zeros(N, dtype=torch.int64, device=dev)
+        for col in cols:
+            u, inv = torch.unique(col, return_inverse=True)
+            if u.numel() > 256:
+                return False
+            code = torch.unique(code * u.numel() + inv, return_inverse=True)[1]
+        K = int(code.max().item()) + 1
+        if K > 256:
+            return False
+        rep = torch.empty(K, dtype=torch.int64, device=dev)
+        rep.scatter_(0, code, torch.arange(N, device=dev))
+        mat = torch.stack([col[rep] for col in cols], dim=1).contiguous()
+        iA, itA = self._iA2d, self._itA2d
+        metric2d = torch.cat([iA[0], iA[1], iA[2], itA[0], itA[1], itA[2]]).contiguous()
+        # exactness check against the folded arrays
+        shp = {0: (Nz, Ny, 1), 1: (Nz, 1, Nx), 2: (1, Ny, Nx)}
+        c3 = code.view(Nz, Ny, Nx)
+        for c in range(3):
+            ia = iA[c].view(shp[c])
+            ita = itA[c].view(shp[c])
+            a_c = ia * mat[:, c][c3]
+            if self.pmc_zero:
+                a_c = torch.where(face_mask_kji(c, self.pmc_zero, Nx, Ny, Nz, dev), a_c,
+                                  torch.zeros((), dtype=a_c.dtype, device=dev))
+            b_c = mat[:, 3 + c][c3] * ita
+            if self.pec_zero:
+                b_c = torch.where(face_mask_kji(c, self.pec_zero, Nx, Ny, Nz, dev), b_c,
+                                  torch.zeros((), dtype=b_c.dtype, device=dev))
+            if not (torch.equal(a_c.reshape(-1), self._a[c * N:(c + 1) * N])
+                    and torch.equal(b_c.reshape(-1), self._b[c * N:(c + 1) * N])):
+                return False
+        self._code = code.to(torch.uint8).contiguous()
+        self._mat, self._nmat, self._metric2d = mat, K, metric2d
         return True
 
     def _wraps_dead(self):
@@ -522,6 +593,11 @@ class SolverFIT3D(RoutinesMixin):
             args.path = 1
             args.E2, args.H2 = _lib.ptr(self._E2), _lib.ptr(self._H2)
             args.kz = int(self.fused_kz)
+            args.variant = int(self.fused_variant)
+            if self.use_material_code and self._material_table():
+                args.code, args.mat, args.metric2d = (_lib.ptr(self._code), _lib.ptr(self._mat),
+                                                      _lib.ptr(self._metric2d))
+                args.nmat = self._nmat
             if args.jmode == _lib.WF_J_FULL and not args.conductive:
                 raise RuntimeError("internal: fused path without a J mode")
             if args.conductive:
