@@ -26,6 +26,22 @@ METRIC = ("Gcell-updates/s + %HBM peak (FP32 vs FP64) at 256x256x512; adjoint/fw
 HBM_PEAK_GBS = 8000.0
 VALUES_PER_CELL = 18  # E r/w 3+3, H r/w 3+3, face coeffs 3, edge coeffs 3
 
+# HBM bytes per launch of the step kernel on the default workload, measured
+# with rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over this script
+# (profiles/k_fused4_cavity_pipe_256x256x512.md): FETCH_SIZE is scaled by 2.0,
+# the ratio of the bytes a 1 GiB device copy reads to what the counter reports
+# for it; WRITE_SIZE matches the copy's bytes and is taken as is. Keyed by
+# (grid, dtype, path, variant, kz); other settings report null.
+MEASURED_TRAFFIC = {
+    ((256, 256, 512), "f32", "fused", 0, 0): {"fetch_B": 958_450_893, "write_B": 812_400_640},
+    ((256, 256, 512), "f64", "fused", 0, 0): {"fetch_B": 2_276_237_517, "write_B": 1_610_629_120},
+}
+
+
+def measured_traffic(grid, dtype, path, variant, kz):
+    t = MEASURED_TRAFFIC.get((tuple(grid), dtype, path, variant, kz))
+    return None if t is None else t["fetch_B"] + t["write_B"]
+
 
 def build_solver(Nx, Ny, Nz, dtype, path):
     import wakis_amd as wa
@@ -38,12 +54,37 @@ def build_solver(Nx, Ny, Nz, dtype, path):
                           use_stl=True, bg="pec", dtype=dtype, verbose=0, path=path)
 
 
-def time_gpu(Nx, Ny, Nz, dtype, steps, warmup, path):
+def hbm_copy_gbs(dev, nbytes=1 << 30, reps=5):
+    """Device-to-device copy of `nbytes` (read + write counted): the HBM rate a
+    plain streaming kernel reaches on this GPU, and the reference dispatch for
+    rocprofv3 FETCH_SIZE/WRITE_SIZE scaling."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize(dev)
+    stream = torch.cuda.current_stream(dev)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record(stream)
+    torch.cuda.synchronize(dev)
+    ms = e0.elapsed_time(e1)
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes * reps / (ms * 1e-3) / 1e9
+
+
+def time_gpu(Nx, Ny, Nz, dtype, steps, warmup, path, variant=0, kz=0):
     import torch
 
     import wakis_amd as wa
     from wakis_amd import _lib
     s = build_solver(Nx, Ny, Nz, dtype, path)
+    s.fused_variant = variant
+    s.fused_kz = kz
     tot = warmup + steps
     beam = wa.Beam(q=1e-9, sigmaz=10e-3, beta=1.0)
     dev = s.device
@@ -118,21 +159,29 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-grid", default="128,128,256")
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=120)
+    ap.add_argument("--variant", type=int, default=0,
+                    help="fused kernel: 0 auto, 1 scalar, 2/3/7/8 rows per workgroup")
+    ap.add_argument("--kz", type=int, default=0, help="z-planes per workgroup (0 auto)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this "
-              "build (the z-slab sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: this engine runs on one GPU; runs on several GPUs are not "
+              "supported (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
     Nx, Ny, Nz = (int(v) for v in args.grid.split(","))
+    import torch
+    copy_gbs = hbm_copy_gbs(torch.device("cuda", 0))
+    print(f"[bench] hbm copy {copy_gbs:.1f} GB/s", file=sys.stderr, flush=True)
     print(f"[bench] fp32 {Nx}x{Ny}x{Nz} steps={args.steps} warmup={args.warmup}",
           file=sys.stderr, flush=True)
-    r32 = time_gpu(Nx, Ny, Nz, np.float32, args.steps, args.warmup, args.path)
+    r32 = time_gpu(Nx, Ny, Nz, np.float32, args.steps, args.warmup, args.path, args.variant,
+                   args.kz)
     print(f"[bench] fp32 {r32}", file=sys.stderr, flush=True)
     r64 = None
     if not args.no_fp64:
         k64 = max(1, args.steps // 3)
-        r64 = time_gpu(Nx, Ny, Nz, np.float64, k64, args.warmup, args.path)
+        r64 = time_gpu(Nx, Ny, Nz, np.float64, k64, args.warmup, args.path, args.variant,
+                       args.kz)
         print(f"[bench] fp64 {r64}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
@@ -158,14 +207,19 @@ def main():
                    "cells": r32["N"], "path": r32["path"]},
         "roofline": {"bound": "hbm", "achieved": round(r32["achieved_GBs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(r32["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_cell_update": r32["bytes_per_cell"]},
+                     "frac": round(r32["achieved_GBs"] / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic((Nx, Ny, Nz), "f32", r32["path"], args.variant,
+                                                 args.kz),
+                     "bytes_per_cell_update": r32["bytes_per_cell"],
+                     "hbm_copy_GBs": round(copy_gbs, 1)},
     }
     if r64 is not None:
         line["fp64"] = {"value": round(r64["gcell_s"], 3), "unit": "Gcell-updates/s",
                         "ms_per_step": round(r64["ms_per_step"], 5), "steps": r64["steps"],
                         "achieved_GBs": round(r64["achieved_GBs"], 1),
-                        "frac": round(r64["achieved_GBs"] / HBM_PEAK_GBS, 4)}
+                        "frac": round(r64["achieved_GBs"] / HBM_PEAK_GBS, 4),
+                        "traffic": measured_traffic((Nx, Ny, Nz), "f64", r64["path"],
+                                                    args.variant, args.kz)}
         line["fp32_over_fp64"] = round(r32["gcell_s"] / r64["gcell_s"], 3)
     if cpu is not None:
         line["cpu_baseline"] = {

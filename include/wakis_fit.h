@@ -102,10 +102,11 @@ typedef struct wf_step_args {
                              iA_z[Nx*Ny] (i+Nx*j), then itA_x, itA_y, itA_z likewise
                              (itA after the periodic/ABC copies)                     */
   int32_t nmat;           /* 1..256                                                   */
-  int32_t variant;        /* fused kernel choice: 0 auto (x-vectorised when Nx is a
-                             multiple of 16/sizeof(dtype): 8 rows/workgroup FP32, 2
-                             FP64), 1 scalar, 2 vectorised 8 rows (FP32; 4 for FP64),
-                             3 vectorised 2 rows, 4 vectorised 4 rows             */
+  int32_t variant;        /* fused kernel choice: 0 auto (x-vectorised kernel when Nx
+                             is a multiple of 16/sizeof(dtype), with the row count
+                             measured fastest for the dtype), 1 scalar kernel,
+                             2, 3, 7 or 8 (FP32 only): x-vectorised kernel with
+                             that many rows per workgroup                          */
 } wf_step_args;
 
 int wf_version(void);
@@ -163,6 +164,27 @@ int wf_wake_trans(const double *WP3d, int64_t ns, double ds, double dxsum, doubl
  * x is [nsig][len], out is [nsig][nbins][2] (re, im). */
 int wf_dft(const double *x, int32_t nsig, int64_t len, int64_t N, int64_t nbins,
            double *out, void *stream);
+
+/* FIT field energy of the leapfrog. No reference function computes it; the
+ * reference documentation only sketches an energy density
+ * (docs/physicsguide.md:181-182). out (device, 2 doubles):
+ *   out[0] = 1/2 sum over edges with ieps > 0 and itA > 0 of L/(ieps*itA) E^2
+ *   out[1] = 1/2 sum over faces of tL/(imu*iA) Hprev*H
+ * (L, tL: the handle's edge lengths; iA, itA: metric2d, laid out as in
+ * wf_step_args; weights formed in FP64 from the storage-type values). With
+ * E = E^n, Hprev = H^(n-1/2), H = H^(n+1/2) the sum is the discrete energy the
+ * lossless leapfrog conserves exactly (up to rounding); with conductivity it
+ * decreases. Deterministic FP64 compensated reduction, one workspace per
+ * handle: calls on one handle must not overlap. */
+int wf_energy(wf_handle *h, const void *E, const void *Hprev, const void *H, const void *ieps,
+              const void *imu, const void *metric2d, double *out, void *stream);
+
+/* Loss factor k = -sum_n WP[n]*lam[n]*ds (V/pC for WP in V/pC, lam in 1/m):
+ * the integral -int WP(s) lambda(s) ds over the wake's s axis. No reference
+ * function computes it. Compensated FP64 sum on the device, out is a device
+ * double. */
+int wf_loss_factor(const double *WP, const double *lam, int64_t n, double ds, double *out,
+                   void *stream);
 
 #ifdef __cplusplus
 }

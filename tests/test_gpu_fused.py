@@ -18,7 +18,7 @@ PEC_ALL = (["pec", "pec", "pec"], ["pec", "pec", "pec"])
 @pytest.mark.parametrize("conductive", [False, True])
 @pytest.mark.parametrize("kz", [0, 5, 1])
 @pytest.mark.parametrize("use_code", [True, False])
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 7, 8])
 def test_fused_matches_oracle(dims, dtype, conductive, kz, use_code, variant):
     print("fused", dims, np.dtype(dtype).name, conductive, kz, use_code, variant, flush=True)
     case = gc.random_case(PEC_ALL, dtype, conductive, seed=sum(dims) + kz, dims=dims)
@@ -36,6 +36,10 @@ def test_fused_matches_oracle(dims, dtype, conductive, kz, use_code, variant):
         sa.path = "general"    # the first step (J not sigma*E yet) runs exactly
         sa.one_step()
         sa.path = "fused"
+    if variant == 8 and dtype == np.float64 and dims[0] % 2 == 0:
+        with pytest.raises(ValueError, match="variant"):  # 8-row tiles: FP32 only
+            sa.one_step()
+        return
     for _ in range(5):
         so.one_step()
         sa.one_step()

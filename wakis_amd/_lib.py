@@ -58,6 +58,9 @@ EXPORTS = {
     "wf_wake_trans": (c_int32, [c_void_p, c_int64, c_double, c_double, c_double, c_void_p,
                                 c_void_p, c_void_p]),
     "wf_dft": (c_int32, [c_void_p, c_int32, c_int64, c_int64, c_int64, c_void_p, c_void_p]),
+    "wf_energy": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                            c_void_p, c_void_p, c_void_p]),
+    "wf_loss_factor": (c_int32, [c_void_p, c_void_p, c_int64, c_double, c_void_p, c_void_p]),
 }
 
 _lib = None

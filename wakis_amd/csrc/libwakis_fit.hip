@@ -13,6 +13,7 @@
 #include "step_fused.h"
 #include "step_fused4.h"
 #include "wake.h"
+#include "reduce.h"
 
 namespace {
 
@@ -50,6 +51,7 @@ struct wf_handle {
   int dtype;
   int device;
   void *metric_buf;  // device: 6 vectors in the storage type
+  double *red_buf;   // device: per-block partials of the energy reduction
   wf::MetricDev<double> md;
   wf::MetricDev<float> mf;
 };
@@ -104,6 +106,12 @@ int wf_create(const wf_grid *gd, wf_handle **out) {
     delete h;
     return fail(WF_EHIP, "wf_create: hipMemcpy: %s", hipGetErrorString(e));
   }
+  e = hipMalloc((void **)&h->red_buf, sizeof(double) * 4 * wf::RED_BLOCKS);
+  if (e != hipSuccess) {
+    hipFree(h->metric_buf);
+    delete h;
+    return fail(WF_EHIP, "wf_create: hipMalloc: %s", hipGetErrorString(e));
+  }
   if (esz == 8) {
     const double *b = (const double *)h->metric_buf;
     h->md = {b + off[0], b + off[1], b + off[2], b + off[3], b + off[4], b + off[5]};
@@ -118,6 +126,7 @@ int wf_create(const wf_grid *gd, wf_handle **out) {
 int wf_destroy(wf_handle *h) {
   if (!h) return WF_OK;
   if (h->metric_buf) hipFree(h->metric_buf);
+  if (h->red_buf) hipFree(h->red_buf);
   delete h;
   return WF_OK;
 }
@@ -250,13 +259,18 @@ int step_fused(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStre
   // 3N array within 4 GiB
   const bool idx32v = (uint64_t)3 * (uint64_t)g.N * sizeof(S) < (1ull << 32);
   if (g.Nx % (int)(16 / sizeof(S)) == 0 && idx32v && a->variant != 1) {
-    if constexpr (sizeof(S) == 4) {
-      if (a->variant == 2 || a->variant == 0) return step_fused4<S, 8>(h, a, m, st);
-    } else {
-      if (a->variant == 0) return step_fused4<S, 2>(h, a, m, st);
+    // variant = rows per workgroup (0: the measured best for the type)
+    const int rows = a->variant ? a->variant : (sizeof(S) == 4 ? 8 : 2);
+    switch (rows) {
+      case 2: return step_fused4<S, 2>(h, a, m, st);
+      case 3: return step_fused4<S, 3>(h, a, m, st);
+      case 7: return step_fused4<S, 7>(h, a, m, st);
+      case 8:
+        if constexpr (sizeof(S) == 4) return step_fused4<S, 8>(h, a, m, st);
+        break;
+      default: break;
     }
-    if (a->variant == 3) return step_fused4<S, 2>(h, a, m, st);
-    return step_fused4<S, 4>(h, a, m, st);  // variant 4 (and 2 for FP64)
+    return fail(WF_EINVAL, "wf_step: variant (rows per workgroup) not built for this dtype");
   }
   wf::FusedAux x;
   x.ixs = (a->jmode == WF_J_LINE || a->beam_table || a->conductive) ? a->ixs : -1;
@@ -456,6 +470,40 @@ int wf_dft(const double *x, int32_t nsig, int64_t len, int64_t N, int64_t nbins,
   if (nbins > 2147483647LL || nsig > 65535) return fail(WF_ERANGE, "wf_dft: too many bins");
   hipLaunchKernelGGL(wf::k_dft, dim3((unsigned)nbins, (unsigned)nsig), dim3(256), 0,
                      (hipStream_t)stream, x, len, N, nbins, out);
+  WF_LAUNCH_CHECK();
+  return WF_OK;
+}
+
+int wf_energy(wf_handle *h, const void *E, const void *Hprev, const void *H, const void *ieps,
+              const void *imu, const void *metric2d, double *out, void *stream) {
+  if (!h || !E || !Hprev || !H || !ieps || !imu || !metric2d || !out)
+    return fail(WF_EINVAL, "wf_energy: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t want = (h->g.N + wf::RED_THREADS - 1) / wf::RED_THREADS;
+  const int nb = (int)(want < wf::RED_BLOCKS ? want : wf::RED_BLOCKS);
+  if (h->dtype == WF_F64)
+    hipLaunchKernelGGL(wf::k_energy<double>, dim3(nb), dim3(wf::RED_THREADS), 0, st, h->g,
+                       h->md, (const double *)E, (const double *)Hprev, (const double *)H,
+                       (const double *)ieps, (const double *)imu, (const double *)metric2d,
+                       h->red_buf);
+  else
+    hipLaunchKernelGGL(wf::k_energy<float>, dim3(nb), dim3(wf::RED_THREADS), 0, st, h->g,
+                       h->mf, (const float *)E, (const float *)Hprev, (const float *)H,
+                       (const float *)ieps, (const float *)imu, (const float *)metric2d,
+                       h->red_buf);
+  WF_LAUNCH_CHECK();
+  hipLaunchKernelGGL(wf::k_combine2, dim3(1), dim3(wf::RED_THREADS), 0, st,
+                     (const double *)h->red_buf, nb, out);
+  WF_LAUNCH_CHECK();
+  return WF_OK;
+}
+
+int wf_loss_factor(const double *WP, const double *lam, int64_t n, double ds, double *out,
+                   void *stream) {
+  if (!WP || !lam || !out) return fail(WF_EINVAL, "wf_loss_factor: null argument");
+  if (n < 0) return fail(WF_ERANGE, "wf_loss_factor: negative length");
+  hipLaunchKernelGGL(wf::k_dot, dim3(1), dim3(wf::RED_THREADS), 0, (hipStream_t)stream, WP, lam,
+                     n, -ds, out);
   WF_LAUNCH_CHECK();
   return WF_OK;
 }

@@ -159,7 +159,10 @@ class SolverFIT3D(RoutinesMixin):
         self._wrap_key = None
         self._E2 = self._H2 = None
         self.fused_kz = 0  # z-planes per workgroup of the fused kernel (0: automatic)
-        self.fused_variant = 0  # 0 auto, 1 scalar, 2/3/4 vectorised with 8/2/4 rows
+        self.fused_variant = 0  # 0 auto, 1 scalar, 2/3/7/8: vectorised, rows per workgroup
+        # energy(): keep E^n and H^(n-1/2) on the exact path too (one copy per step)
+        self.keep_previous_fields = False
+        self._prev_key = None
         self.use_material_code = True  # fused path: uint8 code + coefficient table
         self._lut_key = None
         self._code = self._mat = self._metric2d = None
@@ -342,20 +345,21 @@ class SolverFIT3D(RoutinesMixin):
                 if self.pmc_zero:
                     a_c = a_c * face_mask_kji(c, self.pmc_zero, Nx, Ny, Nz, dev).to(tdt)
                 self._a[c * N:(c + 1) * N].copy_(a_c.reshape(-1))
+            aux = torch.broadcast_to(tL[u] * tL[w], (Nz, Ny, Nx))
+            itA = torch.where(aux != 0, 1.0 / torch.where(aux != 0, aux, 1.0),
+                              torch.zeros((), dtype=f64, device=dev)).clone()
+            for e in self._metric_edits:  # itA[-1 along e] = iA[0 along e]
+                if e == c:
+                    continue
+                ax = 2 - e  # (Nz,Ny,Nx) axis of grid axis e
+                itA.select(ax, itA.shape[ax] - 1).copy_(iA.select(ax, 0))
+            self._itA2d[c] = itA[plane[c]].to(tdt).contiguous().reshape(-1)
             if tensor in ("all", "ieps"):
-                aux = torch.broadcast_to(tL[u] * tL[w], (Nz, Ny, Nx))
-                itA = torch.where(aux != 0, 1.0 / torch.where(aux != 0, aux, 1.0),
-                                  torch.zeros((), dtype=f64, device=dev)).clone()
-                for e in self._metric_edits:  # itA[-1 along e] = iA[0 along e]
-                    if e == c:
-                        continue
-                    ax = 2 - e  # (Nz,Ny,Nx) axis of grid axis e
-                    itA.select(ax, itA.shape[ax] - 1).copy_(iA.select(ax, 0))
-                self._itA2d[c] = itA[plane[c]].to(tdt).contiguous().reshape(-1)
                 b_c = self.ieps._block(c).view(Nz, Ny, Nx) * itA.to(tdt)
                 if self.pec_zero:
                     b_c = b_c * face_mask_kji(c, self.pec_zero, Nx, Ny, Nz, dev).to(tdt)
                 self._b[c * N:(c + 1) * N].copy_(b_c.reshape(-1))
+        self._metric2d_all = torch.cat(self._iA2d + self._itA2d).contiguous()
         torch.cuda.synchronize(dev)
         self._fold_count = getattr(self, "_fold_count", 0) + 1
         self._fold_versions = (self.ieps.version, self.imu.version)
@@ -602,14 +606,47 @@ class SolverFIT3D(RoutinesMixin):
                 raise RuntimeError("internal: fused path without a J mode")
             if args.conductive:
                 args.jmode = _lib.WF_J_FULL
+        keep = (not fused) and self.keep_previous_fields and args.nsteps == 1
+        if keep:
+            if self._E2 is None:
+                self._E2 = torch.empty_like(self.E.array)
+                self._H2 = torch.empty_like(self.H.array)
+            self._E2.copy_(self.E.array)
+            self._H2.copy_(self.H.array)
         _lib.check(_lib.lib().wf_step(self._handle, ctypes.byref(args), _lib.stream_ptr()),
                    "wf_step")
         self.last_path = "fused" if fused else "general"
         if fused and args.result_swapped:
             self.E.array, self._E2 = self._E2, self.E.array
             self.H.array, self._H2 = self._H2, self.H.array
+        # the spare buffers now hold E^n, H^(n-1/2) of the last step (fused: the
+        # ping-pong inputs; exact path: the copies above)
+        self._prev_key = (self.E.version, self.H.version) if (fused or keep) else None
         if self.use_conductivity:
             self._sigmaE_versions = (self.E.version, self.J.version)
+
+    def energy(self, parts=False):
+        """Discrete FIT energy of the last step, in joules (no reference
+        counterpart; docs/physicsguide.md:181-182 sketches an energy density).
+
+        W^n = 1/2 sum eps*tA*L*(E^n)^2 + 1/2 sum mu*A*tL*H^(n-1/2)*H^(n+1/2)
+        over the live edges/faces: the quantity the lossless leapfrog conserves
+        exactly (to rounding); with conductivity it decreases. Needs the fields
+        before the last step: available after a fused step, or on the exact
+        path with ``keep_previous_fields = True`` and single steps. FP64
+        compensated reduction on the device. ``parts=True`` returns
+        (electric, magnetic).
+        """
+        if self._prev_key is None or self._prev_key != (self.E.version, self.H.version):
+            raise RuntimeError("energy(): the fields before the last step are not kept; step "
+                               "again (fused path) or set keep_previous_fields = True")
+        out = torch.zeros(2, dtype=torch.float64, device=self.device)
+        _lib.check(_lib.lib().wf_energy(
+            self._handle, _lib.ptr(self._E2), _lib.ptr(self._H2), _lib.ptr(self.H.array),
+            _lib.ptr(self.ieps.array), _lib.ptr(self.imu.array), _lib.ptr(self._metric2d_all),
+            _lib.ptr(out), _lib.stream_ptr()), "wf_energy")
+        uE, uH = out.tolist()
+        return (uE, uH) if parts else uE + uH
 
     def one_step(self):
         """One leapfrog step (solverFIT3D.py:482-503)."""

@@ -45,6 +45,7 @@ class WakeSolver:
         self.s = self.lambdas = self.WP = self.WP_3d = None
         self.n_transverse_cells = 1
         self.WPx = self.WPy = self.f = self.Z = self.Zx = self.Zy = self.lambdaf = None
+        self.loss_factor = None
         self.save = save
         self.folder = results_folder if results_folder.endswith("/") else results_folder + "/"
         if self.save:
@@ -74,9 +75,11 @@ class WakeSolver:
             self.calc_trans_WP()
             self.calc_long_Z()
             self.calc_trans_Z()
+            self.calc_loss_factor()
         elif compute_plane.lower() == "longitudinal":
             self.calc_long_WP()
             self.calc_long_Z()
+            self.calc_loss_factor()
         self.log("Calculation terminated in %ds" % (time.time() - t0))
 
     # ---------------------------------------------------------------- WP
@@ -236,6 +239,22 @@ class WakeSolver:
         self.fx = self.fy = f
         self._save_txt("Zx.txt", self.fx, self.Zx, "Zx [Ohm]")
         self._save_txt("Zy.txt", self.fy, self.Zy, "Zy [Ohm]")
+
+    def calc_loss_factor(self):
+        """Loss factor k = -integral WP(s) lambda(s) ds in V/pC (no reference
+        counterpart): a compensated FP64 sum over the s axis on the device,
+        ds = s[2] - s[1] as for the transverse integral."""
+        if self.lambdas is None or len(self.lambdas) != len(self.s):
+            self.calc_lambdas_analytic()
+        dev = torch.device("cuda")
+        wp = torch.as_tensor(np.ascontiguousarray(self.WP, dtype=np.float64), device=dev)
+        lam = torch.as_tensor(np.ascontiguousarray(self.lambdas, dtype=np.float64), device=dev)
+        out = torch.zeros(1, dtype=torch.float64, device=dev)
+        _lib.check(_lib.lib().wf_loss_factor(_lib.ptr(wp), _lib.ptr(lam), len(self.s),
+                                             float(self.s[2] - self.s[1]), _lib.ptr(out),
+                                             _lib.stream_ptr()), "wf_loss_factor")
+        self.loss_factor = float(out.item())
+        return self.loss_factor
 
     def _save_txt(self, name, a, b, label):
         if self.save:
