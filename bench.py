@@ -33,8 +33,8 @@ VALUES_PER_CELL = 18  # E r/w 3+3, H r/w 3+3, face coeffs 3, edge coeffs 3
 # for it; WRITE_SIZE matches the copy's bytes and is taken as is. Keyed by
 # (grid, dtype, path, variant, kz); other settings report null.
 MEASURED_TRAFFIC = {
-    ((256, 256, 512), "f32", "fused", 0, 0): {"fetch_B": 958_450_893, "write_B": 812_400_640},
-    ((256, 256, 512), "f64", "fused", 0, 0): {"fetch_B": 2_276_237_517, "write_B": 1_610_629_120},
+    ((256, 256, 512), "f32", "fused", 0, 0): {"fetch_B": 926_966_374, "write_B": 805_458_637},
+    ((256, 256, 512), "f64", "fused", 0, 0): {"fetch_B": 2_184_980_070, "write_B": 1_610_732_749},
 }
 
 

@@ -5,7 +5,8 @@
  * NULL for the default stream). Every entry point returns 0 on success and a
  * negative WF_E* code on failure; wf_last_error() gives the message of the
  * last failure on the calling thread. No entry point allocates device memory
- * except wf_create (small metric vectors); none synchronises the stream except
+ * except wf_create (small metric vectors and a 32 KB reduction workspace); none
+ * synchronises the stream except
  * where stated.
  *
  * Field layout (identical to the reference Field, wakis/field.py:19-24, 57-91):
@@ -107,6 +108,12 @@ typedef struct wf_step_args {
                              measured fastest for the dtype), 1 scalar kernel,
                              2, 3, 7 or 8 (FP32 only): x-vectorised kernel with
                              that many rows per workgroup                          */
+  int32_t zero_masked;    /* fused path: 1 = E is zero on every edge the PEC mask
+                             zeroes and H on every face the PMC mask zeroes, no
+                             source writes there, and (with `code`) metric2d has
+                             the masks folded in (iA_c = 0 on PMC-masked faces,
+                             itA_c = 0 on PEC-masked edges). The x-vectorised
+                             kernel then applies no masks; results are identical */
 } wf_step_args;
 
 int wf_version(void);

@@ -37,7 +37,7 @@ class WfStepArgs(ctypes.Structure):
         ("path", c_int32), ("compute_f64", c_int32),
         ("E2", c_void_p), ("H2", c_void_p), ("kz", c_int32), ("result_swapped", c_int32),
         ("code", c_void_p), ("mat", c_void_p), ("metric2d", c_void_p), ("nmat", c_int32),
-        ("variant", c_int32),
+        ("variant", c_int32), ("zero_masked", c_int32),
     ]
 
 

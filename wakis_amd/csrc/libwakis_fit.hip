@@ -222,10 +222,19 @@ int step_fused4(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStr
     const S *row = a->beam_table ? (const S *)a->beam_table + (a->beam_row0 + s) * g.Nz : nullptr;
     x.cap_sample = a->cap_sample0 + s;
     S *cap = (S *)a->capture;
-#define WF_G_LAUNCH3(I, JM, CO, LU, DE)                                                       \
-  hipLaunchKernelGGL((wf::k_fused4<S, JM, CO, LU, DE, TY>), dim3((unsigned)nb),               \
+#define WF_G_LAUNCH4(I, JM, CO, LU, DE, MK)                                                   \
+  hipLaunchKernelGGL((wf::k_fused4<S, JM, CO, LU, DE, TY, MK>), dim3((unsigned)nb),           \
                      dim3(64 * (TY + 1)), lds, st, g, m, Ebuf[cur], Hbuf[cur], Ebuf[1 - cur], \
                      Hbuf[1 - cur], (S *)a->J, cf, dt, x, row, cap)
+#define WF_G_LAUNCH3(I, JM, CO, LU, DE)                                                       \
+  do {                                                                                        \
+    if constexpr (LU) {                                                                       \
+      if (a->zero_masked) WF_G_LAUNCH4(I, JM, CO, LU, DE, false);                             \
+      else WF_G_LAUNCH4(I, JM, CO, LU, DE, true);                                             \
+    } else {                                                                                  \
+      WF_G_LAUNCH4(I, JM, CO, LU, DE, true);                                                  \
+    }                                                                                         \
+  } while (0)
 #define WF_G_LAUNCH2(I, JM, CO, LU)                                                          \
   if (g.m1desc) WF_G_LAUNCH3(I, JM, CO, LU, true);                                           \
   else WF_G_LAUNCH3(I, JM, CO, LU, false)
@@ -245,6 +254,7 @@ int step_fused4(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStr
 #undef WF_G_LAUNCH
 #undef WF_G_LAUNCH2
 #undef WF_G_LAUNCH3
+#undef WF_G_LAUNCH4
     WF_LAUNCH_CHECK();
     cur = 1 - cur;
   }

@@ -59,7 +59,12 @@ __device__ __forceinline__ void stv(S *__restrict__ p, I idx, const S (&v)[V]) {
   }
 }
 
-template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY>
+// MK = false: the caller guarantees that E is zero on every PEC-masked edge and
+// H on every PMC-masked face (and that no source writes there), and passes
+// coefficients with the row masks folded in. The staged values then need no
+// boundary masks: with zero mask words every mask test below folds to "true"
+// at compile time, which frees the registers the per-lane masks take.
+template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY, bool MK>
 __global__ void __launch_bounds__(64 * (TY + 1))
 k_fused4(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restrict__ Hin,
          S *__restrict__ Eout, S *__restrict__ Hout, S *__restrict__ J, Coefs<S> cf, S dt,
@@ -100,7 +105,7 @@ k_fused4(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restri
   const bool own = (w >= 1) && lact;
   const I N = (I)g.N, sz = (I)g.sz;
   const I col = lact ? (I)ic + (I)g.sy * (I)j : (I)0;
-  const int pz = g.pecz, hz = g.pmcz;
+  const int pz = MK ? g.pecz : 0, hz = MK ? g.pmcz : 0;
   const int jt = j0 + TY;
   const bool top = (w == 0) && (jt < g.Ny) && (ic < g.Nx);
   const I tcol = top ? (I)ic + (I)g.sy * (I)jt : (I)0;

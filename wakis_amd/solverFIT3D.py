@@ -163,6 +163,9 @@ class SolverFIT3D(RoutinesMixin):
         # energy(): keep E^n and H^(n-1/2) on the exact path too (one copy per step)
         self.keep_previous_fields = False
         self._prev_key = None
+        # fused kernel without in-kernel boundary masks when the masked values are zero
+        self.skip_masks = True
+        self._mz_key = self._mz_ok = None
         self.use_material_code = True  # fused path: uint8 code + coefficient table
         self._lut_key = None
         self._code = self._mat = self._metric2d = None
@@ -468,28 +471,64 @@ class SolverFIT3D(RoutinesMixin):
         rep = torch.empty(K, dtype=torch.int64, device=dev)
         rep.scatter_(0, code, torch.arange(N, device=dev))
         mat = torch.stack([col[rep] for col in cols], dim=1).contiguous()
-        iA, itA = self._iA2d, self._itA2d
-        metric2d = torch.cat([iA[0], iA[1], iA[2], itA[0], itA[1], itA[2]]).contiguous()
+        # 2-D metric tables with the row masks folded in (iA zero on PMC-masked
+        # faces, itA zero on PEC-masked edges): a mask of component c depends
+        # only on the two transverse indices, like the table
+        plane = {0: (slice(None), slice(None), 0), 1: (slice(None), 0, slice(None)),
+                 2: (0, slice(None), slice(None))}
+        iA, itA = [], []
+        for c in range(3):
+            ia, ita = self._iA2d[c], self._itA2d[c]
+            zero = torch.zeros((), dtype=ia.dtype, device=dev)
+            if self.pmc_zero:
+                m2 = face_mask_kji(c, self.pmc_zero, Nx, Ny, Nz, dev)[plane[c]].reshape(-1)
+                ia = torch.where(m2, ia, zero)
+            if self.pec_zero:
+                m2 = face_mask_kji(c, self.pec_zero, Nx, Ny, Nz, dev)[plane[c]].reshape(-1)
+                ita = torch.where(m2, ita, zero)
+            iA.append(ia)
+            itA.append(ita)
+        metric2d = torch.cat(iA + itA).contiguous()
         # exactness check against the folded arrays
         shp = {0: (Nz, Ny, 1), 1: (Nz, 1, Nx), 2: (1, Ny, Nx)}
         c3 = code.view(Nz, Ny, Nx)
         for c in range(3):
-            ia = iA[c].view(shp[c])
-            ita = itA[c].view(shp[c])
-            a_c = ia * mat[:, c][c3]
-            if self.pmc_zero:
-                a_c = torch.where(face_mask_kji(c, self.pmc_zero, Nx, Ny, Nz, dev), a_c,
-                                  torch.zeros((), dtype=a_c.dtype, device=dev))
-            b_c = mat[:, 3 + c][c3] * ita
-            if self.pec_zero:
-                b_c = torch.where(face_mask_kji(c, self.pec_zero, Nx, Ny, Nz, dev), b_c,
-                                  torch.zeros((), dtype=b_c.dtype, device=dev))
+            a_c = iA[c].view(shp[c]) * mat[:, c][c3]
+            b_c = mat[:, 3 + c][c3] * itA[c].view(shp[c])
             if not (torch.equal(a_c.reshape(-1), self._a[c * N:(c + 1) * N])
                     and torch.equal(b_c.reshape(-1), self._b[c * N:(c + 1) * N])):
                 return False
         self._code = code.to(torch.uint8).contiguous()
         self._mat, self._nmat, self._metric2d = mat, K, metric2d
         return True
+
+    def _masked_zero(self):
+        """True if E is zero on every PEC-masked edge, H on every PMC-masked
+        face, and the beam line is not on a masked edge.
+
+        Those values then stay zero (their rows have zero coefficients and no
+        source), so the fast kernel may skip the boundary masks.
+        """
+        key = (self.E.version, self.H.version, self.J.version, self._fold_count,
+               self._beam_line)
+        if key == self._mz_key:
+            return self._mz_ok
+        Nx, Ny, Nz, N, dev = self.Nx, self.Ny, self.Nz, self.N, self.device
+        ok = True
+        for c in range(3):
+            if self.pec_zero:
+                keep = face_mask_kji(c, self.pec_zero, Nx, Ny, Nz, dev)
+                Ec = self.E.array[c * N:(c + 1) * N].view(Nz, Ny, Nx)
+                ok = ok and not bool((Ec[~keep] != 0).any().item())
+                if c == 2 and self._beam_line is not None:
+                    ixs, iys = self._beam_line
+                    ok = ok and bool(keep[:, iys, ixs].all().item())
+            if self.pmc_zero:
+                keep = face_mask_kji(c, self.pmc_zero, Nx, Ny, Nz, dev)
+                Hc = self.H.array[c * N:(c + 1) * N].view(Nz, Ny, Nx)
+                ok = ok and not bool((Hc[~keep] != 0).any().item())
+        self._mz_key, self._mz_ok = key, ok
+        return ok
 
     def _wraps_dead(self):
         """True if every flat-index wrap term of M1/M2 contributes exactly zero.
@@ -598,6 +637,7 @@ class SolverFIT3D(RoutinesMixin):
             args.E2, args.H2 = _lib.ptr(self._E2), _lib.ptr(self._H2)
             args.kz = int(self.fused_kz)
             args.variant = int(self.fused_variant)
+            args.zero_masked = 1 if (self.skip_masks and self._masked_zero()) else 0
             if self.use_material_code and self._material_table():
                 args.code, args.mat, args.metric2d = (_lib.ptr(self._code), _lib.ptr(self._mat),
                                                       _lib.ptr(self._metric2d))
