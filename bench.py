@@ -33,8 +33,8 @@ VALUES_PER_CELL = 18  # E r/w 3+3, H r/w 3+3, face coeffs 3, edge coeffs 3
 # for it; WRITE_SIZE matches the copy's bytes and is taken as is. Keyed by
 # (grid, dtype, path, variant, kz); other settings report null.
 MEASURED_TRAFFIC = {
-    ((256, 256, 512), "f32", "fused", 0, 0): {"fetch_B": 926_966_374, "write_B": 805_458_637},
-    ((256, 256, 512), "f64", "fused", 0, 0): {"fetch_B": 2_184_980_070, "write_B": 1_610_732_749},
+    ((256, 256, 512), "f32", "fused", 0, 0): {"fetch_B": 940_352_717, "write_B": 805_468_058},
+    ((256, 256, 512), "f64", "fused", 0, 0): {"fetch_B": 2_013_099_622, "write_B": 1_610_758_963},
 }
 
 
@@ -77,7 +77,7 @@ def hbm_copy_gbs(dev, nbytes=1 << 30, reps=5):
     return 2.0 * nbytes * reps / (ms * 1e-3) / 1e9
 
 
-def time_gpu(Nx, Ny, Nz, dtype, steps, warmup, path, variant=0, kz=0):
+def time_gpu(Nx, Ny, Nz, dtype, steps, warmup, path, variant=0, kz=0, wake_check=False):
     import torch
 
     import wakis_amd as wa
@@ -125,9 +125,36 @@ def time_gpu(Nx, Ny, Nz, dtype, steps, warmup, path, variant=0, kz=0):
                gcell_s=N * steps / (ms * 1e-3) / 1e9, finite=finite,
                bytes_per_cell=VALUES_PER_CELL * np.dtype(dtype).itemsize)
     out["achieved_GBs"] = N * out["bytes_per_cell"] / (out["ms_per_step"] * 1e-3) / 1e9
+    if wake_check:
+        out["WP"] = wake_of_capture(s, cap, tot)
     del s, table, cap
     torch.cuda.empty_cache()
     return out
+
+
+WAKE_CHECK_LENGTH = 0.1  # m of s axis for the truncated-integral comparison
+
+
+def wake_of_capture(s, cap, nsamples):
+    """WP_z(s) on the beam axis from the 3x3 Ez capture of a timed run, with the
+    device WakeSolver kernels (FP64 line integral), over s in
+    [-ti*v, WAKE_CHECK_LENGTH): the integral truncated at the last captured
+    sample."""
+    import wakis_amd as wa
+    wake = wa.WakeSolver(q=1e-9, sigmaz=10e-3, beta=1.0, skip_cells=10, save=False,
+                         verbose=0)
+    wake.wakelength = WAKE_CHECK_LENGTH
+    ixt = int(np.abs(s.x).argmin())
+    iyt = int(np.abs(s.y).argmin())
+    t = np.arange(nsamples, dtype=np.float64) * float(s.dt)
+    wake._set_capture(cap, t, s.x[ixt - 1:ixt + 2], s.y[iyt - 1:iyt + 2], s.z, s.dx, s.dy,
+                      s.dz, s.dtype, nsamples)
+    wake.calc_long_WP_3d()
+    return wake.WP
+
+
+def _brief(r):
+    return {k: v for k, v in r.items() if k != "WP"}
 
 
 def time_cpu(Nx, Ny, Nz, dtype, steps, warmup):
@@ -149,6 +176,48 @@ def time_cpu(Nx, Ny, Nz, dtype, steps, warmup):
                 gcell_s=s.N * steps / el / 1e9)
 
 
+def batch_sweep(n=64, steps=2000, seed=0, dtype=np.float32):
+    """The batched sweep: n box cavities (wakis_amd.batch.random_box_cavities)
+    with about `steps` wakefield steps each, run one after another with the
+    single-grid kernel and then all together with wakesolve_batch; both timed
+    end to end (set-up, stepping, wake post-processing)."""
+    import torch
+
+    from wakis_amd import batch
+    specs = batch.random_box_cavities(seed, n=n)
+
+    def make():
+        sv = [batch.cavity_solver(sp, dtype=dtype) for sp in specs]
+        wl = [steps * float(s.dt) * s.wake.v - s.wake.ti * s.wake.v - (s.z.max() - s.z.min())
+              for s in sv]
+        return sv, wl
+
+    seq, wl = make()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s, w in zip(seq, wl):
+        s.wakesolve(wakelength=w)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    cells = sum(s.N * s.Nt for s in seq)
+    bat, wl = make()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    info = batch.wakesolve_batch(bat, wl)
+    torch.cuda.synchronize()
+    t_bat = time.perf_counter() - t0
+    same = all(np.array_equal(a.wake.WP_3d, b.wake.WP_3d) for a, b in zip(seq, bat))
+    return {"metric": "batched wakefield sweep throughput", "unit": "Gcell-updates/s",
+            "grids": n, "steps_per_grid": [int(min(s.Nt for s in seq)), int(max(s.Nt for s in seq))],
+            "cells_total": int(sum(s.N for s in seq)), "cell_updates": int(cells),
+            "sequential": round(cells / t_seq / 1e9, 3), "batched": round(cells / t_bat / 1e9, 3),
+            "speedup": round(t_seq / t_bat, 3),
+            "batched_step_loop": round(info["cell_updates"] * info["main_steps"] / info["steps"]
+                                       / (info["ms_main"] * 1e-3) / 1e9, 3),
+            "shape": info["shape"], "wp_bit_identical": bool(same), "dtype": np.dtype(dtype).name,
+            "data": "synthetic (random_box_cavities, seed %d)" % seed}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,26 +232,36 @@ def main():
     ap.add_argument("--variant", type=int, default=0,
                     help="fused kernel: 0 auto, 1 scalar, 2/3/7/8 rows per workgroup")
     ap.add_argument("--kz", type=int, default=0, help="z-planes per workgroup (0 auto)")
+    ap.add_argument("--no-wake-check", action="store_true",
+                    help="skip the FP32-vs-FP64 WP_z comparison (FP64 then runs steps/3)")
+    ap.add_argument("--batch", action="store_true",
+                    help="only the batched sweep of 64 box cavities (one JSON line)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU; runs on several GPUs are not "
               "supported (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.batch:
+        print(json.dumps(batch_sweep()), flush=True)
+        return
     Nx, Ny, Nz = (int(v) for v in args.grid.split(","))
     import torch
     copy_gbs = hbm_copy_gbs(torch.device("cuda", 0))
     print(f"[bench] hbm copy {copy_gbs:.1f} GB/s", file=sys.stderr, flush=True)
     print(f"[bench] fp32 {Nx}x{Ny}x{Nz} steps={args.steps} warmup={args.warmup}",
           file=sys.stderr, flush=True)
+    check = not args.no_fp64 and not args.no_wake_check
     r32 = time_gpu(Nx, Ny, Nz, np.float32, args.steps, args.warmup, args.path, args.variant,
-                   args.kz)
-    print(f"[bench] fp32 {r32}", file=sys.stderr, flush=True)
+                   args.kz, wake_check=check)
+    print(f"[bench] fp32 {_brief(r32)}", file=sys.stderr, flush=True)
     r64 = None
     if not args.no_fp64:
-        k64 = max(1, args.steps // 3)
+        # same number of steps as FP32 when the wake check runs (the FP64 run is the
+        # reference for the FP32 wake), else a third of them
+        k64 = args.steps if check else max(1, args.steps // 3)
         r64 = time_gpu(Nx, Ny, Nz, np.float64, k64, args.warmup, args.path, args.variant,
-                       args.kz)
-        print(f"[bench] fp64 {r64}", file=sys.stderr, flush=True)
+                       args.kz, wake_check=check)
+        print(f"[bench] fp64 {_brief(r64)}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cx, cy, cz = (int(v) for v in args.cpu_grid.split(","))
@@ -221,6 +300,16 @@ def main():
                         "traffic": measured_traffic((Nx, Ny, Nz), "f64", r64["path"],
                                                     args.variant, args.kz)}
         line["fp32_over_fp64"] = round(r32["gcell_s"] / r64["gcell_s"], 3)
+        if check:
+            n = min(len(r32["WP"]), len(r64["WP"]))  # s axes from dt32 / dt64
+            wp32, wp64 = r32["WP"][:n], r64["WP"][:n]
+            line["wake_check"] = {
+                "quantity": "WP_z(s) on axis, s in [-ti*v, %.2f m), integral truncated at the "
+                            "last of %d steps" % (WAKE_CHECK_LENGTH, args.warmup + args.steps),
+                "reference": "same run with FP64 fields (the FP64 kernel is bit-identical to "
+                             "the FP64 oracle)",
+                "rel_l2_fp32_vs_fp64": float(np.linalg.norm(wp32 - wp64) / np.linalg.norm(wp64)),
+                "n_s": int(len(wp64))}
     if cpu is not None:
         line["cpu_baseline"] = {
             "value": round(cpu["gcell_s"], 6), "unit": "Gcell-updates/s", "cores": 1,

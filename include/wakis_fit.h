@@ -193,6 +193,71 @@ int wf_energy(wf_handle *h, const void *E, const void *Hprev, const void *H, con
 int wf_loss_factor(const double *WP, const double *lam, int64_t n, double ds, double *out,
                    void *stream);
 
+/* Batches. No reference counterpart (the reference runs one grid per process);
+ * for parameter sweeps over many small grids. Each grid keeps its handle and
+ * a fused-path wf_step_args (path 1, material codes, zero_masked = 1, E2/H2;
+ * beam table and capture optional), all with the same dtype, conductivity, J
+ * mode and boundary order. args[i].nsteps is the number of steps grid i takes
+ * part in; beam_row0 and cap_sample0 refer to step 0. One kernel launch per
+ * step covers the tiles of every grid (3 rows per workgroup, cells_per_lane 1
+ * or the 16-byte width, 0 = 16-byte width; kz z-planes per workgroup, 0 = 32).
+ * Even steps read E/H and write E2/H2, odd steps the reverse: after steps
+ * [0, S) grid i's fields are in E2/H2 iff min(S, nsteps_i) is odd. Results are
+ * bit-identical to wf_step on each grid. wf_batch_create copies the
+ * descriptors to the device (the only allocation); the batch must not outlive
+ * the handles and buffers it names. */
+typedef struct wf_batch wf_batch;
+int wf_batch_create(wf_handle *const *handles, const wf_step_args *args, int32_t n,
+                    int32_t cells_per_lane, int32_t kz, wf_batch **out);
+/* Steps [step0, step0 + nsteps) of every grid. use_graph: replay a captured
+ * HIP graph of 16 steps (built on the first such call) instead of launching
+ * each step; same results. */
+int wf_batch_run(wf_batch *b, int32_t step0, int32_t nsteps, int32_t use_graph, void *stream);
+int wf_batch_destroy(wf_batch *b);
+
+/* Reverse mode. No reference counterpart; for gradients of wake figures of
+ * merit with respect to per-cell eps_r and sigma. */
+typedef struct wf_adjoint_args {
+  double *lE, *lH, *lJ;   /* device FP64 3N adjoints of E, H, J: in at step n+1, out at n */
+  const void *E1, *H1;    /* forward E^{n+1}, H^{n+1} (field dtype)                       */
+  const void *J0;         /* forward J^n as step n used it (field dtype)                   */
+  const void *a, *b;      /* the step's folded coefficients (wf_step_args.a / .b)          */
+  const void *ieps, *sigma;
+  const void *metric2d;   /* 2-D metric tables without masks (wf_step_args.metric2d layout) */
+  double dt;
+  int32_t conductive;     /* 1: J = sigma*E is a state (lJ used)                           */
+  int32_t ixs, iys;       /* beam line (J there is a source); -1 if none                   */
+  int32_t r0[3], r1[3];   /* gradient region [r0, r1) in cells                             */
+  double *g_ieps;         /* device FP64 [3][region cells] (x fastest), accumulated, or NULL */
+  double *g_sigma;        /* same for sigma (conductive)                                   */
+  const double *gcap;     /* device FP64 capture gradient [nlines][cap_samples][Nz], or NULL */
+  int32_t cap_ix0, cap_iy0, cap_nx, cap_ny;
+  int64_t cap_samples;
+  int64_t cap_sample;     /* sample n = E^n, added to lE after the step                   */
+} wf_adjoint_args;
+
+/* Reverse of step n of SolverFIT3D._one_step (solverFIT3D.py:482-503): maps
+ * the adjoints of (E, H, J)^{n+1} to those of (E, H, J)^n with the transposed
+ * stencils (M1^T, M2^T, same masks and flat-index terms as the forward), adds
+ * the capture gradient of sample n, and accumulates d/d ieps and d/d sigma of
+ * the cells in the region. Exact-path semantics (beam line J is a source). */
+int wf_adjoint_step(wf_handle *h, const wf_adjoint_args *a, void *stream);
+
+/* Transpose of wf_wake_wp's line integral without the line-to-line carry:
+ * gcap[l][m][k0+k] += sum over s indices n with it(n,k) == m of gI[l][n]*dz[k]
+ * (gI: gradient w.r.t. the raw line sums; the carry and the 1/qe12 scaling
+ * are the caller's). Arguments as wf_wake_wp. */
+int wf_wake_wp_adjoint(const double *gI, int32_t nlines, int64_t cap_samples, int32_t Nz,
+                       int32_t k0, int32_t nz, const double *z, const double *dz, const double *s,
+                       int64_t ns, double v, double zmin_v, double t0, double ti, double dt,
+                       int64_t nt, double *gcap, void *stream);
+
+/* Transpose of wf_dft for one signal: out[n] = Re sum_f w[f] exp(-2 pi i f n/N)
+ * for n < len, w given as [nbins][2] (re, im); exact phase reduction,
+ * compensated FP64 sums. */
+int wf_idft_real(const double *w, int64_t nbins, int64_t N, int64_t len, double *out,
+                 void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,7 +6,8 @@
 * ``cavity_pipe``: PEC cubic cavity 50 mm with a round pipe r=5 mm along z in a
   60x60x160 mm domain (the 48x48x128 and 256x256x512 throughput workloads),
   PEC on x/y, ABC on z.
-* ``resonator``: the 30^3 PEC vacuum box of test_002_analytic_resonator.py:90-126.
+* ``pillbox``: cylindrical cavity with a beam pipe and a conductive shell wall in a
+  PEC background (the lossy pillbox + PML workload).
 """
 
 import json
@@ -59,3 +60,29 @@ def cavity_pipe_solver(Nx, Ny, Nz, dtype=np.float64):
     x, y, z, mask = cavity_pipe_geometry(Nx, Ny, Nz)
     return fo.OracleSolver(x, y, z, ["pec", "pec", "abc"], ["pec", "pec", "abc"],
                            bg="pec", solids=[(mask, "vacuum")], dtype=dtype)
+
+
+def pillbox_geometry(Nx, Ny, Nz, r_cav=40e-3, half_len=30e-3, r_pipe=10e-3, wall_cells=3,
+                     half_x=45e-3, half_z=90e-3):
+    """Lossy pillbox: vacuum cylinder (radius r_cav, |z| < half_len) plus a beam
+    pipe (radius r_pipe, all z); a shell of ``wall_cells`` cells around the vacuum
+    is the conductive wall, the rest is PEC background. Returns (x, y, z,
+    vacuum_mask, wall_mask)."""
+    x, y, z = fo.uniform_axes(-half_x, half_x, -half_x, half_x, -half_z, half_z, Nx, Ny, Nz)
+
+    def inside(X, Y, Z):
+        R2 = X ** 2 + Y ** 2
+        return ((R2 < r_cav ** 2) & (np.abs(Z) < half_len)) | (R2 < r_pipe ** 2)
+
+    vac = fo.corner_any_inside(x, y, z, inside)
+    grown = vac.copy()
+    for _ in range(wall_cells):  # 6-neighbour dilation, one cell per pass
+        g = grown.copy()
+        g[1:] |= grown[:-1]
+        g[:-1] |= grown[1:]
+        g[:, 1:] |= grown[:, :-1]
+        g[:, :-1] |= grown[:, 1:]
+        g[:, :, 1:] |= grown[:, :, :-1]
+        g[:, :, :-1] |= grown[:, :, 1:]
+        grown = g
+    return x, y, z, vac, grown & ~vac

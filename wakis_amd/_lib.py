@@ -41,6 +41,19 @@ class WfStepArgs(ctypes.Structure):
     ]
 
 
+class WfAdjointArgs(ctypes.Structure):
+    _fields_ = [
+        ("lE", c_void_p), ("lH", c_void_p), ("lJ", c_void_p),
+        ("E1", c_void_p), ("H1", c_void_p), ("J0", c_void_p),
+        ("a", c_void_p), ("b", c_void_p), ("ieps", c_void_p), ("sigma", c_void_p),
+        ("metric2d", c_void_p), ("dt", c_double), ("conductive", c_int32),
+        ("ixs", c_int32), ("iys", c_int32), ("r0", c_int32 * 3), ("r1", c_int32 * 3),
+        ("g_ieps", c_void_p), ("g_sigma", c_void_p), ("gcap", c_void_p),
+        ("cap_ix0", c_int32), ("cap_iy0", c_int32), ("cap_nx", c_int32), ("cap_ny", c_int32),
+        ("cap_samples", c_int64), ("cap_sample", c_int64),
+    ]
+
+
 EXPORTS = {
     "wf_version": (c_int32, []),
     "wf_last_error": (ctypes.c_char_p, []),
@@ -61,6 +74,15 @@ EXPORTS = {
     "wf_energy": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                             c_void_p, c_void_p, c_void_p]),
     "wf_loss_factor": (c_int32, [c_void_p, c_void_p, c_int64, c_double, c_void_p, c_void_p]),
+    "wf_batch_create": (c_int32, [POINTER(c_void_p), POINTER(WfStepArgs), c_int32, c_int32,
+                                  c_int32, POINTER(c_void_p)]),
+    "wf_batch_run": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p]),
+    "wf_batch_destroy": (c_int32, [c_void_p]),
+    "wf_adjoint_step": (c_int32, [c_void_p, POINTER(WfAdjointArgs), c_void_p]),
+    "wf_wake_wp_adjoint": (c_int32, [c_void_p, c_int32, c_int64, c_int32, c_int32, c_int32,
+                                     c_void_p, c_void_p, c_void_p, c_int64, c_double, c_double,
+                                     c_double, c_double, c_double, c_int64, c_void_p, c_void_p]),
+    "wf_idft_real": (c_int32, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p]),
 }
 
 _lib = None

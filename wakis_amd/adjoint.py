@@ -1,0 +1,236 @@
+"""Gradients of wake figures of merit with respect to per-cell eps_r and sigma.
+
+No reference counterpart. The figure of merit of a wakefield run is
+
+    L = w_loss * k_loss + w_band * sum_{f < f_band} |Z_z(f)|^2 * df
+
+with k_loss = -sum_n WP(s_n) lambda(s_n) ds (WakeSolver.calc_loss_factor) and
+Z_z the longitudinal impedance on the kept DFT bins (calc_long_Z). The
+parameters are eps_r and sigma of every cell of a block of cells (all three
+components of a cell share them, as a material fill does).
+
+The gradient is the discrete adjoint of exactly what the forward computes:
+the wake post-processing is reversed with wf_idft_real (impedance bins) and
+wf_wake_wp_adjoint (line integrals, with the reference's line-to-line carry),
+and the time stepping with wf_adjoint_step (transposed stencils), going back
+over the run in segments: sqrt(Nt)-spaced checkpoints of (E, H, J) are kept
+during the forward pass and each segment is recomputed from its checkpoint
+before it is reversed. ``wake_loss`` wraps it as a torch.autograd.Function.
+
+The time step must not depend on sigma: build the solver with
+``tau_clamp=False`` (and a dt for which dt*sigma/(eps_r*eps0) stays well below
+2 over the parameter range).
+"""
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+from scipy.constants import epsilon_0 as eps_0
+
+from . import _lib
+
+
+class WakeAdjoint:
+    def __init__(self, solver, region, wakelength, f_band=3e9, w_loss=1.0, w_band=1.0,
+                 segment=None):
+        """``region`` = ((i0, i1), (j0, j1), (k0, k1)) cells, half-open."""
+        s = solver
+        if s.wake is None:
+            raise ValueError("WakeAdjoint: the solver needs a WakeSolver")
+        if not s.use_conductivity:
+            raise ValueError("WakeAdjoint: sigma is a parameter: build the solver with a "
+                             "conductive material or background")
+        (self.i0, self.i1), (self.j0, self.j1), (self.k0, self.k1) = region
+        if not (0 <= self.i0 < self.i1 <= s.Nx and 0 <= self.j0 < self.j1 <= s.Ny
+                and 0 <= self.k0 < self.k1 <= s.Nz):
+            raise IndexError("WakeAdjoint: region outside the grid")
+        self.s = s
+        self.shape = (self.i1 - self.i0, self.j1 - self.j0, self.k1 - self.k0)
+        self.wakelength = wakelength
+        self.f_band, self.w_loss, self.w_band = f_band, w_loss, w_band
+        self.segment = segment
+        self._fwd = None
+
+    # ------------------------------------------------------------- forward
+    def _set_params(self, eps_r, sigma):
+        s = self.s
+        er = np.asarray(eps_r, dtype=np.float64).reshape(self.shape)
+        sg = np.asarray(sigma, dtype=np.float64).reshape(self.shape)
+        ie = (1.0 / (er * eps_0)).astype(s.dtype)
+        sl = (slice(self.i0, self.i1), slice(self.j0, self.j1), slice(self.k0, self.k1))
+        for c in "xyz":
+            s.ieps[sl + (c,)] = ie
+            s.sigma[sl + (c,)] = sg.astype(s.dtype)
+        s.update_tensors()
+        s.reset_fields()
+        s.step_0 = True
+
+    def forward(self, eps_r, sigma):
+        """Run the wakefield simulation with these parameters; return L."""
+        s = self.s
+        self._set_params(eps_r, sigma)
+        ctx = s._wake_setup(self.wakelength, "both")
+        Nt = ctx["Nt"]
+        K = self.segment or max(1, int(math.ceil(math.sqrt(max(Nt, 1)))))
+        ckpt = []
+        for n0 in range(0, Nt, K):
+            ckpt.append((n0, s.E.array.clone(), s.H.array.clone(), s.J.array.clone()))
+            k = min(K, Nt - n0)
+            s._run(s._wake_args(ctx, n0, k))
+        torch.cuda.synchronize(s.device)
+        s._wake_finish(ctx, "both")
+        w = s.wake
+        self._fwd = dict(ctx=ctx, ckpt=ckpt, K=K, Nt=Nt)
+        return self._loss_value()
+
+    def _band(self):
+        w = self.s.wake
+        return np.flatnonzero(w.f < self.f_band)
+
+    def _loss_value(self):
+        w = self.s.wake
+        k = w.loss_factor
+        band = self._band()
+        df = w.f[1] - w.f[0] if len(w.f) > 1 else 0.0
+        zb = float(np.sum(np.abs(w.Z[band]) ** 2) * df)
+        self.k_loss, self.band_power = k, zb
+        return self.w_loss * k + self.w_band * zb
+
+    # ------------------------------------------------------------ backward
+    def _grad_wp(self):
+        """dL/dWP on the centre line (host arrays of len(s))."""
+        s, w = self.s, self.s.wake
+        dev = s.device
+        ds = w.s[2] - w.s[1]
+        g = -self.w_loss * w.lambdas * ds
+        band = self._band()
+        if self.w_band != 0 and len(band):
+            dsm = np.mean(w.s[1:] - w.s[:-1])
+            N = int((w.v / dsm) // w.fmax * 1001)
+            df = w.f[1] - w.f[0]
+            # L_band = sum |Z_f|^2 df, Z_f = -1e12 dsm DFT(WP)_f / lambdaf_f
+            coef = -2.0 * df * 1e12 * dsm * np.conj(w.Z[band]) / w.lambdaf[band]
+            nb = int(band.max()) + 1
+            wv = np.zeros((nb, 2))
+            wv[band, 0] = coef.real
+            wv[band, 1] = coef.imag
+            wt = torch.as_tensor(wv, device=dev)
+            out = torch.zeros(len(w.s), dtype=torch.float64, device=dev)
+            _lib.check(_lib.lib().wf_idft_real(_lib.ptr(wt), nb, N, len(w.s), _lib.ptr(out),
+                                               _lib.stream_ptr()), "wf_idft_real")
+            g = g + self.w_band * out.cpu().numpy()
+        return g
+
+    def _grad_capture(self, gWP):
+        """dL/d(capture) through the line integrals with the line-to-line carry."""
+        s, w = self.s, self.s.wake
+        cap = w._cap
+        nl = cap.shape[0]
+        Q = w.q * 1e12
+        centre = nl // 2
+        gI = np.zeros((nl, len(w.s)))
+        for l in range(centre + 1):  # WP_3d[l] = (WP_3d[l-1] + I_l) / Q, loss on the centre
+            gI[l] = gWP / Q ** (centre - l + 1)
+        s_ax, dt, zmin = w._s_axis()
+        zz = slice(w.skip_cells, -w.skip_cells) if w.skip_cells != 0 else np.s_[:]
+        z = np.ascontiguousarray(w.zf[zz], dtype=np.float64)
+        nz = len(z)
+        k0 = w.skip_cells if w.skip_cells != 0 else 0
+        nt = min(len(w.t), w._cap_samples)
+        dev = cap.device
+        gcap = torch.zeros(cap.shape, dtype=torch.float64, device=dev)
+        t = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=dev)
+        gIt, zt, dzt, st_ = t(gI), t(z), t(w.dz[:max(nz, 1)]), t(s_ax)
+        _lib.check(_lib.lib().wf_wake_wp_adjoint(
+            _lib.ptr(gIt), nl, cap.shape[1], cap.shape[2], k0, nz, _lib.ptr(zt), _lib.ptr(dzt),
+            _lib.ptr(st_), len(s_ax), float(w.v), float(zmin / w.v), float(w.t[0]),
+            float(w.ti), float(dt), nt, _lib.ptr(gcap), _lib.stream_ptr()), "wf_wake_wp_adjoint")
+        return gcap
+
+    def backward(self):
+        """(dL/d eps_r, dL/d sigma) of the last forward, as FP64 torch tensors."""
+        if self._fwd is None:
+            raise RuntimeError("WakeAdjoint.backward: run forward first")
+        s = self.s
+        f = self._fwd
+        ctx, K, Nt = f["ctx"], f["K"], f["Nt"]
+        dev = s.device
+        gcap = self._grad_capture(self._grad_wp())
+        n3 = 3 * s.N
+        lE = torch.zeros(n3, dtype=torch.float64, device=dev)
+        lH = torch.zeros_like(lE)
+        lJ = torch.zeros_like(lE)
+        nreg = int(np.prod(self.shape))
+        g_ieps = torch.zeros(3 * nreg, dtype=torch.float64, device=dev)
+        g_sig = torch.zeros_like(g_ieps)
+        ix0, iy0, nx, ny = ctx["win"]
+        beam = ctx["beam"]
+        ixs, iys = int(beam.ixs), int(beam.iys)
+        table = ctx["table"]
+        a = _lib.WfAdjointArgs()
+        a.lE, a.lH, a.lJ = _lib.ptr(lE), _lib.ptr(lH), _lib.ptr(lJ)
+        a.dt = float(s.dt)
+        a.conductive = 1
+        a.ixs, a.iys = ixs, iys
+        a.r0[:] = (self.i0, self.j0, self.k0)
+        a.r1[:] = (self.i1, self.j1, self.k1)
+        a.g_ieps, a.g_sigma = _lib.ptr(g_ieps), _lib.ptr(g_sig)
+        a.gcap = _lib.ptr(gcap)
+        a.cap_ix0, a.cap_iy0, a.cap_nx, a.cap_ny = ix0, iy0, nx, ny
+        a.cap_samples = gcap.shape[1]
+        lib = _lib.lib()
+        N = s.N
+        line = slice(2 * N + ixs + s.Nx * iys, 3 * N, s.Nx * s.Ny)  # J_z on the beam line
+        for (n0, E0, H0, J0) in reversed(f["ckpt"]):
+            k = min(K, Nt - n0)
+            s.E.array.copy_(E0)
+            s.H.array.copy_(H0)
+            s.J.array.copy_(J0)
+            states = [(E0, H0, J0)]
+            for n in range(n0, n0 + k):
+                s._run(s._wake_args(ctx, n, 1))
+                states.append((s.E.array.clone(), s.H.array.clone(), s.J.array.clone()))
+            # the coefficients stay those of the forward (the same solver)
+            a.a, a.b = _lib.ptr(s._a), _lib.ptr(s._b)
+            a.ieps, a.sigma = _lib.ptr(s.ieps.array), _lib.ptr(s.sigma.array)
+            a.metric2d = _lib.ptr(s._metric2d_all)
+            for n in range(n0 + k - 1, n0 - 1, -1):
+                E1, H1, _ = states[n - n0 + 1]
+                Jn = states[n - n0][2].clone()
+                Jn[line] = table[n]  # the J the step used on the beam line
+                a.E1, a.H1, a.J0 = _lib.ptr(E1), _lib.ptr(H1), _lib.ptr(Jn)
+                a.cap_sample = n
+                _lib.check(lib.wf_adjoint_step(s._handle, ctypes.byref(a), _lib.stream_ptr()),
+                           "wf_adjoint_step")
+            del states
+        torch.cuda.synchronize(dev)
+        sh = self.shape
+        # per cell (x fastest): sum the three components; ieps = 1/(eps_r eps0)
+        gi = g_ieps.view(3, sh[2], sh[1], sh[0]).sum(0).permute(2, 1, 0)
+        gs = g_sig.view(3, sh[2], sh[1], sh[0]).sum(0).permute(2, 1, 0)
+        ie = s.ieps.to_matrix("x")[self.i0:self.i1, self.j0:self.j1,
+                                   self.k0:self.k1].to(torch.float64)
+        # d ieps / d eps_r = -eps0 ieps^2
+        g_eps = gi * (-eps_0 * ie * ie)
+        return g_eps.contiguous(), gs.contiguous()
+
+
+class _WakeLossFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, eps_r, sigma, adj):
+        L = adj.forward(eps_r.detach().cpu().numpy(), sigma.detach().cpu().numpy())
+        ctx.adj = adj
+        ctx.dev = eps_r.device
+        return torch.tensor(L, dtype=torch.float64, device=eps_r.device)
+
+    @staticmethod
+    def backward(ctx, gL):
+        ge, gs = ctx.adj.backward()
+        return (gL * ge).to(ctx.dev), (gL * gs).to(ctx.dev), None
+
+
+def wake_loss(eps_r, sigma, adj):
+    """L(eps_r, sigma) as a differentiable torch scalar (see WakeAdjoint)."""
+    return _WakeLossFn.apply(eps_r, sigma, adj)

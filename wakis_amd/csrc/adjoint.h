@@ -1,0 +1,206 @@
+// Discrete adjoint of the leapfrog step (reverse mode, FP64 adjoint fields).
+//
+// Forward step n (SolverFIT3D._one_step, solverFIT3D.py:482-503):
+//   H' = H - dt M1 E,   E' = E + dt (M2 H' - ieps J),   J' = sigma E'  (conductive)
+// with M1 = Dpmc iDa iDmu C Dpec Ds (row coefficient a = fl(iA imu) with the PMC
+// row mask) and M2 = iDeps itDa Dpec C^T Dpmc tDs (row coefficient b = fl(ieps
+// itA) with the PEC row mask). On the beam line J is a source, not a state.
+// Reverse step n, from the adjoints (lE, lH, lJ) of step n+1:
+//   z   = lE + sigma lJ                         (J' = sigma E')
+//   lH <- lH + dt M2^T z,   M2^T z = pmc . tL . C (b . z)
+//   lE <- z - dt M1^T lH,   M1^T y = pec . L . C^T (a . y)
+//   lJ <- -dt (1 - beam line) ieps . z          (conductive)
+// and the parameter gradients of that step, with E', H' = E^{n+1}, H^{n+1}
+// and J = J^n:
+//   d/d ieps_q : dt z_q (K2 H'_q - J_q),  K2 = itDa Dpec C^T Dpmc tDs (M2 without ieps)
+//   d/d sigma_q: lJ'_q E'_q
+// The C and C^T rows (with the flat-index wrap and truncation of the
+// reference operator) are h_terms / e_terms of the forward path.
+#pragma once
+#include "fit_common.h"
+#include "step_general.h"
+
+namespace wf {
+
+struct AdjAux {
+  int conductive, ixs, iys;
+  int r0[3], r1[3];  // gradient region (cells), half-open
+  int cap_ix0, cap_iy0, cap_nx, cap_ny;
+  int64_t cap_samples, cap_sample;
+};
+
+template <typename S>
+__device__ __forceinline__ double adj_z(const double *lE, const double *lJ, const S *sigma,
+                                        int64_t q, int cond) {
+  return cond ? lE[q] + (double)sigma[q] * lJ[q] : lE[q];
+}
+
+// parameter gradients of one step over the region (reads lE, lJ of step n+1)
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_adj_grad(GridDev g, MetricDev<S> m, const double *__restrict__ lE,
+           const double *__restrict__ lJ, const S *__restrict__ E1, const S *__restrict__ H1,
+           const S *__restrict__ J0, const S *__restrict__ sigma, const S *__restrict__ metric2d,
+           double dt, AdjAux x, double *__restrict__ g_ieps, double *__restrict__ g_sigma) {
+  const int rx = x.r1[0] - x.r0[0], ry = x.r1[1] - x.r0[1], rz = x.r1[2] - x.r0[2];
+  const int64_t nreg = (int64_t)rx * ry * rz;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nreg) return;
+  Cell c0;
+  c0.i = x.r0[0] + (int)(t % rx);
+  c0.j = x.r0[1] + (int)((t / rx) % ry);
+  c0.k = x.r0[2] + (int)(t / ((int64_t)rx * ry));
+  const int64_t n = flat(c0, g);
+  const int64_t yz = (int64_t)g.Ny * g.Nz, xz = (int64_t)g.Nx * g.Nz, xy = (int64_t)g.Nx * g.Ny;
+  const S *itA[3] = {metric2d + yz + xz + xy, metric2d + 2 * yz + xz + xy,
+                     metric2d + 2 * yz + 2 * xz + xy};
+  const int64_t t2[3] = {(int64_t)c0.j + (int64_t)g.Ny * c0.k,
+                         (int64_t)c0.i + (int64_t)g.Nx * c0.k,
+                         (int64_t)c0.i + (int64_t)g.Nx * c0.j};
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp) {
+    const int64_t q = comp * g.N + n;
+    double k2 = 0.0;
+    if (face_mask(g.pecz, comp, c0.i, c0.j, c0.k, g.Nx, g.Ny, g.Nz)) {
+      Term tt[4];
+      e_terms(comp, c0, g, tt);
+      double acc = 0.0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (!tt[u].ok) continue;
+        if (!face_mask(g.pmcz, tt[u].comp, tt[u].c.i, tt[u].c.j, tt[u].c.k, g.Nx, g.Ny, g.Nz))
+          continue;
+        const double w = (double)len_of(m.tLx, m.tLy, m.tLz, tt[u].comp, tt[u].c);
+        const double p = w * (double)H1[tt[u].comp * g.N + flat(tt[u].c, g)];
+        acc = tt[u].sign > 0 ? acc + p : acc - p;
+      }
+      k2 = (double)itA[comp][t2[comp]] * acc;
+    }
+    const double z = adj_z(lE, lJ, sigma, q, x.conductive);
+    g_ieps[comp * nreg + t] += dt * z * (k2 - (double)J0[q]);
+    if (x.conductive) g_sigma[comp * nreg + t] += lJ[q] * (double)E1[q];
+  }
+}
+
+// lH <- lH + dt M2^T (lE + sigma lJ)
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_adj_h(GridDev g, MetricDev<S> m, double *__restrict__ lH, const double *__restrict__ lE,
+        const double *__restrict__ lJ, const S *__restrict__ b, const S *__restrict__ sigma,
+        double dt, int cond) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.N) return;
+  const Cell c0 = cell_of(n, g);
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp) {
+    const int64_t r = comp * g.N + n;
+    if (!face_mask(g.pmcz, comp, c0.i, c0.j, c0.k, g.Nx, g.Ny, g.Nz)) continue;
+    Term tt[4];
+    h_terms(comp, c0, g, tt);
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (!tt[u].ok) continue;
+      const int64_t q = tt[u].comp * g.N + flat(tt[u].c, g);
+      const double p = (double)b[q] * adj_z(lE, lJ, sigma, q, cond);
+      acc = tt[u].sign > 0 ? acc + p : acc - p;
+    }
+    const double tl = (double)len_of(m.tLx, m.tLy, m.tLz, comp, c0);
+    lH[r] = lH[r] + dt * (tl * acc);
+  }
+}
+
+// lE <- z - dt M1^T lH (+ capture gradient of sample n);  lJ <- -dt (1-B) ieps z
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_adj_e(GridDev g, MetricDev<S> m, double *__restrict__ lE, double *__restrict__ lJ,
+        const double *__restrict__ lH, const S *__restrict__ a, const S *__restrict__ ieps,
+        const S *__restrict__ sigma, double dt, AdjAux x, const double *__restrict__ gcap) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.N) return;
+  const Cell c0 = cell_of(n, g);
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp) {
+    const int64_t q = comp * g.N + n;
+    const double z = adj_z(lE, lJ, sigma, q, x.conductive);
+    double m1t = 0.0;
+    if (face_mask(g.pecz, comp, c0.i, c0.j, c0.k, g.Nx, g.Ny, g.Nz)) {
+      Term tt[4];
+      e_terms(comp, c0, g, tt);
+      double acc = 0.0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (!tt[u].ok) continue;
+        const int64_t r = tt[u].comp * g.N + flat(tt[u].c, g);
+        const double p = (double)a[r] * lH[r];
+        acc = tt[u].sign > 0 ? acc + p : acc - p;
+      }
+      m1t = (double)len_of(m.Lx, m.Ly, m.Lz, comp, c0) * acc;
+    }
+    double le = z - dt * m1t;
+    if (gcap && comp == 2 && x.cap_sample < x.cap_samples) {
+      const int li = c0.i - x.cap_ix0, lj = c0.j - x.cap_iy0;
+      if (li >= 0 && li < x.cap_nx && lj >= 0 && lj < x.cap_ny) {
+        const int64_t l = (int64_t)li * x.cap_ny + lj;
+        le += gcap[(l * x.cap_samples + x.cap_sample) * g.Nz + c0.k];
+      }
+    }
+    lE[q] = le;
+    if (x.conductive) {
+      const bool online = (comp == 2) && (c0.i == x.ixs) && (c0.j == x.iys);
+      lJ[q] = online ? 0.0 : -dt * (double)ieps[q] * z;
+    }
+  }
+}
+
+// Capture gradient of the line integrals: gcap[l][m][k0+k] = sum over the s
+// indices n with it(n,k) == m of gI[l][n] * dz[k] (it as in k_wake_wp, which
+// is non-decreasing in n). One thread per (line, k): runs of equal m are
+// summed in order, so the result is deterministic.
+__global__ void k_wake_wp_adjoint(const double *__restrict__ gI, int nlines, int64_t cap_samples,
+                                  int Nz, int k0, int nz, const double *__restrict__ z,
+                                  const double *__restrict__ dz, const double *__restrict__ s,
+                                  int64_t ns, double v, double zmin_v, double t0, double ti,
+                                  double dt, int64_t nt, double *__restrict__ gcap) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nlines * nz) return;
+  const int l = (int)(t / nz), k = (int)(t - (int64_t)l * nz);
+  const double zk = z[k], dzk = dz[k];
+  int64_t cur = -1;
+  double acc = 0.0;
+  for (int64_t n = 0; n < ns; ++n) {
+    const double ts = (zk + s[n]) / v - zmin_v - t0 + ti;
+    int64_t it = (int64_t)(ts / dt);
+    if (it >= nt) break;
+    if (it < 0) it += nt;  // python negative index (not reached for t0 = 0)
+    if (it != cur) {
+      if (cur >= 0) gcap[((int64_t)l * cap_samples + cur) * Nz + k0 + k] += acc;
+      cur = it;
+      acc = 0.0;
+    }
+    acc += gI[(int64_t)l * ns + n] * dzk;
+  }
+  if (cur >= 0) gcap[((int64_t)l * cap_samples + cur) * Nz + k0 + k] += acc;
+}
+
+// out[n] = Re sum_f (w_re[f] + i w_im[f]) exp(-2 pi i f n / N), n < len: the
+// transpose of the kept-bin DFT, phase reduced exactly as (f*n mod N).
+__global__ void k_idft_real(const double *__restrict__ w, int64_t nbins, int64_t N, int64_t len,
+                            double *__restrict__ out) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= len) return;
+  double s = 0.0, c = 0.0;
+  for (int64_t f = 0; f < nbins; ++f) {
+    const int64_t ph = (f * n) % N;  // exact phase reduction (f*n < 2^63 for kept bins)
+    double sn, cs;
+    sincospi(2.0 * (double)ph / (double)N, &sn, &cs);
+    const double x = w[2 * f] * cs + w[2 * f + 1] * sn;  // Re((wr + i wi)(cs - i sn))
+    const double tt = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - tt) + x;
+    else c += (x - tt) + s;
+    s = tt;
+  }
+  out[n] = s + c;
+}
+
+}  // namespace wf

@@ -14,6 +14,7 @@
 #include "step_fused4.h"
 #include "wake.h"
 #include "reduce.h"
+#include "adjoint.h"
 
 namespace {
 
@@ -194,9 +195,9 @@ int step_fused4(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStr
   x.nty = (g.Ny + TY - 1) / TY;
   if (a->kz > 0) {
     x.KZ = a->kz;
-  } else {  // about 256 workgroups: long z marches, one wave of tiles over the CUs
+  } else {  // about 512 workgroups (two per CU): long z marches, measured best
     const int64_t nxy = (int64_t)x.ntx * x.nty;
-    int64_t kz = (g.Nz * nxy + 255) / 256;
+    int64_t kz = (g.Nz * nxy + 511) / 512;
     x.KZ = (int)(kz < 8 ? 8 : (kz > 64 ? 64 : kz));
   }
   x.ntz = (g.Nz + x.KZ - 1) / x.KZ;
@@ -270,7 +271,7 @@ int step_fused(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStre
   const bool idx32v = (uint64_t)3 * (uint64_t)g.N * sizeof(S) < (1ull << 32);
   if (g.Nx % (int)(16 / sizeof(S)) == 0 && idx32v && a->variant != 1) {
     // variant = rows per workgroup (0: the measured best for the type)
-    const int rows = a->variant ? a->variant : (sizeof(S) == 4 ? 8 : 2);
+    const int rows = a->variant ? a->variant : (sizeof(S) == 4 ? 8 : 3);
     switch (rows) {
       case 2: return step_fused4<S, 2>(h, a, m, st);
       case 3: return step_fused4<S, 3>(h, a, m, st);
@@ -345,54 +346,65 @@ int step_fused(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStre
   return WF_OK;
 }
 
+// Argument checks shared by wf_step and wf_batch_create.
+int check_step_args(const wf_handle *h, const wf_step_args *a, const char *who) {
+  if (!a->E || !a->H || !a->a || !a->b) return fail(WF_EINVAL, "%s: null field/coefficient", who);
+  if (a->nsteps < 0) return fail(WF_EINVAL, "%s: nsteps < 0", who);
+  if (a->jmode < WF_J_NONE || a->jmode > WF_J_FULL) return fail(WF_EINVAL, "%s: bad jmode", who);
+  if (a->conductive && (!a->sigma || !a->J || a->jmode != WF_J_FULL))
+    return fail(WF_EINVAL, "%s: conductive step needs J, sigma and jmode WF_J_FULL", who);
+  if (a->jmode != WF_J_NONE && !a->ieps) return fail(WF_EINVAL, "%s: J term needs ieps", who);
+  if (a->jmode != WF_J_NONE && !a->J && !a->beam_table)
+    return fail(WF_EINVAL, "%s: J term needs J or a beam table", who);
+  if (a->jmode == WF_J_FULL && !a->J) return fail(WF_EINVAL, "%s: WF_J_FULL needs J", who);
+  if ((a->jmode == WF_J_LINE || a->beam_table) &&
+      (a->ixs < 0 || a->ixs >= h->g.Nx || a->iys < 0 || a->iys >= h->g.Ny))
+    return fail(WF_ERANGE, "%s: beam line (%d,%d) outside the grid", who, a->ixs, a->iys);
+  if (a->beam_table && (a->beam_row0 < 0 || a->beam_row0 + a->nsteps > a->beam_rows))
+    return fail(WF_ERANGE, "%s: beam rows [%lld,%lld) outside table of %lld rows", who,
+                (long long)a->beam_row0, (long long)(a->beam_row0 + a->nsteps),
+                (long long)a->beam_rows);
+  if (a->beam_table && !a->J) return fail(WF_EINVAL, "%s: beam table needs J", who);
+  if (a->capture) {
+    if (a->cap_nx < 1 || a->cap_ny < 1 || a->cap_ix0 < 0 || a->cap_iy0 < 0 ||
+        a->cap_ix0 + a->cap_nx > h->g.Nx || a->cap_iy0 + a->cap_ny > h->g.Ny)
+      return fail(WF_ERANGE, "%s: capture window outside the grid", who);
+    if (a->cap_sample0 < 0 || a->cap_samples < 1)
+      return fail(WF_ERANGE, "%s: bad capture sample range", who);
+  }
+  if (a->path == 1) {
+    if (!a->E2 || !a->H2 || a->E2 == a->E || a->H2 == a->H || a->E2 == a->H2)
+      return fail(WF_EINVAL, "%s: fused path needs distinct E2/H2 buffers", who);
+    if (a->jmode == WF_J_FULL && !a->conductive)
+      return fail(WF_EINVAL, "%s: fused path has no general J read (use path 0)", who);
+    if (a->conductive && (!a->J || !a->ieps || !a->sigma))
+      return fail(WF_EINVAL, "%s: fused conductive step needs J, ieps and sigma", who);
+    if (a->kz < 0) return fail(WF_EINVAL, "%s: kz < 0", who);
+    if (a->code && (!a->mat || !a->metric2d || a->nmat < 1 || a->nmat > 256))
+      return fail(WF_EINVAL, "%s: material code needs a table of 1..256 materials "
+                  "and the 2-D metric tables", who);
+    if (a->compute_f64)
+      return fail(WF_EINVAL, "%s: FP64 compute on FP32 storage uses path 0", who);
+  } else if (a->path != 0) {
+    return fail(WF_EINVAL, "%s: path %d not available", who, a->path);
+  }
+  return WF_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 int wf_step(wf_handle *h, wf_step_args *a, void *stream) {
   if (!h || !a) return fail(WF_EINVAL, "wf_step: null argument");
-  if (!a->E || !a->H || !a->a || !a->b) return fail(WF_EINVAL, "wf_step: null field/coefficient");
-  if (a->nsteps < 0) return fail(WF_EINVAL, "wf_step: nsteps < 0");
-  if (a->jmode < WF_J_NONE || a->jmode > WF_J_FULL) return fail(WF_EINVAL, "wf_step: bad jmode");
-  if (a->conductive && (!a->sigma || !a->J || a->jmode != WF_J_FULL))
-    return fail(WF_EINVAL, "wf_step: conductive step needs J, sigma and jmode WF_J_FULL");
-  if (a->jmode != WF_J_NONE && !a->ieps) return fail(WF_EINVAL, "wf_step: J term needs ieps");
-  if (a->jmode != WF_J_NONE && !a->J && !a->beam_table)
-    return fail(WF_EINVAL, "wf_step: J term needs J or a beam table");
-  if (a->jmode == WF_J_FULL && !a->J) return fail(WF_EINVAL, "wf_step: WF_J_FULL needs J");
-  if ((a->jmode == WF_J_LINE || a->beam_table) &&
-      (a->ixs < 0 || a->ixs >= h->g.Nx || a->iys < 0 || a->iys >= h->g.Ny))
-    return fail(WF_ERANGE, "wf_step: beam line (%d,%d) outside the grid", a->ixs, a->iys);
-  if (a->beam_table && (a->beam_row0 < 0 || a->beam_row0 + a->nsteps > a->beam_rows))
-    return fail(WF_ERANGE, "wf_step: beam rows [%lld,%lld) outside table of %lld rows",
-                (long long)a->beam_row0, (long long)(a->beam_row0 + a->nsteps),
-                (long long)a->beam_rows);
-  if (a->beam_table && !a->J) return fail(WF_EINVAL, "wf_step: beam table needs J");
-  if (a->capture) {
-    if (a->cap_nx < 1 || a->cap_ny < 1 || a->cap_ix0 < 0 || a->cap_iy0 < 0 ||
-        a->cap_ix0 + a->cap_nx > h->g.Nx || a->cap_iy0 + a->cap_ny > h->g.Ny)
-      return fail(WF_ERANGE, "wf_step: capture window outside the grid");
-    if (a->cap_sample0 < 0 || a->cap_samples < 1)
-      return fail(WF_ERANGE, "wf_step: bad capture sample range");
-  }
+  const int rc = check_step_args(h, a, "wf_step");
+  if (rc != WF_OK) return rc;
   a->result_swapped = 0;
   hipStream_t st = (hipStream_t)stream;
   if (a->path == 1) {
-    if (!a->E2 || !a->H2 || a->E2 == a->E || a->H2 == a->H || a->E2 == a->H2)
-      return fail(WF_EINVAL, "wf_step: fused path needs distinct E2/H2 buffers");
-    if (a->jmode == WF_J_FULL && !a->conductive)
-      return fail(WF_EINVAL, "wf_step: fused path has no general J read (use path 0)");
-    if (a->conductive && (!a->J || !a->ieps || !a->sigma))
-      return fail(WF_EINVAL, "wf_step: fused conductive step needs J, ieps and sigma");
-    if (a->kz < 0) return fail(WF_EINVAL, "wf_step: kz < 0");
-    if (a->code && (!a->mat || !a->metric2d || a->nmat < 1 || a->nmat > 256))
-      return fail(WF_EINVAL, "wf_step: material code needs a table of 1..256 materials "
-                  "and the 2-D metric tables");
-    if (a->compute_f64) return fail(WF_EINVAL, "wf_step: FP64 compute on FP32 storage uses path 0");
     if (h->dtype == WF_F64) return step_fused<double, double>(h, a, h->md, st);
     return step_fused<float, float>(h, a, h->mf, st);
   }
-  if (a->path != 0) return fail(WF_EINVAL, "wf_step: path %d not available", a->path);
   if (h->dtype == WF_F64) return step_general<double, double>(h, a, h->md, st);
   if (a->compute_f64) return step_general<float, double>(h, a, h->mf, st);
   return step_general<float, float>(h, a, h->mf, st);
@@ -514,6 +526,305 @@ int wf_loss_factor(const double *WP, const double *lam, int64_t n, double ds, do
   if (n < 0) return fail(WF_ERANGE, "wf_loss_factor: negative length");
   hipLaunchKernelGGL(wf::k_dot, dim3(1), dim3(wf::RED_THREADS), 0, (hipStream_t)stream, WP, lam,
                      n, -ds, out);
+  WF_LAUNCH_CHECK();
+  return WF_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: one launch per step over many grids.
+// ---------------------------------------------------------------------------
+struct wf_batch {
+  int dtype = 0, ninst = 0, ntiles = 0, maxsteps = 0;
+  int rows = 3, vw = 1, cond = 0, jm = 0, desc = 0;
+  size_t lds = 0;
+  void *desc_dev = nullptr;   // BatchDesc<S>[ninst]
+  int *step_dev = nullptr;    // step number read by the batch kernel
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_steps = 0;
+};
+
+namespace {
+
+constexpr int BATCH_ROWS = 3;
+constexpr int BATCH_GRAPH_STEPS = 16;
+
+template <typename S>
+int batch_build(wf_handle *const *hs, const wf_step_args *as, int n, int vw, int kz,
+                wf_batch *b) {
+  std::vector<wf::BatchDesc<S>> d(n);
+  int64_t tiles = 0;
+  size_t lds = 0;
+  const int W = 64 * vw;
+  for (int i = 0; i < n; ++i) {
+    const wf_handle *h = hs[i];
+    const wf_step_args *a = &as[i];
+    wf::BatchDesc<S> &e = d[i];
+    e.g = h->g;
+    if constexpr (sizeof(S) == 8) e.m = h->md;
+    else e.m = h->mf;
+    e.E = (S *)a->E; e.H = (S *)a->H; e.E2 = (S *)a->E2; e.H2 = (S *)a->H2; e.J = (S *)a->J;
+    wf::Coefs<S> &cf = e.cf;
+    cf.a = (const S *)a->a; cf.b = (const S *)a->b;
+    cf.ieps = (const S *)a->ieps; cf.sigma = (const S *)a->sigma;
+    cf.code = (const uint8_t *)a->code; cf.mat = (const S *)a->mat; cf.nmat = a->nmat;
+    const S *t = (const S *)a->metric2d;
+    const int64_t yz = (int64_t)h->g.Ny * h->g.Nz, xz = (int64_t)h->g.Nx * h->g.Nz,
+                  xy = (int64_t)h->g.Nx * h->g.Ny;
+    cf.iAx = t; cf.iAy = t + yz; cf.iAz = t + yz + xz;
+    cf.itAx = t + yz + xz + xy; cf.itAy = cf.itAx + yz; cf.itAz = cf.itAy + xz;
+    e.dt = (S)a->dt;
+    wf::FusedAux &x = e.x;
+    x.ixs = (a->jmode == WF_J_LINE || a->beam_table || a->conductive) ? a->ixs : -1;
+    x.iys = (x.ixs >= 0) ? a->iys : -1;
+    x.cap_ix0 = a->cap_ix0; x.cap_iy0 = a->cap_iy0; x.cap_nx = a->cap_nx; x.cap_ny = a->cap_ny;
+    x.cap_samples = a->cap_samples;
+    x.cap_sample = 0;
+    x.ntx = (h->g.Nx + W - 1) / W;
+    x.nty = (h->g.Ny + BATCH_ROWS - 1) / BATCH_ROWS;
+    x.KZ = kz > 0 ? kz : (h->g.Nz < 32 ? h->g.Nz : 32);
+    x.ntz = (h->g.Nz + x.KZ - 1) / x.KZ;
+    e.beam_table = (const S *)a->beam_table;
+    e.beam_row0 = a->beam_row0;
+    e.cap = a->capture ? (S *)a->capture : nullptr;
+    e.cap_sample0 = a->cap_sample0;
+    e.nsteps = a->nsteps;
+    e.tile0 = (int)tiles;
+    tiles += (int64_t)x.ntx * x.nty * x.ntz;
+    lds = std::max(lds, (size_t)a->nmat * 9 * sizeof(S));
+    b->maxsteps = std::max(b->maxsteps, a->nsteps);
+  }
+  if (tiles > 2147483647LL) return fail(WF_ERANGE, "wf_batch_create: too many tiles");
+  b->ntiles = (int)tiles;
+  b->lds = lds;
+  WF_HIP(hipMalloc(&b->desc_dev, sizeof(wf::BatchDesc<S>) * n));
+  WF_HIP(hipMemcpy(b->desc_dev, d.data(), sizeof(wf::BatchDesc<S>) * n, hipMemcpyHostToDevice));
+  WF_HIP(hipMalloc((void **)&b->step_dev, sizeof(int)));
+  return WF_OK;
+}
+
+// launch one batch step (the step number is read on the device)
+template <typename S>
+int batch_launch(const wf_batch *b, hipStream_t st) {
+  const auto *d = (const wf::BatchDesc<S> *)b->desc_dev;
+  const dim3 grid((unsigned)b->ntiles), block(64 * (BATCH_ROWS + 1));
+#define WF_B_LAUNCH(JM, CO, DE, VW)                                                          \
+  hipLaunchKernelGGL((wf::k_fused4_batch<S, JM, CO, true, DE, BATCH_ROWS, false, VW>), grid, \
+                     block, b->lds, st, d, b->ninst, b->ntiles, (const int *)b->step_dev)
+#define WF_B_LAUNCH2(JM, CO, VW)                                                             \
+  do {                                                                                       \
+    if (b->desc) WF_B_LAUNCH(JM, CO, true, VW);                                              \
+    else WF_B_LAUNCH(JM, CO, false, VW);                                                     \
+  } while (0)
+#define WF_B_LAUNCH3(VW)                                                                     \
+  do {                                                                                       \
+    if (b->cond) WF_B_LAUNCH2(0, true, VW);                                                  \
+    else if (b->jm == WF_J_LINE) WF_B_LAUNCH2(1, false, VW);                                 \
+    else WF_B_LAUNCH2(0, false, VW);                                                         \
+  } while (0)
+  if (b->vw == 1) WF_B_LAUNCH3(1);
+  else WF_B_LAUNCH3(16 / (int)sizeof(S));
+#undef WF_B_LAUNCH3
+#undef WF_B_LAUNCH2
+#undef WF_B_LAUNCH
+  WF_LAUNCH_CHECK();
+  hipLaunchKernelGGL(wf::k_batch_next, dim3(1), dim3(1), 0, st, b->step_dev);
+  WF_LAUNCH_CHECK();
+  return WF_OK;
+}
+
+int batch_step_once(const wf_batch *b, hipStream_t st) {
+  return b->dtype == WF_F64 ? batch_launch<double>(b, st) : batch_launch<float>(b, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+int wf_batch_create(wf_handle *const *handles, const wf_step_args *args, int32_t n,
+                    int32_t cells_per_lane, int32_t kz, wf_batch **out) {
+  if (!handles || !args || !out) return fail(WF_EINVAL, "wf_batch_create: null argument");
+  *out = nullptr;
+  if (n < 1) return fail(WF_EINVAL, "wf_batch_create: empty batch");
+  if (kz < 0) return fail(WF_EINVAL, "wf_batch_create: kz < 0");
+  const wf_handle *h0 = handles[0];
+  if (!h0) return fail(WF_EINVAL, "wf_batch_create: null handle");
+  const int dtype = h0->dtype;
+  const int vnat = dtype == WF_F64 ? 2 : 4;
+  const int vw = cells_per_lane == 0 ? vnat : cells_per_lane;
+  if (vw != 1 && vw != vnat)
+    return fail(WF_EINVAL, "wf_batch_create: cells_per_lane must be 0, 1 or %d", vnat);
+  const wf_step_args &a0 = args[0];
+  for (int i = 0; i < n; ++i) {
+    const wf_handle *h = handles[i];
+    const wf_step_args *a = &args[i];
+    if (!h) return fail(WF_EINVAL, "wf_batch_create: null handle %d", i);
+    char who[64];
+    snprintf(who, sizeof(who), "wf_batch_create[%d]", i);
+    const int rc = check_step_args(h, a, who);
+    if (rc != WF_OK) return rc;
+    if (a->path != 1) return fail(WF_EINVAL, "%s: batch items use the fused path (path 1)", who);
+    if (!a->code || !a->zero_masked)
+      return fail(WF_EINVAL, "%s: batch items need material codes and zero_masked = 1", who);
+    if (h->dtype != dtype || a->conductive != a0.conductive || a->jmode != a0.jmode ||
+        h->g.m1desc != h0->g.m1desc)
+      return fail(WF_EINVAL, "%s: dtype, conductivity, J mode and boundary order must match "
+                  "across the batch", who);
+    if (h->g.Nx % vw != 0)
+      return fail(WF_EINVAL, "%s: Nx=%d not a multiple of %d cells per lane", who, h->g.Nx, vw);
+    if ((uint64_t)3 * (uint64_t)h->g.N * (dtype == WF_F64 ? 8 : 4) >= (1ull << 32))
+      return fail(WF_ERANGE, "%s: grid too large for a batch", who);
+  }
+  wf_batch *b = new wf_batch();
+  b->dtype = dtype; b->ninst = n; b->vw = vw;
+  b->cond = a0.conductive; b->jm = a0.jmode; b->desc = h0->g.m1desc;
+  const int rc = dtype == WF_F64 ? batch_build<double>(handles, args, n, vw, kz, b)
+                                 : batch_build<float>(handles, args, n, vw, kz, b);
+  if (rc != WF_OK) {
+    wf_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return WF_OK;
+}
+
+int wf_batch_run(wf_batch *b, int32_t step0, int32_t nsteps, int32_t use_graph, void *stream) {
+  if (!b) return fail(WF_EINVAL, "wf_batch_run: null batch");
+  if (step0 < 0 || nsteps < 0) return fail(WF_EINVAL, "wf_batch_run: negative step range");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(wf::k_batch_set, dim3(1), dim3(1), 0, st, b->step_dev, (int)step0);
+  WF_LAUNCH_CHECK();
+  int done = 0;
+  if (use_graph && nsteps >= BATCH_GRAPH_STEPS) {
+    if (!b->exec) {
+      WF_HIP(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+      WF_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+      int rc = WF_OK;
+      for (int s = 0; s < BATCH_GRAPH_STEPS && rc == WF_OK; ++s) rc = batch_step_once(b, b->cap_stream);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+      if (rc != WF_OK) return rc;
+      if (e != hipSuccess) return fail(WF_EHIP, "wf_batch_run: capture: %s", hipGetErrorString(e));
+      b->graph = g;
+      WF_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      b->graph_steps = BATCH_GRAPH_STEPS;
+    }
+    for (; done + b->graph_steps <= nsteps; done += b->graph_steps)
+      WF_HIP(hipGraphLaunch(b->exec, st));
+  }
+  for (; done < nsteps; ++done) {
+    const int rc = batch_step_once(b, st);
+    if (rc != WF_OK) return rc;
+  }
+  return WF_OK;
+}
+
+int wf_batch_destroy(wf_batch *b) {
+  if (!b) return WF_OK;
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->cap_stream) hipStreamDestroy(b->cap_stream);
+  if (b->desc_dev) hipFree(b->desc_dev);
+  if (b->step_dev) hipFree(b->step_dev);
+  delete b;
+  return WF_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Reverse mode (discrete adjoint of the step and of the wake post-processing)
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename S>
+int adjoint_step(wf_handle *h, const wf_adjoint_args *a, const wf::MetricDev<S> &m,
+                 hipStream_t st) {
+  const wf::GridDev &g = h->g;
+  wf::AdjAux x;
+  x.conductive = a->conductive;
+  x.ixs = a->ixs; x.iys = a->iys;
+  for (int d = 0; d < 3; ++d) { x.r0[d] = a->r0[d]; x.r1[d] = a->r1[d]; }
+  x.cap_ix0 = a->cap_ix0; x.cap_iy0 = a->cap_iy0; x.cap_nx = a->cap_nx; x.cap_ny = a->cap_ny;
+  x.cap_samples = a->cap_samples; x.cap_sample = a->cap_sample;
+  const int64_t nreg = (int64_t)(a->r1[0] - a->r0[0]) * (a->r1[1] - a->r0[1]) *
+                       (a->r1[2] - a->r0[2]);
+  if (nreg > 0 && a->g_ieps) {
+    hipLaunchKernelGGL((wf::k_adj_grad<S>), dim3(nblocks(nreg, 256)), dim3(256), 0, st, g, m,
+                       a->lE, a->lJ, (const S *)a->E1, (const S *)a->H1, (const S *)a->J0,
+                       (const S *)a->sigma, (const S *)a->metric2d, a->dt, x, a->g_ieps,
+                       a->g_sigma);
+    WF_LAUNCH_CHECK();
+  }
+  const unsigned nb = nblocks(g.N, 256);
+  hipLaunchKernelGGL((wf::k_adj_h<S>), dim3(nb), dim3(256), 0, st, g, m, a->lH,
+                     (const double *)a->lE, (const double *)a->lJ, (const S *)a->b,
+                     (const S *)a->sigma, a->dt, a->conductive);
+  WF_LAUNCH_CHECK();
+  hipLaunchKernelGGL((wf::k_adj_e<S>), dim3(nb), dim3(256), 0, st, g, m, a->lE, a->lJ,
+                     (const double *)a->lH, (const S *)a->a, (const S *)a->ieps,
+                     (const S *)a->sigma, a->dt, x, a->gcap);
+  WF_LAUNCH_CHECK();
+  return WF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wf_adjoint_step(wf_handle *h, const wf_adjoint_args *a, void *stream) {
+  if (!h || !a) return fail(WF_EINVAL, "wf_adjoint_step: null argument");
+  if (!a->lE || !a->lH || !a->a || !a->b || !a->ieps)
+    return fail(WF_EINVAL, "wf_adjoint_step: null adjoint field or coefficient");
+  if (a->conductive && (!a->lJ || !a->sigma))
+    return fail(WF_EINVAL, "wf_adjoint_step: conductive step needs lJ and sigma");
+  for (int d = 0; d < 3; ++d) {
+    const int n = d == 0 ? h->g.Nx : (d == 1 ? h->g.Ny : h->g.Nz);
+    if (a->r0[d] < 0 || a->r1[d] > n || a->r0[d] > a->r1[d])
+      return fail(WF_ERANGE, "wf_adjoint_step: gradient region outside the grid");
+  }
+  const bool region = (a->r1[0] > a->r0[0]) && (a->r1[1] > a->r0[1]) && (a->r1[2] > a->r0[2]);
+  if (region && a->g_ieps) {
+    if (!a->E1 || !a->H1 || !a->J0 || !a->metric2d)
+      return fail(WF_EINVAL, "wf_adjoint_step: gradients need E1, H1, J0 and metric2d");
+    if (a->conductive && !a->g_sigma)
+      return fail(WF_EINVAL, "wf_adjoint_step: conductive gradients need g_sigma");
+  }
+  if (a->gcap && (a->cap_nx < 1 || a->cap_ny < 1 || a->cap_ix0 < 0 || a->cap_iy0 < 0 ||
+                  a->cap_ix0 + a->cap_nx > h->g.Nx || a->cap_iy0 + a->cap_ny > h->g.Ny ||
+                  a->cap_sample < 0))
+    return fail(WF_ERANGE, "wf_adjoint_step: capture window outside the grid");
+  hipStream_t st = (hipStream_t)stream;
+  if (h->dtype == WF_F64) return adjoint_step<double>(h, a, h->md, st);
+  return adjoint_step<float>(h, a, h->mf, st);
+}
+
+int wf_wake_wp_adjoint(const double *gI, int32_t nlines, int64_t cap_samples, int32_t Nz,
+                       int32_t k0, int32_t nz, const double *z, const double *dz, const double *s,
+                       int64_t ns, double v, double zmin_v, double t0, double ti, double dt,
+                       int64_t nt, double *gcap, void *stream) {
+  if (!gI || !z || !dz || !s || !gcap) return fail(WF_EINVAL, "wf_wake_wp_adjoint: null argument");
+  if (nlines < 1 || nz < 0 || k0 < 0 || k0 + nz > Nz || nt > cap_samples || dt <= 0)
+    return fail(WF_ERANGE, "wf_wake_wp_adjoint: inconsistent sizes");
+  const int64_t tot = (int64_t)nlines * nz;
+  if (tot == 0) return WF_OK;
+  hipLaunchKernelGGL(wf::k_wake_wp_adjoint, dim3(nblocks(tot, 64)), dim3(64), 0,
+                     (hipStream_t)stream, gI, nlines, cap_samples, Nz, k0, nz, z, dz, s, ns, v,
+                     zmin_v, t0, ti, dt, nt, gcap);
+  WF_LAUNCH_CHECK();
+  return WF_OK;
+}
+
+int wf_idft_real(const double *w, int64_t nbins, int64_t N, int64_t len, double *out,
+                 void *stream) {
+  if (!w || !out) return fail(WF_EINVAL, "wf_idft_real: null argument");
+  if (nbins < 0 || N < 1 || len < 0 || nbins > N) return fail(WF_ERANGE, "wf_idft_real: sizes");
+  if (len == 0) return WF_OK;
+  hipLaunchKernelGGL(wf::k_idft_real, dim3(nblocks(len, 256)), dim3(256), 0, (hipStream_t)stream,
+                     w, nbins, N, len, out);
   WF_LAUNCH_CHECK();
   return WF_OK;
 }

@@ -36,27 +36,40 @@ __device__ __forceinline__ S *atw(S *p, I idx) {
   else
     return p + idx;
 }
-// V consecutive values (16 bytes) starting at a multiple-of-V index
+// V consecutive values starting at a multiple-of-V index: one 16-byte access
+// when V*sizeof(S) == 16, else V element accesses
 template <int V, typename S, typename I>
 __device__ __forceinline__ void ldv(const S *__restrict__ p, I idx, S (&o)[V]) {
-  static_assert(V * sizeof(S) == 16, "16-byte vectors");
   const S *q = at(p, idx);
-  if constexpr (sizeof(S) == 4) {
+  if constexpr (V * sizeof(S) == 16 && sizeof(S) == 4) {
     const float4 t = *reinterpret_cast<const float4 *>(q);
     o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
-  } else {
+  } else if constexpr (V * sizeof(S) == 16) {
     const double2 t = *reinterpret_cast<const double2 *>(q);
     o[0] = t.x; o[1] = t.y;
+  } else {
+#pragma unroll
+    for (int v = 0; v < V; ++v) o[v] = q[v];
   }
 }
 template <int V, typename S, typename I>
 __device__ __forceinline__ void stv(S *__restrict__ p, I idx, const S (&v)[V]) {
   S *q = atw(p, idx);
-  if constexpr (sizeof(S) == 4) {
+  if constexpr (V * sizeof(S) == 16 && sizeof(S) == 4) {
     *reinterpret_cast<float4 *>(q) = make_float4(v[0], v[1], v[2], v[3]);
-  } else {
+  } else if constexpr (V * sizeof(S) == 16) {
     *reinterpret_cast<double2 *>(q) = make_double2(v[0], v[1]);
+  } else {
+#pragma unroll
+    for (int u = 0; u < V; ++u) q[u] = v[u];
   }
+}
+// V consecutive uint8 material codes packed little-endian into one word
+template <int V>
+__device__ __forceinline__ uint32_t ldcode(const uint8_t *__restrict__ p, uint32_t idx) {
+  if constexpr (V == 4) return *reinterpret_cast<const uint32_t *>(p + idx);
+  else if constexpr (V == 2) return *reinterpret_cast<const uint16_t *>(p + idx);
+  else return p[idx];
 }
 
 // MK = false: the caller guarantees that E is zero on every PEC-masked edge and
@@ -64,13 +77,26 @@ __device__ __forceinline__ void stv(S *__restrict__ p, I idx, const S (&v)[V]) {
 // coefficients with the row masks folded in. The staged values then need no
 // boundary masks: with zero mask words every mask test below folds to "true"
 // at compile time, which frees the registers the per-lane masks take.
-template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY, bool MK>
-__global__ void __launch_bounds__(64 * (TY + 1))
-k_fused4(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restrict__ Hin,
-         S *__restrict__ Eout, S *__restrict__ Hout, S *__restrict__ J, Coefs<S> cf, S dt,
-         FusedAux x, const S *__restrict__ beam_row, S *__restrict__ cap) {
+// XCD-aware block -> tile map: the hardware deals consecutive block ids to the
+// 8 XCDs in turn, so block b runs on XCD b % 8; give each XCD a contiguous run
+// of tiles so that neighbouring tiles (which share halo rows) meet in one L2.
+__device__ __forceinline__ int xcd_tile(int blin, int nb) {
+  const int xcd = blin & 7, slot = blin >> 3;
+  const int q8 = nb >> 3, r8 = nb & 7;
+  return xcd * q8 + (xcd < r8 ? xcd : r8) + slot;
+}
+
+// One workgroup's z-march over tile `tile` of the grid (the whole step of that
+// tile); shared by the single-grid and the batched kernels.
+template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY, bool MK, int V>
+__device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S> &m,
+                                            const S *__restrict__ Ein, const S *__restrict__ Hin,
+                                            S *__restrict__ Eout, S *__restrict__ Hout,
+                                            S *__restrict__ J, const Coefs<S> &cf, S dt,
+                                            const FusedAux &x, const S *__restrict__ beam_row,
+                                            S *__restrict__ cap, int tile) {
   using I = uint32_t;
-  constexpr int V = 16 / sizeof(S);            // cells per lane
+  // V cells per lane: 16 / sizeof(S) for 16-byte accesses, or 1 for narrow grids
   constexpr int W = 64 * V;                    // cells per row segment
   constexpr int OFF = V, RW = W + 2 * V;       // cell xi at LDS slot xi + OFF
   constexpr int NM = 9;
@@ -85,11 +111,6 @@ k_fused4(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restri
     for (int t = threadIdx.x; t < cf.nmat * NM; t += 64 * (TY + 1)) sMat[t] = cf.mat[t];
   }
 
-  const int nb = x.ntx * x.nty * x.ntz;
-  const int blin = blockIdx.x;
-  const int xcd = blin & 7, slot = blin >> 3;
-  const int q8 = nb >> 3, r8 = nb & 7;
-  const int tile = xcd * q8 + (xcd < r8 ? xcd : r8) + slot;
   const int tz = tile / (x.ntx * x.nty);
   const int rem = tile - tz * x.ntx * x.nty;
   const int tyb = rem / x.ntx, txb = rem - tyb * x.ntx;
@@ -197,8 +218,7 @@ k_fused4(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restri
     if (lact && (w >= 1 || body)) {
       ldv<V>(Hin, nk, Ho[0]); ldv<V>(Hin, N + nk, Ho[1]); ldv<V>(Hin, 2 * N + nk, Ho[2]);
       if (LUT) {
-        if constexpr (V == 4) code = *reinterpret_cast<const uint32_t *>(cf.code + nk);
-        else code = *reinterpret_cast<const uint16_t *>(cf.code + nk);
+        code = ldcode<V>(cf.code, nk);
         iax = cf.iAx[(int64_t)j + (int64_t)g.Ny * k];
         ldv<V>(cf.iAy, (I)ic + (I)g.Nx * (I)k, c0y);
         if (w >= 1 && body) {
@@ -464,5 +484,61 @@ k_fused4(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restri
     }
   }
 }
+
+template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY, bool MK>
+__global__ void __launch_bounds__(64 * (TY + 1))
+k_fused4(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restrict__ Hin,
+         S *__restrict__ Eout, S *__restrict__ Hout, S *__restrict__ J, Coefs<S> cf, S dt,
+         FusedAux x, const S *__restrict__ beam_row, S *__restrict__ cap) {
+  const int tile = xcd_tile(blockIdx.x, x.ntx * x.nty * x.ntz);
+  fused4_tile<S, JM, COND, LUT, DESC, TY, MK, 16 / sizeof(S)>(g, m, Ein, Hin, Eout, Hout, J,
+                                                             cf, dt, x, beam_row, cap, tile);
+}
+
+// One grid of a batch: everything a step of it needs except the step number.
+template <typename S>
+struct BatchDesc {
+  GridDev g;
+  MetricDev<S> m;
+  S *E, *H, *E2, *H2, *J;      // even steps read E/H and write E2/H2, odd steps the reverse
+  Coefs<S> cf;
+  S dt;
+  FusedAux x;                  // cap_sample is set per step
+  const S *beam_table;         // row of step s: beam_table + (beam_row0 + s) * Nz, or null
+  int64_t beam_row0;
+  S *cap;
+  int64_t cap_sample0;         // capture sample written by step s: cap_sample0 + s
+  int nsteps;                  // the grid takes part in steps 0 .. nsteps-1
+  int tile0;                   // its first tile in the batch's tile numbering
+};
+
+// One step of every grid of a batch in one launch. The step number comes from
+// device memory (k_batch_next advances it), so a captured graph of these
+// launches replays the right steps.
+template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY, bool MK, int V>
+__global__ void __launch_bounds__(64 * (TY + 1))
+k_fused4_batch(const BatchDesc<S> *__restrict__ d, int ninst, int ntiles,
+               const int *__restrict__ step_ptr) {
+  const int step = *step_ptr;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  int lo = 0, hi = ninst - 1;  // last descriptor with tile0 <= tile
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].tile0 <= tile) lo = mid;
+    else hi = mid - 1;
+  }
+  const BatchDesc<S> &b = d[lo];
+  if (step >= b.nsteps) return;  // whole workgroup, before any barrier
+  const bool odd = step & 1;
+  FusedAux x = b.x;
+  x.cap_sample = b.cap_sample0 + step;
+  const S *row = b.beam_table ? b.beam_table + (b.beam_row0 + step) * b.g.Nz : nullptr;
+  fused4_tile<S, JM, COND, LUT, DESC, TY, MK, V>(
+      b.g, b.m, odd ? b.E2 : b.E, odd ? b.H2 : b.H, odd ? b.E : b.E2, odd ? b.H : b.H2, b.J,
+      b.cf, b.dt, x, row, b.cap, tile - b.tile0);
+}
+
+__global__ void k_batch_next(int *step) { step[0] = step[0] + 1; }
+__global__ void k_batch_set(int *step, int v) { step[0] = v; }
 
 }  // namespace wf

@@ -141,6 +141,8 @@ class Field:
             if len(key) != 4:
                 raise IndexError("Need 3 indexes and component to access the field")
             m = self.to_matrix(key[3])
+            if isinstance(value, np.generic):
+                value = value.item()
             v = value if np.isscalar(value) else self._as_tensor(value)
             m[key[0], key[1], key[2]] = v
         elif isinstance(key, (int, np.integer)):
@@ -150,6 +152,8 @@ class Field:
                 raise IndexError(
                     "Lexico-graphic index cannot be higher than product of dimensions")
         elif type(key) is slice:
+            if isinstance(value, np.generic):
+                value = value.item()
             self.array[key] = value if np.isscalar(value) else self._as_tensor(value)
         else:
             raise IndexError("key must be a 3-tuple or an integer")

@@ -44,18 +44,41 @@ class RoutinesMixin:
                   add_space=None, use_field_monitor=False, field_monitor=None, use_edt=None,
                   chunk=None, **kwargs):
         """Wakefield run and WakeSolver post-processing (routines.py:231-482)."""
-        from .sources import Beam
-
         if wake is not None:
             self.wake = wake
-        if self.wake is None:
-            raise AttributeError("Wake solver information not passed to the solver instantiation")
-        if add_space is not None:
+        if add_space is not None and self.wake is not None:
             self.wake.skip_cells = add_space
         if plot or save_J:
             raise NotImplementedError("wakesolve: plotting and Jz.h5 saving are not part of "
                                       "this engine")
+        ctx = self._wake_setup(wakelength, compute_plane)
+        Nt = ctx["Nt"]
+        print("Running electromagnetic time-domain simulation...")
+        t0 = time.time()
+        per_step = callback is not None or (use_field_monitor and field_monitor is not None)
+        step = 1 if per_step else (Nt if chunk is None else int(chunk))
+        n = 0
+        while n < Nt:
+            k = min(step, Nt - n)
+            self._run(self._wake_args(ctx, n, k))
+            self.nstep += k
+            if per_step:
+                if use_field_monitor and field_monitor is not None:
+                    field_monitor.update(self.E, self.dt)
+                if callback is not None:
+                    callback(self, n * self.dt)
+            n += k
+        torch.cuda.synchronize(self.device)
+        self.simulationTime = time.time() - t0
+        self._wake_finish(ctx, compute_plane)
 
+    def _wake_setup(self, wakelength, compute_plane):
+        """Set-up part of wakesolve (routines.py:319-440): time axis, beam table,
+        Ez capture buffer with sample 0, first-step ghost zeroing."""
+        from .sources import Beam
+
+        if self.wake is None:
+            raise AttributeError("Wake solver information not passed to the solver instantiation")
         self.xtest, self.ytest = self.wake.xtest, self.wake.ytest
         self.ixt = np.abs(self.x - self.xtest).argmin()
         self.iyt = np.abs(self.y - self.ytest).argmin()
@@ -79,49 +102,38 @@ class RoutinesMixin:
         dev, tdt = self.device, torch_dtype(self.dtype)
         table = torch.as_tensor(beam.table(self, Nt), device=dev).to(tdt).contiguous()
         self._wake_beam = beam
-        nlines = nx * ny
-        cap = torch.zeros((nlines, max(Nt, 1), self.Nz), dtype=tdt, device=dev)
-        t_axis = np.arange(0, Nt * self.dt, self.dt)
-
-        print("Running electromagnetic time-domain simulation...")
-        t0 = time.time()
-        lib = _lib.lib()
-        st = _lib.stream_ptr()
+        cap = torch.zeros((nx * ny, max(Nt, 1), self.Nz), dtype=tdt, device=dev)
         if Nt > 0:  # sample 0: E before the first step (routines.py:437-440)
-            _lib.check(lib.wf_capture(self._handle, _lib.ptr(self.E.array), _lib.ptr(cap),
-                                      ix0, iy0, nx, ny, Nt, 0, st), "wf_capture")
+            _lib.check(_lib.lib().wf_capture(self._handle, _lib.ptr(self.E.array), _lib.ptr(cap),
+                                             ix0, iy0, nx, ny, Nt, 0, _lib.stream_ptr()),
+                       "wf_capture")
         if self.step_0:
             self._set_ghosts_to_0()
             self.step_0 = False
-        per_step = callback is not None or (use_field_monitor and field_monitor is not None)
-        step = 1 if per_step else (Nt if chunk is None else int(chunk))
-        n = 0
-        while n < Nt:
-            k = min(step, Nt - n)
-            # the beam table rows are what Beam.update writes: J stays "beam line only"
-            self._beam_line = (int(beam.ixs), int(beam.iys)) \
-                if self.J.version == self._J_clean_version else self._beam_line
-            a = self._step_args(k)
-            a.ixs, a.iys = int(beam.ixs), int(beam.iys)
-            if a.jmode == _lib.WF_J_NONE:
-                a.jmode = _lib.WF_J_LINE
-            a.beam_table, a.beam_row0, a.beam_rows = _lib.ptr(table), n, Nt
-            a.capture = _lib.ptr(cap)
-            a.cap_ix0, a.cap_iy0, a.cap_nx, a.cap_ny = ix0, iy0, nx, ny
-            a.cap_samples, a.cap_sample0 = Nt, n + 1
-            self._run(a)
-            self.nstep += k
-            if per_step:
-                if use_field_monitor and field_monitor is not None:
-                    field_monitor.update(self.E, self.dt)
-                if callback is not None:
-                    callback(self, n * self.dt)
-            n += k
-        torch.cuda.synchronize(dev)
-        self.simulationTime = time.time() - t0
+        return dict(Nt=Nt, beam=beam, table=table, cap=cap, win=(ix0, iy0, nx, ny),
+                    t_axis=np.arange(0, Nt * self.dt, self.dt))
 
-        self.wake._set_capture(cap=cap, t=t_axis, xf=self.x[ix0:ix0 + nx],
-                               yf=self.y[iy0:iy0 + ny], zf=z, dx=self.grid.dx,
+    def _wake_args(self, ctx, n, k):
+        """wf_step arguments for steps n .. n+k-1 of the wakesolve run in ctx."""
+        beam, (ix0, iy0, nx, ny), Nt = ctx["beam"], ctx["win"], ctx["Nt"]
+        # the beam table rows are what Beam.update writes: J stays "beam line only"
+        self._beam_line = (int(beam.ixs), int(beam.iys)) \
+            if self.J.version == self._J_clean_version else self._beam_line
+        a = self._step_args(k)
+        a.ixs, a.iys = int(beam.ixs), int(beam.iys)
+        if a.jmode == _lib.WF_J_NONE:
+            a.jmode = _lib.WF_J_LINE
+        a.beam_table, a.beam_row0, a.beam_rows = _lib.ptr(ctx["table"]), n, Nt
+        a.capture = _lib.ptr(ctx["cap"])
+        a.cap_ix0, a.cap_iy0, a.cap_nx, a.cap_ny = ix0, iy0, nx, ny
+        a.cap_samples, a.cap_sample0 = Nt, n + 1
+        return a
+
+    def _wake_finish(self, ctx, compute_plane):
+        """Hand the capture to the WakeSolver and run its post-processing."""
+        ix0, iy0, nx, ny = ctx["win"]
+        self.wake._set_capture(cap=ctx["cap"], t=ctx["t_axis"], xf=self.x[ix0:ix0 + nx],
+                               yf=self.y[iy0:iy0 + ny], zf=self.z, dx=self.grid.dx,
                                dy=self.grid.dy, dz=self.grid.dz, dtype=self.dtype,
-                               nsamples=Nt)
+                               nsamples=ctx["Nt"])
         self.wake.solve(compute_plane=compute_plane)

@@ -70,7 +70,7 @@ class SolverFIT3D(RoutinesMixin):
                  bc_high=["Periodic", "Periodic", "Periodic"],
                  use_stl=False, use_conductors=False, use_gpu=True, use_mpi=False,
                  use_sibc=True, fmax=1e9, dtype=np.float64, n_pml=10, bg=[1.0, 1.0],
-                 verbose=1, path="auto"):
+                 verbose=1, path="auto", tau_clamp=True):
         t0 = time.time()
         if use_mpi:
             raise NotImplementedError("wakis_amd runs the full domain on one GPU (use_mpi=False)")
@@ -136,7 +136,10 @@ class SolverFIT3D(RoutinesMixin):
                                            + 1 / np.min(self.grid.dy) ** 2
                                            + 1 / np.min(self.grid.dz) ** 2))
         self.dt = self.dtype(dt)
-        if self.use_conductivity:
+        # tau_clamp=False keeps dt independent of the conductivities (needed when
+        # sigma is a parameter to differentiate); the caller then keeps
+        # dt*sigma/eps well below 2 for stability
+        if self.use_conductivity and tau_clamp:
             h = self._h
             m = np.logical_and(h["sigma"] != 0, h["ieps"] != 0)
             self.tau = (1 / h["ieps"][m]) / h["sigma"][m]
@@ -626,26 +629,49 @@ class SolverFIT3D(RoutinesMixin):
         self._wrap_key, self._wrap_ok = key, ok
         return ok
 
+    def _prepare_fused(self, args):
+        """Choose the path for a batch and fill the fused-path fields of args.
+
+        Returns True when the batch runs on the fused path (E2/H2 ping-pong,
+        compressed coefficients when exact, no in-kernel masks when the masked
+        values are zero), False for the exact general path.
+        """
+        fused = self._choose_fused(args)
+        if not fused:
+            return False
+        if self._E2 is None:
+            self._E2 = torch.empty_like(self.E.array)
+            self._H2 = torch.empty_like(self.H.array)
+        args.path = 1
+        args.E2, args.H2 = _lib.ptr(self._E2), _lib.ptr(self._H2)
+        args.kz = int(self.fused_kz)
+        args.variant = int(self.fused_variant)
+        args.zero_masked = 1 if (self.skip_masks and self._masked_zero()) else 0
+        if self.use_material_code and self._material_table():
+            args.code, args.mat, args.metric2d = (_lib.ptr(self._code), _lib.ptr(self._mat),
+                                                  _lib.ptr(self._metric2d))
+            args.nmat = self._nmat
+        if args.jmode == _lib.WF_J_FULL and not args.conductive:
+            raise RuntimeError("internal: fused path without a J mode")
+        if args.conductive:
+            args.jmode = _lib.WF_J_FULL
+        return True
+
+    def _finish_run(self, fused, swapped, kept=False):
+        """Bookkeeping after a batch: buffer swap and the invariants it keeps."""
+        self.last_path = "fused" if fused else "general"
+        if fused and swapped:
+            self.E.array, self._E2 = self._E2, self.E.array
+            self.H.array, self._H2 = self._H2, self.H.array
+        # the spare buffers now hold E^n, H^(n-1/2) of the last step (fused: the
+        # ping-pong inputs; exact path: the copies made before the step)
+        self._prev_key = (self.E.version, self.H.version) if (fused or kept) else None
+        if self.use_conductivity:
+            self._sigmaE_versions = (self.E.version, self.J.version)
+
     def _run(self, args):
         """Run one wf_step batch on the fast or the exact path."""
-        fused = self._choose_fused(args)
-        if fused:
-            if self._E2 is None:
-                self._E2 = torch.empty_like(self.E.array)
-                self._H2 = torch.empty_like(self.H.array)
-            args.path = 1
-            args.E2, args.H2 = _lib.ptr(self._E2), _lib.ptr(self._H2)
-            args.kz = int(self.fused_kz)
-            args.variant = int(self.fused_variant)
-            args.zero_masked = 1 if (self.skip_masks and self._masked_zero()) else 0
-            if self.use_material_code and self._material_table():
-                args.code, args.mat, args.metric2d = (_lib.ptr(self._code), _lib.ptr(self._mat),
-                                                      _lib.ptr(self._metric2d))
-                args.nmat = self._nmat
-            if args.jmode == _lib.WF_J_FULL and not args.conductive:
-                raise RuntimeError("internal: fused path without a J mode")
-            if args.conductive:
-                args.jmode = _lib.WF_J_FULL
+        fused = self._prepare_fused(args)
         keep = (not fused) and self.keep_previous_fields and args.nsteps == 1
         if keep:
             if self._E2 is None:
@@ -655,15 +681,7 @@ class SolverFIT3D(RoutinesMixin):
             self._H2.copy_(self.H.array)
         _lib.check(_lib.lib().wf_step(self._handle, ctypes.byref(args), _lib.stream_ptr()),
                    "wf_step")
-        self.last_path = "fused" if fused else "general"
-        if fused and args.result_swapped:
-            self.E.array, self._E2 = self._E2, self.E.array
-            self.H.array, self._H2 = self._H2, self.H.array
-        # the spare buffers now hold E^n, H^(n-1/2) of the last step (fused: the
-        # ping-pong inputs; exact path: the copies above)
-        self._prev_key = (self.E.version, self.H.version) if (fused or keep) else None
-        if self.use_conductivity:
-            self._sigmaE_versions = (self.E.version, self.J.version)
+        self._finish_run(fused, bool(args.result_swapped), keep)
 
     def energy(self, parts=False):
         """Discrete FIT energy of the last step, in joules (no reference
