@@ -218,6 +218,67 @@ def batch_sweep(n=64, steps=2000, seed=0, dtype=np.float32):
             "data": "synthetic (random_box_cavities, seed %d)" % seed}
 
 
+def adjoint_cost(dtype=np.float32, dims=(128, 128, 256), insert=(32, 32, 24), steps=4000,
+                 seed=0):
+    """Reverse-mode cost on the differentiable workload: the 50 mm PEC cavity
+    with pipe on `dims`, a dielectric/lossy block of `insert` cells on the axis
+    with per-cell eps_r ~ U[1, 10] and sigma ~ U[0, 2] S/m (seeded), `steps`
+    wakefield steps at the CFL dt (tau clamp off). Times one plain forward
+    (wakesolve), then the adjoint's forward with checkpoints plus its backward,
+    each bracketed by device synchronisation."""
+    import torch
+
+    import wakis_amd as wa
+    from wakis_amd.adjoint import WakeAdjoint
+    from wakis_amd.solids import Box, Cylinder
+    Nx, Ny, Nz = dims
+    x = np.linspace(-30e-3, 30e-3, Nx + 1)
+    y = np.linspace(-30e-3, 30e-3, Ny + 1)
+    z = np.linspace(-80e-3, 80e-3, Nz + 1)
+    cav = (Box(-25e-3, 25e-3, -25e-3, 25e-3, -25e-3, 25e-3) | Cylinder(5e-3)).cell_mask(x, y, z)
+    lo = [(n - m) // 2 for n, m in zip(dims, insert)]
+    region = tuple((a, a + m) for a, m in zip(lo, insert))
+    ins = np.zeros(dims, dtype=bool)
+    ins[region[0][0]:region[0][1], region[1][0]:region[1][1], region[2][0]:region[2][1]] = True
+    rng = np.random.default_rng(seed)
+    er = rng.uniform(1.0, 10.0, insert)
+    sg = rng.uniform(0.0, 2.0, insert)
+
+    def solver():
+        grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"cav": cav, "ins": ins},
+                            stl_materials={"cav": "vacuum", "ins": [2.0, 1.0, 1.0]}, verbose=0)
+        wake = wa.WakeSolver(q=1e-9, sigmaz=10e-3, beta=1.0, skip_cells=10, save=False,
+                             verbose=0)
+        return wa.SolverFIT3D(grid, wake=wake, bc_low=["pec", "pec", "abc"],
+                              bc_high=["pec", "pec", "abc"], use_stl=True, bg="pec",
+                              dtype=dtype, verbose=0, use_sibc=False, tau_clamp=False)
+
+    s = solver()
+    v = s.wake.v
+    wl = steps * float(s.dt) * v - s.wake.ti * v - (s.z.max() - s.z.min())
+    adj = WakeAdjoint(s, region, wl, w_loss=1.0, w_band=1e-13)
+    adj._set_params(er, sg)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.wakesolve(wakelength=wl)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    L = adj.forward(er, sg)
+    torch.cuda.synchronize()
+    t_fc = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ge, gs = adj.backward()
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    return {"workload": "cavity+pipe %dx%dx%d, %dx%dx%d eps_r/sigma block, %d steps, %s"
+                        % (dims + insert + (s.Nt, np.dtype(dtype).name)),
+            "fwd_s": round(t_fwd, 3), "fwd_checkpointed_s": round(t_fc, 3),
+            "bwd_s": round(t_bwd, 3), "adjoint_over_fwd": round((t_fc + t_bwd) / t_fwd, 3),
+            "loss": float(L), "grad_finite": bool(torch.isfinite(ge).all() and
+                                                 torch.isfinite(gs).all())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,6 +295,8 @@ def main():
     ap.add_argument("--kz", type=int, default=0, help="z-planes per workgroup (0 auto)")
     ap.add_argument("--no-wake-check", action="store_true",
                     help="skip the FP32-vs-FP64 WP_z comparison (FP64 then runs steps/3)")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the reverse-mode cost measurement")
     ap.add_argument("--batch", action="store_true",
                     help="only the batched sweep of 64 box cavities (one JSON line)")
     args = ap.parse_args()
@@ -262,6 +325,10 @@ def main():
         r64 = time_gpu(Nx, Ny, Nz, np.float64, k64, args.warmup, args.path, args.variant,
                        args.kz, wake_check=check)
         print(f"[bench] fp64 {_brief(r64)}", file=sys.stderr, flush=True)
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_cost()
+        print(f"[bench] adjoint {adj}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cx, cy, cz = (int(v) for v in args.cpu_grid.split(","))
@@ -310,6 +377,8 @@ def main():
                              "the FP64 oracle)",
                 "rel_l2_fp32_vs_fp64": float(np.linalg.norm(wp32 - wp64) / np.linalg.norm(wp64)),
                 "n_s": int(len(wp64))}
+    if adj is not None:
+        line["adjoint"] = adj
     if cpu is not None:
         line["cpu_baseline"] = {
             "value": round(cpu["gcell_s"], 6), "unit": "Gcell-updates/s", "cores": 1,

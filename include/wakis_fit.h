@@ -218,14 +218,19 @@ int wf_batch_destroy(wf_batch *b);
 /* Reverse mode. No reference counterpart; for gradients of wake figures of
  * merit with respect to per-cell eps_r and sigma. */
 typedef struct wf_adjoint_args {
-  double *lE, *lH, *lJ;   /* device FP64 3N adjoints of E, H, J: in at step n+1, out at n */
+  double *W;              /* device FP64 3N: adjoint of E plus sigma * adjoint of J
+                             (lE + sigma lJ), in at step n+1, out at n               */
+  double *lH;             /* device FP64 3N adjoint of H, in at step n+1, out at n      */
+  double *lJ_region;      /* device FP64 [3][region cells]: adjoint of J on the region
+                             (conductive; in at n+1, out at n)                        */
+  const void *E0;         /* forward E^n (field dtype)                                    */
   const void *E1, *H1;    /* forward E^{n+1}, H^{n+1} (field dtype)                       */
-  const void *J0;         /* forward J^n as step n used it (field dtype)                   */
+  const void *beam_row;   /* beam current of step n on the beam line (Nz values), or NULL */
   const void *a, *b;      /* the step's folded coefficients (wf_step_args.a / .b)          */
   const void *ieps, *sigma;
   const void *metric2d;   /* 2-D metric tables without masks (wf_step_args.metric2d layout) */
   double dt;
-  int32_t conductive;     /* 1: J = sigma*E is a state (lJ used)                           */
+  int32_t conductive;     /* 1: J = sigma*E is a state (lJ_region used)                    */
   int32_t ixs, iys;       /* beam line (J there is a source); -1 if none                   */
   int32_t r0[3], r1[3];   /* gradient region [r0, r1) in cells                             */
   double *g_ieps;         /* device FP64 [3][region cells] (x fastest), accumulated, or NULL */
@@ -240,7 +245,8 @@ typedef struct wf_adjoint_args {
  * the adjoints of (E, H, J)^{n+1} to those of (E, H, J)^n with the transposed
  * stencils (M1^T, M2^T, same masks and flat-index terms as the forward), adds
  * the capture gradient of sample n, and accumulates d/d ieps and d/d sigma of
- * the cells in the region. Exact-path semantics (beam line J is a source). */
+ * the cells in the region. The J of step n is taken as wakesolve's: beam_row
+ * on the beam line, fl(sigma*E^n) elsewhere (0 when not conductive). */
 int wf_adjoint_step(wf_handle *h, const wf_adjoint_args *a, void *stream);
 
 /* Transpose of wf_wake_wp's line integral without the line-to-line carry:
@@ -257,6 +263,14 @@ int wf_wake_wp_adjoint(const double *gI, int32_t nlines, int64_t cap_samples, in
  * compensated FP64 sums. */
 int wf_idft_real(const double *w, int64_t nbins, int64_t N, int64_t len, double *out,
                  void *stream);
+
+/* One FieldMonitor.update (field_monitors.py:59-102): for the 3N values of E
+ * (field dtype) and each of nfreq monitored frequencies,
+ * acc[c][f][n] += E_c[n] * phase[f], phase given as device [nfreq][2] (re, im)
+ * = exp(-2 pi i f t) evaluated by the caller; acc is device complex128 laid
+ * out [3][nfreq][N][2]. */
+int wf_field_monitor_update(const void *E, int32_t dtype, int64_t N, const double *phase,
+                            int32_t nfreq, double *acc, void *stream);
 
 #ifdef __cplusplus
 }

@@ -7,12 +7,13 @@ wakelength 0.5 m (2062 steps), skip_cells = 10, impedance up to 5 GHz.
 Fields and all wake arithmetic in FP64 through the oracle (oracle/), the
 CPU restatement of the reference's scipy path. Writes
 tests/golden/config1_fp64_oracle.npz: s, WP_3d, WPx, WPy, f, Z, Zx, Zy,
-lambdas, and a stride-97 sample plus seeded weighted sums of the final
-E, H, J.
+lambdas, and a stride-97 sample plus exactly rounded seeded weighted sums
+of the final E, H, J.
 
 Usage: python tests/golden/make_config1_golden.py   (about two minutes)
 """
 
+import math
 import sys
 import time
 from pathlib import Path
@@ -33,8 +34,10 @@ STRIDE = 97
 
 
 def field_digest(F, seed):
+    """Stride-97 sample and the exactly rounded sum of F * w (math.fsum), a
+    weighted checksum independent of BLAS threading."""
     w = np.random.default_rng(seed).standard_normal(F.size)
-    return F[::STRIDE].copy(), float(np.dot(F, w))
+    return F[::STRIDE].copy(), math.fsum((F * w).tolist())
 
 
 def main():

@@ -6,14 +6,15 @@ Beam/PlaneWave sources and analytic solids. The compute path is the HIP
 library libwakis_fit.so (include/wakis_fit.h); there is no CPU fallback.
 """
 
-from . import solids, sources
+from . import field_monitors, solids, sources
 from ._version import __version__
 from .field import Field
+from .field_monitors import FieldMonitor
 from .gridFIT3D import GridFIT3D
 from .materials import material_lib
 from .solverFIT3D import SolverFIT3D
 from .sources import Beam, PlaneWave
 from .wakeSolver import WakeSolver
 
-__all__ = ["Field", "GridFIT3D", "SolverFIT3D", "WakeSolver", "Beam", "PlaneWave",
-           "material_lib", "solids", "sources", "__version__"]
+__all__ = ["Field", "FieldMonitor", "GridFIT3D", "SolverFIT3D", "WakeSolver", "Beam",
+           "PlaneWave", "material_lib", "field_monitors", "solids", "sources", "__version__"]

@@ -43,8 +43,8 @@ class WfStepArgs(ctypes.Structure):
 
 class WfAdjointArgs(ctypes.Structure):
     _fields_ = [
-        ("lE", c_void_p), ("lH", c_void_p), ("lJ", c_void_p),
-        ("E1", c_void_p), ("H1", c_void_p), ("J0", c_void_p),
+        ("W", c_void_p), ("lH", c_void_p), ("lJ_region", c_void_p),
+        ("E0", c_void_p), ("E1", c_void_p), ("H1", c_void_p), ("beam_row", c_void_p),
         ("a", c_void_p), ("b", c_void_p), ("ieps", c_void_p), ("sigma", c_void_p),
         ("metric2d", c_void_p), ("dt", c_double), ("conductive", c_int32),
         ("ixs", c_int32), ("iys", c_int32), ("r0", c_int32 * 3), ("r1", c_int32 * 3),
@@ -83,6 +83,8 @@ EXPORTS = {
                                      c_void_p, c_void_p, c_void_p, c_int64, c_double, c_double,
                                      c_double, c_double, c_double, c_int64, c_void_p, c_void_p]),
     "wf_idft_real": (c_int32, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p]),
+    "wf_field_monitor_update": (c_int32, [c_void_p, c_int32, c_int64, c_void_p, c_int32,
+                                          c_void_p, c_void_p]),
 }
 
 _lib = None

@@ -159,53 +159,65 @@ class WakeAdjoint:
         dev = s.device
         gcap = self._grad_capture(self._grad_wp())
         n3 = 3 * s.N
-        lE = torch.zeros(n3, dtype=torch.float64, device=dev)
-        lH = torch.zeros_like(lE)
-        lJ = torch.zeros_like(lE)
+        W = torch.zeros(n3, dtype=torch.float64, device=dev)
+        lH = torch.zeros_like(W)
         nreg = int(np.prod(self.shape))
+        lJr = torch.zeros(3 * nreg, dtype=torch.float64, device=dev)
         g_ieps = torch.zeros(3 * nreg, dtype=torch.float64, device=dev)
         g_sig = torch.zeros_like(g_ieps)
         ix0, iy0, nx, ny = ctx["win"]
         beam = ctx["beam"]
-        ixs, iys = int(beam.ixs), int(beam.iys)
         table = ctx["table"]
         a = _lib.WfAdjointArgs()
-        a.lE, a.lH, a.lJ = _lib.ptr(lE), _lib.ptr(lH), _lib.ptr(lJ)
+        a.W, a.lH, a.lJ_region = _lib.ptr(W), _lib.ptr(lH), _lib.ptr(lJr)
         a.dt = float(s.dt)
         a.conductive = 1
-        a.ixs, a.iys = ixs, iys
+        a.ixs, a.iys = int(beam.ixs), int(beam.iys)
         a.r0[:] = (self.i0, self.j0, self.k0)
         a.r1[:] = (self.i1, self.j1, self.k1)
         a.g_ieps, a.g_sigma = _lib.ptr(g_ieps), _lib.ptr(g_sig)
         a.gcap = _lib.ptr(gcap)
         a.cap_ix0, a.cap_iy0, a.cap_nx, a.cap_ny = ix0, iy0, nx, ny
         a.cap_samples = gcap.shape[1]
-        lib = _lib.lib()
-        N = s.N
-        line = slice(2 * N + ixs + s.Nx * iys, 3 * N, s.Nx * s.Ny)  # J_z on the beam line
+        # the coefficients stay those of the forward (the same solver)
+        a.a, a.b = _lib.ptr(s._a), _lib.ptr(s._b)
+        a.ieps, a.sigma = _lib.ptr(s.ieps.array), _lib.ptr(s.sigma.array)
+        a.metric2d = _lib.ptr(s._metric2d_all)
+        lib, st = _lib.lib(), _lib.stream_ptr()
+        J_final = s.J.array.clone()
+        # one step's forward arguments; each recomputed step reads state i of the
+        # segment buffers and writes state i+1 (fused path), or steps a copy in place
+        step = s._wake_args(ctx, 0, 1)
+        fused = s._prepare_fused(step)
+        Eb = torch.empty((K + 1, n3), dtype=s.E.array.dtype, device=dev)
+        Hb = torch.empty_like(Eb)
+        row_bytes = table.shape[1] * table.element_size()
         for (n0, E0, H0, J0) in reversed(f["ckpt"]):
             k = min(K, Nt - n0)
-            s.E.array.copy_(E0)
-            s.H.array.copy_(H0)
+            Eb[0].copy_(E0)
+            Hb[0].copy_(H0)
             s.J.array.copy_(J0)
-            states = [(E0, H0, J0)]
-            for n in range(n0, n0 + k):
-                s._run(s._wake_args(ctx, n, 1))
-                states.append((s.E.array.clone(), s.H.array.clone(), s.J.array.clone()))
-            # the coefficients stay those of the forward (the same solver)
-            a.a, a.b = _lib.ptr(s._a), _lib.ptr(s._b)
-            a.ieps, a.sigma = _lib.ptr(s.ieps.array), _lib.ptr(s.sigma.array)
-            a.metric2d = _lib.ptr(s._metric2d_all)
-            for n in range(n0 + k - 1, n0 - 1, -1):
-                E1, H1, _ = states[n - n0 + 1]
-                Jn = states[n - n0][2].clone()
-                Jn[line] = table[n]  # the J the step used on the beam line
-                a.E1, a.H1, a.J0 = _lib.ptr(E1), _lib.ptr(H1), _lib.ptr(Jn)
+            for i in range(k):
+                n = n0 + i
+                if fused:
+                    step.E, step.H = _lib.ptr(Eb[i]), _lib.ptr(Hb[i])
+                    step.E2, step.H2 = _lib.ptr(Eb[i + 1]), _lib.ptr(Hb[i + 1])
+                else:
+                    Eb[i + 1].copy_(Eb[i])
+                    Hb[i + 1].copy_(Hb[i])
+                    step.E, step.H = _lib.ptr(Eb[i + 1]), _lib.ptr(Hb[i + 1])
+                step.nsteps = 1
+                step.beam_row0, step.cap_sample0 = n, n + 1
+                _lib.check(lib.wf_step(s._handle, ctypes.byref(step), st), "wf_step")
+            for i in range(k - 1, -1, -1):
+                n = n0 + i
+                a.E0, a.E1, a.H1 = _lib.ptr(Eb[i]), _lib.ptr(Eb[i + 1]), _lib.ptr(Hb[i + 1])
+                a.beam_row = ctypes.c_void_p(table.data_ptr() + n * row_bytes)
                 a.cap_sample = n
-                _lib.check(lib.wf_adjoint_step(s._handle, ctypes.byref(a), _lib.stream_ptr()),
-                           "wf_adjoint_step")
-            del states
+                _lib.check(lib.wf_adjoint_step(s._handle, ctypes.byref(a), st), "wf_adjoint_step")
+        s.J.array.copy_(J_final)
         torch.cuda.synchronize(dev)
+        del Eb, Hb
         sh = self.shape
         # per cell (x fastest): sum the three components; ieps = 1/(eps_r eps0)
         gi = g_ieps.view(3, sh[2], sh[1], sh[0]).sum(0).permute(2, 1, 0)

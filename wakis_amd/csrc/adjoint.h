@@ -5,13 +5,15 @@
 // with M1 = Dpmc iDa iDmu C Dpec Ds (row coefficient a = fl(iA imu) with the PMC
 // row mask) and M2 = iDeps itDa Dpec C^T Dpmc tDs (row coefficient b = fl(ieps
 // itA) with the PEC row mask). On the beam line J is a source, not a state.
-// Reverse step n, from the adjoints (lE, lH, lJ) of step n+1:
-//   z   = lE + sigma lJ                         (J' = sigma E')
-//   lH <- lH + dt M2^T z,   M2^T z = pmc . tL . C (b . z)
-//   lE <- z - dt M1^T lH,   M1^T y = pec . L . C^T (a . y)
-//   lJ <- -dt (1 - beam line) ieps . z          (conductive)
+// Reverse step n, from the adjoints (lE, lH, lJ) of step n+1, carried as
+// W = lE + sigma lJ (what the step's E update sees, since J' = sigma E') and
+// lH, with lJ itself kept only on the gradient region:
+//   lH <- lH + dt M2^T W,   M2^T W = pmc . tL . C (b . W)
+//   lE  = W - dt M1^T lH,   M1^T y = pec . L . C^T (a . y)
+//   lJ  = -dt (1 - beam line) ieps . W          (conductive)
+//   W  <- lE + sigma lJ
 // and the parameter gradients of that step, with E', H' = E^{n+1}, H^{n+1}
-// and J = J^n:
+// and J = J^n (the beam current on the beam line, fl(sigma E^n) elsewhere):
 //   d/d ieps_q : dt z_q (K2 H'_q - J_q),  K2 = itDa Dpec C^T Dpmc tDs (M2 without ieps)
 //   d/d sigma_q: lJ'_q E'_q
 // The C and C^T rows (with the flat-index wrap and truncation of the
@@ -29,19 +31,15 @@ struct AdjAux {
   int64_t cap_samples, cap_sample;
 };
 
-template <typename S>
-__device__ __forceinline__ double adj_z(const double *lE, const double *lJ, const S *sigma,
-                                        int64_t q, int cond) {
-  return cond ? lE[q] + (double)sigma[q] * lJ[q] : lE[q];
-}
-
-// parameter gradients of one step over the region (reads lE, lJ of step n+1)
+// parameter gradients of one step over the region (reads W and the region's lJ
+// of step n+1)
 template <typename S>
 __global__ void __launch_bounds__(256)
-k_adj_grad(GridDev g, MetricDev<S> m, const double *__restrict__ lE,
-           const double *__restrict__ lJ, const S *__restrict__ E1, const S *__restrict__ H1,
-           const S *__restrict__ J0, const S *__restrict__ sigma, const S *__restrict__ metric2d,
-           double dt, AdjAux x, double *__restrict__ g_ieps, double *__restrict__ g_sigma) {
+k_adj_grad(GridDev g, MetricDev<S> m, const double *__restrict__ W,
+           const double *__restrict__ lJr, const S *__restrict__ E0, const S *__restrict__ E1,
+           const S *__restrict__ H1, const S *__restrict__ beam_row, const S *__restrict__ sigma,
+           const S *__restrict__ metric2d, double dt, AdjAux x, double *__restrict__ g_ieps,
+           double *__restrict__ g_sigma) {
   const int rx = x.r1[0] - x.r0[0], ry = x.r1[1] - x.r0[1], rz = x.r1[2] - x.r0[2];
   const int64_t nreg = (int64_t)rx * ry * rz;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -76,18 +74,21 @@ k_adj_grad(GridDev g, MetricDev<S> m, const double *__restrict__ lE,
       }
       k2 = (double)itA[comp][t2[comp]] * acc;
     }
-    const double z = adj_z(lE, lJ, sigma, q, x.conductive);
-    g_ieps[comp * nreg + t] += dt * z * (k2 - (double)J0[q]);
-    if (x.conductive) g_sigma[comp * nreg + t] += lJ[q] * (double)E1[q];
+    // the J step n used: the beam current on the beam line, sigma*E^n elsewhere
+    const bool online = (comp == 2) && (c0.i == x.ixs) && (c0.j == x.iys);
+    double j0 = 0.0;
+    if (online && beam_row) j0 = (double)beam_row[c0.k];
+    else if (x.conductive) j0 = (double)(S)(sigma[q] * E0[q]);
+    g_ieps[comp * nreg + t] += dt * W[q] * (k2 - j0);
+    if (x.conductive) g_sigma[comp * nreg + t] += lJr[comp * nreg + t] * (double)E1[q];
   }
 }
 
-// lH <- lH + dt M2^T (lE + sigma lJ)
+// lH <- lH + dt M2^T W
 template <typename S>
 __global__ void __launch_bounds__(256)
-k_adj_h(GridDev g, MetricDev<S> m, double *__restrict__ lH, const double *__restrict__ lE,
-        const double *__restrict__ lJ, const S *__restrict__ b, const S *__restrict__ sigma,
-        double dt, int cond) {
+k_adj_h(GridDev g, MetricDev<S> m, double *__restrict__ lH, const double *__restrict__ W,
+        const S *__restrict__ b, double dt) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= g.N) return;
   const Cell c0 = cell_of(n, g);
@@ -102,7 +103,7 @@ k_adj_h(GridDev g, MetricDev<S> m, double *__restrict__ lH, const double *__rest
     for (int u = 0; u < 4; ++u) {
       if (!tt[u].ok) continue;
       const int64_t q = tt[u].comp * g.N + flat(tt[u].c, g);
-      const double p = (double)b[q] * adj_z(lE, lJ, sigma, q, cond);
+      const double p = (double)b[q] * W[q];
       acc = tt[u].sign > 0 ? acc + p : acc - p;
     }
     const double tl = (double)len_of(m.tLx, m.tLy, m.tLz, comp, c0);
@@ -110,10 +111,11 @@ k_adj_h(GridDev g, MetricDev<S> m, double *__restrict__ lH, const double *__rest
   }
 }
 
-// lE <- z - dt M1^T lH (+ capture gradient of sample n);  lJ <- -dt (1-B) ieps z
+// lE = W - dt M1^T lH (+ capture gradient of sample n), lJ = -dt (1-B) ieps W,
+// W <- lE + sigma lJ (lJ also stored on the gradient region)
 template <typename S>
 __global__ void __launch_bounds__(256)
-k_adj_e(GridDev g, MetricDev<S> m, double *__restrict__ lE, double *__restrict__ lJ,
+k_adj_e(GridDev g, MetricDev<S> m, double *__restrict__ W, double *__restrict__ lJr,
         const double *__restrict__ lH, const S *__restrict__ a, const S *__restrict__ ieps,
         const S *__restrict__ sigma, double dt, AdjAux x, const double *__restrict__ gcap) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -122,7 +124,7 @@ k_adj_e(GridDev g, MetricDev<S> m, double *__restrict__ lE, double *__restrict__
 #pragma unroll
   for (int comp = 0; comp < 3; ++comp) {
     const int64_t q = comp * g.N + n;
-    const double z = adj_z(lE, lJ, sigma, q, x.conductive);
+    const double z = W[q];
     double m1t = 0.0;
     if (face_mask(g.pecz, comp, c0.i, c0.j, c0.k, g.Nx, g.Ny, g.Nz)) {
       Term tt[4];
@@ -145,10 +147,19 @@ k_adj_e(GridDev g, MetricDev<S> m, double *__restrict__ lE, double *__restrict__
         le += gcap[(l * x.cap_samples + x.cap_sample) * g.Nz + c0.k];
       }
     }
-    lE[q] = le;
     if (x.conductive) {
       const bool online = (comp == 2) && (c0.i == x.ixs) && (c0.j == x.iys);
-      lJ[q] = online ? 0.0 : -dt * (double)ieps[q] * z;
+      const double lj = online ? 0.0 : -dt * (double)ieps[q] * z;
+      W[q] = le + (double)sigma[q] * lj;
+      if (lJr && c0.i >= x.r0[0] && c0.i < x.r1[0] && c0.j >= x.r0[1] && c0.j < x.r1[1] &&
+          c0.k >= x.r0[2] && c0.k < x.r1[2]) {
+        const int rx = x.r1[0] - x.r0[0], ry = x.r1[1] - x.r0[1], rz = x.r1[2] - x.r0[2];
+        const int64_t t = (int64_t)(c0.i - x.r0[0]) +
+                          (int64_t)rx * ((c0.j - x.r0[1]) + (int64_t)ry * (c0.k - x.r0[2]));
+        lJr[(int64_t)comp * rx * ry * rz + t] = lj;
+      }
+    } else {
+      W[q] = le;
     }
   }
 }

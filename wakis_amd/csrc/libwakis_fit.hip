@@ -15,6 +15,7 @@
 #include "wake.h"
 #include "reduce.h"
 #include "adjoint.h"
+#include "monitor.h"
 
 namespace {
 
@@ -754,17 +755,16 @@ int adjoint_step(wf_handle *h, const wf_adjoint_args *a, const wf::MetricDev<S> 
                        (a->r1[2] - a->r0[2]);
   if (nreg > 0 && a->g_ieps) {
     hipLaunchKernelGGL((wf::k_adj_grad<S>), dim3(nblocks(nreg, 256)), dim3(256), 0, st, g, m,
-                       a->lE, a->lJ, (const S *)a->E1, (const S *)a->H1, (const S *)a->J0,
-                       (const S *)a->sigma, (const S *)a->metric2d, a->dt, x, a->g_ieps,
-                       a->g_sigma);
+                       a->W, a->lJ_region, (const S *)a->E0, (const S *)a->E1, (const S *)a->H1,
+                       (const S *)a->beam_row, (const S *)a->sigma, (const S *)a->metric2d,
+                       a->dt, x, a->g_ieps, a->g_sigma);
     WF_LAUNCH_CHECK();
   }
   const unsigned nb = nblocks(g.N, 256);
   hipLaunchKernelGGL((wf::k_adj_h<S>), dim3(nb), dim3(256), 0, st, g, m, a->lH,
-                     (const double *)a->lE, (const double *)a->lJ, (const S *)a->b,
-                     (const S *)a->sigma, a->dt, a->conductive);
+                     (const double *)a->W, (const S *)a->b, a->dt);
   WF_LAUNCH_CHECK();
-  hipLaunchKernelGGL((wf::k_adj_e<S>), dim3(nb), dim3(256), 0, st, g, m, a->lE, a->lJ,
+  hipLaunchKernelGGL((wf::k_adj_e<S>), dim3(nb), dim3(256), 0, st, g, m, a->W, a->lJ_region,
                      (const double *)a->lH, (const S *)a->a, (const S *)a->ieps,
                      (const S *)a->sigma, a->dt, x, a->gcap);
   WF_LAUNCH_CHECK();
@@ -777,10 +777,10 @@ extern "C" {
 
 int wf_adjoint_step(wf_handle *h, const wf_adjoint_args *a, void *stream) {
   if (!h || !a) return fail(WF_EINVAL, "wf_adjoint_step: null argument");
-  if (!a->lE || !a->lH || !a->a || !a->b || !a->ieps)
+  if (!a->W || !a->lH || !a->a || !a->b || !a->ieps)
     return fail(WF_EINVAL, "wf_adjoint_step: null adjoint field or coefficient");
-  if (a->conductive && (!a->lJ || !a->sigma))
-    return fail(WF_EINVAL, "wf_adjoint_step: conductive step needs lJ and sigma");
+  if (a->conductive && !a->sigma)
+    return fail(WF_EINVAL, "wf_adjoint_step: conductive step needs sigma");
   for (int d = 0; d < 3; ++d) {
     const int n = d == 0 ? h->g.Nx : (d == 1 ? h->g.Ny : h->g.Nz);
     if (a->r0[d] < 0 || a->r1[d] > n || a->r0[d] > a->r1[d])
@@ -788,10 +788,11 @@ int wf_adjoint_step(wf_handle *h, const wf_adjoint_args *a, void *stream) {
   }
   const bool region = (a->r1[0] > a->r0[0]) && (a->r1[1] > a->r0[1]) && (a->r1[2] > a->r0[2]);
   if (region && a->g_ieps) {
-    if (!a->E1 || !a->H1 || !a->J0 || !a->metric2d)
-      return fail(WF_EINVAL, "wf_adjoint_step: gradients need E1, H1, J0 and metric2d");
-    if (a->conductive && !a->g_sigma)
-      return fail(WF_EINVAL, "wf_adjoint_step: conductive gradients need g_sigma");
+    if (!a->E0 || !a->E1 || !a->H1 || !a->metric2d)
+      return fail(WF_EINVAL, "wf_adjoint_step: gradients need E0, E1, H1 and metric2d");
+    if (a->conductive && (!a->g_sigma || !a->lJ_region))
+      return fail(WF_EINVAL, "wf_adjoint_step: conductive gradients need g_sigma and "
+                  "lJ_region");
   }
   if (a->gcap && (a->cap_nx < 1 || a->cap_ny < 1 || a->cap_ix0 < 0 || a->cap_iy0 < 0 ||
                   a->cap_ix0 + a->cap_nx > h->g.Nx || a->cap_iy0 + a->cap_ny > h->g.Ny ||
@@ -825,6 +826,28 @@ int wf_idft_real(const double *w, int64_t nbins, int64_t N, int64_t len, double 
   if (len == 0) return WF_OK;
   hipLaunchKernelGGL(wf::k_idft_real, dim3(nblocks(len, 256)), dim3(256), 0, (hipStream_t)stream,
                      w, nbins, N, len, out);
+  WF_LAUNCH_CHECK();
+  return WF_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int wf_field_monitor_update(const void *E, int32_t dtype, int64_t N, const double *phase,
+                            int32_t nfreq, double *acc, void *stream) {
+  if (!E || !phase || !acc) return fail(WF_EINVAL, "wf_field_monitor_update: null argument");
+  if (N < 1 || nfreq < 0) return fail(WF_ERANGE, "wf_field_monitor_update: sizes");
+  if (dtype != WF_F64 && dtype != WF_F32)
+    return fail(WF_EINVAL, "wf_field_monitor_update: dtype must be WF_F64 or WF_F32");
+  if (nfreq == 0) return WF_OK;
+  const dim3 grid(nblocks(3 * N, 256));
+  if (dtype == WF_F64)
+    hipLaunchKernelGGL(wf::k_monitor<double>, grid, dim3(256), 0, (hipStream_t)stream,
+                       (const double *)E, N, phase, nfreq, acc);
+  else
+    hipLaunchKernelGGL(wf::k_monitor<float>, grid, dim3(256), 0, (hipStream_t)stream,
+                       (const float *)E, N, phase, nfreq, acc);
   WF_LAUNCH_CHECK();
   return WF_OK;
 }

@@ -41,6 +41,7 @@ class Field:
         fill = torch.ones if use_ones else torch.zeros
         self.array = fill(3 * self.N, dtype=torch_dtype(self.dtype), device=self.device)
         self.version = 0  # bumped by every write through this API
+        self.xp = torch  # the array module (numpy or cupy in the reference)
 
     # -- component views (field.py:78-116) --------------------------------
     def _block(self, c):
@@ -164,7 +165,7 @@ class Field:
         f = Field.__new__(Field)
         f.Nx, f.Ny, f.Nz, f.N = self.Nx, self.Ny, self.Nz, self.N
         f.dtype = np.dtype(dtype).type if dtype is not None else self.dtype
-        f.on_gpu, f.device, f.version = True, self.device, 0
+        f.on_gpu, f.device, f.version, f.xp = True, self.device, 0, torch
         f.array = array.to(torch_dtype(f.dtype)).contiguous()
         return f
 

@@ -100,6 +100,8 @@ class WakeSolver:
                                "(Ez.h5 input needs h5py, not part of this engine)")
         if self.counter_moving:
             raise NotImplementedError("counter_moving wake integration is not implemented")
+        if self._cap.shape[0] < nlines or self._cap_samples < 1:
+            raise IndexError("no Ez captured on the test window (it leaves the grid)")
         s, dt, zmin = self._s_axis()
         zz = slice(self.skip_cells, -self.skip_cells) if self.skip_cells != 0 else np.s_[:]
         z = np.ascontiguousarray(self.zf[zz], dtype=np.float64)
