@@ -224,8 +224,9 @@ def adjoint_cost(dtype=np.float32, dims=(128, 128, 256), insert=(32, 32, 24), st
     with pipe on `dims`, a dielectric/lossy block of `insert` cells on the axis
     with per-cell eps_r ~ U[1, 10] and sigma ~ U[0, 2] S/m (seeded), `steps`
     wakefield steps at the CFL dt (tau clamp off). Times one plain forward
-    (wakesolve), then the adjoint's forward with checkpoints plus its backward,
-    each bracketed by device synchronisation."""
+    (wakesolve), then the adjoint's forward (keeping the block's trajectory)
+    plus its backward, each bracketed by device synchronisation; then the
+    same gradient with sqrt(Nt) checkpoints and segment recompute."""
     import torch
 
     import wakis_amd as wa
@@ -271,10 +272,22 @@ def adjoint_cost(dtype=np.float32, dims=(128, 128, 256), insert=(32, 32, 24), st
     ge, gs = adj.backward()
     torch.cuda.synchronize()
     t_bwd = time.perf_counter() - t0
+    mode = adj.used_mode
+    adj = None
+    # the same gradient with sqrt(Nt) checkpoints and segment recompute
+    ck = WakeAdjoint(s, region, wl, w_loss=1.0, w_band=1e-13, mode="checkpoint")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Lc = ck.forward(er, sg)
+    gec, gsc = ck.backward()
+    torch.cuda.synchronize()
+    t_ck = time.perf_counter() - t0
     return {"workload": "cavity+pipe %dx%dx%d, %dx%dx%d eps_r/sigma block, %d steps, %s"
                         % (dims + insert + (s.Nt, np.dtype(dtype).name)),
-            "fwd_s": round(t_fwd, 3), "fwd_checkpointed_s": round(t_fc, 3),
+            "mode": mode, "fwd_s": round(t_fwd, 3), "fwd_with_states_s": round(t_fc, 3),
             "bwd_s": round(t_bwd, 3), "adjoint_over_fwd": round((t_fc + t_bwd) / t_fwd, 3),
+            "checkpoint_mode_over_fwd": round(t_ck / t_fwd, 3),
+            "modes_bitwise_equal": bool(Lc == L and torch.equal(ge, gec) and torch.equal(gs, gsc)),
             "loss": float(L), "grad_finite": bool(torch.isfinite(ge).all() and
                                                  torch.isfinite(gs).all())}
 

@@ -114,6 +114,15 @@ typedef struct wf_step_args {
                              the masks folded in (iA_c = 0 on PMC-masked faces,
                              itA_c = 0 on PEC-masked edges). The x-vectorised
                              kernel then applies no masks; results are identical */
+  /* Optional region trajectory (reverse mode): after step s of this call,
+   * E on the cells [traj_lo, traj_hi) and H on [traj_lo - 1, traj_hi), all
+   * three components (box order: x fastest, then y, z, component), are
+   * copied to slot traj_slot0 + s + 1 of traj_E / traj_H, device buffers of
+   * [traj_slots][3 * box cells] in the field dtype. traj_lo >= 1 on every
+   * axis. NULL traj_E: no trajectory. Not available in batches. */
+  void *traj_E, *traj_H;
+  int32_t traj_lo[3], traj_hi[3];
+  int64_t traj_slots, traj_slot0;
 } wf_step_args;
 
 int wf_version(void);
@@ -137,6 +146,13 @@ int wf_step(wf_handle *h, wf_step_args *a, void *stream);
 
 /* Copy the current Ez window into sample `sample` of the capture buffer:
  * wakesolve's save_to_h5(E[xx,yy,zz,'z']) (routines.py:419-425). */
+/* One trajectory slot (the layout of wf_step_args.traj_*) from the fields E,
+ * H; used for slot 0 (the state before the first step). No reference
+ * counterpart. */
+int wf_trajectory_gather(wf_handle *h, const void *E, const void *H, void *traj_E, void *traj_H,
+                         const int32_t *lo, const int32_t *hi, int64_t slots, int64_t slot,
+                         void *stream);
+
 int wf_capture(wf_handle *h, const void *E, void *capture, int32_t ix0, int32_t iy0,
                int32_t nx, int32_t ny, int64_t samples, int64_t sample, void *stream);
 
@@ -239,6 +255,9 @@ typedef struct wf_adjoint_args {
   int32_t cap_ix0, cap_iy0, cap_nx, cap_ny;
   int64_t cap_samples;
   int64_t cap_sample;     /* sample n = E^n, added to lE after the step                   */
+  int32_t traj;           /* 1: E0, E1 are slots n, n+1 of wf_step's traj_E and H1 slot
+                             n+1 of traj_H, with the region as the box (r0 >= 1); 0: full
+                             3N fields                                                     */
 } wf_adjoint_args;
 
 /* Reverse of step n of SolverFIT3D._one_step (solverFIT3D.py:482-503): maps

@@ -27,7 +27,7 @@ REGION = ((6, 10), (6, 10), (17, 23))
 WAKELENGTH = 0.03
 
 
-def _setup(w_loss, w_band):
+def _setup(w_loss, w_band, mode="auto"):
     import wakis_amd as wa
     from wakis_amd.adjoint import WakeAdjoint
     x, y, z, mask = cases.cavity_pipe_geometry(*DIMS)
@@ -40,7 +40,7 @@ def _setup(w_loss, w_band):
     s = wa.SolverFIT3D(grid, wake=wake, bc_low=["pec", "pec", "abc"],
                        bc_high=["pec", "pec", "abc"], use_stl=True, bg="pec",
                        dtype=np.float64, verbose=0, use_sibc=False, tau_clamp=False)
-    return s, WakeAdjoint(s, REGION, WAKELENGTH, w_loss=w_loss, w_band=w_band)
+    return s, WakeAdjoint(s, REGION, WAKELENGTH, w_loss=w_loss, w_band=w_band, mode=mode)
 
 
 def _params(seed=0):
@@ -205,3 +205,37 @@ def test_wake_loss_autograd_function():
     ge, gs = adj.backward()
     assert np.allclose(e.grad.numpy(), ge.cpu().numpy())
     assert np.allclose(g.grad.numpy(), gs.cpu().numpy())
+
+
+def test_trajectory_and_checkpoint_modes_agree_bitwise():
+    """The region trajectory and the checkpoint/recompute schedule feed the
+    reverse step the same forward values: identical gradients."""
+    print("adjoint trajectory vs checkpoint", flush=True)
+    out = {}
+    for mode in ("trajectory", "checkpoint"):
+        s, adj = _setup(1.0, 1e-13, mode=mode)
+        er, sg = _params(11)
+        L = adj.forward(er, sg)
+        assert adj.used_mode == mode
+        out[mode] = (L,) + tuple(g.cpu().numpy() for g in adj.backward())
+    (La, ea, sa), (Lb, eb, sb) = out["trajectory"], out["checkpoint"]
+    assert La == Lb
+    assert np.array_equal(ea, eb) and np.array_equal(sa, sb)
+
+
+def test_trajectory_mode_needs_interior_region():
+    import wakis_amd as wa
+    from wakis_amd.adjoint import WakeAdjoint
+    s, _ = _setup(1.0, 0.0)
+    region = ((0, 4), (6, 10), (17, 23))
+    with pytest.raises(ValueError):
+        WakeAdjoint(s, region, WAKELENGTH, mode="trajectory").forward(*_params_of(region))
+    adj = WakeAdjoint(s, region, WAKELENGTH)  # auto falls back to checkpoints
+    adj.forward(*_params_of(region))
+    assert adj.used_mode == "checkpoint"
+
+
+def _params_of(region, seed=1):
+    rng = np.random.default_rng(seed)
+    shape = tuple(b - a for a, b in region)
+    return rng.uniform(1.0, 4.0, shape), rng.uniform(0.0, 1.0, shape)

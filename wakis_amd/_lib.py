@@ -38,6 +38,9 @@ class WfStepArgs(ctypes.Structure):
         ("E2", c_void_p), ("H2", c_void_p), ("kz", c_int32), ("result_swapped", c_int32),
         ("code", c_void_p), ("mat", c_void_p), ("metric2d", c_void_p), ("nmat", c_int32),
         ("variant", c_int32), ("zero_masked", c_int32),
+        ("traj_E", c_void_p), ("traj_H", c_void_p),
+        ("traj_lo", c_int32 * 3), ("traj_hi", c_int32 * 3),
+        ("traj_slots", c_int64), ("traj_slot0", c_int64),
     ]
 
 
@@ -50,7 +53,7 @@ class WfAdjointArgs(ctypes.Structure):
         ("ixs", c_int32), ("iys", c_int32), ("r0", c_int32 * 3), ("r1", c_int32 * 3),
         ("g_ieps", c_void_p), ("g_sigma", c_void_p), ("gcap", c_void_p),
         ("cap_ix0", c_int32), ("cap_iy0", c_int32), ("cap_nx", c_int32), ("cap_ny", c_int32),
-        ("cap_samples", c_int64), ("cap_sample", c_int64),
+        ("cap_samples", c_int64), ("cap_sample", c_int64), ("traj", c_int32),
     ]
 
 
@@ -79,6 +82,9 @@ EXPORTS = {
     "wf_batch_run": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p]),
     "wf_batch_destroy": (c_int32, [c_void_p]),
     "wf_adjoint_step": (c_int32, [c_void_p, POINTER(WfAdjointArgs), c_void_p]),
+    "wf_trajectory_gather": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                       POINTER(c_int32), POINTER(c_int32), c_int64, c_int64,
+                                       c_void_p]),
     "wf_wake_wp_adjoint": (c_int32, [c_void_p, c_int32, c_int64, c_int32, c_int32, c_int32,
                                      c_void_p, c_void_p, c_void_p, c_int64, c_double, c_double,
                                      c_double, c_double, c_double, c_int64, c_void_p, c_void_p]),

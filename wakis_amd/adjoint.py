@@ -12,10 +12,19 @@ components of a cell share them, as a material fill does).
 The gradient is the discrete adjoint of exactly what the forward computes:
 the wake post-processing is reversed with wf_idft_real (impedance bins) and
 wf_wake_wp_adjoint (line integrals, with the reference's line-to-line carry),
-and the time stepping with wf_adjoint_step (transposed stencils), going back
-over the run in segments: sqrt(Nt)-spaced checkpoints of (E, H, J) are kept
-during the forward pass and each segment is recomputed from its checkpoint
-before it is reversed. ``wake_loss`` wraps it as a torch.autograd.Function.
+and the time stepping with wf_adjoint_step (transposed stencils).
+
+The reverse step is linear in the adjoint fields and its coefficients do not
+depend on the forward fields; only the parameter gradients read them, and only
+on the block (E) and the block plus one cell below it on each axis (H). So the
+forward pass keeps that trajectory of every step (wf_step's traj_* capture,
+a few hundred KB per step for a 32x32x24 block) and the backward pass walks
+the steps back without recomputing anything ("trajectory" mode). When the
+trajectory would not fit in half of the free device memory, or the block
+touches a low face of the grid, sqrt(Nt)-spaced checkpoints of (E, H, J) are
+kept instead and each segment is recomputed from its checkpoint before it is
+reversed ("checkpoint" mode). ``wake_loss`` wraps it as a
+torch.autograd.Function.
 
 The time step must not depend on sigma: build the solver with
 ``tau_clamp=False`` (and a dt for which dt*sigma/(eps_r*eps0) stays well below
@@ -34,8 +43,11 @@ from . import _lib
 
 class WakeAdjoint:
     def __init__(self, solver, region, wakelength, f_band=3e9, w_loss=1.0, w_band=1.0,
-                 segment=None):
-        """``region`` = ((i0, i1), (j0, j1), (k0, k1)) cells, half-open."""
+                 segment=None, mode="auto"):
+        """``region`` = ((i0, i1), (j0, j1), (k0, k1)) cells, half-open.
+        ``mode``: "trajectory", "checkpoint" or "auto" (trajectory when it fits)."""
+        if mode not in ("auto", "trajectory", "checkpoint"):
+            raise ValueError("WakeAdjoint: mode must be 'auto', 'trajectory' or 'checkpoint'")
         s = solver
         if s.wake is None:
             raise ValueError("WakeAdjoint: the solver needs a WakeSolver")
@@ -51,7 +63,11 @@ class WakeAdjoint:
         self.wakelength = wakelength
         self.f_band, self.w_loss, self.w_band = f_band, w_loss, w_band
         self.segment = segment
+        self.mode = mode
         self._fwd = None
+        e = self.shape
+        self._nE = e[0] * e[1] * e[2]
+        self._nH = (e[0] + 1) * (e[1] + 1) * (e[2] + 1)
 
     # ------------------------------------------------------------- forward
     def _set_params(self, eps_r, sigma):
@@ -67,23 +83,66 @@ class WakeAdjoint:
         s.reset_fields()
         s.step_0 = True
 
+    def _use_trajectory(self, Nt):
+        if self.mode == "checkpoint":
+            return False
+        inside = min(self.i0, self.j0, self.k0) >= 1
+        if self.mode == "trajectory":
+            if not inside:
+                raise ValueError("WakeAdjoint: trajectory mode needs the region to start at "
+                                 "cell 1 or later on every axis")
+            return True
+        need = (Nt + 1) * 3 * (self._nE + self._nH) * self.s.E.array.element_size()
+        free, _ = torch.cuda.mem_get_info(self.s.device)
+        return inside and need <= free // 2
+
+    def _lohi(self):
+        lo = (ctypes.c_int32 * 3)(self.i0, self.j0, self.k0)
+        hi = (ctypes.c_int32 * 3)(self.i1, self.j1, self.k1)
+        return lo, hi
+
     def forward(self, eps_r, sigma):
         """Run the wakefield simulation with these parameters; return L."""
         s = self.s
+        self._fwd = None  # release the previous run's trajectory or checkpoints
         self._set_params(eps_r, sigma)
         ctx = s._wake_setup(self.wakelength, "both")
         Nt = ctx["Nt"]
-        K = self.segment or max(1, int(math.ceil(math.sqrt(max(Nt, 1)))))
-        ckpt = []
-        for n0 in range(0, Nt, K):
-            ckpt.append((n0, s.E.array.clone(), s.H.array.clone(), s.J.array.clone()))
-            k = min(K, Nt - n0)
-            s._run(s._wake_args(ctx, n0, k))
+        if self._use_trajectory(Nt):
+            dev, dt_ = s.device, s.E.array.dtype
+            tE = torch.empty((Nt + 1, 3 * self._nE), dtype=dt_, device=dev)
+            tH = torch.empty((Nt + 1, 3 * self._nH), dtype=dt_, device=dev)
+            lo, hi = self._lohi()
+            _lib.check(_lib.lib().wf_trajectory_gather(
+                s._handle, _lib.ptr(s.E.array), _lib.ptr(s.H.array), _lib.ptr(tE), _lib.ptr(tH),
+                lo, hi, Nt + 1, 0, _lib.stream_ptr()), "wf_trajectory_gather")
+            if Nt > 0:
+                args = s._wake_args(ctx, 0, Nt)
+                args.traj_E, args.traj_H = _lib.ptr(tE), _lib.ptr(tH)
+                args.traj_lo[:] = tuple(lo)
+                args.traj_hi[:] = tuple(hi)
+                args.traj_slots, args.traj_slot0 = Nt + 1, 0
+                s._run(args)
+            fwd = dict(ctx=ctx, Nt=Nt, traj=(tE, tH))
+        else:
+            K = self.segment or max(1, int(math.ceil(math.sqrt(max(Nt, 1)))))
+            ckpt = []
+            for n0 in range(0, Nt, K):
+                ckpt.append((n0, s.E.array.clone(), s.H.array.clone(), s.J.array.clone()))
+                k = min(K, Nt - n0)
+                s._run(s._wake_args(ctx, n0, k))
+            fwd = dict(ctx=ctx, ckpt=ckpt, K=K, Nt=Nt)
         torch.cuda.synchronize(s.device)
         s._wake_finish(ctx, "both")
-        w = s.wake
-        self._fwd = dict(ctx=ctx, ckpt=ckpt, K=K, Nt=Nt)
+        self._fwd = fwd
         return self._loss_value()
+
+    @property
+    def used_mode(self):
+        """"trajectory" or "checkpoint": how the last forward kept its states."""
+        if self._fwd is None:
+            return None
+        return "trajectory" if "traj" in self._fwd else "checkpoint"
 
     def _band(self):
         w = self.s.wake
@@ -155,7 +214,7 @@ class WakeAdjoint:
             raise RuntimeError("WakeAdjoint.backward: run forward first")
         s = self.s
         f = self._fwd
-        ctx, K, Nt = f["ctx"], f["K"], f["Nt"]
+        ctx, Nt = f["ctx"], f["Nt"]
         dev = s.device
         gcap = self._grad_capture(self._grad_wp())
         n3 = 3 * s.N
@@ -184,6 +243,22 @@ class WakeAdjoint:
         a.ieps, a.sigma = _lib.ptr(s.ieps.array), _lib.ptr(s.sigma.array)
         a.metric2d = _lib.ptr(s._metric2d_all)
         lib, st = _lib.lib(), _lib.stream_ptr()
+        row_bytes = table.shape[1] * table.element_size()
+        if "traj" in f:
+            tE, tH = f["traj"]
+            a.traj = 1
+            pE, pH = tE.data_ptr(), tH.data_ptr()
+            sE, sH = tE.stride(0) * tE.element_size(), tH.stride(0) * tH.element_size()
+            pT = table.data_ptr()
+            for n in range(Nt - 1, -1, -1):
+                a.E0, a.E1 = pE + n * sE, pE + (n + 1) * sE
+                a.H1 = pH + (n + 1) * sH
+                a.beam_row = pT + n * row_bytes
+                a.cap_sample = n
+                _lib.check(lib.wf_adjoint_step(s._handle, ctypes.byref(a), st), "wf_adjoint_step")
+            torch.cuda.synchronize(dev)
+            return self._finish_grad(g_ieps, g_sig)
+        K = f["K"]
         J_final = s.J.array.clone()
         # one step's forward arguments; each recomputed step reads state i of the
         # segment buffers and writes state i+1 (fused path), or steps a copy in place
@@ -191,7 +266,6 @@ class WakeAdjoint:
         fused = s._prepare_fused(step)
         Eb = torch.empty((K + 1, n3), dtype=s.E.array.dtype, device=dev)
         Hb = torch.empty_like(Eb)
-        row_bytes = table.shape[1] * table.element_size()
         for (n0, E0, H0, J0) in reversed(f["ckpt"]):
             k = min(K, Nt - n0)
             Eb[0].copy_(E0)
@@ -218,6 +292,10 @@ class WakeAdjoint:
         s.J.array.copy_(J_final)
         torch.cuda.synchronize(dev)
         del Eb, Hb
+        return self._finish_grad(g_ieps, g_sig)
+
+    def _finish_grad(self, g_ieps, g_sig):
+        s = self.s
         sh = self.shape
         # per cell (x fastest): sum the three components; ieps = 1/(eps_r eps0)
         gi = g_ieps.view(3, sh[2], sh[1], sh[0]).sum(0).permute(2, 1, 0)

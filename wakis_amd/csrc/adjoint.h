@@ -29,10 +29,58 @@ struct AdjAux {
   int r0[3], r1[3];  // gradient region (cells), half-open
   int cap_ix0, cap_iy0, cap_nx, cap_ny;
   int64_t cap_samples, cap_sample;
+  int traj;  // E0/E1/H1 in the trajectory box layout
 };
 
+// Region trajectory box: E on [lo, lo + e), H on [lo - 1, lo + e) (h = e + 1
+// per axis), three components each, x fastest, then y, z, component.
+struct TrajBox {
+  int lo[3], e[3];
+  int64_t nE, nH;  // cells per component
+};
+
+__host__ __device__ inline TrajBox traj_box(const int *lo, const int *hi) {
+  TrajBox b;
+  for (int d = 0; d < 3; ++d) { b.lo[d] = lo[d]; b.e[d] = hi[d] - lo[d]; }
+  b.nE = (int64_t)b.e[0] * b.e[1] * b.e[2];
+  b.nH = (int64_t)(b.e[0] + 1) * (b.e[1] + 1) * (b.e[2] + 1);
+  return b;
+}
+
+// index of H component comp of cell (i, j, k) in the H box
+__device__ __forceinline__ int64_t traj_h_index(const TrajBox &b, int comp, int i, int j, int k) {
+  const int hx = b.e[0] + 1, hy = b.e[1] + 1;
+  return comp * b.nH + (i - b.lo[0] + 1) + (int64_t)hx * ((j - b.lo[1] + 1) +
+                                                           (int64_t)hy * (k - b.lo[2] + 1));
+}
+
+// one trajectory slot: tE[3 nE] from E, tH[3 nH] from H
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_traj_gather(GridDev g, const S *__restrict__ E, const S *__restrict__ H, TrajBox b,
+              S *__restrict__ tE, S *__restrict__ tH) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < 3 * b.nE) {
+    const int comp = (int)(t / b.nE);
+    const int64_t r = t - comp * b.nE;
+    const int i = b.lo[0] + (int)(r % b.e[0]);
+    const int j = b.lo[1] + (int)((r / b.e[0]) % b.e[1]);
+    const int k = b.lo[2] + (int)(r / ((int64_t)b.e[0] * b.e[1]));
+    tE[t] = E[comp * g.N + i + (int64_t)g.Nx * (j + (int64_t)g.Ny * k)];
+  } else if (t < 3 * (b.nE + b.nH)) {
+    const int64_t u = t - 3 * b.nE;
+    const int comp = (int)(u / b.nH);
+    const int64_t r = u - comp * b.nH;
+    const int hx = b.e[0] + 1, hy = b.e[1] + 1;
+    const int i = b.lo[0] - 1 + (int)(r % hx);
+    const int j = b.lo[1] - 1 + (int)((r / hx) % hy);
+    const int k = b.lo[2] - 1 + (int)(r / ((int64_t)hx * hy));
+    tH[u] = H[comp * g.N + i + (int64_t)g.Nx * (j + (int64_t)g.Ny * k)];
+  }
+}
+
 // parameter gradients of one step over the region (reads W and the region's lJ
-// of step n+1)
+// of step n+1); E0, E1, H1 are full fields, or trajectory slots (x.traj)
 template <typename S>
 __global__ void __launch_bounds__(256)
 k_adj_grad(GridDev g, MetricDev<S> m, const double *__restrict__ W,
@@ -49,6 +97,7 @@ k_adj_grad(GridDev g, MetricDev<S> m, const double *__restrict__ W,
   c0.j = x.r0[1] + (int)((t / rx) % ry);
   c0.k = x.r0[2] + (int)(t / ((int64_t)rx * ry));
   const int64_t n = flat(c0, g);
+  const TrajBox tb = traj_box(x.r0, x.r1);
   const int64_t yz = (int64_t)g.Ny * g.Nz, xz = (int64_t)g.Nx * g.Nz, xy = (int64_t)g.Nx * g.Ny;
   const S *itA[3] = {metric2d + yz + xz + xy, metric2d + 2 * yz + xz + xy,
                      metric2d + 2 * yz + 2 * xz + xy};
@@ -69,18 +118,22 @@ k_adj_grad(GridDev g, MetricDev<S> m, const double *__restrict__ W,
         if (!face_mask(g.pmcz, tt[u].comp, tt[u].c.i, tt[u].c.j, tt[u].c.k, g.Nx, g.Ny, g.Nz))
           continue;
         const double w = (double)len_of(m.tLx, m.tLy, m.tLz, tt[u].comp, tt[u].c);
-        const double p = w * (double)H1[tt[u].comp * g.N + flat(tt[u].c, g)];
+        const int64_t hq = x.traj ? traj_h_index(tb, tt[u].comp, tt[u].c.i, tt[u].c.j,
+                                                 tt[u].c.k)
+                                  : tt[u].comp * g.N + flat(tt[u].c, g);
+        const double p = w * (double)H1[hq];
         acc = tt[u].sign > 0 ? acc + p : acc - p;
       }
       k2 = (double)itA[comp][t2[comp]] * acc;
     }
     // the J step n used: the beam current on the beam line, sigma*E^n elsewhere
     const bool online = (comp == 2) && (c0.i == x.ixs) && (c0.j == x.iys);
+    const int64_t eq = x.traj ? comp * nreg + t : q;
     double j0 = 0.0;
     if (online && beam_row) j0 = (double)beam_row[c0.k];
-    else if (x.conductive) j0 = (double)(S)(sigma[q] * E0[q]);
+    else if (x.conductive) j0 = (double)(S)(sigma[q] * E0[eq]);
     g_ieps[comp * nreg + t] += dt * W[q] * (k2 - j0);
-    if (x.conductive) g_sigma[comp * nreg + t] += lJr[comp * nreg + t] * (double)E1[q];
+    if (x.conductive) g_sigma[comp * nreg + t] += lJr[comp * nreg + t] * (double)E1[eq];
   }
 }
 

@@ -150,6 +150,16 @@ int wf_zero_ghosts(wf_handle *h, void *E, void *H, void *stream) {
 
 namespace {
 
+// trajectory slot `slot` of a->traj_E / traj_H from E, H (after a step)
+template <typename S>
+void traj_store(const wf_handle *h, const wf_step_args *a, const S *E, const S *H, int64_t slot,
+                hipStream_t st) {
+  const wf::TrajBox b = wf::traj_box(a->traj_lo, a->traj_hi);
+  const int64_t tot = 3 * (b.nE + b.nH);
+  hipLaunchKernelGGL((wf::k_traj_gather<S>), dim3(nblocks(tot, 256)), dim3(256), 0, st, h->g,
+                     E, H, b, (S *)a->traj_E + slot * 3 * b.nE, (S *)a->traj_H + slot * 3 * b.nH);
+}
+
 template <typename S, typename Cm>
 int step_general(wf_handle *h, const wf_step_args *a, const wf::MetricDev<S> &m,
                  hipStream_t st) {
@@ -177,6 +187,10 @@ int step_general(wf_handle *h, const wf_step_args *a, const wf::MetricDev<S> &m,
     else WF_E_LAUNCH(0, false);
 #undef WF_E_LAUNCH
     WF_LAUNCH_CHECK();
+    if (a->traj_E) {
+      traj_store<S>(h, a, (const S *)a->E, (const S *)a->H, a->traj_slot0 + s + 1, st);
+      WF_LAUNCH_CHECK();
+    }
   }
   return WF_OK;
 }
@@ -258,6 +272,10 @@ int step_fused4(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStr
 #undef WF_G_LAUNCH3
 #undef WF_G_LAUNCH4
     WF_LAUNCH_CHECK();
+    if (a->traj_E) {
+      traj_store<S>(h, a, Ebuf[1 - cur], Hbuf[1 - cur], a->traj_slot0 + s + 1, st);
+      WF_LAUNCH_CHECK();
+    }
     cur = 1 - cur;
   }
   a->result_swapped = cur;
@@ -341,6 +359,10 @@ int step_fused(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStre
 #undef WF_F_LAUNCH2
 #undef WF_F_LAUNCH3
     WF_LAUNCH_CHECK();
+    if (a->traj_E) {
+      traj_store<S>(h, a, Ebuf[1 - cur], Hbuf[1 - cur], a->traj_slot0 + s + 1, st);
+      WF_LAUNCH_CHECK();
+    }
     cur = 1 - cur;
   }
   a->result_swapped = cur;
@@ -348,6 +370,17 @@ int step_fused(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStre
 }
 
 // Argument checks shared by wf_step and wf_batch_create.
+// trajectory box check: NULL if fine, else the reason
+const char *check_traj(const wf_handle *h, const void *tE, const void *tH, const int32_t *lo,
+                       const int32_t *hi) {
+  if (!tE || !tH) return "needs both traj_E and traj_H";
+  const int n[3] = {h->g.Nx, h->g.Ny, h->g.Nz};
+  for (int d = 0; d < 3; ++d)
+    if (lo[d] < 1 || hi[d] > n[d] || lo[d] >= hi[d])
+      return "box must satisfy 1 <= lo < hi <= grid size on every axis";
+  return nullptr;
+}
+
 int check_step_args(const wf_handle *h, const wf_step_args *a, const char *who) {
   if (!a->E || !a->H || !a->a || !a->b) return fail(WF_EINVAL, "%s: null field/coefficient", who);
   if (a->nsteps < 0) return fail(WF_EINVAL, "%s: nsteps < 0", who);
@@ -372,6 +405,14 @@ int check_step_args(const wf_handle *h, const wf_step_args *a, const char *who) 
       return fail(WF_ERANGE, "%s: capture window outside the grid", who);
     if (a->cap_sample0 < 0 || a->cap_samples < 1)
       return fail(WF_ERANGE, "%s: bad capture sample range", who);
+  }
+  if (a->traj_E) {
+    const char *why = check_traj(h, a->traj_E, a->traj_H, a->traj_lo, a->traj_hi);
+    if (why) return fail(WF_ERANGE, "%s: trajectory %s", who, why);
+    if (a->traj_slot0 < 0 || a->traj_slot0 + a->nsteps + 1 > a->traj_slots)
+      return fail(WF_ERANGE, "%s: trajectory slots [%lld,%lld] outside a buffer of %lld", who,
+                  (long long)(a->traj_slot0 + 1), (long long)(a->traj_slot0 + a->nsteps),
+                  (long long)a->traj_slots);
   }
   if (a->path == 1) {
     if (!a->E2 || !a->H2 || a->E2 == a->E || a->H2 == a->H || a->E2 == a->H2)
@@ -409,6 +450,24 @@ int wf_step(wf_handle *h, wf_step_args *a, void *stream) {
   if (h->dtype == WF_F64) return step_general<double, double>(h, a, h->md, st);
   if (a->compute_f64) return step_general<float, double>(h, a, h->mf, st);
   return step_general<float, float>(h, a, h->mf, st);
+}
+
+int wf_trajectory_gather(wf_handle *h, const void *E, const void *H, void *traj_E, void *traj_H,
+                         const int32_t *lo, const int32_t *hi, int64_t slots, int64_t slot,
+                         void *stream) {
+  if (!h || !E || !H || !lo || !hi) return fail(WF_EINVAL, "wf_trajectory_gather: null argument");
+  const char *why = check_traj(h, traj_E, traj_H, lo, hi);
+  if (why) return fail(WF_ERANGE, "wf_trajectory_gather: %s", why);
+  if (slot < 0 || slot >= slots) return fail(WF_ERANGE, "wf_trajectory_gather: slot out of range");
+  wf_step_args a;
+  memset(&a, 0, sizeof(a));
+  a.traj_E = traj_E; a.traj_H = traj_H;
+  for (int d = 0; d < 3; ++d) { a.traj_lo[d] = lo[d]; a.traj_hi[d] = hi[d]; }
+  hipStream_t st = (hipStream_t)stream;
+  if (h->dtype == WF_F64) traj_store<double>(h, &a, (const double *)E, (const double *)H, slot, st);
+  else traj_store<float>(h, &a, (const float *)E, (const float *)H, slot, st);
+  WF_LAUNCH_CHECK();
+  return WF_OK;
 }
 
 int wf_capture(wf_handle *h, const void *E, void *capture, int32_t ix0, int32_t iy0, int32_t nx,
@@ -668,6 +727,7 @@ int wf_batch_create(wf_handle *const *handles, const wf_step_args *args, int32_t
     const int rc = check_step_args(h, a, who);
     if (rc != WF_OK) return rc;
     if (a->path != 1) return fail(WF_EINVAL, "%s: batch items use the fused path (path 1)", who);
+    if (a->traj_E) return fail(WF_EINVAL, "%s: batch items take no trajectory", who);
     if (!a->code || !a->zero_masked)
       return fail(WF_EINVAL, "%s: batch items need material codes and zero_masked = 1", who);
     if (h->dtype != dtype || a->conductive != a0.conductive || a->jmode != a0.jmode ||
@@ -751,6 +811,7 @@ int adjoint_step(wf_handle *h, const wf_adjoint_args *a, const wf::MetricDev<S> 
   for (int d = 0; d < 3; ++d) { x.r0[d] = a->r0[d]; x.r1[d] = a->r1[d]; }
   x.cap_ix0 = a->cap_ix0; x.cap_iy0 = a->cap_iy0; x.cap_nx = a->cap_nx; x.cap_ny = a->cap_ny;
   x.cap_samples = a->cap_samples; x.cap_sample = a->cap_sample;
+  x.traj = a->traj;
   const int64_t nreg = (int64_t)(a->r1[0] - a->r0[0]) * (a->r1[1] - a->r0[1]) *
                        (a->r1[2] - a->r0[2]);
   if (nreg > 0 && a->g_ieps) {
@@ -790,6 +851,8 @@ int wf_adjoint_step(wf_handle *h, const wf_adjoint_args *a, void *stream) {
   if (region && a->g_ieps) {
     if (!a->E0 || !a->E1 || !a->H1 || !a->metric2d)
       return fail(WF_EINVAL, "wf_adjoint_step: gradients need E0, E1, H1 and metric2d");
+    if (a->traj && (a->r0[0] < 1 || a->r0[1] < 1 || a->r0[2] < 1))
+      return fail(WF_ERANGE, "wf_adjoint_step: trajectory regions start at cell 1 or later");
     if (a->conductive && (!a->g_sigma || !a->lJ_region))
       return fail(WF_EINVAL, "wf_adjoint_step: conductive gradients need g_sigma and "
                   "lJ_region");
