@@ -103,9 +103,9 @@ typedef struct wf_step_args {
                              iA_z[Nx*Ny] (i+Nx*j), then itA_x, itA_y, itA_z likewise
                              (itA after the periodic/ABC copies)                     */
   int32_t nmat;           /* 1..256                                                   */
-  int32_t variant;        /* fused kernel choice: 0 auto (x-vectorised kernel when Nx
-                             is a multiple of 16/sizeof(dtype), with the row count
-                             measured fastest for the dtype), 1 scalar kernel,
+  int32_t variant;        /* fused kernel choice: 0 auto (x-vectorised kernel with the
+                             row count measured fastest: FP64 3; FP32 8, or 7 when
+                             conductive or without material codes), 1 scalar kernel,
                              2, 3, 7 or 8 (FP32 only): x-vectorised kernel with
                              that many rows per workgroup                          */
   int32_t zero_masked;    /* fused path: 1 = E is zero on every edge the PEC mask
@@ -123,6 +123,11 @@ typedef struct wf_step_args {
   void *traj_E, *traj_H;
   int32_t traj_lo[3], traj_hi[3];
   int64_t traj_slots, traj_slot0;
+  int32_t cells_per_lane; /* fused x-vectorised kernel: cells per lane, 0 = automatic
+                             (16 bytes per lane, fewer when the row of Nx cells would
+                             leave lanes idle), 1, 2 (FP32) or 16/sizeof(dtype); 1 and
+                             2 are built with the automatic row count only. Results
+                             are identical for every choice                        */
 } wf_step_args;
 
 int wf_version(void);

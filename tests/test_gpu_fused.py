@@ -50,6 +50,41 @@ def test_fused_matches_oracle(dims, dtype, conductive, kz, use_code, variant):
         assert np.array_equal(got, ref), f"{name}: rel-L2 {gc.rel_l2(got, ref):.3e}"
 
 
+@pytest.mark.parametrize("dims", [(12, 10, 14), (70, 13, 37), (128, 6, 20), (36, 9, 11)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("conductive", [False, True])
+@pytest.mark.parametrize("use_code", [True, False])
+@pytest.mark.parametrize("cpl", [0, 1, 2, 4])
+def test_fused_cells_per_lane_matches_oracle(dims, dtype, conductive, use_code, cpl):
+    """Every cells-per-lane choice of the x-vectorised kernel is bit-identical
+    to the oracle (automatic: the one that leaves the fewest idle lanes)."""
+    print("fused cells/lane", dims, np.dtype(dtype).name, conductive, use_code, cpl, flush=True)
+    case = gc.random_case(PEC_ALL, dtype, conductive, seed=sum(dims) + cpl, dims=dims)
+    case["fields"][2] = np.zeros_like(case["fields"][2])
+    so = gc.oracle_solver(case)
+    sa = gc.amd_solver(case, path="fused")
+    sa.use_material_code = use_code
+    sa.fused_cells_per_lane = cpl
+    if conductive:
+        so.J[:] = 0
+        so.one_step()
+        sa.path = "general"
+        sa.one_step()
+        sa.path = "fused"
+    vn = 16 // np.dtype(dtype).itemsize
+    if cpl not in (0, 1, vn) and not (cpl == 2 and vn == 4) or (cpl and dims[0] % cpl):
+        with pytest.raises(ValueError, match="cells_per_lane"):
+            sa.one_step()
+        return
+    for _ in range(4):
+        so.one_step()
+        sa.one_step()
+        assert sa.last_path == "fused"
+    for name in ("E", "H", "J"):
+        got, ref = getattr(sa, name).to_numpy(), getattr(so, name)
+        assert np.array_equal(got, ref), f"{name}: rel-L2 {gc.rel_l2(got, ref):.3e}"
+
+
 def _zero_masked_values(case):
     """Zero E on PEC-masked edges (and H on PMC-masked faces) of a random case."""
     dims = tuple(len(a) - 1 for a in (case["x"], case["y"], case["z"]))

@@ -40,7 +40,7 @@ class WfStepArgs(ctypes.Structure):
         ("variant", c_int32), ("zero_masked", c_int32),
         ("traj_E", c_void_p), ("traj_H", c_void_p),
         ("traj_lo", c_int32 * 3), ("traj_hi", c_int32 * 3),
-        ("traj_slots", c_int64), ("traj_slot0", c_int64),
+        ("traj_slots", c_int64), ("traj_slot0", c_int64), ("cells_per_lane", c_int32),
     ]
 
 

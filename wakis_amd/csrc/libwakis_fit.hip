@@ -52,6 +52,14 @@ struct wf_handle {
   wf::GridDev g;
   int dtype;
   int device;
+  int ncu;  // compute units of the device
+  struct Occ {
+    const void *kern;
+    int threads;
+    size_t lds;
+    int per_cu;
+  };
+  std::vector<Occ> occ;  // workgroups per CU of the fused kernels launched so far
   void *metric_buf;  // device: 6 vectors in the storage type
   double *red_buf;   // device: per-block partials of the energy reduction
   wf::MetricDev<double> md;
@@ -83,6 +91,10 @@ int wf_create(const wf_grid *gd, wf_handle **out) {
   h->g.m1desc = gd->m1_descending ? 1 : 0;
   h->dtype = gd->dtype;
   hipGetDevice(&h->device);
+  h->ncu = 0;
+  if (hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, h->device) !=
+          hipSuccess || h->ncu < 1)
+    h->ncu = 1;
   const int n[6] = {gd->Nx, gd->Ny, gd->Nz, gd->Nx, gd->Ny, gd->Nz};
   const double *src[6] = {gd->Lx, gd->Ly, gd->Lz, gd->tLx, gd->tLy, gd->tLz};
   const size_t esz = gd->dtype == WF_F64 ? 8 : 4;
@@ -197,7 +209,62 @@ int step_general(wf_handle *h, const wf_step_args *a, const wf::MetricDev<S> &m,
 
 constexpr int FTX = 64, FTY = 4;
 
-template <typename S, int TY>
+template <typename S>
+using Fused4Fn = void (*)(wf::GridDev, wf::MetricDev<S>, const S *, const S *, S *, S *, S *,
+                          wf::Coefs<S>, S, wf::FusedAux, const S *, S *);
+
+// The k_fused4 instantiation for the step's J mode, conductivity, coefficient
+// form, boundary order and masking.
+template <typename S, int TY, int V>
+Fused4Fn<S> pick_fused4(bool lut, bool cond, bool jline, bool desc, bool mk) {
+#define WF_PICK(JM, CO, LU)                                                                 \
+  do {                                                                                      \
+    if (desc) return mk ? &wf::k_fused4<S, JM, CO, LU, true, TY, true, V>                   \
+                        : &wf::k_fused4<S, JM, CO, LU, true, TY, !LU, V>;                   \
+    return mk ? &wf::k_fused4<S, JM, CO, LU, false, TY, true, V>                            \
+              : &wf::k_fused4<S, JM, CO, LU, false, TY, !LU, V>;                            \
+  } while (0)
+  if (lut) {
+    if (cond) WF_PICK(0, true, true);
+    if (jline) WF_PICK(1, false, true);
+    WF_PICK(0, false, true);
+  }
+  if (cond) WF_PICK(0, true, false);
+  if (jline) WF_PICK(1, false, false);
+  WF_PICK(0, false, false);
+#undef WF_PICK
+}
+
+// Workgroups of `kern` that one CU runs at once (cached per handle).
+int fused_occupancy(wf_handle *h, const void *kern, int threads, size_t lds) {
+  for (const auto &e : h->occ)
+    if (e.kern == kern && e.threads == threads && e.lds == lds) return e.per_cu;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  h->occ.push_back({kern, threads, lds, per_cu});
+  return per_cu;
+}
+
+// z-planes per workgroup. About 512 workgroups (long z marches, measured best),
+// KZ in [8, 64]; then, since each workgroup's march is a serial chain whose
+// time hardly depends on how many others run, a last round of workgroups that
+// would fill at most a quarter of the machine is avoided by the smallest
+// longer KZ that needs one round less.
+int auto_kz(int64_t nxy, int Nz, int64_t conc) {
+  const auto clampkz = [](int64_t k) { return (int)(k < 8 ? 8 : (k > 64 ? 64 : k)); };
+  const int kz0 = clampkz((Nz * nxy + 511) / 512);
+  const auto rounds = [&](int kz) { return (nxy * ((Nz + kz - 1) / kz) + conc - 1) / conc; };
+  const int64_t nwg = nxy * ((Nz + kz0 - 1) / kz0);
+  const int64_t r0 = rounds(kz0);
+  if (r0 < 2 || nwg - (r0 - 1) * conc > conc / 4) return kz0;
+  for (int kz = kz0 + 1; kz <= 64; ++kz)
+    if (rounds(kz) < r0) return kz;
+  return kz0;
+}
+
+template <typename S, int TY, int V>
 int step_fused4(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStream_t st) {
   const wf::GridDev &g = h->g;
   wf::FusedAux x;
@@ -205,15 +272,20 @@ int step_fused4(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStr
   x.iys = (x.ixs >= 0) ? a->iys : -1;
   x.cap_ix0 = a->cap_ix0; x.cap_iy0 = a->cap_iy0; x.cap_nx = a->cap_nx; x.cap_ny = a->cap_ny;
   x.cap_samples = a->cap_samples;
-  constexpr int Wc = 64 * (16 / (int)sizeof(S));
+  constexpr int Wc = 64 * V;
   x.ntx = (g.Nx + Wc - 1) / Wc;
   x.nty = (g.Ny + TY - 1) / TY;
+  const bool lut = a->code && a->mat && a->nmat > 0 && a->metric2d;
+  const size_t lds = lut ? (size_t)a->nmat * 9 * sizeof(S) : 0;
+  const int threads = 64 * (TY + 1);
+  const Fused4Fn<S> kern = pick_fused4<S, TY, V>(lut, a->conductive != 0,
+                                                  a->jmode == WF_J_LINE, g.m1desc != 0,
+                                                  !(lut && a->zero_masked));
   if (a->kz > 0) {
     x.KZ = a->kz;
-  } else {  // about 512 workgroups (two per CU): long z marches, measured best
-    const int64_t nxy = (int64_t)x.ntx * x.nty;
-    int64_t kz = (g.Nz * nxy + 511) / 512;
-    x.KZ = (int)(kz < 8 ? 8 : (kz > 64 ? 64 : kz));
+  } else {
+    const int64_t conc = (int64_t)fused_occupancy(h, (const void *)kern, threads, lds) * h->ncu;
+    x.KZ = auto_kz((int64_t)x.ntx * x.nty, g.Nz, conc);
   }
   x.ntz = (g.Nz + x.KZ - 1) / x.KZ;
   const int64_t nb = (int64_t)x.ntx * x.nty * x.ntz;
@@ -225,52 +297,19 @@ int step_fused4(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStr
   cf.a = (const S *)a->a; cf.b = (const S *)a->b;
   cf.ieps = (const S *)a->ieps; cf.sigma = (const S *)a->sigma;
   cf.code = (const uint8_t *)a->code; cf.mat = (const S *)a->mat; cf.nmat = a->nmat;
-  const bool lut = a->code && a->mat && a->nmat > 0 && a->metric2d;
   if (lut) {
     const S *t = (const S *)a->metric2d;
     const int64_t yz = (int64_t)g.Ny * g.Nz, xz = (int64_t)g.Nx * g.Nz, xy = (int64_t)g.Nx * g.Ny;
     cf.iAx = t; cf.iAy = t + yz; cf.iAz = t + yz + xz;
     cf.itAx = t + yz + xz + xy; cf.itAy = cf.itAx + yz; cf.itAz = cf.itAy + xz;
   }
-  const size_t lds = lut ? (size_t)a->nmat * 9 * sizeof(S) : 0;
   int cur = 0;
   for (int s = 0; s < a->nsteps; ++s) {
     const S *row = a->beam_table ? (const S *)a->beam_table + (a->beam_row0 + s) * g.Nz : nullptr;
     x.cap_sample = a->cap_sample0 + s;
     S *cap = (S *)a->capture;
-#define WF_G_LAUNCH4(I, JM, CO, LU, DE, MK)                                                   \
-  hipLaunchKernelGGL((wf::k_fused4<S, JM, CO, LU, DE, TY, MK>), dim3((unsigned)nb),           \
-                     dim3(64 * (TY + 1)), lds, st, g, m, Ebuf[cur], Hbuf[cur], Ebuf[1 - cur], \
-                     Hbuf[1 - cur], (S *)a->J, cf, dt, x, row, cap)
-#define WF_G_LAUNCH3(I, JM, CO, LU, DE)                                                       \
-  do {                                                                                        \
-    if constexpr (LU) {                                                                       \
-      if (a->zero_masked) WF_G_LAUNCH4(I, JM, CO, LU, DE, false);                             \
-      else WF_G_LAUNCH4(I, JM, CO, LU, DE, true);                                             \
-    } else {                                                                                  \
-      WF_G_LAUNCH4(I, JM, CO, LU, DE, true);                                                  \
-    }                                                                                         \
-  } while (0)
-#define WF_G_LAUNCH2(I, JM, CO, LU)                                                          \
-  if (g.m1desc) WF_G_LAUNCH3(I, JM, CO, LU, true);                                           \
-  else WF_G_LAUNCH3(I, JM, CO, LU, false)
-#define WF_G_LAUNCH(JM, CO, LU)                                                              \
-  do {                                                                                       \
-    WF_G_LAUNCH2(uint32_t, JM, CO, LU);                                                      \
-  } while (0)
-    if (lut) {
-      if (a->conductive) WF_G_LAUNCH(0, true, true);
-      else if (a->jmode == WF_J_LINE) WF_G_LAUNCH(1, false, true);
-      else WF_G_LAUNCH(0, false, true);
-    } else {
-      if (a->conductive) WF_G_LAUNCH(0, true, false);
-      else if (a->jmode == WF_J_LINE) WF_G_LAUNCH(1, false, false);
-      else WF_G_LAUNCH(0, false, false);
-    }
-#undef WF_G_LAUNCH
-#undef WF_G_LAUNCH2
-#undef WF_G_LAUNCH3
-#undef WF_G_LAUNCH4
+    hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(threads), lds, st, g, m, Ebuf[cur],
+                       Hbuf[cur], Ebuf[1 - cur], Hbuf[1 - cur], (S *)a->J, cf, dt, x, row, cap);
     WF_LAUNCH_CHECK();
     if (a->traj_E) {
       traj_store<S>(h, a, Ebuf[1 - cur], Hbuf[1 - cur], a->traj_slot0 + s + 1, st);
@@ -282,26 +321,71 @@ int step_fused4(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStr
   return WF_OK;
 }
 
+// Cells per lane for a row of Nx cells: the largest of vn, vn/2, ..., 1 that
+// divides Nx and leaves the fewest idle lanes in the row's 64-lane segments.
+inline int auto_cells_per_lane(int Nx, int vn) {
+  int best = 1;
+  double beff = -1.0;
+  for (int v = vn; v >= 1; v >>= 1) {
+    if (Nx % v) continue;
+    const int64_t seg = 64 * (int64_t)v;
+    const double eff = (double)Nx / (double)(((Nx + seg - 1) / seg) * seg);
+    if (eff > beff + 1e-12) { beff = eff; best = v; }
+  }
+  return best;
+}
+
 template <typename S, typename Cm>
 int step_fused(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStream_t st) {
   const wf::GridDev &g = h->g;
   // x-vectorised kernel: Nx a multiple of the lane width (16 bytes) and every
   // 3N array within 4 GiB
   const bool idx32v = (uint64_t)3 * (uint64_t)g.N * sizeof(S) < (1ull << 32);
-  if (g.Nx % (int)(16 / sizeof(S)) == 0 && idx32v && a->variant != 1) {
-    // variant = rows per workgroup (0: the measured best for the type)
-    const int rows = a->variant ? a->variant : (sizeof(S) == 4 ? 8 : 3);
-    switch (rows) {
-      case 2: return step_fused4<S, 2>(h, a, m, st);
-      case 3: return step_fused4<S, 3>(h, a, m, st);
-      case 7: return step_fused4<S, 7>(h, a, m, st);
-      case 8:
-        if constexpr (sizeof(S) == 4) return step_fused4<S, 8>(h, a, m, st);
-        break;
-      default: break;
-    }
-    return fail(WF_EINVAL, "wf_step: variant (rows per workgroup) not built for this dtype");
+  constexpr int vn = 16 / (int)sizeof(S);  // cells per lane of a 16-byte access
+  // rows per workgroup measured best: FP64 3; FP32 8, or 7 when the step is
+  // conductive or reads coefficient arrays (8-row tiles then spill registers)
+  const bool lut = a->code && a->mat && a->nmat > 0 && a->metric2d;
+  const int rdef = sizeof(S) == 8 ? 3 : ((lut && !a->conductive) ? 8 : 7);
+  const int rows = a->variant ? a->variant : rdef;
+  // cells per lane: vn for every row count; 1 and 2 (FP32) with the default rows
+  int V = a->cells_per_lane;
+  if (V == 0) {
+    V = vn;
+    if (rows == rdef) V = auto_cells_per_lane(g.Nx, vn);
   }
+  if (idx32v && a->variant != 1 && g.Nx % V == 0) {
+    // variant = rows per workgroup (0: the measured best for the type)
+    if (V == vn) {
+      switch (rows) {
+        case 2: return step_fused4<S, 2, vn>(h, a, m, st);
+        case 3: return step_fused4<S, 3, vn>(h, a, m, st);
+        case 7: return step_fused4<S, 7, vn>(h, a, m, st);
+        case 8:
+          if constexpr (sizeof(S) == 4) return step_fused4<S, 8, vn>(h, a, m, st);
+          break;
+        default: break;
+      }
+      return fail(WF_EINVAL, "wf_step: variant (rows per workgroup) not built for this dtype");
+    }
+    if (rows == rdef) {
+      if constexpr (vn == 4) {
+        if (rows == 8) {
+          if (V == 1) return step_fused4<S, 8, 1>(h, a, m, st);
+          if (V == 2) return step_fused4<S, 8, 2>(h, a, m, st);
+        } else {
+          if (V == 1) return step_fused4<S, 7, 1>(h, a, m, st);
+          if (V == 2) return step_fused4<S, 7, 2>(h, a, m, st);
+        }
+      } else {
+        if (V == 1) return step_fused4<S, 3, 1>(h, a, m, st);
+      }
+    }
+    return fail(WF_EINVAL, "wf_step: cells_per_lane %d not built with %d rows per workgroup",
+                V, rows);
+  }
+  if (a->cells_per_lane && a->variant != 1)
+    return fail(WF_EINVAL, "wf_step: cells_per_lane %d needs Nx a multiple of it and every "
+                "3N array within 4 GiB", a->cells_per_lane);
   wf::FusedAux x;
   x.ixs = (a->jmode == WF_J_LINE || a->beam_table || a->conductive) ? a->ixs : -1;
   x.iys = (x.ixs >= 0) ? a->iys : -1;
@@ -320,7 +404,6 @@ int step_fused(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStre
   cf.a = (const S *)a->a; cf.b = (const S *)a->b;
   cf.ieps = (const S *)a->ieps; cf.sigma = (const S *)a->sigma;
   cf.code = (const uint8_t *)a->code; cf.mat = (const S *)a->mat; cf.nmat = a->nmat;
-  const bool lut = a->code && a->mat && a->nmat > 0 && a->metric2d;
   if (lut) {
     const S *t = (const S *)a->metric2d;
     const int64_t yz = (int64_t)g.Ny * g.Nz, xz = (int64_t)g.Nx * g.Nz, xy = (int64_t)g.Nx * g.Ny;
@@ -422,6 +505,9 @@ int check_step_args(const wf_handle *h, const wf_step_args *a, const char *who) 
     if (a->conductive && (!a->J || !a->ieps || !a->sigma))
       return fail(WF_EINVAL, "%s: fused conductive step needs J, ieps and sigma", who);
     if (a->kz < 0) return fail(WF_EINVAL, "%s: kz < 0", who);
+    if (a->cells_per_lane != 0 && a->cells_per_lane != 1 && a->cells_per_lane != 2 &&
+        a->cells_per_lane != 4)
+      return fail(WF_EINVAL, "%s: cells_per_lane must be 0, 1, 2 or 4", who);
     if (a->code && (!a->mat || !a->metric2d || a->nmat < 1 || a->nmat > 256))
       return fail(WF_EINVAL, "%s: material code needs a table of 1..256 materials "
                   "and the 2-D metric tables", who);

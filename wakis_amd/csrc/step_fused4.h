@@ -485,14 +485,14 @@ __device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S>
   }
 }
 
-template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY, bool MK>
+template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY, bool MK, int V>
 __global__ void __launch_bounds__(64 * (TY + 1))
 k_fused4(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restrict__ Hin,
          S *__restrict__ Eout, S *__restrict__ Hout, S *__restrict__ J, Coefs<S> cf, S dt,
          FusedAux x, const S *__restrict__ beam_row, S *__restrict__ cap) {
   const int tile = xcd_tile(blockIdx.x, x.ntx * x.nty * x.ntz);
-  fused4_tile<S, JM, COND, LUT, DESC, TY, MK, 16 / sizeof(S)>(g, m, Ein, Hin, Eout, Hout, J,
-                                                             cf, dt, x, beam_row, cap, tile);
+  fused4_tile<S, JM, COND, LUT, DESC, TY, MK, V>(g, m, Ein, Hin, Eout, Hout, J, cf, dt, x,
+                                                 beam_row, cap, tile);
 }
 
 // One grid of a batch: everything a step of it needs except the step number.

@@ -163,6 +163,7 @@ class SolverFIT3D(RoutinesMixin):
         self._E2 = self._H2 = None
         self.fused_kz = 0  # z-planes per workgroup of the fused kernel (0: automatic)
         self.fused_variant = 0  # 0 auto, 1 scalar, 2/3/7/8: vectorised, rows per workgroup
+        self.fused_cells_per_lane = 0  # x-vectorised kernel: 0 auto, 1, 2 (FP32), 16/itemsize
         # energy(): keep E^n and H^(n-1/2) on the exact path too (one copy per step)
         self.keep_previous_fields = False
         self._prev_key = None
@@ -646,6 +647,7 @@ class SolverFIT3D(RoutinesMixin):
         args.E2, args.H2 = _lib.ptr(self._E2), _lib.ptr(self._H2)
         args.kz = int(self.fused_kz)
         args.variant = int(self.fused_variant)
+        args.cells_per_lane = int(self.fused_cells_per_lane)
         args.zero_masked = 1 if (self.skip_masks and self._masked_zero()) else 0
         if self.use_material_code and self._material_table():
             args.code, args.mat, args.metric2d = (_lib.ptr(self._code), _lib.ptr(self._mat),
