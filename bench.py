@@ -272,7 +272,7 @@ def adjoint_cost(dtype=np.float32, dims=(128, 128, 256), insert=(32, 32, 24), st
     ge, gs = adj.backward()
     torch.cuda.synchronize()
     t_bwd = time.perf_counter() - t0
-    mode = adj.used_mode
+    mode, kernel, adt = adj.used_mode, adj.used_kernel, adj.used_adjoint_dtype
     adj = None
     # the same gradient with sqrt(Nt) checkpoints and segment recompute
     ck = WakeAdjoint(s, region, wl, w_loss=1.0, w_band=1e-13, mode="checkpoint")
@@ -284,7 +284,7 @@ def adjoint_cost(dtype=np.float32, dims=(128, 128, 256), insert=(32, 32, 24), st
     t_ck = time.perf_counter() - t0
     return {"workload": "cavity+pipe %dx%dx%d, %dx%dx%d eps_r/sigma block, %d steps, %s"
                         % (dims + insert + (s.Nt, np.dtype(dtype).name)),
-            "mode": mode, "fwd_s": round(t_fwd, 3), "fwd_with_states_s": round(t_fc, 3),
+            "mode": mode, "kernel": kernel, "adjoint_fields": adt, "fwd_s": round(t_fwd, 3), "fwd_with_states_s": round(t_fc, 3),
             "bwd_s": round(t_bwd, 3), "adjoint_over_fwd": round((t_fc + t_bwd) / t_fwd, 3),
             "checkpoint_mode_over_fwd": round(t_ck / t_fwd, 3),
             "modes_bitwise_equal": bool(Lc == L and torch.equal(ge, gec) and torch.equal(gs, gsc)),
@@ -338,10 +338,12 @@ def main():
         r64 = time_gpu(Nx, Ny, Nz, np.float64, k64, args.warmup, args.path, args.variant,
                        args.kz, wake_check=check)
         print(f"[bench] fp64 {_brief(r64)}", file=sys.stderr, flush=True)
-    adj = None
+    adj = adj64 = None
     if not args.no_adjoint:
         adj = adjoint_cost()
         print(f"[bench] adjoint {adj}", file=sys.stderr, flush=True)
+        adj64 = adjoint_cost(np.float64)
+        print(f"[bench] adjoint fp64 {adj64}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cx, cy, cz = (int(v) for v in args.cpu_grid.split(","))
@@ -392,6 +394,7 @@ def main():
                 "n_s": int(len(wp64))}
     if adj is not None:
         line["adjoint"] = adj
+        line["adjoint_fp64"] = adj64
     if cpu is not None:
         line["cpu_baseline"] = {
             "value": round(cpu["gcell_s"], 6), "unit": "Gcell-updates/s", "cores": 1,

@@ -239,9 +239,10 @@ int wf_batch_destroy(wf_batch *b);
 /* Reverse mode. No reference counterpart; for gradients of wake figures of
  * merit with respect to per-cell eps_r and sigma. */
 typedef struct wf_adjoint_args {
-  double *W;              /* device FP64 3N: adjoint of E plus sigma * adjoint of J
-                             (lE + sigma lJ), in at step n+1, out at n               */
-  double *lH;             /* device FP64 3N adjoint of H, in at step n+1, out at n      */
+  double *W;              /* device 3N (FP64, or FP32 with fields_f32): adjoint of E
+                             plus sigma * adjoint of J (lE + sigma lJ), in at step n+1,
+                             out at n (out in W2 with fused)                          */
+  double *lH;             /* device 3N adjoint of H, same type and timing as W          */
   double *lJ_region;      /* device FP64 [3][region cells]: adjoint of J on the region
                              (conductive; in at n+1, out at n)                        */
   const void *E0;         /* forward E^n (field dtype)                                    */
@@ -263,6 +264,21 @@ typedef struct wf_adjoint_args {
   int32_t traj;           /* 1: E0, E1 are slots n, n+1 of wf_step's traj_E and H1 slot
                              n+1 of traj_H, with the region as the box (r0 >= 1); 0: full
                              3N fields                                                     */
+  int32_t fused;          /* 0: two kernels, W and lH updated in place. 1 (or 5, 13 rows
+                             per workgroup): one single-pass kernel reading W, lH and
+                             writing W2, lH2 (the caller swaps
+                             the pairs for the next step); drops the flat-index wrap terms,
+                             so only for set-ups where wf_step's fused path applies, with
+                             the region at cell 1 or later in x and y                    */
+  double *W2, *lH2;       /* fused: output buffers, 3N each                               */
+  int32_t fields_f32;     /* fused only: W, lH, W2, lH2 are FP32 arrays (stored FP32,
+                             arithmetic FP64); 0: FP64 arrays                            */
+  void *reg_W;            /* fused only, optional: [slots][3][region cells] in the type of
+                             W; slot reg_slot receives the region's W read by this step   */
+  double *reg_lJ;         /* with reg_W: [slots][3][region cells]; slot reg_slot receives
+                             the region's lJ of step n. With reg_W set, g_ieps must be NULL:
+                             the gradient is accumulated afterwards by wf_adjoint_grad    */
+  int64_t reg_slot;
 } wf_adjoint_args;
 
 /* Reverse of step n of SolverFIT3D._one_step (solverFIT3D.py:482-503): maps
@@ -272,6 +288,31 @@ typedef struct wf_adjoint_args {
  * the cells in the region. The J of step n is taken as wakesolve's: beam_row
  * on the beam line, fl(sigma*E^n) elsewhere (0 when not conductive). */
 int wf_adjoint_step(wf_handle *h, const wf_adjoint_args *a, void *stream);
+
+/* Parameter gradients of nsteps reverse steps in one pass (no reference
+ * counterpart), from the stored slots: forward E of steps n0+s and n0+s+1
+ * (E + s*E_stride, E + (s+1)*E_stride), H of step n0+s+1 (H + (s+1)*H_stride),
+ * the region's W of reverse step n0+s (reg_W slot s) and lJ of step n0+s+1
+ * (reg_lJ slot s+1, zeros after the run's last step), and the beam row of step
+ * n0+s (beam + s*Nz). Adds to g_ieps / g_sigma exactly what nsteps per-step
+ * calls with gradients would add. */
+typedef struct wf_adjoint_grad_args {
+  const void *E, *H;      /* field dtype; trajectory boxes (traj = 1) or full 3N fields */
+  int64_t E_stride, H_stride; /* elements per slot                                     */
+  int32_t traj;
+  const void *beam;       /* beam table rows from step n0, or NULL                      */
+  const void *sigma, *metric2d;
+  const void *reg_W;      /* [nsteps][3][region cells], FP32 if fields_f32 else FP64    */
+  const double *reg_lJ;   /* [nsteps + 1][3][region cells]                              */
+  int32_t fields_f32;
+  int32_t nsteps;
+  double dt;
+  int32_t conductive, ixs, iys;
+  int32_t r0[3], r1[3];
+  double *g_ieps, *g_sigma;
+} wf_adjoint_grad_args;
+
+int wf_adjoint_grad(wf_handle *h, const wf_adjoint_grad_args *a, void *stream);
 
 /* Transpose of wf_wake_wp's line integral without the line-to-line carry:
  * gcap[l][m][k0+k] += sum over s indices n with it(n,k) == m of gI[l][n]*dz[k]

@@ -27,7 +27,7 @@ REGION = ((6, 10), (6, 10), (17, 23))
 WAKELENGTH = 0.03
 
 
-def _setup(w_loss, w_band, mode="auto"):
+def _setup(w_loss, w_band, mode="auto", kernel="auto", dtype=np.float64, adjoint_dtype="auto"):
     import wakis_amd as wa
     from wakis_amd.adjoint import WakeAdjoint
     x, y, z, mask = cases.cavity_pipe_geometry(*DIMS)
@@ -39,8 +39,9 @@ def _setup(w_loss, w_band, mode="auto"):
     wake = wa.WakeSolver(q=1e-9, sigmaz=10e-3, beta=1.0, skip_cells=2, save=False, verbose=0)
     s = wa.SolverFIT3D(grid, wake=wake, bc_low=["pec", "pec", "abc"],
                        bc_high=["pec", "pec", "abc"], use_stl=True, bg="pec",
-                       dtype=np.float64, verbose=0, use_sibc=False, tau_clamp=False)
-    return s, WakeAdjoint(s, REGION, WAKELENGTH, w_loss=w_loss, w_band=w_band, mode=mode)
+                       dtype=dtype, verbose=0, use_sibc=False, tau_clamp=False)
+    return s, WakeAdjoint(s, REGION, WAKELENGTH, w_loss=w_loss, w_band=w_band, mode=mode,
+                          kernel=kernel, adjoint_dtype=adjoint_dtype)
 
 
 def _params(seed=0):
@@ -239,3 +240,63 @@ def _params_of(region, seed=1):
     rng = np.random.default_rng(seed)
     shape = tuple(b - a for a, b in region)
     return rng.uniform(1.0, 4.0, shape), rng.uniform(0.0, 1.0, shape)
+
+
+@pytest.mark.parametrize("mode", ["trajectory", "checkpoint"])
+def test_fused_reverse_kernel_equals_two_pass(mode):
+    """The single-pass reverse kernel sums the same terms in the same order as
+    the two-kernel path: identical gradients (and the default picks it here)."""
+    print("adjoint fused vs two-pass", mode, flush=True)
+    out = {}
+    for kernel in ("fused", "two-pass"):
+        s, adj = _setup(1.0, 1e-13, mode=mode, kernel=kernel)
+        er, sg = _params(13)
+        adj.forward(er, sg)
+        out[kernel] = tuple(g.cpu().numpy() for g in adj.backward())
+        assert adj.used_kernel == kernel
+    assert np.abs(out["fused"][0]).max() > 0
+    for a, b in zip(out["fused"], out["two-pass"]):
+        assert np.array_equal(a, b)
+    s, adj = _setup(1.0, 0.0)
+    adj.forward(*_params(2))
+    adj.backward()
+    assert adj.used_kernel == "fused"
+
+
+def test_fused_reverse_kernel_rejects_low_face_region():
+    from wakis_amd.adjoint import WakeAdjoint
+    s, _ = _setup(1.0, 0.0)
+    region = ((0, 4), (6, 10), (17, 23))
+    adj = WakeAdjoint(s, region, WAKELENGTH, kernel="fused")
+    adj.forward(*_params_of(region))
+    with pytest.raises(ValueError):
+        adj.backward()
+    adj = WakeAdjoint(s, region, WAKELENGTH)
+    adj.forward(*_params_of(region))
+    adj.backward()
+    assert adj.used_kernel == "two-pass"
+
+
+def test_fp32_adjoint_fields_close_to_fp64():
+    """FP32 forward run: adjoint fields stored in FP32 (FP64 arithmetic) give the
+    gradient of FP64-stored adjoint fields to 1e-4 relative L2, and both are
+    within 1e-2 of the FP64 run's gradient (the FP32 forward's own error)."""
+    print("adjoint FP32 storage", flush=True)
+    er, sg = _params(21)
+    got = {}
+    for adt in ("float32", "float64"):
+        s, adj = _setup(1.0, 1e-13, dtype=np.float32, adjoint_dtype=adt)
+        adj.forward(er, sg)
+        got[adt] = tuple(g.cpu().numpy() for g in adj.backward())
+        assert adj.used_kernel == "fused" and adj.used_adjoint_dtype == adt
+    s, adj = _setup(1.0, 1e-13)
+    adj.forward(er, sg)
+    ref = tuple(g.cpu().numpy() for g in adj.backward())
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    for k in range(2):
+        r32 = rel(got["float32"][k], got["float64"][k])
+        r64 = rel(got["float64"][k], ref[k])
+        print("grad", k, "fp32 storage vs fp64 storage", r32, "fp32 run vs fp64 run", r64,
+              flush=True)
+        assert r32 < 1e-4
+        assert r64 < 1e-2

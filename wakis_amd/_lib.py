@@ -54,6 +54,20 @@ class WfAdjointArgs(ctypes.Structure):
         ("g_ieps", c_void_p), ("g_sigma", c_void_p), ("gcap", c_void_p),
         ("cap_ix0", c_int32), ("cap_iy0", c_int32), ("cap_nx", c_int32), ("cap_ny", c_int32),
         ("cap_samples", c_int64), ("cap_sample", c_int64), ("traj", c_int32),
+        ("fused", c_int32), ("W2", c_void_p), ("lH2", c_void_p), ("fields_f32", c_int32),
+        ("reg_W", c_void_p), ("reg_lJ", c_void_p), ("reg_slot", c_int64),
+    ]
+
+
+class WfAdjointGradArgs(ctypes.Structure):
+    """wf_adjoint_grad_args of include/wakis_fit.h."""
+    _fields_ = [
+        ("E", c_void_p), ("H", c_void_p), ("E_stride", c_int64), ("H_stride", c_int64),
+        ("traj", c_int32), ("beam", c_void_p), ("sigma", c_void_p), ("metric2d", c_void_p),
+        ("reg_W", c_void_p), ("reg_lJ", c_void_p), ("fields_f32", c_int32),
+        ("nsteps", c_int32), ("dt", c_double), ("conductive", c_int32), ("ixs", c_int32),
+        ("iys", c_int32), ("r0", c_int32 * 3), ("r1", c_int32 * 3),
+        ("g_ieps", c_void_p), ("g_sigma", c_void_p),
     ]
 
 
@@ -82,6 +96,7 @@ EXPORTS = {
     "wf_batch_run": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p]),
     "wf_batch_destroy": (c_int32, [c_void_p]),
     "wf_adjoint_step": (c_int32, [c_void_p, POINTER(WfAdjointArgs), c_void_p]),
+    "wf_adjoint_grad": (c_int32, [c_void_p, POINTER(WfAdjointGradArgs), c_void_p]),
     "wf_trajectory_gather": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                        POINTER(c_int32), POINTER(c_int32), c_int64, c_int64,
                                        c_void_p]),

@@ -43,11 +43,26 @@ from . import _lib
 
 class WakeAdjoint:
     def __init__(self, solver, region, wakelength, f_band=3e9, w_loss=1.0, w_band=1.0,
-                 segment=None, mode="auto"):
+                 segment=None, mode="auto", kernel="auto", adjoint_dtype="auto"):
         """``region`` = ((i0, i1), (j0, j1), (k0, k1)) cells, half-open.
-        ``mode``: "trajectory", "checkpoint" or "auto" (trajectory when it fits)."""
+        ``mode``: "trajectory", "checkpoint" or "auto" (trajectory when it fits).
+        ``kernel``: "fused" (one single-pass reverse kernel per step), "two-pass"
+        or "auto" (fused when the flat-index wrap terms are dead, as for the
+        fused forward step, and the region starts at cell 1 or later in x and y).
+        ``adjoint_dtype``: storage of the adjoint fields, "float64", "float32"
+        (fused kernel only; arithmetic stays FP64) or "auto" (float32 with the
+        fused kernel when the solver's fields are FP32, else float64)."""
+        if adjoint_dtype not in ("auto", "float32", "float64"):
+            raise ValueError("WakeAdjoint: adjoint_dtype must be 'auto', 'float32' or 'float64'")
+        self.adjoint_dtype = adjoint_dtype
+        self.used_adjoint_dtype = None
         if mode not in ("auto", "trajectory", "checkpoint"):
             raise ValueError("WakeAdjoint: mode must be 'auto', 'trajectory' or 'checkpoint'")
+        if kernel not in ("auto", "fused", "two-pass"):
+            raise ValueError("WakeAdjoint: kernel must be 'auto', 'fused' or 'two-pass'")
+        self.kernel = kernel
+        self.fused_rows = 0  # rows per workgroup of the fused reverse kernel (0: default)
+        self.used_kernel = None
         s = solver
         if s.wake is None:
             raise ValueError("WakeAdjoint: the solver needs a WakeSolver")
@@ -93,6 +108,7 @@ class WakeAdjoint:
                                  "cell 1 or later on every axis")
             return True
         need = (Nt + 1) * 3 * (self._nE + self._nH) * self.s.E.array.element_size()
+        need += (Nt + 1) * 3 * self._nE * 16  # region W and lJ of every reverse step
         free, _ = torch.cuda.mem_get_info(self.s.device)
         return inside and need <= free // 2
 
@@ -136,6 +152,15 @@ class WakeAdjoint:
         s._wake_finish(ctx, "both")
         self._fwd = fwd
         return self._loss_value()
+
+    def _use_fused_kernel(self):
+        if self.kernel == "two-pass":
+            return False
+        ok = self.i0 >= 1 and self.j0 >= 1 and self.s._wraps_dead()
+        if self.kernel == "fused" and not ok:
+            raise ValueError("WakeAdjoint: the fused reverse kernel needs dead flat-index wrap "
+                             "terms and the region at cell 1 or later in x and y")
+        return ok
 
     @property
     def used_mode(self):
@@ -218,7 +243,15 @@ class WakeAdjoint:
         dev = s.device
         gcap = self._grad_capture(self._grad_wp())
         n3 = 3 * s.N
-        W = torch.zeros(n3, dtype=torch.float64, device=dev)
+        adj_fused = self._use_fused_kernel()
+        self.used_kernel = "fused" if adj_fused else "two-pass"
+        f32 = self.adjoint_dtype == "float32" or (self.adjoint_dtype == "auto" and adj_fused
+                                                   and s.E.array.dtype == torch.float32)
+        if f32 and not adj_fused:
+            raise ValueError("WakeAdjoint: FP32 adjoint fields need the fused reverse kernel")
+        self.used_adjoint_dtype = "float32" if f32 else "float64"
+        adt = torch.float32 if f32 else torch.float64
+        W = torch.zeros(n3, dtype=adt, device=dev)
         lH = torch.zeros_like(W)
         nreg = int(np.prod(self.shape))
         lJr = torch.zeros(3 * nreg, dtype=torch.float64, device=dev)
@@ -244,18 +277,55 @@ class WakeAdjoint:
         a.metric2d = _lib.ptr(s._metric2d_all)
         lib, st = _lib.lib(), _lib.stream_ptr()
         row_bytes = table.shape[1] * table.element_size()
+        if adj_fused:
+            W2, lH2 = torch.zeros_like(W), torch.zeros_like(lH)
+            a.fused = int(self.fused_rows) or 1
+            a.W2, a.lH2 = _lib.ptr(W2), _lib.ptr(lH2)
+            a.fields_f32 = int(f32)
+
+        def adjoint_step():
+            _lib.check(lib.wf_adjoint_step(s._handle, ctypes.byref(a), st), "wf_adjoint_step")
+            if adj_fused:  # the outputs are the next step's inputs
+                a.W, a.W2 = a.W2, a.W
+                a.lH, a.lH2 = a.lH2, a.lH
+
+        # fused kernel: the steps store the region's W and lJ, and one pass per
+        # run (or per segment) accumulates the gradients
+        n3r = 3 * nreg
+        ga = _lib.WfAdjointGradArgs()
+        ga.sigma, ga.metric2d = a.sigma, a.metric2d
+        ga.fields_f32 = int(f32)
+        ga.dt, ga.conductive, ga.ixs, ga.iys = a.dt, 1, a.ixs, a.iys
+        ga.r0[:] = tuple(a.r0)
+        ga.r1[:] = tuple(a.r1)
+        ga.g_ieps, ga.g_sigma = _lib.ptr(g_ieps), _lib.ptr(g_sig)
+
+        def grad_pass(E_, Es, H_, Hs, traj, beam_ptr, rW_, rJ_, nsteps):
+            ga.E, ga.E_stride, ga.H, ga.H_stride, ga.traj = E_, Es, H_, Hs, traj
+            ga.beam = beam_ptr
+            ga.reg_W, ga.reg_lJ, ga.nsteps = _lib.ptr(rW_), _lib.ptr(rJ_), nsteps
+            _lib.check(lib.wf_adjoint_grad(s._handle, ctypes.byref(ga), st), "wf_adjoint_grad")
+
         if "traj" in f:
             tE, tH = f["traj"]
             a.traj = 1
             pE, pH = tE.data_ptr(), tH.data_ptr()
             sE, sH = tE.stride(0) * tE.element_size(), tH.stride(0) * tH.element_size()
             pT = table.data_ptr()
+            if adj_fused:
+                rW = torch.empty((max(Nt, 1), n3r), dtype=adt, device=dev)
+                rJ = torch.zeros((Nt + 1, n3r), dtype=torch.float64, device=dev)
+                a.reg_W, a.reg_lJ = _lib.ptr(rW), _lib.ptr(rJ)
+                a.g_ieps = a.g_sigma = None
             for n in range(Nt - 1, -1, -1):
                 a.E0, a.E1 = pE + n * sE, pE + (n + 1) * sE
                 a.H1 = pH + (n + 1) * sH
                 a.beam_row = pT + n * row_bytes
                 a.cap_sample = n
-                _lib.check(lib.wf_adjoint_step(s._handle, ctypes.byref(a), st), "wf_adjoint_step")
+                a.reg_slot = n
+                adjoint_step()
+            if adj_fused:
+                grad_pass(pE, tE.stride(0), pH, tH.stride(0), 1, pT, rW, rJ, Nt)
             torch.cuda.synchronize(dev)
             return self._finish_grad(g_ieps, g_sig)
         K = f["K"]
@@ -266,8 +336,16 @@ class WakeAdjoint:
         fused = s._prepare_fused(step)
         Eb = torch.empty((K + 1, n3), dtype=s.E.array.dtype, device=dev)
         Hb = torch.empty_like(Eb)
+        if adj_fused:
+            rW = torch.empty((K, n3r), dtype=adt, device=dev)
+            rJ = torch.zeros((K + 1, n3r), dtype=torch.float64, device=dev)
+            carry = torch.zeros(n3r, dtype=torch.float64, device=dev)  # lJ after the segment
+            a.reg_W, a.reg_lJ = _lib.ptr(rW), _lib.ptr(rJ)
+            a.g_ieps = a.g_sigma = None
         for (n0, E0, H0, J0) in reversed(f["ckpt"]):
             k = min(K, Nt - n0)
+            if adj_fused:
+                rJ[k].copy_(carry)
             Eb[0].copy_(E0)
             Hb[0].copy_(H0)
             s.J.array.copy_(J0)
@@ -288,7 +366,12 @@ class WakeAdjoint:
                 a.E0, a.E1, a.H1 = _lib.ptr(Eb[i]), _lib.ptr(Eb[i + 1]), _lib.ptr(Hb[i + 1])
                 a.beam_row = ctypes.c_void_p(table.data_ptr() + n * row_bytes)
                 a.cap_sample = n
-                _lib.check(lib.wf_adjoint_step(s._handle, ctypes.byref(a), st), "wf_adjoint_step")
+                a.reg_slot = i
+                adjoint_step()
+            if adj_fused:
+                grad_pass(Eb.data_ptr(), n3, Hb.data_ptr(), n3, 0,
+                          table.data_ptr() + n0 * row_bytes, rW, rJ, k)
+                carry.copy_(rJ[0])
         s.J.array.copy_(J_final)
         torch.cuda.synchronize(dev)
         del Eb, Hb

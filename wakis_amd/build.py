@@ -14,7 +14,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libwakis_fit.so"
 SOURCES = ["libwakis_fit.hip"]
 HEADERS = ["fit_common.h", "step_general.h", "step_fused.h", "step_fused4.h", "wake.h",
-           "reduce.h", "adjoint.h", "monitor.h"]
+           "reduce.h", "adjoint.h", "adjoint_fused.h", "monitor.h"]
 
 # -ffp-contract=off keeps every multiply and add separately rounded, as numpy
 # and scipy's csr_matvec do on the host; -packed-fp32-ops keeps the compiler

@@ -81,9 +81,9 @@ k_traj_gather(GridDev g, const S *__restrict__ E, const S *__restrict__ H, TrajB
 
 // parameter gradients of one step over the region (reads W and the region's lJ
 // of step n+1); E0, E1, H1 are full fields, or trajectory slots (x.traj)
-template <typename S>
+template <typename S, typename A = double>
 __global__ void __launch_bounds__(256)
-k_adj_grad(GridDev g, MetricDev<S> m, const double *__restrict__ W,
+k_adj_grad(GridDev g, MetricDev<S> m, const A *__restrict__ W,
            const double *__restrict__ lJr, const S *__restrict__ E0, const S *__restrict__ E1,
            const S *__restrict__ H1, const S *__restrict__ beam_row, const S *__restrict__ sigma,
            const S *__restrict__ metric2d, double dt, AdjAux x, double *__restrict__ g_ieps,
@@ -132,9 +132,88 @@ k_adj_grad(GridDev g, MetricDev<S> m, const double *__restrict__ W,
     double j0 = 0.0;
     if (online && beam_row) j0 = (double)beam_row[c0.k];
     else if (x.conductive) j0 = (double)(S)(sigma[q] * E0[eq]);
-    g_ieps[comp * nreg + t] += dt * W[q] * (k2 - j0);
+    g_ieps[comp * nreg + t] += dt * (double)W[q] * (k2 - j0);
     if (x.conductive) g_sigma[comp * nreg + t] += lJr[comp * nreg + t] * (double)E1[eq];
   }
+}
+
+// Parameter gradients of nsteps reverse steps in one pass, from stored slots:
+// slot s holds the forward E of steps n0+s (E at s) and n0+s+1 (E at s+1), H of
+// step n0+s+1 (H at s+1), the region's W read by reverse step n0+s (rW at s) and
+// lJ of step n0+s+1 (rJ at s+1); beam row n0+s at beam + s*Nz. E/H slots are
+// trajectory boxes (x.traj) or full 3N fields (E_stride/H_stride elements per
+// slot). Each (cell, component) accumulates over s = nsteps-1 .. 0, the order
+// of the per-step kernel, so the sums are the same.
+template <typename S, typename A>
+__global__ void __launch_bounds__(256)
+k_adj_grad_seq(GridDev g, MetricDev<S> m, const S *__restrict__ E, int64_t Es,
+               const S *__restrict__ H, int64_t Hs, const A *__restrict__ rW,
+               const double *__restrict__ rJ, const S *__restrict__ beam,
+               const S *__restrict__ sigma, const S *__restrict__ metric2d, double dt, AdjAux x,
+               int nsteps, double *__restrict__ g_ieps, double *__restrict__ g_sigma) {
+  const int rx = x.r1[0] - x.r0[0], ry = x.r1[1] - x.r0[1], rz = x.r1[2] - x.r0[2];
+  const int64_t nreg = (int64_t)rx * ry * rz;
+  const int64_t tt3 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tt3 >= 3 * nreg) return;
+  const int comp = (int)(tt3 / nreg);
+  const int64_t t = tt3 - comp * nreg;
+  Cell c0;
+  c0.i = x.r0[0] + (int)(t % rx);
+  c0.j = x.r0[1] + (int)((t / rx) % ry);
+  c0.k = x.r0[2] + (int)(t / ((int64_t)rx * ry));
+  const int64_t n = flat(c0, g);
+  const int64_t q = comp * g.N + n;
+  const TrajBox tb = traj_box(x.r0, x.r1);
+  const int64_t yz = (int64_t)g.Ny * g.Nz, xz = (int64_t)g.Nx * g.Nz, xy = (int64_t)g.Nx * g.Ny;
+  // the step-independent part of K2 H' = itA * sum(+-tL H) of this row
+  double wgt[4] = {0, 0, 0, 0};
+  int64_t hq[4] = {0, 0, 0, 0};
+  double itA = 0.0;
+  if (face_mask(g.pecz, comp, c0.i, c0.j, c0.k, g.Nx, g.Ny, g.Nz)) {
+    Term tt[4];
+    e_terms(comp, c0, g, tt);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (!tt[u].ok) continue;
+      if (!face_mask(g.pmcz, tt[u].comp, tt[u].c.i, tt[u].c.j, tt[u].c.k, g.Nx, g.Ny, g.Nz))
+        continue;
+      const double w = (double)len_of(m.tLx, m.tLy, m.tLz, tt[u].comp, tt[u].c);
+      wgt[u] = tt[u].sign > 0 ? w : -w;
+      hq[u] = x.traj ? traj_h_index(tb, tt[u].comp, tt[u].c.i, tt[u].c.j, tt[u].c.k)
+                     : tt[u].comp * g.N + flat(tt[u].c, g);
+    }
+    const S *itAc = metric2d + (comp == 0 ? yz + xz + xy : (comp == 1 ? 2 * yz + xz + xy
+                                                                      : 2 * yz + 2 * xz + xy));
+    const int64_t t2 = comp == 0 ? (int64_t)c0.j + (int64_t)g.Ny * c0.k
+                                 : (comp == 1 ? (int64_t)c0.i + (int64_t)g.Nx * c0.k
+                                              : (int64_t)c0.i + (int64_t)g.Nx * c0.j);
+    itA = (double)itAc[t2];
+  }
+  const bool online = (comp == 2) && (c0.i == x.ixs) && (c0.j == x.iys);
+  const int64_t eq = x.traj ? comp * nreg + t : q;
+  const S sg = x.conductive ? sigma[q] : S(0);
+  double gi = g_ieps[comp * nreg + t];
+  double gs = x.conductive ? g_sigma[comp * nreg + t] : 0.0;
+  for (int s = nsteps - 1; s >= 0; --s) {
+    const S *H1 = H + (int64_t)(s + 1) * Hs;
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (wgt[u] == 0.0) continue;  // absent, masked or zero-length term
+      const double p = fabs(wgt[u]) * (double)H1[hq[u]];
+      acc = wgt[u] > 0 ? acc + p : acc - p;
+    }
+    const double k2 = itA * acc;
+    double j0 = 0.0;
+    if (online && beam) j0 = (double)beam[(int64_t)s * g.Nz + c0.k];
+    else if (x.conductive) j0 = (double)(S)(sg * E[(int64_t)s * Es + eq]);
+    gi += dt * (double)rW[(int64_t)s * 3 * nreg + comp * nreg + t] * (k2 - j0);
+    if (x.conductive)
+      gs += rJ[(int64_t)(s + 1) * 3 * nreg + comp * nreg + t] *
+            (double)E[(int64_t)(s + 1) * Es + eq];
+  }
+  g_ieps[comp * nreg + t] = gi;
+  if (x.conductive) g_sigma[comp * nreg + t] = gs;
 }
 
 // lH <- lH + dt M2^T W

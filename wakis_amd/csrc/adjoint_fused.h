@@ -1,0 +1,235 @@
+// Single-pass reverse leapfrog step (the two kernels k_adj_h and k_adj_e of
+// adjoint.h in one z-march over the grid).
+//
+// Reverse step n, P = b.W and Q = a.lH', arithmetic in FP64; W and lH are
+// stored as A (FP64, or FP32 to halve their traffic when the forward run is
+// FP32):
+//   lH' = lH + dt tL . (C P)            (PMC-masked rows keep lH)
+//   W'  = W - dt pec . L . (C^T Q) [+ capture gradient] [+ sigma lJ]
+//   lJ  = -dt (1 - beam line) ieps . W  (conductive; stored on the region)
+// C P uses forward differences of P (x+1, y+1, z+1), C^T Q backward
+// differences of Q (x-1, y-1, z-1); the terms are summed in the order of
+// h_terms / e_terms, so with A = double the results equal the two-kernel path
+// bit for bit. The flat-index wrap couplings are dropped: valid when their
+// operator entries are zero or act on frozen zero fields (the condition of the
+// fused forward step), with the gradient region away from the low x and y
+// faces. Offsets are 32-bit byte offsets (3N * sizeof(double) < 4 GiB).
+//
+// Layout: a workgroup covers rows j0-1 .. j0+TY with TY + 2 row waves (wave
+// w <-> row j0-1+w, lane l <-> cell i0+l) plus one halo wave whose lanes hold
+// the columns i0-1 (lanes 0..31) and i0+64 (lanes 32..63) of those rows. It
+// owns rows 1..TY of its 64 columns and marches up a chunk of KZ planes. Each
+// thread loads and stages its own cell, so the loads of a wave are uniform
+// (out-of-grid cells read a valid address and are masked to zero): W and b two
+// planes ahead, lH, a, ieps, sigma one plane ahead, in raw form. Per plane:
+// stage P -> barrier -> lH', Q -> stage Q -> barrier -> W' of the owned cells.
+// W' and lH' go to separate buffers (the caller alternates them), so every
+// tile reads the step's inputs only.
+#pragma once
+#include "adjoint.h"
+#include "fit_common.h"
+#include "step_fused4.h"
+#include "step_general.h"
+
+namespace wf {
+
+struct AdjTile {
+  int ntx, nty, ntz, KZ;
+};
+
+constexpr int ADJ_TX = 64;  // owned cells per tile row
+
+// is the component-c row of cell (i, j, k) kept by the zero-face bits (the
+// face_mask of the general path)?
+__device__ __forceinline__ bool keep_row(int bits, int c, int i, int j, int k, const GridDev &g) {
+  return bits == 0 || face_mask(bits, c, i, j, k, g.Nx, g.Ny, g.Nz);
+}
+
+template <typename S, typename A, int TY>
+__global__ void __launch_bounds__(64 * (TY + 3))
+k_adj_fused(GridDev g, MetricDev<S> m, const A *__restrict__ Win, const A *__restrict__ lHin,
+            A *__restrict__ Wout, A *__restrict__ lHout, double *__restrict__ lJr,
+            const S *__restrict__ a, const S *__restrict__ b, const S *__restrict__ ieps,
+            const S *__restrict__ sigma, double dt, AdjAux x, const double *__restrict__ gcap,
+            AdjTile t, A *__restrict__ rW, double *__restrict__ rJ, int64_t rslot) {
+  static_assert(TY + 3 <= 16, "at most 1024 threads per workgroup");
+  using I = uint32_t;
+  constexpr int NW = TY + 2;              // row waves
+  constexpr int NS = ADJ_TX + 3;          // slots: 0 = i0-1, 1..64 cells, 65 = i0+64, 66 spare
+  __shared__ double sP[3][NW + 1][NS];    // row NW: zeros (above the top row)
+  __shared__ double sQ[3][NW][NS];
+
+  const int tile = xcd_tile(blockIdx.x, t.ntx * t.nty * t.ntz);
+  const int tz = tile / (t.ntx * t.nty);
+  const int rem = tile - tz * t.ntx * t.nty;
+  const int tyb = rem / t.ntx, txb = rem - tyb * t.ntx;
+  const int i0 = txb * ADJ_TX, j0 = tyb * TY, k0 = tz * t.KZ;
+  const int kend = min(k0 + t.KZ, g.Nz);
+
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const bool halo = (w == NW);
+  const int side = lane >> 5;  // halo wave: 0 = column i0-1, 1 = column i0+64
+  const int row = halo ? (lane & 31) : w;
+  const int slot = halo ? (side ? ADJ_TX + 1 : 0) : 1 + lane;
+  const int i = halo ? (side ? i0 + ADJ_TX : i0 - 1) : i0 + lane;
+  const int j = j0 - 1 + row;
+  const bool inrow = row < NW;
+  const bool valid = inrow && (i >= 0) && (i < g.Nx) && (j >= 0) && (j < g.Ny);
+  const bool own = valid && !halo && (w >= 1) && (w <= TY);
+  const bool makes_q = inrow && !(halo && side);  // the column i0+64 needs P only
+  const I N = (I)g.N, sz = (I)g.sz;
+  const I col = valid ? (I)i + (I)g.sy * (I)j : 0;
+  for (int u = threadIdx.x; u < 3 * (NW + 1) * NS; u += 64 * (NW + 1))
+    (&sP[0][0][0])[u] = 0.0;
+
+  const double tLx = valid ? (double)m.tLx[i] : 0.0, tLy = valid ? (double)m.tLy[j] : 0.0;
+  const double Lx = valid ? (double)m.Lx[i] : 0.0, Ly = valid ? (double)m.Ly[j] : 0.0;
+  const bool online = own && (i == x.ixs) && (j == x.iys);
+  const bool capcol = gcap && own && x.cap_sample < x.cap_samples && i >= x.cap_ix0 &&
+                      i < x.cap_ix0 + x.cap_nx && j >= x.cap_iy0 && j < x.cap_iy0 + x.cap_ny;
+  const int64_t capbase =
+      capcol ? ((int64_t)(i - x.cap_ix0) * x.cap_ny + (j - x.cap_iy0)) * x.cap_samples : 0;
+  const bool regcol = own && i >= x.r0[0] && i < x.r1[0] && j >= x.r0[1] && j < x.r1[1];
+  const int rx = x.r1[0] - x.r0[0], ry = x.r1[1] - x.r0[1], rz = x.r1[2] - x.r0[2];
+  const int64_t nreg = (int64_t)rx * ry * rz;
+  const bool cond = x.conductive != 0;
+
+  // plane p's offset of this cell, clamped into the grid (masked at use)
+  const auto off = [&](int p) { return col + sz * (I)min(p, g.Nz - 1); };
+  const int kstart = (k0 > 0) ? k0 - 1 : k0;
+  A Wk[3], Wk1[3], lh[3];
+  S bk[3], bk1[3], ac[3], ie[3], sg[3];
+  {
+    const I n0 = off(kstart), n1 = off(kstart + 1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      Wk[c] = *at(Win, (I)c * N + n0); bk[c] = *at(b, (I)c * N + n0);
+      Wk1[c] = *at(Win, (I)c * N + n1); bk1[c] = *at(b, (I)c * N + n1);
+      lh[c] = *at(lHin, (I)c * N + n0); ac[c] = *at(a, (I)c * N + n0);
+      ie[c] = *at(ieps, (I)c * N + n0); sg[c] = *at(sigma, (I)c * N + n0);
+    }
+  }
+  double Qpx = 0, Qpy = 0;  // Q(k-1) of this cell, x and y components
+  __syncthreads();          // zeroed LDS
+
+  for (int k = kstart; k < kend; ++k) {
+    const bool body = (k >= k0);
+    const bool kp1 = (k + 1 < g.Nz);
+    // ---- prefetch: W, b of plane k+2; lH, a, ieps, sigma of plane k+1
+    A Wn2[3], lhN[3];
+    S bn2[3], acN[3], ieN[3], sgN[3];
+    {
+      const I n2 = off(k + 2), n1 = off(k + 1);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        Wn2[c] = *at(Win, (I)c * N + n2); bn2[c] = *at(b, (I)c * N + n2);
+        lhN[c] = *at(lHin, (I)c * N + n1); acN[c] = *at(a, (I)c * N + n1);
+        if (cond) { ieN[c] = *at(ieps, (I)c * N + n1); sgN[c] = *at(sigma, (I)c * N + n1); }
+        else { ieN[c] = S(0); sgN[c] = S(0); }
+      }
+    }
+    // ---- stage P(k)
+    double Pk[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Pk[c] = valid ? (double)bk[c] * (double)Wk[c] : 0.0;
+    if (inrow) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sP[c][row][slot] = Pk[c];
+    }
+    const double tLz = (double)m.tLz[k], Lz = (double)m.Lz[k];
+    __syncthreads();
+
+    // ---- lH'(k) and Q(k) (row waves: all 64 columns; halo wave: column i0-1)
+    double hn[3] = {0, 0, 0};
+    if (makes_q) {
+      double Pk1[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) Pk1[c] = (valid && kp1) ? (double)bk1[c] * (double)Wk1[c] : 0.0;
+      // h_terms order: x: +Py, -Py(z+1), -Pz, +Pz(y+1); y: -Px, +Px(z+1), +Pz, -Pz(x+1);
+      // z: +Px, -Px(y+1), -Py, +Py(x+1)
+      double s0 = 0.0;
+      s0 = s0 + Pk[1]; s0 = s0 - Pk1[1]; s0 = s0 - Pk[2]; s0 = s0 + sP[2][row + 1][slot];
+      double s1 = 0.0;
+      s1 = s1 - Pk[0]; s1 = s1 + Pk1[0]; s1 = s1 + Pk[2]; s1 = s1 - sP[2][row][slot + 1];
+      double s2 = 0.0;
+      s2 = s2 + Pk[0]; s2 = s2 - sP[0][row + 1][slot]; s2 = s2 - Pk[1];
+      s2 = s2 + sP[1][row][slot + 1];
+      const double h0 = (double)lh[0], h1 = (double)lh[1], h2 = (double)lh[2];
+      hn[0] = keep_row(g.pmcz, 0, i, j, k, g) ? h0 + dt * (tLx * s0) : h0;
+      hn[1] = keep_row(g.pmcz, 1, i, j, k, g) ? h1 + dt * (tLy * s1) : h1;
+      hn[2] = keep_row(g.pmcz, 2, i, j, k, g) ? h2 + dt * (tLz * s2) : h2;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sQ[c][row][slot] = valid ? (double)ac[c] * hn[c] : 0.0;
+    }
+    __syncthreads();
+
+    // ---- W'(k) of the owned cells on body planes
+    double Qx = 0.0, Qy = 0.0;
+    if (!halo) { Qx = sQ[0][row][slot]; Qy = sQ[1][row][slot]; }
+    if (own && body) {
+      const double Qz = sQ[2][row][slot];
+      // e_terms order: x: +Qy(z-1), -Qy, -Qz(y-1), +Qz; y: -Qx(z-1), +Qx, +Qz(x-1), -Qz;
+      // z: +Qx(y-1), -Qx, -Qy(x-1), +Qy
+      double e0 = 0.0;
+      e0 = e0 + Qpy; e0 = e0 - Qy; e0 = e0 - sQ[2][row - 1][slot]; e0 = e0 + Qz;
+      double e1 = 0.0;
+      e1 = e1 - Qpx; e1 = e1 + Qx; e1 = e1 + sQ[2][row][slot - 1]; e1 = e1 - Qz;
+      double e2 = 0.0;
+      e2 = e2 + sQ[0][row - 1][slot]; e2 = e2 - Qx; e2 = e2 - sQ[1][row][slot - 1]; e2 = e2 + Qy;
+      const double m0 = keep_row(g.pecz, 0, i, j, k, g) ? Lx * e0 : 0.0;
+      const double m1 = keep_row(g.pecz, 1, i, j, k, g) ? Ly * e1 : 0.0;
+      const double m2 = keep_row(g.pecz, 2, i, j, k, g) ? Lz * e2 : 0.0;
+      const double wv[3] = {(double)Wk[0], (double)Wk[1], (double)Wk[2]};
+      double le[3] = {wv[0] - dt * m0, wv[1] - dt * m1, wv[2] - dt * m2};
+      if (capcol) le[2] += gcap[(capbase + x.cap_sample) * g.Nz + k];
+      double wn[3];
+      const bool inreg = regcol && k >= x.r0[2] && k < x.r1[2];
+      const int64_t tr = inreg ? (int64_t)(i - x.r0[0]) +
+                                     (int64_t)rx * ((j - x.r0[1]) + (int64_t)ry * (k - x.r0[2]))
+                               : 0;
+      double ljv[3] = {0, 0, 0};
+      if (cond) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          ljv[c] = (c == 2 && online) ? 0.0 : -dt * (double)ie[c] * wv[c];
+          wn[c] = le[c] + (double)sg[c] * ljv[c];
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) wn[c] = le[c];
+      }
+      if (inreg) {
+        if (rW) {  // region W read by this step and its lJ, for the one-pass gradient
+          const int64_t base = rslot * 3 * nreg + tr;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            rW[base + c * nreg] = Wk[c];
+            rJ[base + c * nreg] = ljv[c];
+          }
+        } else if (lJr && cond) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) lJr[c * nreg + tr] = ljv[c];
+        }
+      }
+      const I nk = col + sz * (I)k;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        *atw(Wout, (I)c * N + nk) = (A)wn[c];
+        *atw(lHout, (I)c * N + nk) = (A)hn[c];
+      }
+    }
+    Qpx = Qx;  // Q(k) of this cell seeds the z-1 terms of plane k+1
+    Qpy = Qy;
+    // advance the register pipeline. Plane k+1 stages sP after this plane's
+    // second barrier (every read of this plane's sP is before it) and writes
+    // sQ after its own first barrier (every read of this plane's sQ is before it).
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      Wk[c] = Wk1[c]; bk[c] = bk1[c];
+      Wk1[c] = Wn2[c]; bk1[c] = bn2[c];
+      lh[c] = lhN[c]; ac[c] = acN[c]; ie[c] = ieN[c]; sg[c] = sgN[c];
+    }
+  }
+}
+
+}  // namespace wf

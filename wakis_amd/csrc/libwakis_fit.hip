@@ -15,6 +15,7 @@
 #include "wake.h"
 #include "reduce.h"
 #include "adjoint.h"
+#include "adjoint_fused.h"
 #include "monitor.h"
 
 namespace {
@@ -887,6 +888,31 @@ int wf_batch_destroy(wf_batch *b) {
 // ---------------------------------------------------------------------------
 namespace {
 
+constexpr int ADJ_ROWS = 13;  // owned rows per workgroup of the single-pass reverse kernel
+                               // (16 waves with the halo rows and column; measured best)
+
+template <typename S, typename A, int TY>
+int adjoint_fused(wf_handle *h, const wf_adjoint_args *a, const wf::MetricDev<S> &m,
+                  const wf::AdjAux &x, hipStream_t st) {
+  const wf::GridDev &g = h->g;
+  wf::AdjTile t;
+  t.ntx = (g.Nx + wf::ADJ_TX - 1) / wf::ADJ_TX;
+  t.nty = (g.Ny + TY - 1) / TY;
+  const auto kern = &wf::k_adj_fused<S, A, TY>;
+  const int threads = 64 * (TY + 3);
+  const int64_t conc = (int64_t)fused_occupancy(h, (const void *)kern, threads, 0) * h->ncu;
+  t.KZ = auto_kz((int64_t)t.ntx * t.nty, g.Nz, conc);
+  t.ntz = (g.Nz + t.KZ - 1) / t.KZ;
+  const int64_t nbf = (int64_t)t.ntx * t.nty * t.ntz;
+  if (nbf > 2147483647LL) return fail(WF_ERANGE, "wf_adjoint_step: grid too large");
+  hipLaunchKernelGGL(kern, dim3((unsigned)nbf), dim3(threads), 0, st, g, m,
+                     (const A *)a->W, (const A *)a->lH, (A *)a->W2, (A *)a->lH2, a->lJ_region,
+                     (const S *)a->a, (const S *)a->b, (const S *)a->ieps, (const S *)a->sigma,
+                     a->dt, x, a->gcap, t, (A *)a->reg_W, a->reg_lJ, a->reg_slot);
+  WF_LAUNCH_CHECK();
+  return WF_OK;
+}
+
 template <typename S>
 int adjoint_step(wf_handle *h, const wf_adjoint_args *a, const wf::MetricDev<S> &m,
                  hipStream_t st) {
@@ -901,11 +927,30 @@ int adjoint_step(wf_handle *h, const wf_adjoint_args *a, const wf::MetricDev<S> 
   const int64_t nreg = (int64_t)(a->r1[0] - a->r0[0]) * (a->r1[1] - a->r0[1]) *
                        (a->r1[2] - a->r0[2]);
   if (nreg > 0 && a->g_ieps) {
-    hipLaunchKernelGGL((wf::k_adj_grad<S>), dim3(nblocks(nreg, 256)), dim3(256), 0, st, g, m,
-                       a->W, a->lJ_region, (const S *)a->E0, (const S *)a->E1, (const S *)a->H1,
-                       (const S *)a->beam_row, (const S *)a->sigma, (const S *)a->metric2d,
-                       a->dt, x, a->g_ieps, a->g_sigma);
+    if (a->fields_f32)
+      hipLaunchKernelGGL((wf::k_adj_grad<S, float>), dim3(nblocks(nreg, 256)), dim3(256), 0, st,
+                         g, m, (const float *)a->W, a->lJ_region, (const S *)a->E0,
+                         (const S *)a->E1, (const S *)a->H1, (const S *)a->beam_row,
+                         (const S *)a->sigma, (const S *)a->metric2d, a->dt, x, a->g_ieps,
+                         a->g_sigma);
+    else
+      hipLaunchKernelGGL((wf::k_adj_grad<S, double>), dim3(nblocks(nreg, 256)), dim3(256), 0,
+                         st, g, m, (const double *)a->W, a->lJ_region, (const S *)a->E0,
+                         (const S *)a->E1, (const S *)a->H1, (const S *)a->beam_row,
+                         (const S *)a->sigma, (const S *)a->metric2d, a->dt, x, a->g_ieps,
+                         a->g_sigma);
     WF_LAUNCH_CHECK();
+  }
+  if (a->fused) {
+    const int rows = a->fused == 1 ? ADJ_ROWS : a->fused;
+    if (a->fields_f32) {
+      if (rows == 5) return adjoint_fused<S, float, 5>(h, a, m, x, st);
+      if (rows == 13) return adjoint_fused<S, float, 13>(h, a, m, x, st);
+    } else {
+      if (rows == 5) return adjoint_fused<S, double, 5>(h, a, m, x, st);
+      if (rows == 13) return adjoint_fused<S, double, 13>(h, a, m, x, st);
+    }
+    return fail(WF_EINVAL, "wf_adjoint_step: fused must be 0, 1, 5 or 13 (rows per workgroup)");
   }
   const unsigned nb = nblocks(g.N, 256);
   hipLaunchKernelGGL((wf::k_adj_h<S>), dim3(nb), dim3(256), 0, st, g, m, a->lH,
@@ -943,6 +988,21 @@ int wf_adjoint_step(wf_handle *h, const wf_adjoint_args *a, void *stream) {
       return fail(WF_EINVAL, "wf_adjoint_step: conductive gradients need g_sigma and "
                   "lJ_region");
   }
+  if (a->fields_f32 && !a->fused)
+    return fail(WF_EINVAL, "wf_adjoint_step: FP32 adjoint fields need the fused kernel");
+  if (a->reg_W && (!a->fused || !a->reg_lJ || a->g_ieps || a->reg_slot < 0))
+    return fail(WF_EINVAL, "wf_adjoint_step: reg_W needs the fused kernel, reg_lJ, a slot "
+                "and g_ieps = NULL");
+  if (a->fused) {
+    if ((uint64_t)3 * (uint64_t)h->g.N * sizeof(double) >= (1ull << 32))
+      return fail(WF_ERANGE, "wf_adjoint_step: grid too large for the fused kernel");
+    if (!a->W2 || !a->lH2 || a->W2 == a->W || a->lH2 == a->lH || a->W2 == a->lH2 ||
+        a->W2 == a->lH || a->lH2 == a->W)
+      return fail(WF_EINVAL, "wf_adjoint_step: fused step needs distinct W2, lH2 buffers");
+    if (region && (a->r0[0] < 1 || a->r0[1] < 1))
+      return fail(WF_ERANGE, "wf_adjoint_step: fused step needs the region at cell 1 or "
+                  "later in x and y");
+  }
   if (a->gcap && (a->cap_nx < 1 || a->cap_ny < 1 || a->cap_ix0 < 0 || a->cap_iy0 < 0 ||
                   a->cap_ix0 + a->cap_nx > h->g.Nx || a->cap_iy0 + a->cap_ny > h->g.Ny ||
                   a->cap_sample < 0))
@@ -950,6 +1010,47 @@ int wf_adjoint_step(wf_handle *h, const wf_adjoint_args *a, void *stream) {
   hipStream_t st = (hipStream_t)stream;
   if (h->dtype == WF_F64) return adjoint_step<double>(h, a, h->md, st);
   return adjoint_step<float>(h, a, h->mf, st);
+}
+
+int wf_adjoint_grad(wf_handle *h, const wf_adjoint_grad_args *a, void *stream) {
+  if (!h || !a) return fail(WF_EINVAL, "wf_adjoint_grad: null argument");
+  if (!a->E || !a->H || !a->reg_W || !a->reg_lJ || !a->metric2d || !a->g_ieps)
+    return fail(WF_EINVAL, "wf_adjoint_grad: null array");
+  if (a->conductive && (!a->sigma || !a->g_sigma))
+    return fail(WF_EINVAL, "wf_adjoint_grad: conductive gradients need sigma and g_sigma");
+  if (a->nsteps < 0 || a->E_stride < 0 || a->H_stride < 0)
+    return fail(WF_EINVAL, "wf_adjoint_grad: negative size");
+  for (int d = 0; d < 3; ++d) {
+    const int n = d == 0 ? h->g.Nx : (d == 1 ? h->g.Ny : h->g.Nz);
+    if (a->r0[d] < 0 || a->r1[d] > n || a->r0[d] >= a->r1[d])
+      return fail(WF_ERANGE, "wf_adjoint_grad: region outside the grid or empty");
+  }
+  if (a->traj && (a->r0[0] < 1 || a->r0[1] < 1 || a->r0[2] < 1))
+    return fail(WF_ERANGE, "wf_adjoint_grad: trajectory regions start at cell 1 or later");
+  if (a->nsteps == 0) return WF_OK;
+  wf::AdjAux x;
+  memset(&x, 0, sizeof(x));
+  x.conductive = a->conductive; x.ixs = a->ixs; x.iys = a->iys; x.traj = a->traj;
+  for (int d = 0; d < 3; ++d) { x.r0[d] = a->r0[d]; x.r1[d] = a->r1[d]; }
+  const int64_t nreg = (int64_t)(a->r1[0] - a->r0[0]) * (a->r1[1] - a->r0[1]) *
+                       (a->r1[2] - a->r0[2]);
+  const unsigned nb = nblocks(3 * nreg, 256);
+  hipStream_t st = (hipStream_t)stream;
+#define WF_GRAD_SEQ(S, A, M)                                                                  \
+  hipLaunchKernelGGL((wf::k_adj_grad_seq<S, A>), dim3(nb), dim3(256), 0, st, h->g, M,         \
+                     (const S *)a->E, a->E_stride, (const S *)a->H, a->H_stride,              \
+                     (const A *)a->reg_W, a->reg_lJ, (const S *)a->beam, (const S *)a->sigma,  \
+                     (const S *)a->metric2d, a->dt, x, a->nsteps, a->g_ieps, a->g_sigma)
+  if (h->dtype == WF_F64) {
+    if (a->fields_f32) WF_GRAD_SEQ(double, float, h->md);
+    else WF_GRAD_SEQ(double, double, h->md);
+  } else {
+    if (a->fields_f32) WF_GRAD_SEQ(float, float, h->mf);
+    else WF_GRAD_SEQ(float, double, h->mf);
+  }
+#undef WF_GRAD_SEQ
+  WF_LAUNCH_CHECK();
+  return WF_OK;
 }
 
 int wf_wake_wp_adjoint(const double *gI, int32_t nlines, int64_t cap_samples, int32_t Nz,
