@@ -279,6 +279,9 @@ typedef struct wf_adjoint_args {
                              the region's lJ of step n. With reg_W set, g_ieps must be NULL:
                              the gradient is accumulated afterwards by wf_adjoint_grad    */
   int64_t reg_slot;
+  const void *kappa;      /* fused only, optional: per edge -dt*ieps*sigma (coefficient
+                             dtype), zero on the beam line's z edges; the conduction term
+                             becomes kappa*W and ieps is read on the region only        */
 } wf_adjoint_args;
 
 /* Reverse of step n of SolverFIT3D._one_step (solverFIT3D.py:482-503): maps

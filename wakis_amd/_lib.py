@@ -56,6 +56,7 @@ class WfAdjointArgs(ctypes.Structure):
         ("cap_samples", c_int64), ("cap_sample", c_int64), ("traj", c_int32),
         ("fused", c_int32), ("W2", c_void_p), ("lH2", c_void_p), ("fields_f32", c_int32),
         ("reg_W", c_void_p), ("reg_lJ", c_void_p), ("reg_slot", c_int64),
+        ("kappa", c_void_p),
     ]
 
 

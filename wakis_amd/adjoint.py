@@ -62,6 +62,9 @@ class WakeAdjoint:
             raise ValueError("WakeAdjoint: kernel must be 'auto', 'fused' or 'two-pass'")
         self.kernel = kernel
         self.fused_rows = 0  # rows per workgroup of the fused reverse kernel (0: default)
+        # fused kernel: read kappa = -dt ieps sigma instead of ieps and sigma (one
+        # rounding of the conduction product differs from the two-kernel path)
+        self.use_kappa = True
         self.used_kernel = None
         s = solver
         if s.wake is None:
@@ -282,6 +285,14 @@ class WakeAdjoint:
             a.fused = int(self.fused_rows) or 1
             a.W2, a.lH2 = _lib.ptr(W2), _lib.ptr(lH2)
             a.fields_f32 = int(f32)
+            if self.use_kappa:
+                kap = (-float(s.dt) * s.ieps.array.to(torch.float64)
+                       * s.sigma.array.to(torch.float64))
+                kz = 2 * s.N + int(beam.ixs) + s.Nx * int(beam.iys) + s.Nx * s.Ny * torch.arange(
+                    s.Nz, device=dev)
+                kap[kz] = 0.0  # the beam line's J is a source, not sigma E
+                kap = kap.to(s.ieps.array.dtype)
+                a.kappa = _lib.ptr(kap)
 
         def adjoint_step():
             _lib.check(lib.wf_adjoint_step(s._handle, ctypes.byref(a), st), "wf_adjoint_step")

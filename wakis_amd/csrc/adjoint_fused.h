@@ -23,7 +23,9 @@
 // (out-of-grid cells read a valid address and are masked to zero): W and b two
 // planes ahead, lH, a, ieps, sigma one plane ahead, in raw form. Per plane:
 // stage P -> barrier -> lH', Q -> stage Q -> barrier -> W' of the owned cells.
-// W' and lH' go to separate buffers (the caller alternates them), so every
+// With kappa = -dt ieps sigma (host-computed, zero on the beam line's z edges)
+// the conduction term is kappa W, one array instead of two. W' and lH' go to
+// separate buffers (the caller alternates them), so every
 // tile reads the step's inputs only.
 #pragma once
 #include "adjoint.h"
@@ -51,7 +53,8 @@ k_adj_fused(GridDev g, MetricDev<S> m, const A *__restrict__ Win, const A *__res
             A *__restrict__ Wout, A *__restrict__ lHout, double *__restrict__ lJr,
             const S *__restrict__ a, const S *__restrict__ b, const S *__restrict__ ieps,
             const S *__restrict__ sigma, double dt, AdjAux x, const double *__restrict__ gcap,
-            AdjTile t, A *__restrict__ rW, double *__restrict__ rJ, int64_t rslot) {
+            AdjTile t, A *__restrict__ rW, double *__restrict__ rJ, int64_t rslot,
+            const S *__restrict__ kappa) {
   static_assert(TY + 3 <= 16, "at most 1024 threads per workgroup");
   using I = uint32_t;
   constexpr int NW = TY + 2;              // row waves
@@ -93,6 +96,11 @@ k_adj_fused(GridDev g, MetricDev<S> m, const A *__restrict__ Win, const A *__res
   const int rx = x.r1[0] - x.r0[0], ry = x.r1[1] - x.r0[1], rz = x.r1[2] - x.r0[2];
   const int64_t nreg = (int64_t)rx * ry * rz;
   const bool cond = x.conductive != 0;
+  // kappa = -dt ieps sigma per edge replaces the sigma and ieps reads (ieps is
+  // then read on the region only, for lJ); sg[] holds kappa in that case
+  const bool kap = kappa != nullptr;
+  const S *sgsrc = kap ? kappa : sigma;
+  const auto in_region = [&](int p) { return regcol && p >= x.r0[2] && p < x.r1[2]; };
 
   // plane p's offset of this cell, clamped into the grid (masked at use)
   const auto off = [&](int p) { return col + sz * (I)min(p, g.Nz - 1); };
@@ -106,7 +114,8 @@ k_adj_fused(GridDev g, MetricDev<S> m, const A *__restrict__ Win, const A *__res
       Wk[c] = *at(Win, (I)c * N + n0); bk[c] = *at(b, (I)c * N + n0);
       Wk1[c] = *at(Win, (I)c * N + n1); bk1[c] = *at(b, (I)c * N + n1);
       lh[c] = *at(lHin, (I)c * N + n0); ac[c] = *at(a, (I)c * N + n0);
-      ie[c] = *at(ieps, (I)c * N + n0); sg[c] = *at(sigma, (I)c * N + n0);
+      ie[c] = (!kap || in_region(kstart)) ? *at(ieps, (I)c * N + n0) : S(0);
+      sg[c] = *at(sgsrc, (I)c * N + n0);
     }
   }
   double Qpx = 0, Qpy = 0;  // Q(k-1) of this cell, x and y components
@@ -124,8 +133,12 @@ k_adj_fused(GridDev g, MetricDev<S> m, const A *__restrict__ Win, const A *__res
       for (int c = 0; c < 3; ++c) {
         Wn2[c] = *at(Win, (I)c * N + n2); bn2[c] = *at(b, (I)c * N + n2);
         lhN[c] = *at(lHin, (I)c * N + n1); acN[c] = *at(a, (I)c * N + n1);
-        if (cond) { ieN[c] = *at(ieps, (I)c * N + n1); sgN[c] = *at(sigma, (I)c * N + n1); }
-        else { ieN[c] = S(0); sgN[c] = S(0); }
+        if (cond) {
+          ieN[c] = (!kap || in_region(k + 1)) ? *at(ieps, (I)c * N + n1) : S(0);
+          sgN[c] = *at(sgsrc, (I)c * N + n1);
+        } else {
+          ieN[c] = S(0); sgN[c] = S(0);
+        }
       }
     }
     // ---- stage P(k)
@@ -192,7 +205,7 @@ k_adj_fused(GridDev g, MetricDev<S> m, const A *__restrict__ Win, const A *__res
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           ljv[c] = (c == 2 && online) ? 0.0 : -dt * (double)ie[c] * wv[c];
-          wn[c] = le[c] + (double)sg[c] * ljv[c];
+          wn[c] = kap ? le[c] + (double)sg[c] * wv[c] : le[c] + (double)sg[c] * ljv[c];
         }
       } else {
 #pragma unroll

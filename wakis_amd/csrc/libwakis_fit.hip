@@ -908,7 +908,8 @@ int adjoint_fused(wf_handle *h, const wf_adjoint_args *a, const wf::MetricDev<S>
   hipLaunchKernelGGL(kern, dim3((unsigned)nbf), dim3(threads), 0, st, g, m,
                      (const A *)a->W, (const A *)a->lH, (A *)a->W2, (A *)a->lH2, a->lJ_region,
                      (const S *)a->a, (const S *)a->b, (const S *)a->ieps, (const S *)a->sigma,
-                     a->dt, x, a->gcap, t, (A *)a->reg_W, a->reg_lJ, a->reg_slot);
+                     a->dt, x, a->gcap, t, (A *)a->reg_W, a->reg_lJ, a->reg_slot,
+                     (const S *)a->kappa);
   WF_LAUNCH_CHECK();
   return WF_OK;
 }
@@ -990,6 +991,8 @@ int wf_adjoint_step(wf_handle *h, const wf_adjoint_args *a, void *stream) {
   }
   if (a->fields_f32 && !a->fused)
     return fail(WF_EINVAL, "wf_adjoint_step: FP32 adjoint fields need the fused kernel");
+  if (a->kappa && !a->fused)
+    return fail(WF_EINVAL, "wf_adjoint_step: kappa is read by the fused kernel only");
   if (a->reg_W && (!a->fused || !a->reg_lJ || a->g_ieps || a->reg_slot < 0))
     return fail(WF_EINVAL, "wf_adjoint_step: reg_W needs the fused kernel, reg_lJ, a slot "
                 "and g_ieps = NULL");
