@@ -308,6 +308,10 @@ def main():
     ap.add_argument("--kz", type=int, default=0, help="z-planes per workgroup (0 auto)")
     ap.add_argument("--no-wake-check", action="store_true",
                     help="skip the FP32-vs-FP64 WP_z comparison (FP64 then runs steps/3)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="print only the reverse-mode cost (FP32 and FP64) as one JSON line")
+    ap.add_argument("--adjoint-steps", type=int, default=4000,
+                    help="wakefield steps of the reverse-mode workload")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the reverse-mode cost measurement")
     ap.add_argument("--batch", action="store_true",
@@ -319,6 +323,11 @@ def main():
         sys.exit(2)
     if args.batch:
         print(json.dumps(batch_sweep()), flush=True)
+        return
+    if args.adjoint_only:
+        out = {"fp32": adjoint_cost(steps=args.adjoint_steps),
+               "fp64": adjoint_cost(np.float64, steps=args.adjoint_steps)}
+        print(json.dumps(out), flush=True)
         return
     Nx, Ny, Nz = (int(v) for v in args.grid.split(","))
     import torch
@@ -340,9 +349,9 @@ def main():
         print(f"[bench] fp64 {_brief(r64)}", file=sys.stderr, flush=True)
     adj = adj64 = None
     if not args.no_adjoint:
-        adj = adjoint_cost()
+        adj = adjoint_cost(steps=args.adjoint_steps)
         print(f"[bench] adjoint {adj}", file=sys.stderr, flush=True)
-        adj64 = adjoint_cost(np.float64)
+        adj64 = adjoint_cost(np.float64, steps=args.adjoint_steps)
         print(f"[bench] adjoint fp64 {adj64}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
