@@ -180,7 +180,10 @@ def batch_sweep(n=64, steps=2000, seed=0, dtype=np.float32):
     """The batched sweep: n box cavities (wakis_amd.batch.random_box_cavities)
     with about `steps` wakefield steps each, run one after another with the
     single-grid kernel and then all together with wakesolve_batch; both timed
-    end to end (set-up, stepping, wake post-processing)."""
+    end to end (set-up, stepping, wake post-processing). Then the forward +
+    adjoint of each cavity's loss factor with respect to the eps_r of its
+    dielectric block, one problem after another (WakeAdjoint) and all together
+    (wake_adjoint_batch)."""
     import torch
 
     from wakis_amd import batch
@@ -207,14 +210,53 @@ def batch_sweep(n=64, steps=2000, seed=0, dtype=np.float32):
     torch.cuda.synchronize()
     t_bat = time.perf_counter() - t0
     same = all(np.array_equal(a.wake.WP_3d, b.wake.WP_3d) for a, b in zip(seq, bat))
+    nt_range = [int(min(s.Nt for s in seq)), int(max(s.Nt for s in seq))]
+    cells_total = int(sum(s.N for s in seq))
+    del seq, bat
+    torch.cuda.empty_cache()
+    # forward + adjoint of the loss factor: d k_loss / d eps_r over each dielectric block
+    from wakis_amd.adjoint import WakeAdjoint
+
+    def make_adj():
+        sv, wls = make()
+        return [WakeAdjoint(s, batch.dielectric_region(sp), w, w_loss=1.0, w_band=0.0)
+                for s, sp, w in zip(sv, specs, wls)]
+
+    adjs = make_adj()
+    ers = [np.full(adj.shape, sp["eps_r"]) for adj, sp in zip(adjs, specs)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ref = []
+    for adj, er in zip(adjs, ers):
+        L = adj.forward(er)
+        ref.append((L, adj.backward()[0]))
+    torch.cuda.synchronize()
+    t_seq_adj = time.perf_counter() - t0
+    ad_cells = 2 * sum(adj.s.N * adj.s.Nt for adj in adjs)  # forward + reverse steps
+    del adjs
+    torch.cuda.empty_cache()
+    adjs = make_adj()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses, grads = batch.wake_adjoint_batch(adjs, ers)
+    torch.cuda.synchronize()
+    t_bat_adj = time.perf_counter() - t0
+    same_adj = all(L == Lb and torch.equal(g, gb[0])
+                   for (L, g), Lb, gb in zip(ref, losses, grads))
+    adjoint = {"quantity": "k_loss and d k_loss / d eps_r of every dielectric-block cell",
+               "cell_updates": int(ad_cells),
+               "sequential": round(ad_cells / t_seq_adj / 1e9, 3),
+               "batched": round(ad_cells / t_bat_adj / 1e9, 3),
+               "speedup": round(t_seq_adj / t_bat_adj, 3), "bit_identical": bool(same_adj)}
     return {"metric": "batched wakefield sweep throughput", "unit": "Gcell-updates/s",
-            "grids": n, "steps_per_grid": [int(min(s.Nt for s in seq)), int(max(s.Nt for s in seq))],
-            "cells_total": int(sum(s.N for s in seq)), "cell_updates": int(cells),
+            "grids": n, "steps_per_grid": nt_range,
+            "cells_total": cells_total, "cell_updates": int(cells),
             "sequential": round(cells / t_seq / 1e9, 3), "batched": round(cells / t_bat / 1e9, 3),
             "speedup": round(t_seq / t_bat, 3),
             "batched_step_loop": round(info["cell_updates"] * info["main_steps"] / info["steps"]
                                        / (info["ms_main"] * 1e-3) / 1e9, 3),
             "shape": info["shape"], "wp_bit_identical": bool(same), "dtype": np.dtype(dtype).name,
+            "adjoint": adjoint,
             "data": "synthetic (random_box_cavities, seed %d)" % seed}
 
 

@@ -216,9 +216,10 @@ int wf_loss_factor(const double *WP, const double *lam, int64_t n, double ds, do
 
 /* Batches. No reference counterpart (the reference runs one grid per process);
  * for parameter sweeps over many small grids. Each grid keeps its handle and
- * a fused-path wf_step_args (path 1, material codes, zero_masked = 1, E2/H2;
- * beam table and capture optional), all with the same dtype, conductivity, J
- * mode and boundary order. args[i].nsteps is the number of steps grid i takes
+ * a fused-path wf_step_args (path 1, E2/H2; material codes with
+ * zero_masked = 1 for every grid, or coefficient arrays for every grid; beam
+ * table, capture and region trajectory optional), all with the same dtype,
+ * conductivity, J mode and boundary order. args[i].nsteps is the number of steps grid i takes
  * part in; beam_row0 and cap_sample0 refer to step 0. One kernel launch per
  * step covers the tiles of every grid (3 rows per workgroup, cells_per_lane 1
  * or the 16-byte width, 0 = 16-byte width; kz z-planes per workgroup, 0 = 32).
@@ -279,6 +280,7 @@ typedef struct wf_adjoint_args {
                              the region's lJ of step n. With reg_W set, g_ieps must be NULL:
                              the gradient is accumulated afterwards by wf_adjoint_grad    */
   int64_t reg_slot;
+  int64_t reg_slots;      /* slots in reg_W (reg_slot < reg_slots)                        */
   const void *kappa;      /* fused only, optional: per edge -dt*ieps*sigma (coefficient
                              dtype), zero on the beam line's z edges; the conduction term
                              becomes kappa*W and ieps is read on the region only        */
@@ -316,6 +318,24 @@ typedef struct wf_adjoint_grad_args {
 } wf_adjoint_grad_args;
 
 int wf_adjoint_grad(wf_handle *h, const wf_adjoint_grad_args *a, void *stream);
+
+/* Reverse mode over many grids, one launch per reverse step (no reference
+ * counterpart; the reverse of wf_batch_*). Grid i runs nsteps[i] reverse steps;
+ * reverse step r of the batch is step nsteps[i]-1-r of grid i (grids with fewer
+ * steps start later). args[i] as for wf_adjoint_step with the fused kernel and
+ * the region slots (fused = 1, W/lH and W2/lH2, reg_W/reg_lJ with reg_slots >=
+ * nsteps[i], g_ieps = NULL); cap_sample and reg_slot are set per step. Even r
+ * reads W/lH and writes W2/lH2, odd r the reverse. dtype and fields_f32 must
+ * match across the batch. The gradients are accumulated afterwards by
+ * wf_adjoint_grad, one call per grid. */
+typedef struct wf_batch_adjoint wf_batch_adjoint;
+int wf_batch_adjoint_create(wf_handle *const *handles, const wf_adjoint_args *args,
+                            const int32_t *nsteps, int32_t n, wf_batch_adjoint **out);
+/* Reverse steps [r0, r0 + nr); use_graph: replay a captured HIP graph of 16
+ * steps (built on the first such call). */
+int wf_batch_adjoint_run(wf_batch_adjoint *b, int32_t r0, int32_t nr, int32_t use_graph,
+                         void *stream);
+int wf_batch_adjoint_destroy(wf_batch_adjoint *b);
 
 /* Transpose of wf_wake_wp's line integral without the line-to-line carry:
  * gcap[l][m][k0+k] += sum over s indices n with it(n,k) == m of gI[l][n]*dz[k]

@@ -305,3 +305,41 @@ def test_fp32_adjoint_fields_close_to_fp64():
               flush=True)
         assert r32 < 1e-4
         assert r64 < 1e-2
+
+
+def test_lossless_adjoint_matches_finite_differences():
+    """A block without conductivity (and a solver without any): gradient with
+    respect to eps_r only, against central differences (h = 1e-4)."""
+    import wakis_amd as wa
+    from wakis_amd.adjoint import WakeAdjoint
+    print("adjoint lossless vs FD", flush=True)
+    x, y, z, mask = cases.cavity_pipe_geometry(*DIMS)
+    ins = np.zeros(DIMS, dtype=bool)
+    (i0, i1), (j0, j1), (k0, k1) = REGION
+    ins[i0:i1, j0:j1, k0:k1] = True
+    grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"cav": mask, "ins": ins},
+                        stl_materials={"cav": "vacuum", "ins": [2.0, 1.0]}, verbose=0)
+    wake = wa.WakeSolver(q=1e-9, sigmaz=10e-3, beta=1.0, skip_cells=2, save=False, verbose=0)
+    s = wa.SolverFIT3D(grid, wake=wake, bc_low=["pec", "pec", "abc"],
+                       bc_high=["pec", "pec", "abc"], use_stl=True, bg="pec",
+                       dtype=np.float64, verbose=0, use_sibc=False)
+    assert not s.use_conductivity
+    adj = WakeAdjoint(s, REGION, WAKELENGTH, w_loss=1.0, w_band=1e-13)
+    er, _ = _params(4)
+    adj.forward(er)
+    ge, gs = adj.backward()
+    assert gs is None and adj.used_kernel == "fused"
+    ge = ge.cpu().numpy()
+    rng = np.random.default_rng(9)
+    cells = [tuple(int(rng.integers(0, n)) for n in er.shape) for _ in range(8)]
+    h = 1e-4
+    fd = []
+    for c in cells:
+        Pp, Pm = er.copy(), er.copy()
+        Pp[c] += h
+        Pm[c] -= h
+        fd.append((adj.forward(Pp) - adj.forward(Pm)) / (2 * h))
+    fd = np.array(fd)
+    assert np.abs(fd).max() > 0
+    np.testing.assert_allclose([ge[c] for c in cells], fd, rtol=1e-3,
+                               atol=1e-3 * np.abs(fd).max())

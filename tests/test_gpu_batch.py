@@ -51,3 +51,41 @@ def test_batch_refuses_grids_off_the_fast_path():
     singles[1].E[0, 3, 3, "y"] = 1.0  # tangential E on the x-low PEC face: masked value != 0
     with pytest.raises(batch.BatchNotApplicable):
         batch.wakesolve_batch(singles, 0.02)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("per_cell", [False, True])
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_adjoint_batch_matches_single_problems(dtype, per_cell, use_graph):
+    """Batched forward (with the block trajectories captured in the step loop)
+    and batched reverse steps give each problem's own loss and gradient, bit for
+    bit. per_cell: random eps_r per block cell (coefficient arrays, no material
+    codes) instead of one eps_r per block."""
+    import torch
+    from wakis_amd import batch
+    from wakis_amd.adjoint import WakeAdjoint
+    print("adjoint batch", np.dtype(dtype).name, per_cell, use_graph, flush=True)
+    specs = batch.random_box_cavities(5, n=4, cell=2.0e-3)
+
+    def problems():
+        return [WakeAdjoint(batch.cavity_solver(sp, dtype=dtype), batch.dielectric_region(sp),
+                            0.02 + 0.01 * i, w_loss=1.0, w_band=1e-13)
+                for i, sp in enumerate(specs)]
+
+    singles, batched = problems(), problems()
+    rng = np.random.default_rng(2)
+    ers = [rng.uniform(1.0, 6.0, adj.shape) if per_cell else
+           np.full(adj.shape, sp["eps_r"]) for adj, sp in zip(singles, specs)]
+    ref = []
+    for adj, er in zip(singles, ers):
+        L = adj.forward(er)
+        g = adj.backward()
+        assert adj.used_kernel == "fused" and adj.used_mode == "trajectory"
+        ref.append((L, g))
+    assert len({adj.s.Nt for adj in singles}) > 1
+    assert (singles[0].s._code is None) == per_cell
+    losses, grads = batch.wake_adjoint_batch(batched, ers, use_graph=use_graph)
+    for (L, g), Lb, gb in zip(ref, losses, grads):
+        assert Lb == L
+        assert gb[1] is None
+        assert torch.equal(g[0], gb[0])

@@ -56,7 +56,7 @@ class WfAdjointArgs(ctypes.Structure):
         ("cap_samples", c_int64), ("cap_sample", c_int64), ("traj", c_int32),
         ("fused", c_int32), ("W2", c_void_p), ("lH2", c_void_p), ("fields_f32", c_int32),
         ("reg_W", c_void_p), ("reg_lJ", c_void_p), ("reg_slot", c_int64),
-        ("kappa", c_void_p),
+        ("reg_slots", c_int64), ("kappa", c_void_p),
     ]
 
 
@@ -98,6 +98,10 @@ EXPORTS = {
     "wf_batch_destroy": (c_int32, [c_void_p]),
     "wf_adjoint_step": (c_int32, [c_void_p, POINTER(WfAdjointArgs), c_void_p]),
     "wf_adjoint_grad": (c_int32, [c_void_p, POINTER(WfAdjointGradArgs), c_void_p]),
+    "wf_batch_adjoint_create": (c_int32, [POINTER(c_void_p), POINTER(WfAdjointArgs),
+                                          POINTER(c_int32), c_int32, POINTER(c_void_p)]),
+    "wf_batch_adjoint_run": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p]),
+    "wf_batch_adjoint_destroy": (c_int32, [c_void_p]),
     "wf_trajectory_gather": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                        POINTER(c_int32), POINTER(c_int32), c_int64, c_int64,
                                        c_void_p]),

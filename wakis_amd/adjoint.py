@@ -69,9 +69,9 @@ class WakeAdjoint:
         s = solver
         if s.wake is None:
             raise ValueError("WakeAdjoint: the solver needs a WakeSolver")
-        if not s.use_conductivity:
-            raise ValueError("WakeAdjoint: sigma is a parameter: build the solver with a "
-                             "conductive material or background")
+        # sigma is a parameter only on a conductive solver (a conductive material,
+        # background or PML); otherwise the gradient is with respect to eps_r alone
+        self.conductive = bool(s.use_conductivity)
         (self.i0, self.i1), (self.j0, self.j1), (self.k0, self.k1) = region
         if not (0 <= self.i0 < self.i1 <= s.Nx and 0 <= self.j0 < self.j1 <= s.Ny
                 and 0 <= self.k0 < self.k1 <= s.Nz):
@@ -88,15 +88,19 @@ class WakeAdjoint:
         self._nH = (e[0] + 1) * (e[1] + 1) * (e[2] + 1)
 
     # ------------------------------------------------------------- forward
-    def _set_params(self, eps_r, sigma):
+    def _set_params(self, eps_r, sigma=None):
         s = self.s
         er = np.asarray(eps_r, dtype=np.float64).reshape(self.shape)
-        sg = np.asarray(sigma, dtype=np.float64).reshape(self.shape)
+        if sigma is not None:
+            sg = np.asarray(sigma, dtype=np.float64).reshape(self.shape)
+            if not self.conductive and np.any(sg != 0):
+                raise ValueError("WakeAdjoint: nonzero sigma needs a conductive solver")
         ie = (1.0 / (er * eps_0)).astype(s.dtype)
         sl = (slice(self.i0, self.i1), slice(self.j0, self.j1), slice(self.k0, self.k1))
         for c in "xyz":
             s.ieps[sl + (c,)] = ie
-            s.sigma[sl + (c,)] = sg.astype(s.dtype)
+            if self.conductive and sigma is not None:
+                s.sigma[sl + (c,)] = sg.astype(s.dtype)
         s.update_tensors()
         s.reset_fields()
         s.step_0 = True
@@ -120,8 +124,9 @@ class WakeAdjoint:
         hi = (ctypes.c_int32 * 3)(self.i1, self.j1, self.k1)
         return lo, hi
 
-    def forward(self, eps_r, sigma):
-        """Run the wakefield simulation with these parameters; return L."""
+    def forward(self, eps_r, sigma=None):
+        """Run the wakefield simulation with these parameters (sigma: None keeps
+        the solver's, as for a solver without conductivity); return L."""
         s = self.s
         self._fwd = None  # release the previous run's trajectory or checkpoints
         self._set_params(eps_r, sigma)
@@ -236,8 +241,9 @@ class WakeAdjoint:
             float(w.ti), float(dt), nt, _lib.ptr(gcap), _lib.stream_ptr()), "wf_wake_wp_adjoint")
         return gcap
 
-    def backward(self):
-        """(dL/d eps_r, dL/d sigma) of the last forward, as FP64 torch tensors."""
+    def _reverse_setup(self):
+        """Adjoint fields, wf_adjoint_args and wf_adjoint_grad_args of the last
+        forward run (the reverse pass's state), as a dict."""
         if self._fwd is None:
             raise RuntimeError("WakeAdjoint.backward: run forward first")
         s = self.s
@@ -266,7 +272,7 @@ class WakeAdjoint:
         a = _lib.WfAdjointArgs()
         a.W, a.lH, a.lJ_region = _lib.ptr(W), _lib.ptr(lH), _lib.ptr(lJr)
         a.dt = float(s.dt)
-        a.conductive = 1
+        a.conductive = int(self.conductive)
         a.ixs, a.iys = int(beam.ixs), int(beam.iys)
         a.r0[:] = (self.i0, self.j0, self.k0)
         a.r1[:] = (self.i1, self.j1, self.k1)
@@ -280,12 +286,13 @@ class WakeAdjoint:
         a.metric2d = _lib.ptr(s._metric2d_all)
         lib, st = _lib.lib(), _lib.stream_ptr()
         row_bytes = table.shape[1] * table.element_size()
+        W2 = lH2 = kap = None
         if adj_fused:
             W2, lH2 = torch.zeros_like(W), torch.zeros_like(lH)
             a.fused = int(self.fused_rows) or 1
             a.W2, a.lH2 = _lib.ptr(W2), _lib.ptr(lH2)
             a.fields_f32 = int(f32)
-            if self.use_kappa:
+            if self.use_kappa and self.conductive:
                 kap = (-float(s.dt) * s.ieps.array.to(torch.float64)
                        * s.sigma.array.to(torch.float64))
                 kz = 2 * s.N + int(beam.ixs) + s.Nx * int(beam.iys) + s.Nx * s.Ny * torch.arange(
@@ -306,7 +313,7 @@ class WakeAdjoint:
         ga = _lib.WfAdjointGradArgs()
         ga.sigma, ga.metric2d = a.sigma, a.metric2d
         ga.fields_f32 = int(f32)
-        ga.dt, ga.conductive, ga.ixs, ga.iys = a.dt, 1, a.ixs, a.iys
+        ga.dt, ga.conductive, ga.ixs, ga.iys = a.dt, a.conductive, a.ixs, a.iys
         ga.r0[:] = tuple(a.r0)
         ga.r1[:] = tuple(a.r1)
         ga.g_ieps, ga.g_sigma = _lib.ptr(g_ieps), _lib.ptr(g_sig)
@@ -317,6 +324,22 @@ class WakeAdjoint:
             ga.reg_W, ga.reg_lJ, ga.nsteps = _lib.ptr(rW_), _lib.ptr(rJ_), nsteps
             _lib.check(lib.wf_adjoint_grad(s._handle, ctypes.byref(ga), st), "wf_adjoint_grad")
 
+        return dict(a=a, ga=ga, adjoint_step=adjoint_step, grad_pass=grad_pass, lib=lib, st=st,
+                    fused=adj_fused, f32=f32, adt=adt, n3=n3, nreg=nreg, n3r=n3r, table=table,
+                    row_bytes=row_bytes, g_ieps=g_ieps, g_sig=g_sig,
+                    keep=[W, lH, lJr, gcap, W2, lH2, kap])
+
+    def backward(self):
+        """(dL/d eps_r, dL/d sigma) of the last forward, as FP64 torch tensors
+        (dL/d sigma is None on a solver without conductivity)."""
+        R = self._reverse_setup()
+        s, f = self.s, self._fwd
+        ctx, Nt = f["ctx"], f["Nt"]
+        dev = s.device
+        a, adjoint_step, grad_pass = R["a"], R["adjoint_step"], R["grad_pass"]
+        adj_fused, adt, n3, n3r = R["fused"], R["adt"], R["n3"], R["n3r"]
+        table, row_bytes, lib, st = R["table"], R["row_bytes"], R["lib"], R["st"]
+        g_ieps, g_sig = R["g_ieps"], R["g_sig"]
         if "traj" in f:
             tE, tH = f["traj"]
             a.traj = 1
@@ -326,7 +349,7 @@ class WakeAdjoint:
             if adj_fused:
                 rW = torch.empty((max(Nt, 1), n3r), dtype=adt, device=dev)
                 rJ = torch.zeros((Nt + 1, n3r), dtype=torch.float64, device=dev)
-                a.reg_W, a.reg_lJ = _lib.ptr(rW), _lib.ptr(rJ)
+                a.reg_W, a.reg_lJ, a.reg_slots = _lib.ptr(rW), _lib.ptr(rJ), max(Nt, 1)
                 a.g_ieps = a.g_sigma = None
             for n in range(Nt - 1, -1, -1):
                 a.E0, a.E1 = pE + n * sE, pE + (n + 1) * sE
@@ -351,7 +374,7 @@ class WakeAdjoint:
             rW = torch.empty((K, n3r), dtype=adt, device=dev)
             rJ = torch.zeros((K + 1, n3r), dtype=torch.float64, device=dev)
             carry = torch.zeros(n3r, dtype=torch.float64, device=dev)  # lJ after the segment
-            a.reg_W, a.reg_lJ = _lib.ptr(rW), _lib.ptr(rJ)
+            a.reg_W, a.reg_lJ, a.reg_slots = _lib.ptr(rW), _lib.ptr(rJ), K
             a.g_ieps = a.g_sigma = None
         for (n0, E0, H0, J0) in reversed(f["ckpt"]):
             k = min(K, Nt - n0)
@@ -393,18 +416,21 @@ class WakeAdjoint:
         sh = self.shape
         # per cell (x fastest): sum the three components; ieps = 1/(eps_r eps0)
         gi = g_ieps.view(3, sh[2], sh[1], sh[0]).sum(0).permute(2, 1, 0)
-        gs = g_sig.view(3, sh[2], sh[1], sh[0]).sum(0).permute(2, 1, 0)
+        gs = None
+        if self.conductive:
+            gs = g_sig.view(3, sh[2], sh[1], sh[0]).sum(0).permute(2, 1, 0).contiguous()
         ie = s.ieps.to_matrix("x")[self.i0:self.i1, self.j0:self.j1,
                                    self.k0:self.k1].to(torch.float64)
         # d ieps / d eps_r = -eps0 ieps^2
         g_eps = gi * (-eps_0 * ie * ie)
-        return g_eps.contiguous(), gs.contiguous()
+        return g_eps.contiguous(), gs
 
 
 class _WakeLossFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, eps_r, sigma, adj):
-        L = adj.forward(eps_r.detach().cpu().numpy(), sigma.detach().cpu().numpy())
+        sg = None if sigma is None else sigma.detach().cpu().numpy()
+        L = adj.forward(eps_r.detach().cpu().numpy(), sg)
         ctx.adj = adj
         ctx.dev = eps_r.device
         return torch.tensor(L, dtype=torch.float64, device=eps_r.device)
@@ -412,9 +438,10 @@ class _WakeLossFn(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gL):
         ge, gs = ctx.adj.backward()
-        return (gL * ge).to(ctx.dev), (gL * gs).to(ctx.dev), None
+        return ((gL * ge).to(ctx.dev), None if gs is None else (gL * gs).to(ctx.dev), None)
 
 
 def wake_loss(eps_r, sigma, adj):
-    """L(eps_r, sigma) as a differentiable torch scalar (see WakeAdjoint)."""
+    """L(eps_r, sigma) as a differentiable torch scalar (see WakeAdjoint);
+    ``sigma`` None on a solver without conductivity."""
     return _WakeLossFn.apply(eps_r, sigma, adj)

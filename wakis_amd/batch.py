@@ -65,9 +65,10 @@ def wakesolve_batch(solvers, wakelength, compute_plane="both", cells_per_lane=No
     value or one per solver).
 
     Each solver needs its ``wake`` (WakeSolver) and must qualify for the fused
-    path with material codes and zero-valued masked fields (the usual state of
-    a fresh wakefield run); otherwise BatchNotApplicable is raised before
-    anything runs. ``cells_per_lane``/``kz`` fix the tile shape; with
+    path: with material codes and zero-valued masked fields (the usual state
+    of a fresh wakefield run), or for every grid without material codes
+    (coefficient arrays, masks applied in the kernel); otherwise
+    BatchNotApplicable is raised before anything runs. ``cells_per_lane``/``kz`` fix the tile shape; with
     ``autotune`` the candidates are timed on successive chunks of
     ``tune_steps`` steps of the run itself and the fastest runs the rest.
     Returns a dict with the chosen shape, the step count and the timing.
@@ -84,9 +85,11 @@ def wakesolve_batch(solvers, wakelength, compute_plane="both", cells_per_lane=No
         a = s._wake_args(ctx, 0, ctx["Nt"])
         if not s._prepare_fused(a):
             raise BatchNotApplicable("a grid of the batch does not qualify for the fused path")
-        if not a.code or not a.zero_masked:
-            raise BatchNotApplicable("batched grids need material codes and zero masked fields")
+        if a.code and not a.zero_masked:
+            raise BatchNotApplicable("batched grids with material codes need zero masked fields")
         args.append(a)
+    if len({bool(a.code) for a in args}) != 1:
+        raise BatchNotApplicable("all grids of a batch use material codes, or none")
     nmax = max(ctx["Nt"] for ctx in ctxs)
     if cells_per_lane is not None or kz is not None or not autotune:
         cands = [(cells_per_lane or 0, kz or 0)]
@@ -138,6 +141,123 @@ def wakesolve_batch(solvers, wakelength, compute_plane="both", cells_per_lane=No
     return info
 
 
+class _AdjointBatch:
+    def __init__(self, solvers, args, nsteps):
+        n = len(solvers)
+        handles = (ctypes.c_void_p * n)(*[s._handle.value for s in solvers])
+        arr = (_lib.WfAdjointArgs * n)(*args)
+        nst = (ctypes.c_int32 * n)(*[int(k) for k in nsteps])
+        self.ptr = ctypes.c_void_p()
+        _lib.check(_lib.lib().wf_batch_adjoint_create(handles, arr, nst, n,
+                                                      ctypes.byref(self.ptr)),
+                   "wf_batch_adjoint_create")
+        self._keep = (handles, arr, nst)
+
+    def run(self, r0, nr, use_graph):
+        _lib.check(_lib.lib().wf_batch_adjoint_run(self.ptr, int(r0), int(nr), int(use_graph),
+                                                   _lib.stream_ptr()), "wf_batch_adjoint_run")
+
+    def close(self):
+        if self.ptr is not None and self.ptr.value:
+            _lib.lib().wf_batch_adjoint_destroy(self.ptr)
+        self.ptr = None
+
+    __del__ = close
+
+
+def wake_adjoint_batch(adjs, eps_r, sigma=None, cells_per_lane=0, kz=0, use_graph=True):
+    """Loss and gradients of several ``WakeAdjoint`` problems (one solver and
+    block each) computed together.
+
+    The forward runs are one batched launch per step (``wf_batch_*``), with
+    each block's trajectory captured inside the step loop. The reverse steps
+    are one batched launch per step (``wf_batch_adjoint_*``, the single-pass
+    reverse kernel over all grids). Each grid's gradients are then accumulated
+    in one pass. The results equal each problem's own ``forward`` and
+    ``backward`` (trajectory mode, fused reverse kernel).
+
+    ``eps_r``, ``sigma``: one block array per problem (``sigma`` None for
+    solvers without conductivity). Returns (losses, gradients), where
+    gradients[i] = (dL/d eps_r, dL/d sigma or None). Raises
+    BatchNotApplicable before anything runs if a grid cannot take the batched
+    paths.
+    """
+    n = len(adjs)
+    if n == 0:
+        raise ValueError("wake_adjoint_batch: no problems")
+    sigma = list(sigma) if sigma is not None else [None] * n
+    solvers = [adj.s for adj in adjs]
+    if len({np.dtype(s.dtype) for s in solvers}) != 1:
+        raise BatchNotApplicable("all solvers of a batch need the same dtype")
+    for adj in adjs:
+        if min(adj.i0, adj.j0, adj.k0) < 1:
+            raise BatchNotApplicable("batched blocks start at cell 1 or later on every axis")
+    # ---- forward: parameters, wake set-up, trajectories (slot 0 before the run)
+    args, nts = [], []
+    for adj, er, sg in zip(adjs, eps_r, sigma):
+        s = adj.s
+        adj._fwd = None
+        adj._set_params(er, sg)
+        ctx = s._wake_setup(adj.wakelength, "both")
+        Nt = ctx["Nt"]
+        a = s._wake_args(ctx, 0, Nt)
+        if not s._prepare_fused(a):
+            raise BatchNotApplicable("a grid of the batch does not qualify for the fused path")
+        tE = torch.empty((Nt + 1, 3 * adj._nE), dtype=s.E.array.dtype, device=s.device)
+        tH = torch.empty((Nt + 1, 3 * adj._nH), dtype=s.E.array.dtype, device=s.device)
+        lo, hi = adj._lohi()
+        _lib.check(_lib.lib().wf_trajectory_gather(
+            s._handle, _lib.ptr(s.E.array), _lib.ptr(s.H.array), _lib.ptr(tE), _lib.ptr(tH),
+            lo, hi, Nt + 1, 0, _lib.stream_ptr()), "wf_trajectory_gather")
+        a.traj_E, a.traj_H = _lib.ptr(tE), _lib.ptr(tH)
+        a.traj_lo[:] = tuple(lo)
+        a.traj_hi[:] = tuple(hi)
+        a.traj_slots, a.traj_slot0 = Nt + 1, 0
+        adj._fwd = dict(ctx=ctx, Nt=Nt, traj=(tE, tH))
+        args.append(a)
+        nts.append(Nt)
+    if len({bool(a.code) for a in args}) != 1:
+        raise BatchNotApplicable("all grids of a batch use material codes, or none")
+    if any(a.code and not a.zero_masked for a in args):
+        raise BatchNotApplicable("batched grids with material codes need zero masked fields")
+    nmax = max(nts)
+    b = _Batch(solvers, args, cells_per_lane, kz)
+    b.run(0, nmax, use_graph)
+    b.close()
+    losses = []
+    for adj, Nt in zip(adjs, nts):
+        s = adj.s
+        s._finish_run(True, Nt % 2 == 1)
+        s.nstep += Nt
+        s._wake_finish(adj._fwd["ctx"], "both")
+        losses.append(adj._loss_value())
+    # ---- reverse: per-grid state, batched steps, one gradient pass per grid
+    Rs, rargs, regs = [], [], []
+    for adj, Nt in zip(adjs, nts):
+        R = adj._reverse_setup()
+        if not R["fused"]:
+            raise BatchNotApplicable("the fused reverse kernel does not apply to a grid")
+        a = R["a"]
+        rW = torch.empty((max(Nt, 1), R["n3r"]), dtype=R["adt"], device=adj.s.device)
+        rJ = torch.zeros((Nt + 1, R["n3r"]), dtype=torch.float64, device=adj.s.device)
+        a.reg_W, a.reg_lJ, a.reg_slots = _lib.ptr(rW), _lib.ptr(rJ), max(Nt, 1)
+        a.g_ieps = a.g_sigma = None
+        Rs.append(R)
+        rargs.append(a)
+        regs.append((rW, rJ))
+    rb = _AdjointBatch(solvers, rargs, nts)
+    rb.run(0, nmax, use_graph)
+    rb.close()
+    grads = []
+    for adj, R, (rW, rJ), Nt in zip(adjs, Rs, regs, nts):
+        tE, tH = adj._fwd["traj"]
+        R["grad_pass"](tE.data_ptr(), tE.stride(0), tH.data_ptr(), tH.stride(0), 1,
+                       R["table"].data_ptr(), rW, rJ, Nt)
+        grads.append(adj._finish_grad(R["g_ieps"], R["g_sig"]))
+    torch.cuda.synchronize()
+    return losses, grads
+
+
 # --------------------------------------------------------------------------
 # Synthetic sweep: box cavities with a beam pipe and a dielectric block
 # --------------------------------------------------------------------------
@@ -175,6 +295,13 @@ def random_box_cavities(seed, n=64, cell=1.0e-3):
                         dielectric=diel.cell_mask(x, y, z), eps_r=eps_r,
                         sides=(a, b, c), r_pipe=r))
     return out
+
+
+def dielectric_region(spec):
+    """((i0, i1), (j0, j1), (k0, k1)): the cell bounding box of a
+    ``random_box_cavities`` entry's dielectric block."""
+    idx = np.nonzero(spec["dielectric"])
+    return tuple((int(v.min()), int(v.max()) + 1) for v in idx)
 
 
 def cavity_solver(spec, dtype=np.float32, sigmaz=5e-3, q=1e-9, beta=1.0):

@@ -21,6 +21,7 @@
 #pragma once
 #include "fit_common.h"
 #include "step_general.h"
+#include "step_fused4.h"
 
 namespace wf {
 
@@ -54,12 +55,11 @@ __device__ __forceinline__ int64_t traj_h_index(const TrajBox &b, int comp, int 
                                                            (int64_t)hy * (k - b.lo[2] + 1));
 }
 
-// one trajectory slot: tE[3 nE] from E, tH[3 nH] from H
+// the element t of a trajectory slot (E box, then H box) from the fields E, H
 template <typename S>
-__global__ void __launch_bounds__(256)
-k_traj_gather(GridDev g, const S *__restrict__ E, const S *__restrict__ H, TrajBox b,
-              S *__restrict__ tE, S *__restrict__ tH) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void traj_copy(const GridDev &g, const S *__restrict__ E,
+                                          const S *__restrict__ H, const TrajBox &b, int64_t t,
+                                          S *__restrict__ tE, S *__restrict__ tH) {
   if (t < 3 * b.nE) {
     const int comp = (int)(t / b.nE);
     const int64_t r = t - comp * b.nE;
@@ -77,6 +77,38 @@ k_traj_gather(GridDev g, const S *__restrict__ E, const S *__restrict__ H, TrajB
     const int k = b.lo[2] - 1 + (int)(r / ((int64_t)hx * hy));
     tH[u] = H[comp * g.N + i + (int64_t)g.Nx * (j + (int64_t)g.Ny * k)];
   }
+}
+
+// one trajectory slot: tE[3 nE] from E, tH[3 nH] from H
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_traj_gather(GridDev g, const S *__restrict__ E, const S *__restrict__ H, TrajBox b,
+              S *__restrict__ tE, S *__restrict__ tH) {
+  traj_copy<S>(g, E, H, b, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, tE, tH);
+}
+
+// the trajectory slots of every grid of a batch after the step the batch
+// kernel just ran (step number from device memory, as k_fused4_batch)
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_traj_gather_batch(const BatchDesc<S> *__restrict__ d, int ninst, int64_t total,
+                    const int *__restrict__ step_ptr) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int lo = 0, hi = ninst - 1;  // last descriptor with thr0 <= t
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].thr0 <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const BatchDesc<S> &e = d[lo];
+  const int step = *step_ptr;
+  if (!e.tE || step >= e.nsteps) return;
+  const TrajBox b = traj_box(e.tlo, e.thi);
+  const int64_t slot = e.traj_slot0 + step + 1;
+  const bool odd = step & 1;  // the step wrote E2/H2 when even, E/H when odd
+  traj_copy<S>(e.g, odd ? e.E : e.E2, odd ? e.H : e.H2, b, t - e.thr0, e.tE + slot * 3 * b.nE,
+               e.tH + slot * 3 * b.nH);
 }
 
 // parameter gradients of one step over the region (reads W and the region's lJ

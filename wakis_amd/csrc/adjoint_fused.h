@@ -47,14 +47,46 @@ __device__ __forceinline__ bool keep_row(int bits, int c, int i, int j, int k, c
   return bits == 0 || face_mask(bits, c, i, j, k, g.Nx, g.Ny, g.Nz);
 }
 
+// Everything one reverse step of one grid reads (kernel argument by value).
+template <typename S, typename A>
+struct AdjStep {
+  GridDev g;
+  MetricDev<S> m;
+  const A *Win, *lHin;
+  A *Wout, *lHout;
+  double *lJr;
+  const S *a, *b, *ieps, *sigma, *kappa;
+  double dt;
+  AdjAux x;
+  const double *gcap;
+  AdjTile t;
+  A *rW;
+  double *rJ;
+  int64_t rslot;
+};
+
+// One workgroup's z-march over tile `tile` of one grid's reverse step.
 template <typename S, typename A, int TY>
-__global__ void __launch_bounds__(64 * (TY + 3))
-k_adj_fused(GridDev g, MetricDev<S> m, const A *__restrict__ Win, const A *__restrict__ lHin,
-            A *__restrict__ Wout, A *__restrict__ lHout, double *__restrict__ lJr,
-            const S *__restrict__ a, const S *__restrict__ b, const S *__restrict__ ieps,
-            const S *__restrict__ sigma, double dt, AdjAux x, const double *__restrict__ gcap,
-            AdjTile t, A *__restrict__ rW, double *__restrict__ rJ, int64_t rslot,
-            const S *__restrict__ kappa) {
+__device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int tile) {
+  const GridDev &g = p.g;
+  const MetricDev<S> &m = p.m;
+  const A *__restrict__ Win = p.Win;
+  const A *__restrict__ lHin = p.lHin;
+  A *__restrict__ Wout = p.Wout;
+  A *__restrict__ lHout = p.lHout;
+  double *__restrict__ lJr = p.lJr;
+  const S *__restrict__ a = p.a;
+  const S *__restrict__ b = p.b;
+  const S *__restrict__ ieps = p.ieps;
+  const S *__restrict__ sigma = p.sigma;
+  const S *__restrict__ kappa = p.kappa;
+  const double dt = p.dt;
+  const AdjAux &x = p.x;
+  const double *__restrict__ gcap = p.gcap;
+  const AdjTile &t = p.t;
+  A *__restrict__ rW = p.rW;
+  double *__restrict__ rJ = p.rJ;
+  const int64_t rslot = p.rslot;
   static_assert(TY + 3 <= 16, "at most 1024 threads per workgroup");
   using I = uint32_t;
   constexpr int NW = TY + 2;              // row waves
@@ -62,7 +94,6 @@ k_adj_fused(GridDev g, MetricDev<S> m, const A *__restrict__ Win, const A *__res
   __shared__ double sP[3][NW + 1][NS];    // row NW: zeros (above the top row)
   __shared__ double sQ[3][NW][NS];
 
-  const int tile = xcd_tile(blockIdx.x, t.ntx * t.nty * t.ntz);
   const int tz = tile / (t.ntx * t.nty);
   const int rem = tile - tz * t.ntx * t.nty;
   const int tyb = rem / t.ntx, txb = rem - tyb * t.ntx;
@@ -243,6 +274,46 @@ k_adj_fused(GridDev g, MetricDev<S> m, const A *__restrict__ Win, const A *__res
       lh[c] = lhN[c]; ac[c] = acN[c]; ie[c] = ieN[c]; sg[c] = sgN[c];
     }
   }
+}
+
+template <typename S, typename A, int TY>
+__global__ void __launch_bounds__(64 * (TY + 3)) k_adj_fused(AdjStep<S, A> p) {
+  adj_fused_tile<S, A, TY>(p, xcd_tile(blockIdx.x, p.t.ntx * p.t.nty * p.t.ntz));
+}
+
+// One grid of a reverse batch. Reverse step r of the batch is step Nt-1-r of
+// the grid (grids with fewer steps start later); even r reads W/lH and writes
+// W2/lH2, odd r the reverse.
+template <typename S, typename A>
+struct AdjBatchDesc {
+  AdjStep<S, A> p;  // W/lH in p.Win/p.lHin, W2/lH2 in p.Wout/p.lHout
+  int Nt;
+  int tile0;
+};
+
+template <typename S, typename A, int TY>
+__global__ void __launch_bounds__(64 * (TY + 3))
+k_adj_fused_batch(const AdjBatchDesc<S, A> *__restrict__ d, int ninst, int ntiles,
+                  const int *__restrict__ rptr) {
+  const int r = *rptr;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  int lo = 0, hi = ninst - 1;  // last descriptor with tile0 <= tile
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].tile0 <= tile) lo = mid;
+    else hi = mid - 1;
+  }
+  const int n = d[lo].Nt - 1 - r;
+  if (n < 0) return;  // whole workgroup, before any barrier
+  AdjStep<S, A> p = d[lo].p;
+  if (r & 1) {
+    const A *wi = p.Win, *li = p.lHin;
+    p.Win = p.Wout; p.lHin = p.lHout;
+    p.Wout = const_cast<A *>(wi); p.lHout = const_cast<A *>(li);
+  }
+  p.x.cap_sample = n;
+  p.rslot = n;
+  adj_fused_tile<S, A, TY>(p, tile - d[lo].tile0);
 }
 
 }  // namespace wf

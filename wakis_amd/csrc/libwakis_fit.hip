@@ -684,7 +684,9 @@ int wf_loss_factor(const double *WP, const double *lam, int64_t n, double ds, do
 // ---------------------------------------------------------------------------
 struct wf_batch {
   int dtype = 0, ninst = 0, ntiles = 0, maxsteps = 0;
+  int64_t traj_threads = 0;   // batched trajectory gather threads (0: none)
   int rows = 3, vw = 1, cond = 0, jm = 0, desc = 0;
+  int lut = 1;                // material codes (1) or coefficient arrays (0)
   size_t lds = 0;
   void *desc_dev = nullptr;   // BatchDesc<S>[ninst]
   int *step_dev = nullptr;    // step number read by the batch kernel
@@ -718,11 +720,13 @@ int batch_build(wf_handle *const *hs, const wf_step_args *as, int n, int vw, int
     cf.a = (const S *)a->a; cf.b = (const S *)a->b;
     cf.ieps = (const S *)a->ieps; cf.sigma = (const S *)a->sigma;
     cf.code = (const uint8_t *)a->code; cf.mat = (const S *)a->mat; cf.nmat = a->nmat;
-    const S *t = (const S *)a->metric2d;
-    const int64_t yz = (int64_t)h->g.Ny * h->g.Nz, xz = (int64_t)h->g.Nx * h->g.Nz,
-                  xy = (int64_t)h->g.Nx * h->g.Ny;
-    cf.iAx = t; cf.iAy = t + yz; cf.iAz = t + yz + xz;
-    cf.itAx = t + yz + xz + xy; cf.itAy = cf.itAx + yz; cf.itAz = cf.itAy + xz;
+    if (b->lut) {
+      const S *t = (const S *)a->metric2d;
+      const int64_t yz = (int64_t)h->g.Ny * h->g.Nz, xz = (int64_t)h->g.Nx * h->g.Nz,
+                    xy = (int64_t)h->g.Nx * h->g.Ny;
+      cf.iAx = t; cf.iAy = t + yz; cf.iAz = t + yz + xz;
+      cf.itAx = t + yz + xz + xy; cf.itAy = cf.itAx + yz; cf.itAz = cf.itAy + xz;
+    }
     e.dt = (S)a->dt;
     wf::FusedAux &x = e.x;
     x.ixs = (a->jmode == WF_J_LINE || a->beam_table || a->conductive) ? a->ixs : -1;
@@ -741,7 +745,15 @@ int batch_build(wf_handle *const *hs, const wf_step_args *as, int n, int vw, int
     e.nsteps = a->nsteps;
     e.tile0 = (int)tiles;
     tiles += (int64_t)x.ntx * x.nty * x.ntz;
-    lds = std::max(lds, (size_t)a->nmat * 9 * sizeof(S));
+    e.thr0 = b->traj_threads;
+    if (a->traj_E) {
+      e.tE = (S *)a->traj_E; e.tH = (S *)a->traj_H;
+      for (int q = 0; q < 3; ++q) { e.tlo[q] = a->traj_lo[q]; e.thi[q] = a->traj_hi[q]; }
+      e.traj_slot0 = a->traj_slot0;
+      const wf::TrajBox tb = wf::traj_box(a->traj_lo, a->traj_hi);
+      b->traj_threads += 3 * (tb.nE + tb.nH);
+    }
+    if (b->lut) lds = std::max(lds, (size_t)a->nmat * 9 * sizeof(S));
     b->maxsteps = std::max(b->maxsteps, a->nsteps);
   }
   if (tiles > 2147483647LL) return fail(WF_ERANGE, "wf_batch_create: too many tiles");
@@ -759,8 +771,16 @@ int batch_launch(const wf_batch *b, hipStream_t st) {
   const auto *d = (const wf::BatchDesc<S> *)b->desc_dev;
   const dim3 grid((unsigned)b->ntiles), block(64 * (BATCH_ROWS + 1));
 #define WF_B_LAUNCH(JM, CO, DE, VW)                                                          \
-  hipLaunchKernelGGL((wf::k_fused4_batch<S, JM, CO, true, DE, BATCH_ROWS, false, VW>), grid, \
-                     block, b->lds, st, d, b->ninst, b->ntiles, (const int *)b->step_dev)
+  do {                                                                                       \
+    if (b->lut)                                                                              \
+      hipLaunchKernelGGL((wf::k_fused4_batch<S, JM, CO, true, DE, BATCH_ROWS, false, VW>),   \
+                         grid, block, b->lds, st, d, b->ninst, b->ntiles,                    \
+                         (const int *)b->step_dev);                                          \
+    else                                                                                     \
+      hipLaunchKernelGGL((wf::k_fused4_batch<S, JM, CO, false, DE, BATCH_ROWS, true, VW>),   \
+                         grid, block, 0, st, d, b->ninst, b->ntiles,                         \
+                         (const int *)b->step_dev);                                          \
+  } while (0)
 #define WF_B_LAUNCH2(JM, CO, VW)                                                             \
   do {                                                                                       \
     if (b->desc) WF_B_LAUNCH(JM, CO, true, VW);                                              \
@@ -778,6 +798,12 @@ int batch_launch(const wf_batch *b, hipStream_t st) {
 #undef WF_B_LAUNCH2
 #undef WF_B_LAUNCH
   WF_LAUNCH_CHECK();
+  if (b->traj_threads > 0) {
+    hipLaunchKernelGGL(wf::k_traj_gather_batch<S>, dim3(nblocks(b->traj_threads, 256)),
+                       dim3(256), 0, st, d, b->ninst, b->traj_threads,
+                       (const int *)b->step_dev);
+    WF_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL(wf::k_batch_next, dim3(1), dim3(1), 0, st, b->step_dev);
   WF_LAUNCH_CHECK();
   return WF_OK;
@@ -814,9 +840,11 @@ int wf_batch_create(wf_handle *const *handles, const wf_step_args *args, int32_t
     const int rc = check_step_args(h, a, who);
     if (rc != WF_OK) return rc;
     if (a->path != 1) return fail(WF_EINVAL, "%s: batch items use the fused path (path 1)", who);
-    if (a->traj_E) return fail(WF_EINVAL, "%s: batch items take no trajectory", who);
-    if (!a->code || !a->zero_masked)
-      return fail(WF_EINVAL, "%s: batch items need material codes and zero_masked = 1", who);
+    const bool lut = a->code && a->mat && a->nmat > 0 && a->metric2d;
+    if (lut != (a0.code && a0.mat && a0.nmat > 0 && a0.metric2d))
+      return fail(WF_EINVAL, "%s: all batch items use material codes, or none", who);
+    if (lut && !a->zero_masked)
+      return fail(WF_EINVAL, "%s: batch items with material codes need zero_masked = 1", who);
     if (h->dtype != dtype || a->conductive != a0.conductive || a->jmode != a0.jmode ||
         h->g.m1desc != h0->g.m1desc)
       return fail(WF_EINVAL, "%s: dtype, conductivity, J mode and boundary order must match "
@@ -829,6 +857,7 @@ int wf_batch_create(wf_handle *const *handles, const wf_step_args *args, int32_t
   wf_batch *b = new wf_batch();
   b->dtype = dtype; b->ninst = n; b->vw = vw;
   b->cond = a0.conductive; b->jm = a0.jmode; b->desc = h0->g.m1desc;
+  b->lut = (a0.code && a0.mat && a0.nmat > 0 && a0.metric2d) ? 1 : 0;
   const int rc = dtype == WF_F64 ? batch_build<double>(handles, args, n, vw, kz, b)
                                  : batch_build<float>(handles, args, n, vw, kz, b);
   if (rc != WF_OK) {
@@ -888,8 +917,36 @@ int wf_batch_destroy(wf_batch *b) {
 // ---------------------------------------------------------------------------
 namespace {
 
+wf::AdjAux adj_aux(const wf_adjoint_args *a) {
+  wf::AdjAux x;
+  memset(&x, 0, sizeof(x));
+  x.conductive = a->conductive;
+  x.ixs = a->ixs; x.iys = a->iys;
+  for (int d = 0; d < 3; ++d) { x.r0[d] = a->r0[d]; x.r1[d] = a->r1[d]; }
+  x.cap_ix0 = a->cap_ix0; x.cap_iy0 = a->cap_iy0; x.cap_nx = a->cap_nx; x.cap_ny = a->cap_ny;
+  x.cap_samples = a->cap_samples; x.cap_sample = a->cap_sample;
+  x.traj = a->traj;
+  return x;
+}
+
 constexpr int ADJ_ROWS = 13;  // owned rows per workgroup of the single-pass reverse kernel
                                // (16 waves with the halo rows and column; measured best)
+
+template <typename S, typename A>
+wf::AdjStep<S, A> adj_step_params(const wf_handle *h, const wf_adjoint_args *a,
+                                  const wf::MetricDev<S> &m, const wf::AdjAux &x,
+                                  const wf::AdjTile &t) {
+  wf::AdjStep<S, A> p;
+  p.g = h->g; p.m = m;
+  p.Win = (const A *)a->W; p.lHin = (const A *)a->lH;
+  p.Wout = (A *)a->W2; p.lHout = (A *)a->lH2;
+  p.lJr = a->lJ_region;
+  p.a = (const S *)a->a; p.b = (const S *)a->b;
+  p.ieps = (const S *)a->ieps; p.sigma = (const S *)a->sigma; p.kappa = (const S *)a->kappa;
+  p.dt = a->dt; p.x = x; p.gcap = a->gcap; p.t = t;
+  p.rW = (A *)a->reg_W; p.rJ = a->reg_lJ; p.rslot = a->reg_slot;
+  return p;
+}
 
 template <typename S, typename A, int TY>
 int adjoint_fused(wf_handle *h, const wf_adjoint_args *a, const wf::MetricDev<S> &m,
@@ -905,11 +962,8 @@ int adjoint_fused(wf_handle *h, const wf_adjoint_args *a, const wf::MetricDev<S>
   t.ntz = (g.Nz + t.KZ - 1) / t.KZ;
   const int64_t nbf = (int64_t)t.ntx * t.nty * t.ntz;
   if (nbf > 2147483647LL) return fail(WF_ERANGE, "wf_adjoint_step: grid too large");
-  hipLaunchKernelGGL(kern, dim3((unsigned)nbf), dim3(threads), 0, st, g, m,
-                     (const A *)a->W, (const A *)a->lH, (A *)a->W2, (A *)a->lH2, a->lJ_region,
-                     (const S *)a->a, (const S *)a->b, (const S *)a->ieps, (const S *)a->sigma,
-                     a->dt, x, a->gcap, t, (A *)a->reg_W, a->reg_lJ, a->reg_slot,
-                     (const S *)a->kappa);
+  const wf::AdjStep<S, A> p = adj_step_params<S, A>(h, a, m, x, t);
+  hipLaunchKernelGGL(kern, dim3((unsigned)nbf), dim3(threads), 0, st, p);
   WF_LAUNCH_CHECK();
   return WF_OK;
 }
@@ -918,13 +972,7 @@ template <typename S>
 int adjoint_step(wf_handle *h, const wf_adjoint_args *a, const wf::MetricDev<S> &m,
                  hipStream_t st) {
   const wf::GridDev &g = h->g;
-  wf::AdjAux x;
-  x.conductive = a->conductive;
-  x.ixs = a->ixs; x.iys = a->iys;
-  for (int d = 0; d < 3; ++d) { x.r0[d] = a->r0[d]; x.r1[d] = a->r1[d]; }
-  x.cap_ix0 = a->cap_ix0; x.cap_iy0 = a->cap_iy0; x.cap_nx = a->cap_nx; x.cap_ny = a->cap_ny;
-  x.cap_samples = a->cap_samples; x.cap_sample = a->cap_sample;
-  x.traj = a->traj;
+  const wf::AdjAux x = adj_aux(a);
   const int64_t nreg = (int64_t)(a->r1[0] - a->r0[0]) * (a->r1[1] - a->r0[1]) *
                        (a->r1[2] - a->r0[2]);
   if (nreg > 0 && a->g_ieps) {
@@ -964,52 +1012,63 @@ int adjoint_step(wf_handle *h, const wf_adjoint_args *a, const wf::MetricDev<S> 
   return WF_OK;
 }
 
+// the checks of wf_adjoint_step (also applied to each grid of a reverse batch)
+int check_adjoint_args(const wf_handle *h, const wf_adjoint_args *a, const char *who) {
+  if (!a->W || !a->lH || !a->a || !a->b || !a->ieps)
+    return fail(WF_EINVAL, "%s: null adjoint field or coefficient", who);
+  if (a->conductive && !a->sigma)
+    return fail(WF_EINVAL, "%s: conductive step needs sigma", who);
+  for (int d = 0; d < 3; ++d) {
+    const int n = d == 0 ? h->g.Nx : (d == 1 ? h->g.Ny : h->g.Nz);
+    if (a->r0[d] < 0 || a->r1[d] > n || a->r0[d] > a->r1[d])
+      return fail(WF_ERANGE, "%s: gradient region outside the grid", who);
+  }
+  const bool region = (a->r1[0] > a->r0[0]) && (a->r1[1] > a->r0[1]) && (a->r1[2] > a->r0[2]);
+  if (region && a->g_ieps) {
+    if (!a->E0 || !a->E1 || !a->H1 || !a->metric2d)
+      return fail(WF_EINVAL, "%s: gradients need E0, E1, H1 and metric2d", who);
+    if (a->traj && (a->r0[0] < 1 || a->r0[1] < 1 || a->r0[2] < 1))
+      return fail(WF_ERANGE, "%s: trajectory regions start at cell 1 or later", who);
+    if (a->conductive && (!a->g_sigma || !a->lJ_region))
+      return fail(WF_EINVAL, "%s: conductive gradients need g_sigma and "
+                  "lJ_region", who);
+  }
+  if (a->fields_f32 && !a->fused)
+    return fail(WF_EINVAL, "%s: FP32 adjoint fields need the fused kernel", who);
+  if (a->kappa && !a->fused)
+    return fail(WF_EINVAL, "%s: kappa is read by the fused kernel only", who);
+  if (a->reg_W && (!a->fused || !a->reg_lJ || a->g_ieps || a->reg_slot < 0))
+    return fail(WF_EINVAL, "%s: reg_W needs the fused kernel, reg_lJ, a slot "
+                "and g_ieps = NULL", who);
+  if (a->fused) {
+    if ((uint64_t)3 * (uint64_t)h->g.N * sizeof(double) >= (1ull << 32))
+      return fail(WF_ERANGE, "%s: grid too large for the fused kernel", who);
+    if (!a->W2 || !a->lH2 || a->W2 == a->W || a->lH2 == a->lH || a->W2 == a->lH2 ||
+        a->W2 == a->lH || a->lH2 == a->W)
+      return fail(WF_EINVAL, "%s: fused step needs distinct W2, lH2 buffers", who);
+    if (region && (a->r0[0] < 1 || a->r0[1] < 1))
+      return fail(WF_ERANGE, "%s: fused step needs the region at cell 1 or "
+                  "later in x and y", who);
+  }
+  if (a->gcap && (a->cap_nx < 1 || a->cap_ny < 1 || a->cap_ix0 < 0 || a->cap_iy0 < 0 ||
+                  a->cap_ix0 + a->cap_nx > h->g.Nx || a->cap_iy0 + a->cap_ny > h->g.Ny ||
+                  a->cap_sample < 0))
+    return fail(WF_ERANGE, "%s: capture window outside the grid", who);
+  if (a->reg_W && a->reg_slot >= a->reg_slots)
+    return fail(WF_ERANGE, "%s: reg_slot %lld outside %lld slots", who, (long long)a->reg_slot,
+                (long long)a->reg_slots);
+  return WF_OK;
+}
+
+
 }  // namespace
 
 extern "C" {
 
 int wf_adjoint_step(wf_handle *h, const wf_adjoint_args *a, void *stream) {
   if (!h || !a) return fail(WF_EINVAL, "wf_adjoint_step: null argument");
-  if (!a->W || !a->lH || !a->a || !a->b || !a->ieps)
-    return fail(WF_EINVAL, "wf_adjoint_step: null adjoint field or coefficient");
-  if (a->conductive && !a->sigma)
-    return fail(WF_EINVAL, "wf_adjoint_step: conductive step needs sigma");
-  for (int d = 0; d < 3; ++d) {
-    const int n = d == 0 ? h->g.Nx : (d == 1 ? h->g.Ny : h->g.Nz);
-    if (a->r0[d] < 0 || a->r1[d] > n || a->r0[d] > a->r1[d])
-      return fail(WF_ERANGE, "wf_adjoint_step: gradient region outside the grid");
-  }
-  const bool region = (a->r1[0] > a->r0[0]) && (a->r1[1] > a->r0[1]) && (a->r1[2] > a->r0[2]);
-  if (region && a->g_ieps) {
-    if (!a->E0 || !a->E1 || !a->H1 || !a->metric2d)
-      return fail(WF_EINVAL, "wf_adjoint_step: gradients need E0, E1, H1 and metric2d");
-    if (a->traj && (a->r0[0] < 1 || a->r0[1] < 1 || a->r0[2] < 1))
-      return fail(WF_ERANGE, "wf_adjoint_step: trajectory regions start at cell 1 or later");
-    if (a->conductive && (!a->g_sigma || !a->lJ_region))
-      return fail(WF_EINVAL, "wf_adjoint_step: conductive gradients need g_sigma and "
-                  "lJ_region");
-  }
-  if (a->fields_f32 && !a->fused)
-    return fail(WF_EINVAL, "wf_adjoint_step: FP32 adjoint fields need the fused kernel");
-  if (a->kappa && !a->fused)
-    return fail(WF_EINVAL, "wf_adjoint_step: kappa is read by the fused kernel only");
-  if (a->reg_W && (!a->fused || !a->reg_lJ || a->g_ieps || a->reg_slot < 0))
-    return fail(WF_EINVAL, "wf_adjoint_step: reg_W needs the fused kernel, reg_lJ, a slot "
-                "and g_ieps = NULL");
-  if (a->fused) {
-    if ((uint64_t)3 * (uint64_t)h->g.N * sizeof(double) >= (1ull << 32))
-      return fail(WF_ERANGE, "wf_adjoint_step: grid too large for the fused kernel");
-    if (!a->W2 || !a->lH2 || a->W2 == a->W || a->lH2 == a->lH || a->W2 == a->lH2 ||
-        a->W2 == a->lH || a->lH2 == a->W)
-      return fail(WF_EINVAL, "wf_adjoint_step: fused step needs distinct W2, lH2 buffers");
-    if (region && (a->r0[0] < 1 || a->r0[1] < 1))
-      return fail(WF_ERANGE, "wf_adjoint_step: fused step needs the region at cell 1 or "
-                  "later in x and y");
-  }
-  if (a->gcap && (a->cap_nx < 1 || a->cap_ny < 1 || a->cap_ix0 < 0 || a->cap_iy0 < 0 ||
-                  a->cap_ix0 + a->cap_nx > h->g.Nx || a->cap_iy0 + a->cap_ny > h->g.Ny ||
-                  a->cap_sample < 0))
-    return fail(WF_ERANGE, "wf_adjoint_step: capture window outside the grid");
+  const int rc = check_adjoint_args(h, a, "wf_adjoint_step");
+  if (rc != WF_OK) return rc;
   hipStream_t st = (hipStream_t)stream;
   if (h->dtype == WF_F64) return adjoint_step<double>(h, a, h->md, st);
   return adjoint_step<float>(h, a, h->mf, st);
@@ -1102,6 +1161,166 @@ int wf_field_monitor_update(const void *E, int32_t dtype, int64_t N, const doubl
     hipLaunchKernelGGL(wf::k_monitor<float>, grid, dim3(256), 0, (hipStream_t)stream,
                        (const float *)E, N, phase, nfreq, acc);
   WF_LAUNCH_CHECK();
+  return WF_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Reverse mode over a batch of grids
+// ---------------------------------------------------------------------------
+struct wf_batch_adjoint {
+  int dtype = 0, f32 = 0, ninst = 0, ntiles = 0, maxsteps = 0;
+  void *desc_dev = nullptr;   // AdjBatchDesc<S, A>[ninst]
+  int *step_dev = nullptr;    // reverse step read by the batch kernel
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+constexpr int BATCH_ADJ_ROWS = 5;  // rows per workgroup of the batched reverse kernel
+
+template <typename S, typename A>
+int batch_adj_build(wf_handle *const *hs, const wf_adjoint_args *as, const int32_t *nt, int n,
+                    wf_batch_adjoint *b) {
+  std::vector<wf::AdjBatchDesc<S, A>> d(n);
+  int64_t tiles = 0;
+  for (int i = 0; i < n; ++i) {
+    const wf_handle *h = hs[i];
+    const wf_adjoint_args *a = &as[i];
+    wf::AdjTile t;
+    t.ntx = (h->g.Nx + wf::ADJ_TX - 1) / wf::ADJ_TX;
+    t.nty = (h->g.Ny + BATCH_ADJ_ROWS - 1) / BATCH_ADJ_ROWS;
+    t.KZ = h->g.Nz < 32 ? h->g.Nz : 32;
+    t.ntz = (h->g.Nz + t.KZ - 1) / t.KZ;
+    const wf::MetricDev<S> *m;
+    if constexpr (sizeof(S) == 8) m = &h->md;
+    else m = &h->mf;
+    d[i].p = adj_step_params<S, A>(h, a, *m, adj_aux(a), t);
+    d[i].Nt = nt[i];
+    d[i].tile0 = (int)tiles;
+    tiles += (int64_t)t.ntx * t.nty * t.ntz;
+    b->maxsteps = std::max(b->maxsteps, (int)nt[i]);
+  }
+  if (tiles > 2147483647LL) return fail(WF_ERANGE, "wf_batch_adjoint_create: too many tiles");
+  b->ntiles = (int)tiles;
+  WF_HIP(hipMalloc(&b->desc_dev, sizeof(wf::AdjBatchDesc<S, A>) * n));
+  WF_HIP(hipMemcpy(b->desc_dev, d.data(), sizeof(wf::AdjBatchDesc<S, A>) * n,
+                   hipMemcpyHostToDevice));
+  WF_HIP(hipMalloc((void **)&b->step_dev, sizeof(int)));
+  return WF_OK;
+}
+
+template <typename S, typename A>
+int batch_adj_launch(const wf_batch_adjoint *b, hipStream_t st) {
+  hipLaunchKernelGGL((wf::k_adj_fused_batch<S, A, BATCH_ADJ_ROWS>), dim3((unsigned)b->ntiles),
+                     dim3(64 * (BATCH_ADJ_ROWS + 3)), 0, st,
+                     (const wf::AdjBatchDesc<S, A> *)b->desc_dev, b->ninst, b->ntiles,
+                     (const int *)b->step_dev);
+  WF_LAUNCH_CHECK();
+  hipLaunchKernelGGL(wf::k_batch_next, dim3(1), dim3(1), 0, st, b->step_dev);
+  WF_LAUNCH_CHECK();
+  return WF_OK;
+}
+
+int batch_adj_step_once(const wf_batch_adjoint *b, hipStream_t st) {
+  if (b->dtype == WF_F64)
+    return b->f32 ? batch_adj_launch<double, float>(b, st) : batch_adj_launch<double, double>(b, st);
+  return b->f32 ? batch_adj_launch<float, float>(b, st) : batch_adj_launch<float, double>(b, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+int wf_batch_adjoint_create(wf_handle *const *handles, const wf_adjoint_args *args,
+                            const int32_t *nsteps, int32_t n, wf_batch_adjoint **out) {
+  if (!handles || !args || !nsteps || !out)
+    return fail(WF_EINVAL, "wf_batch_adjoint_create: null argument");
+  *out = nullptr;
+  if (n < 1) return fail(WF_EINVAL, "wf_batch_adjoint_create: empty batch");
+  if (!handles[0]) return fail(WF_EINVAL, "wf_batch_adjoint_create: null handle");
+  const int dtype = handles[0]->dtype, f32 = args[0].fields_f32;
+  for (int i = 0; i < n; ++i) {
+    const wf_handle *h = handles[i];
+    const wf_adjoint_args *a = &args[i];
+    char who[64];
+    snprintf(who, sizeof(who), "wf_batch_adjoint_create[%d]", i);
+    if (!h) return fail(WF_EINVAL, "%s: null handle", who);
+    const int rc = check_adjoint_args(h, a, who);
+    if (rc != WF_OK) return rc;
+    if (!a->fused || !a->reg_W || a->g_ieps)
+      return fail(WF_EINVAL, "%s: batch items use the fused kernel with region slots "
+                  "(fused, reg_W, reg_lJ, g_ieps = NULL)", who);
+    if (nsteps[i] < 0 || nsteps[i] > a->reg_slots)
+      return fail(WF_ERANGE, "%s: %d steps for %lld region slots", who, (int)nsteps[i],
+                  (long long)a->reg_slots);
+    if (a->gcap && nsteps[i] > a->cap_samples)
+      return fail(WF_ERANGE, "%s: %d steps for %lld capture samples", who, (int)nsteps[i],
+                  (long long)a->cap_samples);
+    if (h->dtype != dtype || a->fields_f32 != f32)
+      return fail(WF_EINVAL, "%s: dtype and fields_f32 must match across the batch", who);
+  }
+  wf_batch_adjoint *b = new wf_batch_adjoint();
+  b->dtype = dtype; b->f32 = f32; b->ninst = n;
+  int rc;
+  if (dtype == WF_F64)
+    rc = f32 ? batch_adj_build<double, float>(handles, args, nsteps, n, b)
+             : batch_adj_build<double, double>(handles, args, nsteps, n, b);
+  else
+    rc = f32 ? batch_adj_build<float, float>(handles, args, nsteps, n, b)
+             : batch_adj_build<float, double>(handles, args, nsteps, n, b);
+  if (rc != WF_OK) {
+    wf_batch_adjoint_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return WF_OK;
+}
+
+int wf_batch_adjoint_run(wf_batch_adjoint *b, int32_t r0, int32_t nr, int32_t use_graph,
+                         void *stream) {
+  if (!b) return fail(WF_EINVAL, "wf_batch_adjoint_run: null batch");
+  if (r0 < 0 || nr < 0) return fail(WF_EINVAL, "wf_batch_adjoint_run: negative step range");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(wf::k_batch_set, dim3(1), dim3(1), 0, st, b->step_dev, (int)r0);
+  WF_LAUNCH_CHECK();
+  int done = 0;
+  if (use_graph && nr >= BATCH_GRAPH_STEPS) {
+    if (!b->exec) {
+      WF_HIP(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+      WF_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+      int rc = WF_OK;
+      for (int s = 0; s < BATCH_GRAPH_STEPS && rc == WF_OK; ++s)
+        rc = batch_adj_step_once(b, b->cap_stream);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+      if (rc != WF_OK) return rc;
+      if (e != hipSuccess)
+        return fail(WF_EHIP, "wf_batch_adjoint_run: capture: %s", hipGetErrorString(e));
+      b->graph = g;
+      WF_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    for (; done + BATCH_GRAPH_STEPS <= nr; done += BATCH_GRAPH_STEPS)
+      WF_HIP(hipGraphLaunch(b->exec, st));
+  }
+  for (; done < nr; ++done) {
+    const int rc = batch_adj_step_once(b, st);
+    if (rc != WF_OK) return rc;
+  }
+  return WF_OK;
+}
+
+int wf_batch_adjoint_destroy(wf_batch_adjoint *b) {
+  if (!b) return WF_OK;
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->cap_stream) hipStreamDestroy(b->cap_stream);
+  if (b->desc_dev) hipFree(b->desc_dev);
+  if (b->step_dev) hipFree(b->step_dev);
+  delete b;
   return WF_OK;
 }
 

@@ -510,6 +510,11 @@ struct BatchDesc {
   int64_t cap_sample0;         // capture sample written by step s: cap_sample0 + s
   int nsteps;                  // the grid takes part in steps 0 .. nsteps-1
   int tile0;                   // its first tile in the batch's tile numbering
+  // optional region trajectory (reverse mode): after step s, slot traj_slot0+s+1
+  S *tE, *tH;                  // [slots][3 * box cells], or null
+  int tlo[3], thi[3];
+  int64_t traj_slot0;
+  int64_t thr0;                // first thread of this grid in the batched gather
 };
 
 // One step of every grid of a batch in one launch. The step number comes from
