@@ -466,6 +466,95 @@ class OraclePlaneWave:
             Ex[xs, ys, self.zs] = 0.0
 
 
+class OracleWavePacket:
+    """Gaussian wave packet on a z plane (sources.py:285-455): H_y and E_x of
+    the plane are -A cos(w t + phi) g_xy g_t and A mu0 c cos(w t + phi) g_xy g_t
+    with g_xy over meshgrid(x[xs], y[ys]) and g_t of s = z_min - beta c t
+    around s0 = z_min - t_inj (sources.py:397-429)."""
+
+    def __init__(self, xs=None, ys=None, zs=0, sigmaz=None, sigmaxy=None, tinj=None,
+                 wavelength=None, f=None, amplitude=1.0, beta=1.0, phase=0):
+        self.xs, self.ys, self.zs = xs, ys, zs
+        self.sigmaz, self.sigmaxy = sigmaz, sigmaxy
+        self.amplitude, self.beta, self.phase = amplitude, beta, phase
+        self.f = f if f is not None else c_light / wavelength
+        self.w = 2 * np.pi * self.f
+        self.tinj = tinj if tinj is not None else 6 * sigmaz
+
+    def update(self, solver, t):
+        xs = self.xs if self.xs is not None else slice(0, solver.Nx)
+        ys = self.ys if self.ys is not None else slice(0, solver.Ny)
+        X, Y = np.meshgrid(solver.x[xs], solver.y[ys])
+        gxy = np.exp(-(X ** 2 + Y ** 2) / (2 * self.sigmaxy ** 2))
+        s0 = solver.z.min() - self.tinj
+        s = solver.z.min() - self.beta * c_light * t
+        gt = np.exp(-((s - s0) ** 2) / (2 * self.sigmaz ** 2))
+        cw = np.cos(self.w * t + self.phase)
+        comp(solver.H, solver.dims, "y")[xs, ys, self.zs] = -self.amplitude * cw * gxy * gt
+        comp(solver.E, solver.dims, "x")[xs, ys, self.zs] = (self.amplitude * mu_0 * c_light
+                                                              * cw * gxy * gt)
+
+
+def _oracle_field(solver, field):
+    return {"E": solver.E, "H": solver.H, "J": solver.J}[field]
+
+
+class OracleDipole:
+    """A sin(w t + phi) on one component of E, H or J at one cell
+    (sources.py:458-566); f = nodes c / (z_max - z_min) when not given."""
+
+    def __init__(self, field="E", component="z", xs=None, ys=None, zs=None, nodes=10, f=None,
+                 amplitude=1.0, phase=0):
+        if len(field) == 2:
+            field, component = field[0], field[1]
+        self.field, self.component = field, component
+        self.xs, self.ys, self.zs, self.nodes, self.f = xs, ys, zs, nodes, f
+        self.amplitude, self.phase = amplitude, phase
+
+    def update(self, solver, t):
+        xs = self.xs if self.xs is not None else int(solver.Nx / 2)
+        ys = self.ys if self.ys is not None else int(solver.Ny / 2)
+        zs = self.zs if self.zs is not None else int(solver.Nz / 2)
+        f = self.f if self.f is not None else self.nodes / ((solver.z.max() - solver.z.min())
+                                                             / c_light)
+        comp(_oracle_field(solver, self.field), solver.dims, self.component)[xs, ys, zs] = (
+            self.amplitude * np.sin(2 * np.pi * f * t + self.phase))
+
+
+class OraclePulse:
+    """Pulse of tau = c t - delay on one component (sources.py:569-762):
+    Harris (10 - 15 cos(2 pi tau / L) + 6 cos(4 pi tau / L) - cos(6 pi tau / L))
+    / 32 for tau < L, Gaussian exp(-(tau - L/2)^2 / (2 (L/10)^2)) with L/2
+    written 5 (L/10), rectangular 1 on 0 < tau < L; L = 50 dt by default."""
+
+    def __init__(self, field="E", component="z", xs=None, ys=None, zs=None, shape="Harris",
+                 L=None, amplitude=1.0, delay=0.0):
+        if len(field) == 2:
+            field, component = field[0], field[1]
+        self.field, self.component, self.shape = field, component, shape.lower()
+        self.xs, self.ys, self.zs, self.L = xs, ys, zs, L
+        self.amplitude, self.delay = amplitude, delay
+
+    def profile(self, t, L):
+        tau = t * c_light - self.delay
+        if self.shape == "harris":
+            if tau >= L:
+                return 0.0
+            return (10 - 15 * np.cos(2 * np.pi / L * tau) + 6 * np.cos(4 * np.pi / L * tau)
+                    - np.cos(6 * np.pi / L * tau)) / 32
+        if self.shape == "gaussian":
+            return np.exp(-((tau - 5 * (L / 10)) ** 2) / (2 * (L / 10) ** 2))
+        return 1.0 if 0.0 < tau < L else 0.0
+
+    def update(self, solver, t):
+        xs = self.xs if self.xs is not None else int(solver.Nx / 2)
+        ys = self.ys if self.ys is not None else int(solver.Ny / 2)
+        zs = self.zs if self.zs is not None else int(solver.Nz / 2)
+        L = self.L if self.L is not None else 50 * solver.dt
+        comp(_oracle_field(solver, self.field), solver.dims, self.component)[xs, ys, zs] = (
+            self.amplitude * self.profile(t, L))
+
+
 # --------------------------------------------------------------------------
 # Drivers (routines.py:17-229, 231-482)
 # --------------------------------------------------------------------------

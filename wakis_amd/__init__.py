@@ -2,7 +2,7 @@
 
 Exports the reference's public names for the FIT leapfrog and wakefield path
 (wakis/__init__.py:6-24): Field, GridFIT3D, SolverFIT3D, WakeSolver, plus the
-Beam/PlaneWave sources and analytic solids. The compute path is the HIP
+Beam, PlaneWave, WavePacket, Dipole and Pulse sources and analytic solids. The compute path is the HIP
 library libwakis_fit.so (include/wakis_fit.h); there is no CPU fallback.
 """
 
@@ -13,8 +13,8 @@ from .field_monitors import FieldMonitor
 from .gridFIT3D import GridFIT3D
 from .materials import material_lib
 from .solverFIT3D import SolverFIT3D
-from .sources import Beam, PlaneWave
+from .sources import Beam, Dipole, PlaneWave, Pulse, WavePacket
 from .wakeSolver import WakeSolver
 
 __all__ = ["Field", "FieldMonitor", "GridFIT3D", "SolverFIT3D", "WakeSolver", "Beam",
-           "PlaneWave", "material_lib", "field_monitors", "solids", "sources", "__version__"]
+           "PlaneWave", "WavePacket", "Dipole", "Pulse", "material_lib", "field_monitors", "solids", "sources", "__version__"]
