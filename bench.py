@@ -77,7 +77,8 @@ def hbm_copy_gbs(dev, nbytes=1 << 30, reps=5):
     return 2.0 * nbytes * reps / (ms * 1e-3) / 1e9
 
 
-def time_gpu(Nx, Ny, Nz, dtype, steps, warmup, path, variant=0, kz=0, wake_check=False):
+def time_gpu(Nx, Ny, Nz, dtype, steps, warmup, path, variant=0, kz=0, wake_check=False,
+             keep_fields=False):
     import torch
 
     import wakis_amd as wa
@@ -127,6 +128,9 @@ def time_gpu(Nx, Ny, Nz, dtype, steps, warmup, path, variant=0, kz=0, wake_check
     out["achieved_GBs"] = N * out["bytes_per_cell"] / (out["ms_per_step"] * 1e-3) / 1e9
     if wake_check:
         out["WP"] = wake_of_capture(s, cap, tot)
+    if keep_fields:  # final fields in FP64, for the FP32-vs-FP64 field check
+        out["E"] = s.E.array.to(torch.float64)
+        out["H"] = s.H.array.to(torch.float64)
     del s, table, cap
     torch.cuda.empty_cache()
     return out
@@ -154,7 +158,7 @@ def wake_of_capture(s, cap, nsamples):
 
 
 def _brief(r):
-    return {k: v for k, v in r.items() if k != "WP"}
+    return {k: v for k, v in r.items() if k not in ("WP", "E", "H")}
 
 
 def time_cpu(Nx, Ny, Nz, dtype, steps, warmup):
@@ -260,6 +264,85 @@ def batch_sweep(n=64, steps=2000, seed=0, dtype=np.float32):
             "data": "synthetic (random_box_cavities, seed %d)" % seed}
 
 
+def lossy_pml(dtype, steps=300, warmup=20, dims=(200, 200, 400), keep_fields=False):
+    """The lossy pillbox with PML: cylinder r = 40 mm, |z| < 30 mm, beam pipe
+    r = 10 mm, a 3-cell wall shell of sigma = 1e4 S/m (eps_r = mu_r = 1) around
+    the vacuum in a PEC background; 90 x 90 x 180 mm on `dims` cells, PEC x/y,
+    10-layer PML on both z ends; beam q = 1 nC, sigma_z = 15 mm, beta = 0.8 on
+    axis, its table starting at 0.9 of the injection time so the bunch head is
+    in the domain. dt is the reference's tau clamp (eps0 / sigma_max) and the
+    wall keeps its volume conductivity (no surface-impedance fit). Times
+    `steps` leapfrog steps (J = sigma E everywhere, coefficient arrays)."""
+    import torch
+
+    import wakis_amd as wa
+    from wakis_amd import _lib
+    from wakis_amd.solids import Cylinder
+    Nx, Ny, Nz = dims
+    x = np.linspace(-45e-3, 45e-3, Nx + 1)
+    y = np.linspace(-45e-3, 45e-3, Ny + 1)
+    z = np.linspace(-90e-3, 90e-3, Nz + 1)
+    vac = (Cylinder(40e-3, -30e-3, 30e-3) | Cylinder(10e-3)).cell_mask(x, y, z)
+    grown = vac.copy()
+    for _ in range(3):  # 6-neighbour dilation: one wall cell per pass
+        g = grown.copy()
+        g[1:] |= grown[:-1]
+        g[:-1] |= grown[1:]
+        g[:, 1:] |= grown[:, :-1]
+        g[:, :-1] |= grown[:, 1:]
+        g[:, :, 1:] |= grown[:, :, :-1]
+        g[:, :, :-1] |= grown[:, :, 1:]
+        grown = g
+    wall = grown & ~vac
+    grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"wall": wall, "vac": vac},
+                        stl_materials={"wall": [1.0, 1.0, 1e4], "vac": "vacuum"}, verbose=0)
+    s = wa.SolverFIT3D(grid, bc_low=["pec", "pec", "pml"], bc_high=["pec", "pec", "pml"],
+                       use_stl=True, bg="pec", dtype=dtype, verbose=0, n_pml=10, use_sibc=False)
+    beam = wa.Beam(q=1e-9, sigmaz=15e-3, beta=0.8)
+    dev = s.device
+    tot = warmup + steps
+    beam._locate(s)
+    t = 0.9 * beam.ti + np.arange(tot) * float(s.dt)
+    table = torch.as_tensor(beam._line(s, t[:, None]).astype(s.dtype), device=dev).contiguous()
+    s._set_ghosts_to_0()
+    s.step_0 = False
+    s._beam_line = (int(beam.ixs), int(beam.iys))
+
+    def run(n0, k):
+        a = s._step_args(k)
+        a.ixs, a.iys = int(beam.ixs), int(beam.iys)
+        a.jmode = _lib.WF_J_FULL
+        a.beam_table, a.beam_row0, a.beam_rows = _lib.ptr(table), n0, tot
+        s._run(a)
+
+    run(0, warmup)
+    torch.cuda.synchronize(dev)
+    stream = torch.cuda.current_stream(dev)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    run(warmup, steps)
+    e1.record(stream)
+    torch.cuda.synchronize(dev)
+    ms = e0.elapsed_time(e1) / steps
+    b = 21 * np.dtype(dtype).itemsize
+    out = dict(ms_per_step=round(ms, 5), gcell_s=round(s.N / (ms * 1e-3) / 1e9, 3),
+               bytes_per_cell_update=b, path=s.last_path, dt=float(s.dt),
+               achieved_GBs=round(s.N * b / (ms * 1e-3) / 1e9, 1),
+               frac=round(s.N * b / (ms * 1e-3) / 8e12, 4))
+    if keep_fields:
+        out["E"] = s.E.array.to(torch.float64)
+        out["H"] = s.H.array.to(torch.float64)
+    del s, table
+    torch.cuda.empty_cache()
+    return out
+
+
+def _rel_l2(a, b):
+    import torch
+    return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+
+
 def adjoint_cost(dtype=np.float32, dims=(128, 128, 256), insert=(32, 32, 24), steps=4000,
                  seed=0):
     """Reverse-mode cost on the differentiable workload: the 50 mm PEC cavity
@@ -354,6 +437,8 @@ def main():
                     help="print only the reverse-mode cost (FP32 and FP64) as one JSON line")
     ap.add_argument("--adjoint-steps", type=int, default=4000,
                     help="wakefield steps of the reverse-mode workload")
+    ap.add_argument("--no-lossy", action="store_true",
+                    help="skip the lossy pillbox with PML (200x200x400)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the reverse-mode cost measurement")
     ap.add_argument("--batch", action="store_true",
@@ -379,7 +464,7 @@ def main():
           file=sys.stderr, flush=True)
     check = not args.no_fp64 and not args.no_wake_check
     r32 = time_gpu(Nx, Ny, Nz, np.float32, args.steps, args.warmup, args.path, args.variant,
-                   args.kz, wake_check=check)
+                   args.kz, wake_check=check, keep_fields=check)
     print(f"[bench] fp32 {_brief(r32)}", file=sys.stderr, flush=True)
     r64 = None
     if not args.no_fp64:
@@ -387,8 +472,32 @@ def main():
         # reference for the FP32 wake), else a third of them
         k64 = args.steps if check else max(1, args.steps // 3)
         r64 = time_gpu(Nx, Ny, Nz, np.float64, k64, args.warmup, args.path, args.variant,
-                       args.kz, wake_check=check)
+                       args.kz, wake_check=check, keep_fields=check)
         print(f"[bench] fp64 {_brief(r64)}", file=sys.stderr, flush=True)
+    fields = None
+    if r64 is not None and "E" in r32 and "E" in r64:
+        fields = {"quantity": "E and H after %d steps (warm-up included), FP32 run vs FP64 run"
+                              % (args.warmup + args.steps),
+                  "rel_l2_E": _rel_l2(r32["E"], r64["E"]), "rel_l2_H": _rel_l2(r32["H"], r64["H"])}
+        for r in (r32, r64):
+            r.pop("E"), r.pop("H")
+        torch.cuda.empty_cache()
+        print(f"[bench] fields {fields}", file=sys.stderr, flush=True)
+    lossy = None
+    if not args.no_lossy:
+        l32 = lossy_pml(np.float32, keep_fields=True)
+        l64 = lossy_pml(np.float64, keep_fields=True)
+        lossy = {"workload": "lossy pillbox r=40 mm, 3-cell sigma=1e4 S/m wall, 10-layer PML "
+                             "on z, 200x200x400 cells, beam sigma_z=15 mm beta=0.8, 320 steps",
+                 "dt_policy": "reference tau clamp (dt = eps0/sigma_max = %.4g s); volume "
+                              "conductivity, no surface-impedance fit" % l64["dt"],
+                 "fp32": {k: v for k, v in l32.items() if k not in ("E", "H", "dt")},
+                 "fp64": {k: v for k, v in l64.items() if k not in ("E", "H", "dt")},
+                 "rel_l2_E_fp32_vs_fp64": _rel_l2(l32["E"], l64["E"]),
+                 "rel_l2_H_fp32_vs_fp64": _rel_l2(l32["H"], l64["H"])}
+        del l32, l64
+        torch.cuda.empty_cache()
+        print(f"[bench] lossy {lossy}", file=sys.stderr, flush=True)
     adj = adj64 = None
     if not args.no_adjoint:
         adj = adjoint_cost(steps=args.adjoint_steps)
@@ -443,6 +552,10 @@ def main():
                              "the FP64 oracle)",
                 "rel_l2_fp32_vs_fp64": float(np.linalg.norm(wp32 - wp64) / np.linalg.norm(wp64)),
                 "n_s": int(len(wp64))}
+    if fields is not None:
+        line["field_check"] = fields
+    if lossy is not None:
+        line["lossy_pml"] = lossy
     if adj is not None:
         line["adjoint"] = adj
         line["adjoint_fp64"] = adj64

@@ -85,6 +85,36 @@ def test_fused_cells_per_lane_matches_oracle(dims, dtype, conductive, use_code, 
         assert np.array_equal(got, ref), f"{name}: rel-L2 {gc.rel_l2(got, ref):.3e}"
 
 
+@pytest.mark.parametrize("dims", [(12, 10, 14), (64, 8, 33), (70, 13, 37)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("use_code", [True, False])
+def test_fused_multistep_conductive_call_matches_oracle(dims, dtype, use_code):
+    """One wf_step call of several conductive steps: J = sigma E is stored only
+    by the call's last step (the next step recomputes sigma E from E), and a
+    beam line without a beam table keeps its J between the steps; E, H and J
+    equal the oracle's step-by-step run."""
+    print("fused multistep conductive", dims, np.dtype(dtype).name, use_code, flush=True)
+    case = gc.random_case(PEC_ALL, dtype, True, seed=sum(dims) + 17, dims=dims)
+    case["fields"][2] = np.zeros_like(case["fields"][2])
+    so = gc.oracle_solver(case)
+    sa = gc.amd_solver(case, path="fused")
+    sa.use_material_code = use_code
+    line = (dims[0] // 2, dims[1] // 2)
+    so.J[:] = 0
+    so.one_step()
+    sa.path = "general"
+    sa.one_step()  # J = sigma E now holds
+    sa.path = "fused"
+    sa._beam_line = line  # a beam line with no beam table: its J is read back
+    for _ in range(6):
+        so.one_step()
+    sa._run(sa._step_args(6))
+    assert sa.last_path == "fused"
+    for name in ("E", "H", "J"):
+        got, ref = getattr(sa, name).to_numpy(), getattr(so, name)
+        assert np.array_equal(got, ref), f"{name}: rel-L2 {gc.rel_l2(got, ref):.3e}"
+
+
 def _zero_masked_values(case):
     """Zero E on PEC-masked edges (and H on PMC-masked faces) of a random case."""
     dims = tuple(len(a) - 1 for a in (case["x"], case["y"], case["z"]))

@@ -308,6 +308,7 @@ int step_fused4(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStr
   for (int s = 0; s < a->nsteps; ++s) {
     const S *row = a->beam_table ? (const S *)a->beam_table + (a->beam_row0 + s) * g.Nz : nullptr;
     x.cap_sample = a->cap_sample0 + s;
+    x.skip_j = (s + 1 < a->nsteps) ? 1 : 0;
     S *cap = (S *)a->capture;
     hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(threads), lds, st, g, m, Ebuf[cur],
                        Hbuf[cur], Ebuf[1 - cur], Hbuf[1 - cur], (S *)a->J, cf, dt, x, row, cap);

@@ -24,6 +24,10 @@ struct FusedAux {
   int cap_ix0, cap_iy0, cap_nx, cap_ny;
   int64_t cap_samples, cap_sample;
   int KZ, ntx, nty, ntz;
+  // k_fused4, conductive steps: 1 = store J = sigma E only on the beam line
+  // without a beam table (the next step of the same call recomputes sigma E
+  // from E; J is stored in full by the call's last step)
+  int skip_j;
 };
 
 // z-face override of the face mask for components x and y at plane k:

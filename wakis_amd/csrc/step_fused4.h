@@ -460,13 +460,16 @@ __device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S>
       }
       stv<V>(Eout, nk, En[0]); stv<V>(Eout, N + nk, En[1]); stv<V>(Eout, 2 * N + nk, En[2]);
       stv<V>(Hout, nk, Hn[0]); stv<V>(Hout, N + nk, Hn[1]); stv<V>(Hout, 2 * N + nk, Hn[2]);
-      if (COND) {
+      if (COND && !x.skip_j) {
         S Jn[3][V];
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
           for (int q = 0; q < V; ++q) Jn[c][q] = (S)(sg[c][q] * En[c][q]);
         stv<V>(J, nk, Jn[0]); stv<V>(J, N + nk, Jn[1]); stv<V>(J, 2 * N + nk, Jn[2]);
+      } else if (COND && qbeam >= 0 && !beam_row) {
+        // the line's J is read back by the next step when no beam table feeds it
+        *atw(J, 2 * N + nk + qbeam) = (S)(sg[2][qbeam] * En[2][qbeam]);
       }
 #pragma unroll
       for (int q = 0; q < V; ++q) { Hpx[q] = HX[q + 1]; Hpy[q] = HY[q + 1]; }
@@ -537,6 +540,7 @@ k_fused4_batch(const BatchDesc<S> *__restrict__ d, int ninst, int ntiles,
   const bool odd = step & 1;
   FusedAux x = b.x;
   x.cap_sample = b.cap_sample0 + step;
+  x.skip_j = (step + 1 < b.nsteps) ? 1 : 0;
   const S *row = b.beam_table ? b.beam_table + (b.beam_row0 + step) * b.g.Nz : nullptr;
   fused4_tile<S, JM, COND, LUT, DESC, TY, MK, V>(
       b.g, b.m, odd ? b.E2 : b.E, odd ? b.H2 : b.H, odd ? b.E : b.E2, odd ? b.H : b.H2, b.J,
