@@ -363,6 +363,8 @@ class WakeAdjoint:
             torch.cuda.synchronize(dev)
             return self._finish_grad(g_ieps, g_sig)
         K = f["K"]
+        if min(self.i0, self.j0, self.k0) >= 1:
+            return self._backward_segments(R)
         J_final = s.J.array.clone()
         # one step's forward arguments; each recomputed step reads state i of the
         # segment buffers and writes state i+1 (fused path), or steps a copy in place
@@ -410,6 +412,63 @@ class WakeAdjoint:
         torch.cuda.synchronize(dev)
         del Eb, Hb
         return self._finish_grad(g_ieps, g_sig)
+
+    def _backward_segments(self, R):
+        """Checkpoint mode for a block away from the low faces: each segment is
+        recomputed from its checkpoint by one multi-step wf_step call that
+        captures only the block's trajectory (as the trajectory mode's forward
+        does), then reversed from that trajectory. The solver's fields are
+        restored to the end of the run afterwards."""
+        s, f = self.s, self._fwd
+        ctx, Nt, K = f["ctx"], f["Nt"], f["K"]
+        dev = s.device
+        a, adjoint_step, grad_pass = R["a"], R["adjoint_step"], R["grad_pass"]
+        adj_fused, adt, n3r = R["fused"], R["adt"], R["n3r"]
+        table, row_bytes = R["table"], R["row_bytes"]
+        lib, st = R["lib"], R["st"]
+        finals = [F.array.clone() for F in (s.E, s.H, s.J)]
+        dt_ = s.E.array.dtype
+        tE = torch.empty((K + 1, 3 * self._nE), dtype=dt_, device=dev)
+        tH = torch.empty((K + 1, 3 * self._nH), dtype=dt_, device=dev)
+        pE, pH, pT = tE.data_ptr(), tH.data_ptr(), table.data_ptr()
+        sE, sH = tE.stride(0) * tE.element_size(), tH.stride(0) * tH.element_size()
+        lo, hi = self._lohi()
+        a.traj = 1
+        if adj_fused:
+            rW = torch.empty((K, n3r), dtype=adt, device=dev)
+            rJ = torch.zeros((K + 1, n3r), dtype=torch.float64, device=dev)
+            carry = torch.zeros(n3r, dtype=torch.float64, device=dev)  # lJ after the segment
+            a.reg_W, a.reg_lJ, a.reg_slots = _lib.ptr(rW), _lib.ptr(rJ), K
+            a.g_ieps = a.g_sigma = None
+        for (n0, E0, H0, J0) in reversed(f["ckpt"]):
+            k = min(K, Nt - n0)
+            s._load_run_state(E0, H0, J0)
+            _lib.check(lib.wf_trajectory_gather(s._handle, _lib.ptr(s.E.array),
+                                                _lib.ptr(s.H.array), pE, pH, lo, hi, K + 1, 0,
+                                                st), "wf_trajectory_gather")
+            args = s._wake_args(ctx, n0, k)
+            args.capture = None  # the capture of the forward run stays as it is
+            args.traj_E, args.traj_H = pE, pH
+            args.traj_lo[:] = tuple(lo)
+            args.traj_hi[:] = tuple(hi)
+            args.traj_slots, args.traj_slot0 = K + 1, 0
+            s._run(args)
+            if adj_fused:
+                rJ[k].copy_(carry)
+            for i in range(k - 1, -1, -1):
+                n = n0 + i
+                a.E0, a.E1 = pE + i * sE, pE + (i + 1) * sE
+                a.H1 = pH + (i + 1) * sH
+                a.beam_row = pT + n * row_bytes
+                a.cap_sample = n
+                a.reg_slot = i
+                adjoint_step()
+            if adj_fused:
+                grad_pass(pE, tE.stride(0), pH, tH.stride(0), 1, pT + n0 * row_bytes, rW, rJ, k)
+                carry.copy_(rJ[0])
+        s._load_run_state(*finals)
+        torch.cuda.synchronize(dev)
+        return self._finish_grad(R["g_ieps"], R["g_sig"])
 
     def _finish_grad(self, g_ieps, g_sig):
         s = self.s

@@ -111,6 +111,10 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
   const bool valid = inrow && (i >= 0) && (i < g.Nx) && (j >= 0) && (j < g.Ny);
   const bool own = valid && !halo && (w >= 1) && (w <= TY);
   const bool makes_q = inrow && !(halo && side);  // the column i0+64 needs P only
+  // what a thread's cell needs besides W and b: lH and a where lH' is used (own
+  // rows, the row below, the column i0-1), ieps/sigma/kappa on own rows only
+  const bool need_lh = (!halo && w <= TY) || (halo && !side && inrow);
+  const bool need_c = !halo && w >= 1 && w <= TY;
   const I N = (I)g.N, sz = (I)g.sz;
   const I col = valid ? (I)i + (I)g.sy * (I)j : 0;
   for (int u = threadIdx.x; u < 3 * (NW + 1) * NS; u += 64 * (NW + 1))
@@ -144,9 +148,10 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
     for (int c = 0; c < 3; ++c) {
       Wk[c] = *at(Win, (I)c * N + n0); bk[c] = *at(b, (I)c * N + n0);
       Wk1[c] = *at(Win, (I)c * N + n1); bk1[c] = *at(b, (I)c * N + n1);
-      lh[c] = *at(lHin, (I)c * N + n0); ac[c] = *at(a, (I)c * N + n0);
-      ie[c] = (!kap || in_region(kstart)) ? *at(ieps, (I)c * N + n0) : S(0);
-      sg[c] = *at(sgsrc, (I)c * N + n0);
+      lh[c] = need_lh ? *at(lHin, (I)c * N + n0) : A(0);
+      ac[c] = need_lh ? *at(a, (I)c * N + n0) : S(0);
+      ie[c] = (need_c && (!kap || in_region(kstart))) ? *at(ieps, (I)c * N + n0) : S(0);
+      sg[c] = need_c ? *at(sgsrc, (I)c * N + n0) : S(0);
     }
   }
   double Qpx = 0, Qpy = 0;  // Q(k-1) of this cell, x and y components
@@ -163,8 +168,9 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         Wn2[c] = *at(Win, (I)c * N + n2); bn2[c] = *at(b, (I)c * N + n2);
-        lhN[c] = *at(lHin, (I)c * N + n1); acN[c] = *at(a, (I)c * N + n1);
-        if (cond) {
+        lhN[c] = need_lh ? *at(lHin, (I)c * N + n1) : A(0);
+        acN[c] = need_lh ? *at(a, (I)c * N + n1) : S(0);
+        if (cond && need_c) {
           ieN[c] = (!kap || in_region(k + 1)) ? *at(ieps, (I)c * N + n1) : S(0);
           sgN[c] = *at(sgsrc, (I)c * N + n1);
         } else {
