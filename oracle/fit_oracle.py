@@ -217,16 +217,42 @@ def corner_any_inside(x, y, z, inside):
 # --------------------------------------------------------------------------
 # Solver (solverFIT3D.py:40-354, 416-503, 820-884)
 # --------------------------------------------------------------------------
+def sibc_surface(mask):
+    """Surface cells of a solid for the SIBC substitution (gridFIT3D.py:518-533):
+    the cells where the gradient of the solid's 0/1 cell field is nonzero, the
+    gradient restated as index-space finite differences (central in the
+    interior, one-sided on the first and last cell of each axis). Parity
+    unpinned: the reference computes it with pyvista's compute_derivative,
+    which is not available to check against; the tests pin only the rule's
+    invariants (a shell inside the solid, one cell thick on flat faces).
+    """
+    m = np.asarray(mask).astype(np.int64)
+    out = np.zeros(m.shape, dtype=bool)
+    for ax in range(3):
+        n = m.shape[ax]
+        if n < 2:
+            continue
+        d = np.zeros_like(m)
+        sl = lambda a, b: tuple(slice(a, b) if i == ax else slice(None) for i in range(3))
+        d[sl(1, n - 1)] = m[sl(2, n)] - m[sl(0, n - 2)]
+        d[sl(0, 1)] = m[sl(1, 2)] - m[sl(0, 1)]
+        d[sl(n - 1, n)] = m[sl(n - 1, n)] - m[sl(n - 2, n - 1)]
+        out |= d != 0
+    return out
+
+
 class OracleSolver:
     """Restatement of ``SolverFIT3D`` on the CPU scipy path (no MKL/GPU/MPI).
 
     ``solids`` is a list of ``(mask3d_bool, material)`` applied in order with
-    the arithmetic of ``_apply_stl_materials`` (solverFIT3D.py:416-480); the
-    SIBC substitution (needs a pyvista surface gradient) is not restated.
+    the arithmetic of ``_apply_stl_materials`` (solverFIT3D.py:416-480),
+    including the surface-impedance substitution when ``use_sibc`` (the surface
+    rule of ``sibc_surface``, parity unpinned).
     """
 
     def __init__(self, x, y, z, bc_low, bc_high, bg=(1.0, 1.0), solids=(),
-                 dtype=np.float64, cfln=0.5, dt=None, n_pml=10):
+                 dtype=np.float64, cfln=0.5, dt=None, n_pml=10, use_sibc=False, fmax=1e9):
+        self.use_sibc, self.fmax = use_sibc, fmax
         self.x_nodes, self.y_nodes, self.z_nodes = x, y, z
         self.Nx, self.Ny, self.Nz = len(x) - 1, len(y) - 1, len(z) - 1
         self.dims = (self.Nx, self.Ny, self.Nz)
@@ -299,12 +325,22 @@ class OracleSolver:
         self.assemble()
 
     def _apply_solid(self, mask, mat):
-        """Material override arithmetic of solverFIT3D.py:416-480 (no SIBC)."""
+        """Material override arithmetic of solverFIT3D.py:416-480."""
         if isinstance(mat, str):
             mat = MATERIAL_LIB[mat.lower()]
         maski = mask.astype(int)
         eps = mat[0] * eps_0
         mu = mat[1] * mu_0
+        if len(mat) == 3:
+            # conductivity the grid resolves without SIBC (solverFIT3D.py:445-447)
+            dn = np.sqrt(2) * min(self.dx.min(), self.dy.min(), self.dz.min())
+            if self.use_sibc and mat[2] > 10 / (np.pi * self.fmax * mu * dn ** 2):
+                # surface cells, surface impedance as conductivity and as the
+                # (absolute) permittivity (solverFIT3D.py:455-466)
+                maski = sibc_surface(mask).astype(int)
+                imp = np.sqrt(np.pi * self.fmax * mu / mat[2])
+                mat = (mat[0], mat[1], 1 / imp)
+                eps = 1 / imp
 
         def scaled_add(field, scale3d):
             # Field.__mul__ with a 3-D matrix, then Field.__add__ (field.py:350-460)

@@ -78,21 +78,24 @@ def test_config1_fp32_wake_within_1e4_of_fp64_oracle():
     assert err_z <= 1e-4
 
 
-def _pillbox_pair(dtype, Nx=40, Ny=40, Nz=80, steps=150):
+def _pillbox_pair(dtype, Nx=40, Ny=40, Nz=80, steps=150, sibc=False):
     import wakis_amd as wa
     x, y, z, vac, wall = cases.pillbox_geometry(Nx, Ny, Nz)
     lossy = [1.0, 1.0, 1e4]
     bc = (["pec", "pec", "pml"], ["pec", "pec", "pml"])
     so = fo.OracleSolver(x, y, z, *bc, bg="pec", solids=[(wall, lossy), (vac, "vacuum")],
-                         dtype=dtype, n_pml=10)
+                         dtype=dtype, n_pml=10, use_sibc=sibc)
     grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"wall": wall, "vac": vac},
                         stl_materials={"wall": lossy, "vac": "vacuum"}, verbose=0)
-    # use_sibc=False: keep the volume conductivity of the wall (the reference
-    # would replace it by its pyvista surface-impedance fit, which the oracle
-    # does not restate)
+    # use_sibc=False keeps the volume conductivity of the wall; True (the
+    # reference's default) replaces it by a surface-impedance layer
     sa = wa.SolverFIT3D(grid, bc_low=bc[0], bc_high=bc[1], use_stl=True, bg="pec",
-                        dtype=dtype, verbose=0, n_pml=10, use_sibc=False)
+                        dtype=dtype, verbose=0, n_pml=10, use_sibc=sibc)
     assert sa.dt == so.dt and sa.use_conductivity and so.use_conductivity
+    if sibc:
+        assert np.array_equal(grid.grid["wall"], fo.sibc_surface(wall))
+        for name in ("ieps", "sigma"):
+            assert np.array_equal(getattr(sa, name).to_numpy(), getattr(so, name)), name
     bo = fo.OracleBeam(q=1e-9, sigmaz=15e-3, beta=0.8)
     ba = wa.Beam(q=1e-9, sigmaz=15e-3, beta=0.8)
     # start with the bunch centre just below the domain so the short run sees it
@@ -106,10 +109,11 @@ def _pillbox_pair(dtype, Nx=40, Ny=40, Nz=80, steps=150):
     return so, sa
 
 
+@pytest.mark.parametrize("sibc", [False, True])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_lossy_pillbox_pml_bit_identical(dtype):
-    print("pillbox", np.dtype(dtype).name, flush=True)
-    so, sa = _pillbox_pair(dtype)
+def test_lossy_pillbox_pml_bit_identical(dtype, sibc):
+    print("pillbox", np.dtype(dtype).name, "sibc" if sibc else "volume", flush=True)
+    so, sa = _pillbox_pair(dtype, sibc=sibc)
     assert sa.last_path == "fused"
     assert np.any(so.E != 0)
     for name in ("E", "H", "J"):

@@ -78,6 +78,26 @@ class GridFIT3D:
                 lib = material_lib[mat.lower()]
                 self.stl_materials[key] = list(lib)
 
+    def _mark_cells_in_surface(self, key):
+        """Keep only the surface cells of solid ``key`` (gridFIT3D.py:518-533), for the
+        SIBC substitution of a highly conductive solid: a cell stays marked where the
+        finite-difference gradient of the solid's 0/1 cell field is nonzero, i.e. where
+        the field differs between its two neighbours along some axis (central
+        difference), or from its one neighbour on the first and last cell of an axis.
+        The reference takes the gradient with pyvista's ``compute_derivative``."""
+        m = np.asarray(self.grid[key]).astype(np.int8)
+        surf = np.zeros(m.shape, dtype=bool)
+        for ax in range(3):
+            if m.shape[ax] < 2:
+                continue
+            a = np.moveaxis(m, ax, 0)
+            s = np.moveaxis(surf, ax, 0)  # view: writes land in surf
+            s[1:-1] |= a[2:] != a[:-2]
+            s[0] |= a[1] != a[0]
+            s[-1] |= a[-1] != a[-2]
+        self.grid[key] = surf
+        return surf
+
     # Full 3N metric Fields (gridFIT3D.py:258-300), built on demand on the host.
     def _metric(self, which):
         Nx, Ny, Nz = self.Nx, self.Ny, self.Nz

@@ -249,7 +249,9 @@ class SolverFIT3D(RoutinesMixin):
         return np.reshape(arr[c * self.N:(c + 1) * self.N], self.dims, order="F")
 
     def _apply_stl_materials(self):
-        """solverFIT3D.py:416-480 with analytic cell masks (no SIBC)."""
+        """solverFIT3D.py:416-480 with analytic cell masks; SIBC keeps the solid's
+        surface cells (``GridFIT3D._mark_cells_in_surface``) and gives them the surface impedance as
+        conductivity and (absolute) permittivity (solverFIT3D.py:455-466)."""
         h = self._h
         for key in self.grid.stl_solids.keys():
             mask = np.asarray(self.grid.grid[key]).astype(int)
@@ -277,10 +279,10 @@ class SolverFIT3D(RoutinesMixin):
                 dn = np.sqrt(2) * min(self.dx.min(), self.dy.min(), self.dz.min())
                 sigma_max = 10 / (np.pi * self.fmax * mu * dn ** 2)
                 if self.use_sibc and sig > sigma_max:
-                    raise NotImplementedError(
-                        f"solid {key!r}: sigma={sig} S/m exceeds {sigma_max:.3g} S/m and "
-                        "the reference would switch to its pyvista-based SIBC surface; "
-                        "pass use_sibc=False to keep the volume conductivity")
+                    mask = self.grid._mark_cells_in_surface(key).astype(int)
+                    imp = np.sqrt(np.pi * self.fmax * mu / sig)
+                    sig = 1 / imp  # SIBC surface conductivity
+                    eps = 1 / imp
                 h["sigma"] = scaled_add(h["sigma"], -1.0 * mask)
                 h["sigma"] = mat_add(h["sigma"], mask * sig)
                 self.use_conductivity = True
