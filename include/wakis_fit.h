@@ -214,6 +214,13 @@ int wf_energy(wf_handle *h, const void *E, const void *Hprev, const void *H, con
 int wf_loss_factor(const double *WP, const double *lam, int64_t n, double ds, double *out,
                    void *stream);
 
+/* Number of non-finite values (Inf or NaN) among x[0, n), x of the handle's
+ * dtype, into the device int64 *out. No reference counterpart: the reference
+ * has no failure detection and steps on with NaN/Inf fields. For run health
+ * checks between steps (a NaN written into a field, an unstable dt). Uses the
+ * handle's reduction workspace: calls on one handle must not overlap. */
+int wf_count_nonfinite(wf_handle *h, const void *x, int64_t n, int64_t *out, void *stream);
+
 /* Batches. No reference counterpart (the reference runs one grid per process);
  * for parameter sweeps over many small grids. Each grid keeps its handle and
  * a fused-path wf_step_args (path 1, E2/H2; material codes with

@@ -668,6 +668,26 @@ int wf_energy(wf_handle *h, const void *E, const void *Hprev, const void *H, con
   return WF_OK;
 }
 
+int wf_count_nonfinite(wf_handle *h, const void *x, int64_t n, int64_t *out, void *stream) {
+  if (!h || !out || (!x && n > 0)) return fail(WF_EINVAL, "wf_count_nonfinite: null argument");
+  if (n < 0) return fail(WF_ERANGE, "wf_count_nonfinite: negative length");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t want = (n + wf::RED_THREADS - 1) / wf::RED_THREADS;
+  const int nb = (int)(want < 1 ? 1 : (want < wf::RED_BLOCKS ? want : wf::RED_BLOCKS));
+  long long *part = (long long *)h->red_buf;  // RED_BLOCKS*4 doubles: room for nb counts
+  if (h->dtype == WF_F64)
+    hipLaunchKernelGGL(wf::k_count_nonfinite<double>, dim3(nb), dim3(wf::RED_THREADS), 0, st,
+                       (const double *)x, n, part);
+  else
+    hipLaunchKernelGGL(wf::k_count_nonfinite<float>, dim3(nb), dim3(wf::RED_THREADS), 0, st,
+                       (const float *)x, n, part);
+  WF_LAUNCH_CHECK();
+  hipLaunchKernelGGL(wf::k_sum_counts, dim3(1), dim3(wf::RED_THREADS), 0, st,
+                     (const long long *)part, nb, (long long *)out);
+  WF_LAUNCH_CHECK();
+  return WF_OK;
+}
+
 int wf_loss_factor(const double *WP, const double *lam, int64_t n, double ds, double *out,
                    void *stream) {
   if (!WP || !lam || !out) return fail(WF_EINVAL, "wf_loss_factor: null argument");

@@ -115,4 +115,45 @@ k_dot(const double *__restrict__ x, const double *__restrict__ y, int64_t n, dou
   if (threadIdx.x == 0) out[0] = scale * (a[0].s + a[0].c);
 }
 
+// non-finite values (Inf or NaN: all exponent bits set) of x[0, n): per-block
+// counts into part[blockIdx.x], in a fixed tree order
+__device__ __forceinline__ int nonfinite(float v) {
+  return (__float_as_uint(v) & 0x7f800000u) == 0x7f800000u;
+}
+__device__ __forceinline__ int nonfinite(double v) {
+  return (__double_as_longlong(v) & 0x7ff0000000000000ll) == 0x7ff0000000000000ll;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(RED_THREADS)
+k_count_nonfinite(const S *__restrict__ x, int64_t n, long long *__restrict__ part) {
+  __shared__ long long sh[RED_THREADS];
+  long long c = 0;
+  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * RED_THREADS)
+    c += nonfinite(x[i]);
+  sh[threadIdx.x] = c;
+  __syncthreads();
+  for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// one block: out[0] = sum of nb per-block counts
+__global__ void __launch_bounds__(RED_THREADS)
+k_sum_counts(const long long *__restrict__ part, int nb, long long *__restrict__ out) {
+  __shared__ long long sh[RED_THREADS];
+  long long c = 0;
+  for (int b = threadIdx.x; b < nb; b += RED_THREADS) c += part[b];
+  sh[threadIdx.x] = c;
+  __syncthreads();
+  for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
 }  // namespace wf

@@ -42,8 +42,17 @@ class RoutinesMixin:
     def wakesolve(self, wakelength, wake=None, callback=None, compute_plane="both", plot=False,
                   plot_func=None, plot_from=None, plot_every=1, plot_until=None, save_J=False,
                   add_space=None, use_field_monitor=False, field_monitor=None, use_edt=None,
-                  chunk=None, **kwargs):
-        """Wakefield run and WakeSolver post-processing (routines.py:231-482)."""
+                  chunk=None, checkpoint=None, checkpoint_every=None, resume=None, **kwargs):
+        """Wakefield run and WakeSolver post-processing (routines.py:231-482).
+
+        Beyond the reference: ``checkpoint_every=m`` with ``checkpoint=path``
+        writes the run state (E, H, J, the step index and the Ez samples
+        captured so far) to ``path`` every m steps, replacing the file
+        atomically; ``resume=path`` continues such a run in a new solver built
+        the same way, and the result is bitwise that of the uninterrupted run.
+        The fields are checked for Inf/NaN on the device at the end
+        (``fields_finite``).
+        """
         if wake is not None:
             self.wake = wake
         if add_space is not None and self.wake is not None:
@@ -51,15 +60,23 @@ class RoutinesMixin:
         if plot or save_J:
             raise NotImplementedError("wakesolve: plotting and Jz.h5 saving are not part of "
                                       "this engine")
+        every = None
+        if checkpoint_every is not None:
+            every = int(checkpoint_every)
+            if every < 1 or checkpoint is None:
+                raise ValueError("wakesolve: checkpoint_every needs a positive step count "
+                                 "and a checkpoint file name")
         ctx = self._wake_setup(wakelength, compute_plane)
         Nt = ctx["Nt"]
+        n = 0 if resume is None else self._wake_resume(ctx, resume)
         print("Running electromagnetic time-domain simulation...")
         t0 = time.time()
         per_step = callback is not None or (use_field_monitor and field_monitor is not None)
         step = 1 if per_step else (Nt if chunk is None else int(chunk))
-        n = 0
         while n < Nt:
             k = min(step, Nt - n)
+            if every is not None:
+                k = min(k, every - n % every)
             self._run(self._wake_args(ctx, n, k))
             self.nstep += k
             if per_step:
@@ -68,9 +85,64 @@ class RoutinesMixin:
                 if callback is not None:
                     callback(self, n * self.dt)
             n += k
+            if every is not None and n % every == 0 and n < Nt:
+                self._wake_checkpoint(ctx, checkpoint, n)
         torch.cuda.synchronize(self.device)
         self.simulationTime = time.time() - t0
+        self.fields_finite = self.check_finite(("E", "H"))
+        if not self.fields_finite:
+            print("[!] Warning: the fields hold Inf/NaN values after the time stepping")
         self._wake_finish(ctx, compute_plane)
+
+    def _wake_state_key(self, ctx):
+        return dict(wake_Nt=np.int64(ctx["Nt"]), wake_dt=np.float64(self.dt),
+                    wake_dims=np.asarray(self.dims, dtype=np.int64),
+                    wake_dtype=np.str_(np.dtype(self.dtype).name),
+                    wake_win=np.asarray(ctx["win"], dtype=np.int64))
+
+    def _wake_checkpoint(self, ctx, path, n):
+        """Write the state after n steps of the run in ctx: E, H, J (the keys of
+        save_state, so load_state reads the file too), the step index, the Ez
+        samples 0..n and the fast-path facts of J. Written to a temporary file
+        and renamed, so an interrupted write keeps the previous checkpoint."""
+        import os
+        bl = self._beam_line if self._beam_line is not None else (-1, -1)
+        state = dict(E=self.E.to_numpy(), H=self.H.to_numpy(), J=self.J.to_numpy(),
+                     wake_n=np.int64(n), wake_cap=ctx["cap"][:, :n + 1].cpu().numpy(),
+                     nstep=np.int64(self.nstep),
+                     j_beam_only=np.bool_(self.J.version == self._J_clean_version),
+                     j_sigma_e=np.bool_(self.use_conductivity and self._sigmaE_holds()),
+                     beam_line=np.asarray(bl, dtype=np.int64), **self._wake_state_key(ctx))
+        tmp = f"{path}.part"
+        with open(tmp, "wb") as f:
+            np.savez(f, **state)
+        os.replace(tmp, path)
+
+    def _wake_resume(self, ctx, path):
+        """Load a _wake_checkpoint file into this run; returns the next step."""
+        with np.load(path, allow_pickle=False) as st:
+            want = self._wake_state_key(ctx)
+            for key, val in want.items():
+                if key not in st or not np.array_equal(st[key], val):
+                    raise ValueError(f"wakesolve: {path} was written by a different run "
+                                     f"({key}: {st[key] if key in st else None} != {val})")
+            n = int(st["wake_n"])
+            if not 0 < n <= ctx["Nt"]:
+                raise ValueError(f"wakesolve: {path} holds step {n} of {ctx['Nt']}")
+            for name in ("E", "H", "J"):
+                getattr(self, name).fromarray(st[name])
+            cap = st["wake_cap"]
+            if cap.shape != tuple(ctx["cap"][:, :n + 1].shape):
+                raise ValueError(f"wakesolve: {path} holds an Ez capture of shape {cap.shape}")
+            ctx["cap"][:, :n + 1].copy_(torch.as_tensor(cap))
+            self.nstep = int(st["nstep"])
+            if bool(st["j_beam_only"]):  # J is zero off the beam line, as in the saved run
+                self._J_clean_version = self.J.version
+                bl = tuple(int(v) for v in st["beam_line"])
+                self._beam_line = None if bl[0] < 0 else bl
+            if bool(st["j_sigma_e"]):
+                self._sigmaE_versions = (self.E.version, self.J.version)
+        return n
 
     def _wake_setup(self, wakelength, compute_plane):
         """Set-up part of wakesolve (routines.py:319-440): time axis, beam table,

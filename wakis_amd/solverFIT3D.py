@@ -733,6 +733,22 @@ class SolverFIT3D(RoutinesMixin):
         uE, uH = out.tolist()
         return (uE, uH) if parts else uE + uH
 
+    def count_nonfinite(self, fields=("E", "H", "J")):
+        """Number of Inf/NaN values in the named fields, counted on the device
+        (no reference counterpart: the reference steps on with non-finite
+        fields). Returns a dict {name: count}."""
+        out = torch.zeros(len(fields), dtype=torch.int64, device=self.device)
+        for i, name in enumerate(fields):
+            F = getattr(self, name)
+            _lib.check(_lib.lib().wf_count_nonfinite(
+                self._handle, _lib.ptr(F.array), F.array.numel(), _lib.ptr(out[i:]),
+                _lib.stream_ptr()), "wf_count_nonfinite")
+        return dict(zip(fields, out.tolist()))
+
+    def check_finite(self, fields=("E", "H", "J")):
+        """True if the named fields hold no Inf or NaN (device check)."""
+        return not any(self.count_nonfinite(fields).values())
+
     def one_step(self):
         """One leapfrog step (solverFIT3D.py:482-503)."""
         if self.step_0:
