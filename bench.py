@@ -97,7 +97,7 @@ def time_gpu(Nx, Ny, Nz, dtype, steps, warmup, path, variant=0, kz=0, wake_check
     s.step_0 = False
     s._beam_line = (int(beam.ixs), int(beam.iys))
 
-    def run(n0, k):
+    def args(n0, k):
         a = s._step_args(k)
         a.ixs, a.iys = int(beam.ixs), int(beam.iys)
         a.jmode = _lib.WF_J_LINE
@@ -105,8 +105,12 @@ def time_gpu(Nx, Ny, Nz, dtype, steps, warmup, path, variant=0, kz=0, wake_check
         a.capture = _lib.ptr(cap)
         a.cap_ix0, a.cap_iy0, a.cap_nx, a.cap_ny = ixt - 1, iyt - 1, 3, 3
         a.cap_samples, a.cap_sample0 = tot, n0 + 1
-        s._run(a)
+        return a
 
+    def run(n0, k):
+        s._run(args(n0, k))
+
+    s.tune_fused(lambda k: args(0, k))  # workgroup shape by measurement (fields restored)
     run(0, warmup)
     torch.cuda.synchronize(dev)
     used_path = s.last_path
@@ -124,7 +128,8 @@ def time_gpu(Nx, Ny, Nz, dtype, steps, warmup, path, variant=0, kz=0, wake_check
     N = s.N
     out = dict(N=N, steps=steps, ms_total=ms, ms_per_step=ms / steps, wall_s=wall, path=used_path,
                gcell_s=N * steps / (ms * 1e-3) / 1e9, finite=finite,
-               bytes_per_cell=VALUES_PER_CELL * np.dtype(dtype).itemsize)
+               bytes_per_cell=VALUES_PER_CELL * np.dtype(dtype).itemsize,
+               full_rows=s.fused_full_rows, tuning=_tuning(s))
     out["achieved_GBs"] = N * out["bytes_per_cell"] / (out["ms_per_step"] * 1e-3) / 1e9
     if wake_check:
         out["WP"] = wake_of_capture(s, cap, tot)
@@ -264,15 +269,17 @@ def batch_sweep(n=64, steps=2000, seed=0, dtype=np.float32):
             "data": "synthetic (random_box_cavities, seed %d)" % seed}
 
 
-def lossy_pml(dtype, steps=300, warmup=20, dims=(200, 200, 400), keep_fields=False):
+def lossy_pml(dtype, steps=300, warmup=20, dims=(200, 200, 400), keep_fields=False, sibc=True):
     """The lossy pillbox with PML: cylinder r = 40 mm, |z| < 30 mm, beam pipe
     r = 10 mm, a 3-cell wall shell of sigma = 1e4 S/m (eps_r = mu_r = 1) around
     the vacuum in a PEC background; 90 x 90 x 180 mm on `dims` cells, PEC x/y,
     10-layer PML on both z ends; beam q = 1 nC, sigma_z = 15 mm, beta = 0.8 on
     axis, its table starting at 0.9 of the injection time so the bunch head is
-    in the domain. dt is the reference's tau clamp (eps0 / sigma_max) and the
-    wall keeps its volume conductivity (no surface-impedance fit). Times
-    `steps` leapfrog steps (J = sigma E everywhere, coefficient arrays)."""
+    in the domain. With `sibc` (the reference's default) the wall is replaced
+    by its surface cells with sigma = eps = 1/Z_s, Z_s = sqrt(pi fmax mu0 /
+    sigma), fmax = 1 GHz, and dt is the CFL step; without it the wall keeps its
+    volume conductivity and dt is the reference's tau clamp (eps0 / sigma).
+    Times `steps` leapfrog steps (J = sigma E everywhere)."""
     import torch
 
     import wakis_amd as wa
@@ -297,7 +304,7 @@ def lossy_pml(dtype, steps=300, warmup=20, dims=(200, 200, 400), keep_fields=Fal
     grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"wall": wall, "vac": vac},
                         stl_materials={"wall": [1.0, 1.0, 1e4], "vac": "vacuum"}, verbose=0)
     s = wa.SolverFIT3D(grid, bc_low=["pec", "pec", "pml"], bc_high=["pec", "pec", "pml"],
-                       use_stl=True, bg="pec", dtype=dtype, verbose=0, n_pml=10, use_sibc=False)
+                       use_stl=True, bg="pec", dtype=dtype, verbose=0, n_pml=10, use_sibc=sibc)
     beam = wa.Beam(q=1e-9, sigmaz=15e-3, beta=0.8)
     dev = s.device
     tot = warmup + steps
@@ -308,13 +315,17 @@ def lossy_pml(dtype, steps=300, warmup=20, dims=(200, 200, 400), keep_fields=Fal
     s.step_0 = False
     s._beam_line = (int(beam.ixs), int(beam.iys))
 
-    def run(n0, k):
+    def args(n0, k):
         a = s._step_args(k)
         a.ixs, a.iys = int(beam.ixs), int(beam.iys)
         a.jmode = _lib.WF_J_FULL
         a.beam_table, a.beam_row0, a.beam_rows = _lib.ptr(table), n0, tot
-        s._run(a)
+        return a
 
+    def run(n0, k):
+        s._run(args(n0, k))
+
+    s.tune_fused(lambda k: args(0, k))  # workgroup shape by measurement (fields restored)
     run(0, warmup)
     torch.cuda.synchronize(dev)
     stream = torch.cuda.current_stream(dev)
@@ -329,13 +340,21 @@ def lossy_pml(dtype, steps=300, warmup=20, dims=(200, 200, 400), keep_fields=Fal
     out = dict(ms_per_step=round(ms, 5), gcell_s=round(s.N / (ms * 1e-3) / 1e9, 3),
                bytes_per_cell_update=b, path=s.last_path, dt=float(s.dt),
                achieved_GBs=round(s.N * b / (ms * 1e-3) / 1e9, 1),
-               frac=round(s.N * b / (ms * 1e-3) / 8e12, 4))
+               frac=round(s.N * b / (ms * 1e-3) / 8e12, 4),
+               full_rows=s.fused_full_rows, tuning=_tuning(s))
     if keep_fields:
         out["E"] = s.E.array.to(torch.float64)
         out["H"] = s.H.array.to(torch.float64)
     del s, table
     torch.cuda.empty_cache()
     return out
+
+
+def _tuning(s):
+    """ms per step of each workgroup shape tune_fused() timed (-1: row segments,
+    N: whole-row workgroups of N rows)."""
+    t = s.fused_tuning or {}
+    return {str(k): round(v, 4) for k, v in sorted(t.items())}
 
 
 def _rel_l2(a, b):
@@ -491,8 +510,9 @@ def main():
         l64 = lossy_pml(np.float64, keep_fields=True)
         lossy = {"workload": "lossy pillbox r=40 mm, 3-cell sigma=1e4 S/m wall, 10-layer PML "
                              "on z, 200x200x400 cells, beam sigma_z=15 mm beta=0.8, 320 steps",
-                 "dt_policy": "reference tau clamp (dt = eps0/sigma_max = %.4g s); volume "
-                              "conductivity, no surface-impedance fit" % l64["dt"],
+                 "dt_policy": "use_sibc=True (the reference default): wall surface cells "
+                              "with sigma = eps = 1/Z_s at fmax = 1 GHz; dt = %.4g s (CFL, "
+                              "no tau clamp)" % l64["dt"],
                  "fp32": {k: v for k, v in l32.items() if k not in ("E", "H", "dt")},
                  "fp64": {k: v for k, v in l64.items() if k not in ("E", "H", "dt")},
                  "rel_l2_E_fp32_vs_fp64": _rel_l2(l32["E"], l64["E"]),
@@ -527,7 +547,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"wakefield leapfrog, PEC cavity 50mm + pipe r=5mm, "
                                f"{Nx}x{Ny}x{Nz} cells, beam on axis, PEC x/y ABC z",
-                   "cells": r32["N"], "path": r32["path"]},
+                   "cells": r32["N"], "path": r32["path"], "full_rows": r32["full_rows"]},
         "roofline": {"bound": "hbm", "achieved": round(r32["achieved_GBs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r32["achieved_GBs"] / HBM_PEAK_GBS, 4),

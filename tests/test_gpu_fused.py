@@ -11,6 +11,7 @@ from tests.test_stencil_rules import BCS
 pytestmark = pytest.mark.gpu
 
 PEC_ALL = (["pec", "pec", "pec"], ["pec", "pec", "pec"])
+PMC_MIX = (["pec", "pmc", "pec"], ["pmc", "pec", "pec"])
 
 
 @pytest.mark.parametrize("dims", [(12, 10, 14), (70, 13, 37), (64, 8, 33)])
@@ -115,6 +116,125 @@ def test_fused_multistep_conductive_call_matches_oracle(dims, dtype, use_code):
         assert np.array_equal(got, ref), f"{name}: rel-L2 {gc.rel_l2(got, ref):.3e}"
 
 
+def _full_rows_fit(dims, dtype, rows):
+    """Whether a full-row tile of `rows` rows fits (512 threads, 64 KiB LDS
+    without the material table), as the engine decides."""
+    vn = 16 // np.dtype(dtype).itemsize
+    Nx, Ny = dims[0], dims[1]
+    if Nx % vn:
+        return False
+    q = Nx // vn
+
+    def fits(r):
+        thr = -(-(r + 1) * q // 64) * 64
+        return thr <= 512 and (6 * r + 9) * (Nx + 2 * vn) * np.dtype(dtype).itemsize <= 60000
+
+    return fits(1) if rows == 1 else fits(rows)
+
+
+@pytest.mark.parametrize("dims", [(12, 10, 14), (200, 9, 12), (40, 17, 21), (36, 3, 11),
+                                  (70, 13, 37)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("conductive", [False, True])
+@pytest.mark.parametrize("use_code", [True, False])
+@pytest.mark.parametrize("rows", [1, 2, 5, 9])
+def test_fused_full_rows_matches_oracle(dims, dtype, conductive, use_code, rows):
+    """Workgroups of whole rows (thread -> (row, cells) map, no x halos) are
+    bit-identical to the oracle for every row count that fits; the others are
+    refused."""
+    print("fused full rows", dims, np.dtype(dtype).name, conductive, use_code, rows, flush=True)
+    case = gc.random_case(PEC_ALL, dtype, conductive, seed=sum(dims) + 3 * rows, dims=dims)
+    case["fields"][2] = np.zeros_like(case["fields"][2])
+    so = gc.oracle_solver(case)
+    sa = gc.amd_solver(case, path="fused")
+    sa.use_material_code = use_code
+    sa.fused_full_rows = rows
+    if conductive:
+        so.J[:] = 0
+        so.one_step()
+        sa.path = "general"
+        sa.one_step()
+        sa.path = "fused"
+    if not _full_rows_fit(dims, dtype, rows):
+        with pytest.raises(ValueError, match="full-row"):
+            sa.one_step()
+        return
+    for _ in range(5):
+        so.one_step()
+        sa.one_step()
+        assert sa.last_path == "fused"
+    for name in ("E", "H", "J"):
+        got, ref = getattr(sa, name).to_numpy(), getattr(so, name)
+        assert np.array_equal(got, ref), f"{name}: rel-L2 {gc.rel_l2(got, ref):.3e}"
+
+
+@pytest.mark.parametrize("bc", [PEC_ALL, PMC_MIX])
+@pytest.mark.parametrize("zero_masked", [False, True])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("rows", [1, 3])
+def test_fused_full_rows_masks_match_oracle(bc, zero_masked, dtype, rows):
+    """Full-row tiles with PEC/PMC masks applied in the kernel or folded away;
+    a multi-step conductive call with a beam line and no beam table."""
+    print("fused full rows masks", bc[0], zero_masked, np.dtype(dtype).name, rows, flush=True)
+    dims = (40, 11, 17)
+    case = gc.random_case(bc, dtype, True, seed=29 + rows, dims=dims)
+    case["fields"][2] = np.zeros_like(case["fields"][2])
+    if zero_masked:
+        _zero_masked_values(case)
+    so = gc.oracle_solver(case)
+    sa = gc.amd_solver(case, path="fused")
+    sa.fused_full_rows = rows
+    so.J[:] = 0
+    so.one_step()
+    sa.path = "general"
+    sa.one_step()
+    sa.path = "fused"
+    sa._beam_line = (dims[0] // 2, dims[1] // 2)
+    for _ in range(4):
+        so.one_step()
+    sa._run(sa._step_args(4))
+    assert sa.last_path == "fused"
+    assert sa._masked_zero() == zero_masked
+    for name in ("E", "H", "J"):
+        got, ref = getattr(sa, name).to_numpy(), getattr(so, name)
+        assert np.array_equal(got, ref), f"{name}: rel-L2 {gc.rel_l2(got, ref):.3e}"
+
+
+@pytest.mark.parametrize("conductive", [False, True])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_tune_fused_restores_state_and_keeps_results(conductive, dtype):
+    """tune_fused() times every candidate shape, leaves E, H, J as they were,
+    and the run that follows equals the oracle bitwise."""
+    print("tune fused", conductive, np.dtype(dtype).name, flush=True)
+    dims = (40, 21, 30)
+    case = gc.random_case(PEC_ALL, dtype, conductive, seed=41, dims=dims)
+    case["fields"][2] = np.zeros_like(case["fields"][2])
+    so = gc.oracle_solver(case)
+    sa = gc.amd_solver(case, path="fused")
+    if conductive:
+        so.J[:] = 0
+        so.one_step()
+        sa.path = "general"
+        sa.one_step()
+        sa.path = "fused"
+    before = [getattr(sa, n).to_numpy() for n in ("E", "H", "J")]
+    best = sa.tune_fused()
+    assert best is not None and best in sa.fused_tuning and -1 in sa.fused_tuning
+    assert len(sa.fused_tuning) >= 3 and sa.fused_full_rows == best
+    for n, b in zip(("E", "H", "J"), before):
+        assert np.array_equal(getattr(sa, n).to_numpy(), b), n
+    assert sa.tune_fused() is None  # the choice is now fixed
+    so.one_step()
+    sa.one_step()  # (first step: ghost zeroing, as in the oracle)
+    for _ in range(4):
+        so.one_step()
+    sa._run(sa._step_args(4))
+    assert sa.last_path == "fused"
+    for name in ("E", "H", "J"):
+        got, ref = getattr(sa, name).to_numpy(), getattr(so, name)
+        assert np.array_equal(got, ref), f"{name}: rel-L2 {gc.rel_l2(got, ref):.3e}"
+
+
 def _zero_masked_values(case):
     """Zero E on PEC-masked edges (and H on PMC-masked faces) of a random case."""
     dims = tuple(len(a) - 1 for a in (case["x"], case["y"], case["z"]))
@@ -124,8 +244,6 @@ def _zero_masked_values(case):
     if pmc is not None:
         case["fields"][1] = np.where(pmc == 0, 0, case["fields"][1]).astype(case["dtype"])
 
-
-PMC_MIX = (["pec", "pmc", "pec"], ["pmc", "pec", "pec"])
 
 
 @pytest.mark.parametrize("bc", [PEC_ALL, PMC_MIX])
@@ -174,7 +292,7 @@ def test_fused_falls_back_when_wraps_live():
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("zbc", ["abc", "pml"])
 @pytest.mark.parametrize("use_code", [True, False])
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, "full_rows"])
 def test_fused_beam_loop_matches_oracle(dtype, zbc, use_code, variant):
     """Beam.update + one_step user loop on the cavity/pipe (beam-line J mode;
     with PML on z the conductive fused step), with and without the material
@@ -190,7 +308,11 @@ def test_fused_beam_loop_matches_oracle(dtype, zbc, use_code, variant):
     sa = wa.SolverFIT3D(grid, bc_low=bc_lo, bc_high=bc_hi, use_stl=True, bg="pec",
                         dtype=dtype, verbose=0, path="fused", n_pml=4)
     sa.use_material_code = use_code
-    sa.fused_variant = variant
+    if variant == "full_rows":
+        sa.fused_full_rows = 1
+    else:
+        sa.fused_variant = variant
+        sa.fused_full_rows = -1
     assert sa.dt == so.dt
     bo, ba = fo.OracleBeam(q=1e-9, sigmaz=5e-3), wa.Beam(q=1e-9, sigmaz=5e-3)
     for n in range(80):

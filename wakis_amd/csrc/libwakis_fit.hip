@@ -216,14 +216,14 @@ using Fused4Fn = void (*)(wf::GridDev, wf::MetricDev<S>, const S *, const S *, S
 
 // The k_fused4 instantiation for the step's J mode, conductivity, coefficient
 // form, boundary order and masking.
-template <typename S, int TY, int V>
+template <typename S, int TY, int V, bool FR>
 Fused4Fn<S> pick_fused4(bool lut, bool cond, bool jline, bool desc, bool mk) {
 #define WF_PICK(JM, CO, LU)                                                                 \
   do {                                                                                      \
-    if (desc) return mk ? &wf::k_fused4<S, JM, CO, LU, true, TY, true, V>                   \
-                        : &wf::k_fused4<S, JM, CO, LU, true, TY, !LU, V>;                   \
-    return mk ? &wf::k_fused4<S, JM, CO, LU, false, TY, true, V>                            \
-              : &wf::k_fused4<S, JM, CO, LU, false, TY, !LU, V>;                            \
+    if (desc) return mk ? &wf::k_fused4<S, JM, CO, LU, true, TY, true, V, FR>               \
+                        : &wf::k_fused4<S, JM, CO, LU, true, TY, !LU, V, FR>;               \
+    return mk ? &wf::k_fused4<S, JM, CO, LU, false, TY, true, V, FR>                        \
+              : &wf::k_fused4<S, JM, CO, LU, false, TY, !LU, V, FR>;                        \
   } while (0)
   if (lut) {
     if (cond) WF_PICK(0, true, true);
@@ -265,23 +265,76 @@ int auto_kz(int64_t nxy, int Nz, int64_t conc) {
   return kz0;
 }
 
-template <typename S, int TY, int V>
-int step_fused4(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStream_t st) {
+// Full-row tiles: dynamic LDS bytes of a tile of `rows` own rows (E and H rows
+// of Nx + 2V slots, then the material table).
+inline size_t full_rows_lds(int Nx, int V, int rows, size_t sz, size_t table) {
+  return (size_t)(6 * rows + 9) * (size_t)(Nx + 2 * V) * sz + table;
+}
+constexpr int FR_BOUND = 7;            // full-row kernels: up to 64 * 8 threads
+constexpr size_t FR_LDS_MAX = 65536;   // and 64 KiB of LDS per workgroup
+
+// Own rows of a full-row tile, within the thread and LDS bounds (0 if no tile
+// fits). The march of a workgroup is latency bound and its two barriers per
+// plane stall all its waves together, so tiles that let a CU hold two or more
+// workgroups come first (measured: 200-cell rows, 4 rows in 4 waves twice per
+// CU run 1.25x faster than 9 rows in 8 waves once per CU); among those, the
+// most own rows resident per CU, then the fewest idle lanes. `per_cu(T)`
+// gives the workgroups per CU of a T-row tile. *eff = own cells per lane.
+template <typename F>
+int full_rows_pick(int Nx, int Ny, int V, size_t sz, size_t table, F per_cu, double *eff) {
+  const int Q = Nx / V;
+  int best = 0, bocc = 0;
+  int64_t bres = -1;
+  double be = 0.0;
+  for (int T = 1; T <= Ny; ++T) {
+    const int thr = ((T + 1) * Q + 63) / 64 * 64;
+    if (thr > 64 * (FR_BOUND + 1) || full_rows_lds(Nx, V, T, sz, table) > FR_LDS_MAX) break;
+    const int occ = per_cu(T, thr, full_rows_lds(Nx, V, T, sz, table));
+    const int occ2 = occ < 2 ? occ : 2;
+    const int64_t res = (int64_t)occ * T;
+    const double e = (double)T * Q / thr;
+    if (occ2 > bocc || (occ2 == bocc && (res > bres || (res == bres && e > be + 1e-12)))) {
+      best = T; bocc = occ2; bres = res; be = e;
+    }
+  }
+  *eff = be;
+  return best;
+}
+
+template <typename S, int TY, int V, bool FR = false>
+int step_fused4(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStream_t st,
+                int fr_rows = 0) {
   const wf::GridDev &g = h->g;
   wf::FusedAux x;
   x.ixs = (a->jmode == WF_J_LINE || a->beam_table || a->conductive) ? a->ixs : -1;
   x.iys = (x.ixs >= 0) ? a->iys : -1;
   x.cap_ix0 = a->cap_ix0; x.cap_iy0 = a->cap_iy0; x.cap_nx = a->cap_nx; x.cap_ny = a->cap_ny;
   x.cap_samples = a->cap_samples;
-  constexpr int Wc = 64 * V;
-  x.ntx = (g.Nx + Wc - 1) / Wc;
-  x.nty = (g.Ny + TY - 1) / TY;
   const bool lut = a->code && a->mat && a->nmat > 0 && a->metric2d;
-  const size_t lds = lut ? (size_t)a->nmat * 9 * sizeof(S) : 0;
-  const int threads = 64 * (TY + 1);
-  const Fused4Fn<S> kern = pick_fused4<S, TY, V>(lut, a->conductive != 0,
-                                                  a->jmode == WF_J_LINE, g.m1desc != 0,
-                                                  !(lut && a->zero_masked));
+  const size_t table = lut ? (size_t)a->nmat * 9 * sizeof(S) : 0;
+  size_t lds = table;
+  int threads = 64 * (TY + 1);
+  if constexpr (FR) {
+    // whole rows per tile: Q = Nx / V lanes per row, fr_rows own rows + the halo row
+    x.rows = fr_rows;
+    x.qrow = g.Nx / V;
+    x.ntx = 1;
+    x.nty = (g.Ny + fr_rows - 1) / fr_rows;
+    threads = ((fr_rows + 1) * x.qrow + 63) / 64 * 64;
+    lds = full_rows_lds(g.Nx, V, fr_rows, sizeof(S), table);
+    if (fr_rows < 1 || threads > 64 * (TY + 1) || lds > FR_LDS_MAX)
+      return fail(WF_EINVAL, "wf_step: full-row tile of %d rows does not fit (%d threads, "
+                  "%zu bytes of LDS)", fr_rows, threads, lds);
+  } else {
+    constexpr int Wc = 64 * V;
+    x.rows = TY;
+    x.qrow = 64;
+    x.ntx = (g.Nx + Wc - 1) / Wc;
+    x.nty = (g.Ny + TY - 1) / TY;
+  }
+  const Fused4Fn<S> kern = pick_fused4<S, TY, V, FR>(lut, a->conductive != 0,
+                                                      a->jmode == WF_J_LINE, g.m1desc != 0,
+                                                      !(lut && a->zero_masked));
   if (a->kz > 0) {
     x.KZ = a->kz;
   } else {
@@ -349,6 +402,41 @@ int step_fused(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStre
   const bool lut = a->code && a->mat && a->nmat > 0 && a->metric2d;
   const int rdef = sizeof(S) == 8 ? 3 : ((lut && !a->conductive) ? 8 : 7);
   const int rows = a->variant ? a->variant : rdef;
+  // full-row tiles (16-byte lanes): always with full_rows = N >= 1 (N rows per
+  // workgroup), never with -1, and by default when the picked tile has at
+  // least 2 rows and leaves clearly fewer lanes idle than the 64-lane row
+  // segments would
+  if (a->full_rows > 0 || (a->full_rows == 0 && a->variant == 0 && a->cells_per_lane == 0)) {
+    const bool fits = idx32v && g.Nx % vn == 0;
+    if (a->full_rows > 0 && !fits)
+      return fail(WF_EINVAL, "wf_step: full-row tiles need Nx a multiple of %d and every 3N "
+                  "array within 4 GiB", vn);
+    if (fits) {
+      double efr = 0.0;
+      const size_t table = lut ? (size_t)a->nmat * 9 * sizeof(S) : 0;
+      const Fused4Fn<S> kfr = pick_fused4<S, FR_BOUND, vn, true>(
+          lut, a->conductive != 0, a->jmode == WF_J_LINE, g.m1desc != 0,
+          !(lut && a->zero_masked));
+      const auto per_cu = [&](int, int thr, size_t lds) {
+        return fused_occupancy(h, (const void *)kfr, thr, lds);
+      };
+      int T = full_rows_pick(g.Nx, g.Ny, vn, sizeof(S), table, per_cu, &efr);
+      bool use = a->full_rows > 0;
+      if (a->full_rows >= 1) T = a->full_rows;
+      if (!use && T >= 2) {
+        const int Va = auto_cells_per_lane(g.Nx, vn);
+        const int64_t seg = 64 * (int64_t)Va;
+        const double enr = (double)g.Nx / (double)(((g.Nx + seg - 1) / seg) * seg) *
+                           (double)rdef / (rdef + 1.0);
+        use = efr > enr + 0.02;
+      }
+      if (use) {
+        if (T < 1)
+          return fail(WF_EINVAL, "wf_step: no full-row tile fits rows of %d cells", g.Nx);
+        return step_fused4<S, FR_BOUND, vn, true>(h, a, m, st, T);
+      }
+    }
+  }
   // cells per lane: vn for every row count; 1 and 2 (FP32) with the default rows
   int V = a->cells_per_lane;
   if (V == 0) {

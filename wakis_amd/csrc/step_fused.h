@@ -28,6 +28,8 @@ struct FusedAux {
   // without a beam table (the next step of the same call recomputes sigma E
   // from E; J is stored in full by the call's last step)
   int skip_j;
+  // k_fused4 with full-row tiles: own rows per tile and lanes per row (Nx / V)
+  int rows, qrow;
 };
 
 // z-face override of the face mask for components x and y at plane k:

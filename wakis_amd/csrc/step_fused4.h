@@ -10,6 +10,14 @@
 // workgroup marches up its chunk of z planes with E two planes and the other
 // per-plane inputs one plane ahead. PEC/PMC column masks and absent
 // neighbours are applied once, when a value is staged (as 0).
+//
+// Full-row tiles (FR): for rows whose length is not close to a multiple of the
+// 64-lane segment (e.g. 200 cells: 22% of the lanes idle), a workgroup takes
+// whole rows instead. Thread t then owns the V cells at x = V * (t % Q) of
+// tile row t / Q (Q = Nx / V lanes per row; tile row 0 is the halo row), the
+// tile has x.rows own rows, there are no x halos, and the LDS rows (Nx + 2V
+// slots) live in dynamic LDS ahead of the material table. Only the thread ->
+// (row, cell) map and the LDS strides change; the arithmetic is the same.
 #pragma once
 #include "fit_common.h"
 #include "step_fused.h"
@@ -88,7 +96,7 @@ __device__ __forceinline__ int xcd_tile(int blin, int nb) {
 
 // One workgroup's z-march over tile `tile` of the grid (the whole step of that
 // tile); shared by the single-grid and the batched kernels.
-template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY, bool MK, int V>
+template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY, bool MK, int V, bool FR>
 __device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S> &m,
                                             const S *__restrict__ Ein, const S *__restrict__ Hin,
                                             S *__restrict__ Eout, S *__restrict__ Hout,
@@ -97,29 +105,50 @@ __device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S>
                                             S *__restrict__ cap, int tile) {
   using I = uint32_t;
   // V cells per lane: 16 / sizeof(S) for 16-byte accesses, or 1 for narrow grids
-  constexpr int W = 64 * V;                    // cells per row segment
-  constexpr int OFF = V, RW = W + 2 * V;       // cell xi at LDS slot xi + OFF
+  constexpr int OFF = V;                       // cell xi at LDS slot xi + OFF
   constexpr int NM = 9;
+  // segment cells, own rows, LDS row length and lanes per row (FR: run time)
+  const int W = FR ? g.Nx : 64 * V;
+  const int TYr = FR ? x.rows : TY;
+  const int RW = W + 2 * V;
+  const int Q = FR ? x.qrow : 64;
   // one buffer each: a plane's E staging happens after the barrier that ends
   // every read of the previous plane's E, and its H writes after the barrier
-  // that ends every read of the previous plane's H
-  __shared__ S sE[3][TY + 2][RW];  // rows: 0 = j0-1, 1..TY own, TY+1 = j0+TY
-  __shared__ S sH[3][TY + 1][RW];  // rows: 0 = j0-1, 1..TY own (PMC-masked)
+  // that ends every read of the previous plane's H.
+  // sE rows: 0 = j0-1, 1..TYr own, TYr+1 = j0+TYr; sH rows: 0 = j0-1, 1..TYr
+  // own (PMC-masked)
   extern __shared__ unsigned char s_dyn[];
-  S *sMat = reinterpret_cast<S *>(s_dyn);
+  S *sEp, *sHp, *sMat;
+  if constexpr (FR) {
+    sEp = reinterpret_cast<S *>(s_dyn);
+    sHp = sEp + 3 * (TYr + 2) * RW;
+    sMat = sHp + 3 * (TYr + 1) * RW;
+  } else {
+    __shared__ S sE_[3 * (TY + 2) * (64 * V + 2 * V)];
+    __shared__ S sH_[3 * (TY + 1) * (64 * V + 2 * V)];
+    sEp = sE_;
+    sHp = sH_;
+    sMat = reinterpret_cast<S *>(s_dyn);
+  }
+  const auto sE = [&](int c, int r) { return sEp + (c * (TYr + 2) + r) * RW; };
+  const auto sH = [&](int c, int r) { return sHp + (c * (TYr + 1) + r) * RW; };
   if (LUT) {
-    for (int t = threadIdx.x; t < cf.nmat * NM; t += 64 * (TY + 1)) sMat[t] = cf.mat[t];
+    const int nthr = FR ? (int)blockDim.x : 64 * (TY + 1);
+    for (int t = threadIdx.x; t < cf.nmat * NM; t += nthr) sMat[t] = cf.mat[t];
   }
 
   const int tz = tile / (x.ntx * x.nty);
   const int rem = tile - tz * x.ntx * x.nty;
   const int tyb = rem / x.ntx, txb = rem - tyb * x.ntx;
-  const int i0 = txb * W, j0 = tyb * TY, k0 = tz * x.KZ;
+  const int i0 = txb * W, j0 = tyb * TYr, k0 = tz * x.KZ;
   const int kend = min(k0 + x.KZ, g.Nz);
 
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = FR ? (int)threadIdx.x % Q : (int)(threadIdx.x & 63);
+  const int w = FR ? (int)threadIdx.x / Q : (int)(threadIdx.x >> 6);
+  const int LL = Q - 1;                          // last lane of a row segment
+  const bool wok = !FR || w <= TYr;              // FR: threads past the tile rows idle
   const int j = (w == 0) ? j0 - 1 : j0 + w - 1;  // row of this wave
-  const bool rowok = (j >= 0) && (j < g.Ny);
+  const bool rowok = wok && (j >= 0) && (j < g.Ny);
   const int xi = V * lane;
   const int ic = i0 + xi;
   const bool lact = rowok && (ic < g.Nx);          // Nx % V == 0: all V cells or none
@@ -127,12 +156,12 @@ __device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S>
   const I N = (I)g.N, sz = (I)g.sz;
   const I col = lact ? (I)ic + (I)g.sy * (I)j : (I)0;
   const int pz = MK ? g.pecz : 0, hz = MK ? g.pmcz : 0;
-  const int jt = j0 + TY;
+  const int jt = j0 + TYr;
   const bool top = (w == 0) && (jt < g.Ny) && (ic < g.Nx);
   const I tcol = top ? (I)ic + (I)g.sy * (I)jt : (I)0;
   const bool hasL = rowok && (lane == 0) && (i0 > 0);
-  const bool hasR = rowok && (lane == 63) && (i0 + W < g.Nx);
-  const bool topR = (w == 0) && (lane == 63) && (jt < g.Ny) && (i0 + W < g.Nx);
+  const bool hasR = rowok && (lane == LL) && (i0 + W < g.Nx);
+  const bool topR = (w == 0) && (lane == LL) && (jt < g.Ny) && (i0 + W < g.Nx);
   const bool topL = (w == 0) && (lane == 0) && (jt < g.Ny) && (i0 > 0);
 
   const bool mXj = rowok && ypart(pz, j, g.Ny);
@@ -287,35 +316,37 @@ __device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S>
       mEy[q] = lact && zsel(zk, mYi[q]);
       mEy1[q] = lact && kp1 && zsel(zk1, mYi[q]);
     }
+    if (wok) {
 #pragma unroll
-    for (int q = 0; q < V; ++q) {
-      sE[0][w][OFF + xi + q] = mEx ? Ek[0][q] : S(0);
-      sE[1][w][OFF + xi + q] = mEy[q] ? Ek[1][q] : S(0);
-      sE[2][w][OFF + xi + q] = mZi[q] ? Ek[2][q] : S(0);
-    }
-    if (lane == 0) {
-      sE[0][w][OFF - 1] = (hasL && mEx) ? hxL[0] : S(0);
-      sE[1][w][OFF - 1] = (hasL && zsel(zk, mYL)) ? hxL[1] : S(0);
-      sE[2][w][OFF - 1] = mZL ? hxL[2] : S(0);
-    }
-    if (lane == 63) {
-      sE[0][w][OFF + W] = (hasR && mEx) ? hxR[0] : S(0);
-      sE[1][w][OFF + W] = (hasR && zsel(zk, mYR)) ? hxR[1] : S(0);
-      sE[2][w][OFF + W] = mZR ? hxR[2] : S(0);
+      for (int q = 0; q < V; ++q) {
+        sE(0, w)[OFF + xi + q] = mEx ? Ek[0][q] : S(0);
+        sE(1, w)[OFF + xi + q] = mEy[q] ? Ek[1][q] : S(0);
+        sE(2, w)[OFF + xi + q] = mZi[q] ? Ek[2][q] : S(0);
+      }
+      if (lane == 0) {
+        sE(0, w)[OFF - 1] = (hasL && mEx) ? hxL[0] : S(0);
+        sE(1, w)[OFF - 1] = (hasL && zsel(zk, mYL)) ? hxL[1] : S(0);
+        sE(2, w)[OFF - 1] = mZL ? hxL[2] : S(0);
+      }
+      if (lane == LL) {
+        sE(0, w)[OFF + W] = (hasR && mEx) ? hxR[0] : S(0);
+        sE(1, w)[OFF + W] = (hasR && zsel(zk, mYR)) ? hxR[1] : S(0);
+        sE(2, w)[OFF + W] = mZR ? hxR[2] : S(0);
+      }
     }
     if (w == 0) {
       const bool mEt = (jt < g.Ny) && zsel(zk, mXt);
 #pragma unroll
       for (int q = 0; q < V; ++q) {
-        sE[0][TY + 1][OFF + xi + q] = (top && mEt) ? Tp[0][q] : S(0);
-        sE[1][TY + 1][OFF + xi + q] = (top && zsel(zk, mYt[q])) ? Tp[1][q] : S(0);
-        sE[2][TY + 1][OFF + xi + q] = mZt[q] ? Tp[2][q] : S(0);
+        sE(0, TYr + 1)[OFF + xi + q] = (top && mEt) ? Tp[0][q] : S(0);
+        sE(1, TYr + 1)[OFF + xi + q] = (top && zsel(zk, mYt[q])) ? Tp[1][q] : S(0);
+        sE(2, TYr + 1)[OFF + xi + q] = mZt[q] ? Tp[2][q] : S(0);
       }
-      if (lane == 0) sE[0][TY + 1][OFF - 1] = (topL && mEt) ? htL : S(0);
-      if (lane == 63) {
-        sE[0][TY + 1][OFF + W] = (topR && mEt) ? htR[0] : S(0);
-        sE[1][TY + 1][OFF + W] = (topR && zsel(zk, mYR)) ? htR[1] : S(0);
-        sE[2][TY + 1][OFF + W] = mZtR ? htR[2] : S(0);
+      if (lane == 0) sE(0, TYr + 1)[OFF - 1] = (topL && mEt) ? htL : S(0);
+      if (lane == LL) {
+        sE(0, TYr + 1)[OFF + W] = (topR && mEt) ? htR[0] : S(0);
+        sE(1, TYr + 1)[OFF + W] = (topR && zsel(zk, mYR)) ? htR[1] : S(0);
+        sE(2, TYr + 1)[OFF + W] = mZtR ? htR[2] : S(0);
       }
     }
     const S Lz = m.Lz[k];
@@ -356,17 +387,17 @@ __device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S>
       S EX[V + 1], EY[V + 1], EZ[V + 1];
 #pragma unroll
       for (int q = 0; q < V + 1; ++q) {
-        EX[q] = sE[0][w][OFF + xi + q]; EY[q] = sE[1][w][OFF + xi + q]; EZ[q] = sE[2][w][OFF + xi + q];
+        EX[q] = sE(0, w)[OFF + xi + q]; EY[q] = sE(1, w)[OFF + xi + q]; EZ[q] = sE(2, w)[OFF + xi + q];
       }
 #pragma unroll
       for (int q = 0; q < V; ++q) {
         const S ex1 = mEx1 ? Ek1[0][q] : S(0), ey1 = mEy1[q] ? Ek1[1][q] : S(0);
         Hn[0][q] = h_row<S, S, DESC>(Ho[0][q], ao[0][q], dt, Ly, Lz, EY[q], ey1, EZ[q],
-                                     sE[2][w + 1][OFF + xi + q], +1, -1);
+                                     sE(2, w + 1)[OFF + xi + q], +1, -1);
         Hn[1][q] = h_row<S, S, DESC>(Ho[1][q], ao[1][q], dt, Lx[q], Lz, EX[q], ex1, EZ[q],
                                      EZ[q + 1], -1, +1);
         Hn[2][q] = h_row<S, S, DESC>(Ho[2][q], ao[2][q], dt, Lx[q], Ly, EX[q],
-                                     sE[0][w + 1][OFF + xi + q], EY[q], EY[q + 1], +1, -1);
+                                     sE(0, w + 1)[OFF + xi + q], EY[q], EY[q + 1], +1, -1);
       }
     }
     {
@@ -384,11 +415,13 @@ __device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S>
           if (!mz(hz, ic + q, j, g)) hm[2][q] = S(0);
         }
       }
+      if (wok) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
+        for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int q = 0; q < V; ++q) sH[c][w][OFF + xi + q] = hm[c][q];
-      if (lane == 0) {
+          for (int q = 0; q < V; ++q) sH(c, w)[OFF + xi + q] = hm[c][q];
+      }
+      if (wok && lane == 0) {
         S hy = 0, hzv = 0;
         if (hasL && w >= 1 && body) {
           S lay, laz;
@@ -404,20 +437,20 @@ __device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S>
             if (!zsel(zh, xpart(hz, i0 - 1, g.Nx))) lay = S(0);
             if (!mz(hz, i0 - 1, j, g)) laz = S(0);
           }
-          const S exl = sE[0][w][OFF - 1], eyl = sE[1][w][OFF - 1], ezl = sE[2][w][OFF - 1];
+          const S exl = sE(0, w)[OFF - 1], eyl = sE(1, w)[OFF - 1], ezl = sE(2, w)[OFF - 1];
           const S ex1 = mEx1 ? LE1 : S(0);
-          hy = h_row<S, S, DESC>(LH0, lay, dt, LxL, Lz, exl, ex1, ezl, sE[2][w][OFF], -1, +1);
-          hzv = h_row<S, S, DESC>(LH1, laz, dt, LxL, Ly, exl, sE[0][w + 1][OFF - 1], eyl,
-                                  sE[1][w][OFF], +1, -1);
+          hy = h_row<S, S, DESC>(LH0, lay, dt, LxL, Lz, exl, ex1, ezl, sE(2, w)[OFF], -1, +1);
+          hzv = h_row<S, S, DESC>(LH1, laz, dt, LxL, Ly, exl, sE(0, w + 1)[OFF - 1], eyl,
+                                  sE(1, w)[OFF], +1, -1);
           if (hz) {
             const int zh = zface(hz, k, g.Nz);
             if (!zsel(zh, xpart(hz, i0 - 1, g.Nx))) hy = S(0);
             if (!mz(hz, i0 - 1, j, g)) hzv = S(0);
           }
         }
-        sH[0][w][OFF - 1] = S(0);
-        sH[1][w][OFF - 1] = hy;
-        sH[2][w][OFF - 1] = hzv;
+        sH(0, w)[OFF - 1] = S(0);
+        sH(1, w)[OFF - 1] = hy;
+        sH(2, w)[OFF - 1] = hzv;
       }
     }
     __syncthreads();
@@ -428,17 +461,17 @@ __device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S>
       S HX[V + 1], HY[V + 1], HZ[V + 1];  // [0] = -x neighbour
 #pragma unroll
       for (int q = 0; q < V + 1; ++q) {
-        HX[q] = sH[0][w][OFF + xi + q - 1];
-        HY[q] = sH[1][w][OFF + xi + q - 1];
-        HZ[q] = sH[2][w][OFF + xi + q - 1];
+        HX[q] = sH(0, w)[OFF + xi + q - 1];
+        HY[q] = sH(1, w)[OFF + xi + q - 1];
+        HZ[q] = sH(2, w)[OFF + xi + q - 1];
       }
       S En[3][V];
 #pragma unroll
       for (int q = 0; q < V; ++q) {
-        S d0 = e_sum<S>(bo[0][q], tLy, tLz, Hpy[q], HY[q + 1], sH[2][w - 1][OFF + xi + q],
+        S d0 = e_sum<S>(bo[0][q], tLy, tLz, Hpy[q], HY[q + 1], sH(2, w - 1)[OFF + xi + q],
                         HZ[q + 1], +1, -1);
         S d1 = e_sum<S>(bo[1][q], tLx[q], tLz, Hpx[q], HX[q + 1], HZ[q], HZ[q + 1], -1, +1);
-        S d2 = e_sum<S>(bo[2][q], tLx[q], tLy, sH[0][w - 1][OFF + xi + q], HX[q + 1], HY[q],
+        S d2 = e_sum<S>(bo[2][q], tLx[q], tLy, sH(0, w - 1)[OFF + xi + q], HX[q + 1], HY[q],
                         HY[q + 1], +1, -1);
         const bool online = (q == qbeam);
         S jline = 0;
@@ -478,24 +511,25 @@ __device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S>
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int q = 0; q < V; ++q) { Ek[c][q] = Ek1[c][q]; Ek1[c][q] = En2[c][q]; }
-    if (w >= 1 && !body) {  // warm-up plane: H_new(k0-1) seeds the z-1 terms
+    if (w >= 1 && wok && !body) {  // warm-up plane: H_new(k0-1) seeds the z-1 terms
 #pragma unroll
       for (int q = 0; q < V; ++q) {
-        Hpx[q] = sH[0][w][OFF + xi + q];
-        Hpy[q] = sH[1][w][OFF + xi + q];
+        Hpx[q] = sH(0, w)[OFF + xi + q];
+        Hpy[q] = sH(1, w)[OFF + xi + q];
       }
     }
   }
 }
 
-template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY, bool MK, int V>
+// FR: TY only bounds the workgroup (64 * (TY + 1) threads); x.rows is the tile
+template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY, bool MK, int V, bool FR>
 __global__ void __launch_bounds__(64 * (TY + 1))
 k_fused4(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restrict__ Hin,
          S *__restrict__ Eout, S *__restrict__ Hout, S *__restrict__ J, Coefs<S> cf, S dt,
          FusedAux x, const S *__restrict__ beam_row, S *__restrict__ cap) {
   const int tile = xcd_tile(blockIdx.x, x.ntx * x.nty * x.ntz);
-  fused4_tile<S, JM, COND, LUT, DESC, TY, MK, V>(g, m, Ein, Hin, Eout, Hout, J, cf, dt, x,
-                                                 beam_row, cap, tile);
+  fused4_tile<S, JM, COND, LUT, DESC, TY, MK, V, FR>(g, m, Ein, Hin, Eout, Hout, J, cf, dt, x,
+                                                     beam_row, cap, tile);
 }
 
 // One grid of a batch: everything a step of it needs except the step number.
@@ -542,7 +576,7 @@ k_fused4_batch(const BatchDesc<S> *__restrict__ d, int ninst, int ntiles,
   x.cap_sample = b.cap_sample0 + step;
   x.skip_j = (step + 1 < b.nsteps) ? 1 : 0;
   const S *row = b.beam_table ? b.beam_table + (b.beam_row0 + step) * b.g.Nz : nullptr;
-  fused4_tile<S, JM, COND, LUT, DESC, TY, MK, V>(
+  fused4_tile<S, JM, COND, LUT, DESC, TY, MK, V, false>(
       b.g, b.m, odd ? b.E2 : b.E, odd ? b.H2 : b.H, odd ? b.E : b.E2, odd ? b.H : b.H2, b.J,
       b.cf, b.dt, x, row, b.cap, tile - b.tile0);
 }

@@ -164,6 +164,9 @@ class SolverFIT3D(RoutinesMixin):
         self.fused_kz = 0  # z-planes per workgroup of the fused kernel (0: automatic)
         self.fused_variant = 0  # 0 auto, 1 scalar, 2/3/7/8: vectorised, rows per workgroup
         self.fused_cells_per_lane = 0  # x-vectorised kernel: 0 auto, 1, 2 (FP32), 16/itemsize
+        # whole-row workgroups: 0 auto, -1 never, N >= 1: N rows (tune_fused() measures)
+        self.fused_full_rows = 0
+        self.fused_tuning = None  # {full_rows: ms per step} of the last tune_fused()
         # energy(): keep E^n and H^(n-1/2) on the exact path too (one copy per step)
         self.keep_previous_fields = False
         self._prev_key = None
@@ -650,6 +653,7 @@ class SolverFIT3D(RoutinesMixin):
         args.kz = int(self.fused_kz)
         args.variant = int(self.fused_variant)
         args.cells_per_lane = int(self.fused_cells_per_lane)
+        args.full_rows = int(self.fused_full_rows)
         args.zero_masked = 1 if (self.skip_masks and self._masked_zero()) else 0
         if self.use_material_code and self._material_table():
             args.code, args.mat, args.metric2d = (_lib.ptr(self._code), _lib.ptr(self._mat),
@@ -709,6 +713,71 @@ class SolverFIT3D(RoutinesMixin):
         _lib.check(_lib.lib().wf_step(self._handle, ctypes.byref(args), _lib.stream_ptr()),
                    "wf_step")
         self._finish_run(fused, bool(args.result_swapped), keep)
+
+    def tune_fused(self, make_args=None, steps=3):
+        """Choose the fused kernel's workgroup shape by timing it (no reference
+        counterpart). Candidates: the 64-lane row segments, and whole-row
+        workgroups with the most rows for each wave count that fits. Each runs
+        one untimed and `steps` timed steps of ``make_args(k)`` (default: plain
+        steps); E, H and J are restored afterwards, and every shape gives
+        bitwise the same fields. Sets ``fused_full_rows`` to the fastest and
+        returns it (None when the fused path would not run or the choice was
+        fixed by the caller)."""
+        if (self.fused_full_rows != 0 or self.fused_variant != 0 or self.fused_cells_per_lane
+                or self.path == "general"):
+            return None
+        make = make_args if make_args is not None else self._step_args
+        if not self._choose_fused(make(steps)):
+            return None
+        size = np.dtype(self.dtype).itemsize
+        vn = 16 // size
+        if self.Nx % vn:
+            return None
+        q = self.Nx // vn
+        table = (self._nmat * 9 * size) if self.use_material_code else 0
+        by_waves = {}
+        for T in range(1, self.Ny + 1):
+            thr = -(-(T + 1) * q // 64) * 64
+            if thr > 512 or (6 * T + 9) * (self.Nx + 2 * vn) * size + table > 65536:
+                break
+            by_waves[thr // 64] = T
+        if not by_waves:
+            return None
+        cands = [-1] + sorted(by_waves.values())
+        saved = [F.array.clone() for F in (self.E, self.H, self.J)]
+        held = self.use_conductivity and self._sigmaE_holds()
+        jclean, line = self.J.version == self._J_clean_version, self._beam_line
+        stream = torch.cuda.current_stream(self.device)
+        times = {}
+        try:
+            for c in cands:
+                self.fused_full_rows = c
+                try:
+                    self._run(make(1))
+                except ValueError:
+                    continue
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                self._run(make(steps))
+                e1.record(stream)
+                e1.synchronize()
+                times[c] = e0.elapsed_time(e1) / steps
+        finally:
+            for F, X in zip((self.E, self.H, self.J), saved):
+                F.array.copy_(X)
+                F.version += 1
+            self._mz_key = self._wrap_key = self._prev_key = None
+            if jclean:  # J is again zero off the beam line
+                self._J_clean_version = self.J.version
+            self._beam_line = line
+            if self.use_conductivity:
+                self._sigmaE_versions = ((self.E.version, self.J.version) if held
+                                         else (-1, -1))
+        best = min(times, key=times.get) if times else 0
+        self.fused_full_rows = best
+        self.fused_tuning = times
+        return best
 
     def energy(self, parts=False):
         """Discrete FIT energy of the last step, in joules (no reference
