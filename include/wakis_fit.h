@@ -128,15 +128,18 @@ typedef struct wf_step_args {
                              leave lanes idle), 1, 2 (FP32) or 16/sizeof(dtype); 1 and
                              2 are built with the automatic row count only. Results
                              are identical for every choice                        */
-  int32_t full_rows;      /* fused x-vectorised kernel, 16-byte lanes: workgroups of
-                             whole rows (thread t: cells V*(t % (Nx/V)) of tile row
-                             t / (Nx/V)) instead of 64-lane row segments. 0 = when
-                             the segments would leave clearly more lanes idle (e.g.
-                             Nx = 200), -1 = never, N >= 1 = always, with N own
-                             rows per workgroup.
+  int32_t full_rows;      /* fused x-vectorised kernel, 16-byte lanes: N >= 1 =
+                             workgroups of N whole rows (thread t: cells
+                             V*(t % (Nx/V)) of tile row t / (Nx/V)) instead of
+                             64-lane row segments (0 or negative). Pays when Nx is
+                             not close to a multiple of the segment (Nx = 200:
+                             1.25-1.4x); which is faster is measured, not derived.
                              Needs Nx a multiple of 16/sizeof(dtype), at most 512
-                             threads and 64 KiB of LDS per workgroup. Results are
-                             identical for every choice                            */
+                             threads and 64 KiB of LDS per workgroup. In a batch
+                             (wf_batch_create), N >= 1 on every item = whole-row
+                             tiles with the rows per grid that fill the batch's
+                             256-thread workgroup. Results are identical for every
+                             choice                                                */
 } wf_step_args;
 
 int wf_version(void);

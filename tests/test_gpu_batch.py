@@ -21,17 +21,20 @@ def _pairs(dtype, n=5, seed=3):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("shape", [(1, 16), (0, 32), None])
+@pytest.mark.parametrize("shape", [(1, 16), (0, 32), (0, 16, "rows"), (0, 32, "rows"), None])
 @pytest.mark.parametrize("use_graph", [True, False])
-def test_batch_matches_single_grid_runs(dtype, shape, use_graph):
+@pytest.mark.parametrize("codes", [True, False])
+def test_batch_matches_single_grid_runs(dtype, shape, use_graph, codes):
     from wakis_amd import batch
-    print("batch", np.dtype(dtype).name, shape, use_graph, flush=True)
+    print("batch", np.dtype(dtype).name, shape, use_graph, codes, flush=True)
     singles, batched = _pairs(dtype)
+    for s in singles + batched:
+        s.use_material_code = codes
     wl = [0.02 + 0.01 * i for i in range(len(singles))]  # uneven step counts
     for s, w in zip(singles, wl):
         s.wakesolve(wakelength=w)
     kw = dict(autotune=True, tune_steps=8) if shape is None else dict(
-        cells_per_lane=shape[0], kz=shape[1])
+        cells_per_lane=shape[0], kz=shape[1], full_rows=len(shape) > 2)
     info = batch.wakesolve_batch(batched, wl, use_graph=use_graph, **kw)
     assert len({s.Nt for s in singles}) > 1
     assert info["steps"] == max(s.Nt for s in singles)

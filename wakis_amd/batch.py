@@ -28,10 +28,12 @@ def _native_width(dtype):
 
 
 class _Batch:
-    def __init__(self, solvers, args, cells_per_lane, kz):
+    def __init__(self, solvers, args, cells_per_lane, kz, full_rows=False):
         n = len(solvers)
         handles = (ctypes.c_void_p * n)(*[s._handle.value for s in solvers])
         arr = (_lib.WfStepArgs * n)(*args)
+        for a in arr:  # whole-row tiles (rows per grid picked by the library), or segments
+            a.full_rows = 1 if full_rows else 0
         self.ptr = ctypes.c_void_p()
         _lib.check(_lib.lib().wf_batch_create(handles, arr, n, int(cells_per_lane), int(kz),
                                               ctypes.byref(self.ptr)), "wf_batch_create")
@@ -50,17 +52,21 @@ class _Batch:
 
 
 def _candidates(solvers):
-    """Tile shapes to try: cells per lane (1, or the 16-byte width when every
-    grid allows it) times z-planes per workgroup."""
-    widths = [1]
+    """Tile shapes to try, (cells per lane, z-planes per workgroup, whole rows):
+    row segments of 1 or (when every grid allows it) 16 bytes per lane, and
+    whole-row workgroups when every row fits one (Nx / width <= 128), each with
+    16 and 32 z-planes per workgroup."""
+    widths = [(1, False)]
     nat = _native_width(solvers[0].dtype)
     if all(s.Nx % nat == 0 for s in solvers):
-        widths.append(nat)
-    return [(w, kz) for w in widths for kz in (16, 32)]
+        widths.append((nat, False))
+        if all(s.Nx // nat <= 128 for s in solvers):
+            widths.append((nat, True))
+    return [(w, kz, fr) for w, fr in widths for kz in (16, 32)]
 
 
 def wakesolve_batch(solvers, wakelength, compute_plane="both", cells_per_lane=None, kz=None,
-                    use_graph=True, autotune=True, tune_steps=32, verbose=0):
+                    use_graph=True, autotune=True, tune_steps=32, verbose=0, full_rows=None):
     """``wakesolve`` of every solver, advanced together (``wakelength``: one
     value or one per solver).
 
@@ -68,7 +74,8 @@ def wakesolve_batch(solvers, wakelength, compute_plane="both", cells_per_lane=No
     path: with material codes and zero-valued masked fields (the usual state
     of a fresh wakefield run), or for every grid without material codes
     (coefficient arrays, masks applied in the kernel); otherwise
-    BatchNotApplicable is raised before anything runs. ``cells_per_lane``/``kz`` fix the tile shape; with
+    BatchNotApplicable is raised before anything runs. ``cells_per_lane``/``kz``/
+    ``full_rows`` (whole-row workgroups, rows per grid picked) fix the tile shape; with
     ``autotune`` the candidates are timed on successive chunks of
     ``tune_steps`` steps of the run itself and the fastest runs the rest.
     Returns a dict with the chosen shape, the step count and the timing.
@@ -91,8 +98,8 @@ def wakesolve_batch(solvers, wakelength, compute_plane="both", cells_per_lane=No
     if len({bool(a.code) for a in args}) != 1:
         raise BatchNotApplicable("all grids of a batch use material codes, or none")
     nmax = max(ctx["Nt"] for ctx in ctxs)
-    if cells_per_lane is not None or kz is not None or not autotune:
-        cands = [(cells_per_lane or 0, kz or 0)]
+    if cells_per_lane is not None or kz is not None or full_rows is not None or not autotune:
+        cands = [(cells_per_lane or 0, kz or 0, bool(full_rows))]
     else:
         cands = _candidates(solvers)
     ev0 = torch.cuda.Event(enable_timing=True)
@@ -102,15 +109,15 @@ def wakesolve_batch(solvers, wakelength, compute_plane="both", cells_per_lane=No
     timings = []
     t0 = time.perf_counter()
     if len(cands) > 1:
-        for cpl, kzc in cands:
+        for cpl, kzc, fr in cands:
             if step + tune_steps > nmax:
                 break
-            b = _Batch(solvers, args, cpl, kzc)
+            b = _Batch(solvers, args, cpl, kzc, fr)
             ev0.record(stream)
             b.run(step, tune_steps, False)
             ev1.record(stream)
             torch.cuda.synchronize()
-            timings.append((ev0.elapsed_time(ev1), cpl, kzc))
+            timings.append((ev0.elapsed_time(ev1), cpl, kzc, fr))
             b.close()
             step += tune_steps
         best = min(timings)[1:] if timings else cands[0]
@@ -132,8 +139,10 @@ def wakesolve_batch(solvers, wakelength, compute_plane="both", cells_per_lane=No
         s.simulationTime = wall
         s._wake_finish(ctx, compute_plane)
     cells = sum(s.N * ctx["Nt"] for s, ctx in zip(solvers, ctxs))
-    info = dict(shape=dict(cells_per_lane=best[0], kz=best[1]), steps=nmax,
-                tuning=[dict(ms=t, cells_per_lane=c, kz=k) for t, c, k in timings],
+    info = dict(shape=dict(cells_per_lane=best[0], kz=best[1], full_rows=bool(best[2])),
+                steps=nmax,
+                tuning=[dict(ms=t, cells_per_lane=c, kz=k, full_rows=bool(f))
+                        for t, c, k, f in timings],
                 ms_main=ms_main, main_steps=main_steps, wall_s=wall, cell_updates=cells)
     if verbose:
         print(f"wakesolve_batch: {len(solvers)} grids, {nmax} steps, shape {best}, "

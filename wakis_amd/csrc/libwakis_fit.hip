@@ -273,34 +273,6 @@ inline size_t full_rows_lds(int Nx, int V, int rows, size_t sz, size_t table) {
 constexpr int FR_BOUND = 7;            // full-row kernels: up to 64 * 8 threads
 constexpr size_t FR_LDS_MAX = 65536;   // and 64 KiB of LDS per workgroup
 
-// Own rows of a full-row tile, within the thread and LDS bounds (0 if no tile
-// fits). The march of a workgroup is latency bound and its two barriers per
-// plane stall all its waves together, so tiles that let a CU hold two or more
-// workgroups come first (measured: 200-cell rows, 4 rows in 4 waves twice per
-// CU run 1.25x faster than 9 rows in 8 waves once per CU); among those, the
-// most own rows resident per CU, then the fewest idle lanes. `per_cu(T)`
-// gives the workgroups per CU of a T-row tile. *eff = own cells per lane.
-template <typename F>
-int full_rows_pick(int Nx, int Ny, int V, size_t sz, size_t table, F per_cu, double *eff) {
-  const int Q = Nx / V;
-  int best = 0, bocc = 0;
-  int64_t bres = -1;
-  double be = 0.0;
-  for (int T = 1; T <= Ny; ++T) {
-    const int thr = ((T + 1) * Q + 63) / 64 * 64;
-    if (thr > 64 * (FR_BOUND + 1) || full_rows_lds(Nx, V, T, sz, table) > FR_LDS_MAX) break;
-    const int occ = per_cu(T, thr, full_rows_lds(Nx, V, T, sz, table));
-    const int occ2 = occ < 2 ? occ : 2;
-    const int64_t res = (int64_t)occ * T;
-    const double e = (double)T * Q / thr;
-    if (occ2 > bocc || (occ2 == bocc && (res > bres || (res == bres && e > be + 1e-12)))) {
-      best = T; bocc = occ2; bres = res; be = e;
-    }
-  }
-  *eff = be;
-  return best;
-}
-
 template <typename S, int TY, int V, bool FR = false>
 int step_fused4(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStream_t st,
                 int fr_rows = 0) {
@@ -402,40 +374,14 @@ int step_fused(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStre
   const bool lut = a->code && a->mat && a->nmat > 0 && a->metric2d;
   const int rdef = sizeof(S) == 8 ? 3 : ((lut && !a->conductive) ? 8 : 7);
   const int rows = a->variant ? a->variant : rdef;
-  // full-row tiles (16-byte lanes): always with full_rows = N >= 1 (N rows per
-  // workgroup), never with -1, and by default when the picked tile has at
-  // least 2 rows and leaves clearly fewer lanes idle than the 64-lane row
-  // segments would
-  if (a->full_rows > 0 || (a->full_rows == 0 && a->variant == 0 && a->cells_per_lane == 0)) {
-    const bool fits = idx32v && g.Nx % vn == 0;
-    if (a->full_rows > 0 && !fits)
+  // whole-row tiles (16-byte lanes) with full_rows = N >= 1 rows per workgroup;
+  // row segments otherwise (which shape is faster is measured by the caller:
+  // no static rule fitted the measurements)
+  if (a->full_rows > 0) {
+    if (!idx32v || g.Nx % vn != 0)
       return fail(WF_EINVAL, "wf_step: full-row tiles need Nx a multiple of %d and every 3N "
                   "array within 4 GiB", vn);
-    if (fits) {
-      double efr = 0.0;
-      const size_t table = lut ? (size_t)a->nmat * 9 * sizeof(S) : 0;
-      const Fused4Fn<S> kfr = pick_fused4<S, FR_BOUND, vn, true>(
-          lut, a->conductive != 0, a->jmode == WF_J_LINE, g.m1desc != 0,
-          !(lut && a->zero_masked));
-      const auto per_cu = [&](int, int thr, size_t lds) {
-        return fused_occupancy(h, (const void *)kfr, thr, lds);
-      };
-      int T = full_rows_pick(g.Nx, g.Ny, vn, sizeof(S), table, per_cu, &efr);
-      bool use = a->full_rows > 0;
-      if (a->full_rows >= 1) T = a->full_rows;
-      if (!use && T >= 2) {
-        const int Va = auto_cells_per_lane(g.Nx, vn);
-        const int64_t seg = 64 * (int64_t)Va;
-        const double enr = (double)g.Nx / (double)(((g.Nx + seg - 1) / seg) * seg) *
-                           (double)rdef / (rdef + 1.0);
-        use = efr > enr + 0.02;
-      }
-      if (use) {
-        if (T < 1)
-          return fail(WF_EINVAL, "wf_step: no full-row tile fits rows of %d cells", g.Nx);
-        return step_fused4<S, FR_BOUND, vn, true>(h, a, m, st, T);
-      }
-    }
+    return step_fused4<S, FR_BOUND, vn, true>(h, a, m, st, a->full_rows);
   }
   // cells per lane: vn for every row count; 1 and 2 (FP32) with the default rows
   int V = a->cells_per_lane;
@@ -796,6 +742,7 @@ struct wf_batch {
   int64_t traj_threads = 0;   // batched trajectory gather threads (0: none)
   int rows = 3, vw = 1, cond = 0, jm = 0, desc = 0;
   int lut = 1;                // material codes (1) or coefficient arrays (0)
+  int fr = 0;                 // whole-row tiles (16-byte lanes), rows per grid
   size_t lds = 0;
   void *desc_dev = nullptr;   // BatchDesc<S>[ninst]
   int *step_dev = nullptr;    // step number read by the batch kernel
@@ -843,8 +790,26 @@ int batch_build(wf_handle *const *hs, const wf_step_args *as, int n, int vw, int
     x.cap_ix0 = a->cap_ix0; x.cap_iy0 = a->cap_iy0; x.cap_nx = a->cap_nx; x.cap_ny = a->cap_ny;
     x.cap_samples = a->cap_samples;
     x.cap_sample = 0;
-    x.ntx = (h->g.Nx + W - 1) / W;
-    x.nty = (h->g.Ny + BATCH_ROWS - 1) / BATCH_ROWS;
+    if (b->fr) {
+      // whole rows: as many as fill the batch workgroup (and 64 KiB of LDS)
+      const size_t table = b->lut ? (size_t)a->nmat * 9 * sizeof(S) : 0;
+      x.qrow = h->g.Nx / vw;
+      int T = 64 * (BATCH_ROWS + 1) / x.qrow - 1;
+      if (T > h->g.Ny) T = h->g.Ny;
+      while (T >= 1 && full_rows_lds(h->g.Nx, vw, T, sizeof(S), table) > FR_LDS_MAX) --T;
+      if (T < 1)
+        return fail(WF_EINVAL, "wf_batch_create[%d]: rows of %d cells do not fit a whole-row "
+                    "tile", i, h->g.Nx);
+      x.rows = T;
+      x.ntx = 1;
+      x.nty = (h->g.Ny + T - 1) / T;
+      lds = std::max(lds, full_rows_lds(h->g.Nx, vw, T, sizeof(S), table));
+    } else {
+      x.rows = BATCH_ROWS;
+      x.qrow = 64;
+      x.ntx = (h->g.Nx + W - 1) / W;
+      x.nty = (h->g.Ny + BATCH_ROWS - 1) / BATCH_ROWS;
+    }
     x.KZ = kz > 0 ? kz : (h->g.Nz < 32 ? h->g.Nz : 32);
     x.ntz = (h->g.Nz + x.KZ - 1) / x.KZ;
     e.beam_table = (const S *)a->beam_table;
@@ -862,7 +827,7 @@ int batch_build(wf_handle *const *hs, const wf_step_args *as, int n, int vw, int
       const wf::TrajBox tb = wf::traj_box(a->traj_lo, a->traj_hi);
       b->traj_threads += 3 * (tb.nE + tb.nH);
     }
-    if (b->lut) lds = std::max(lds, (size_t)a->nmat * 9 * sizeof(S));
+    if (b->lut && !b->fr) lds = std::max(lds, (size_t)a->nmat * 9 * sizeof(S));
     b->maxsteps = std::max(b->maxsteps, a->nsteps);
   }
   if (tiles > 2147483647LL) return fail(WF_ERANGE, "wf_batch_create: too many tiles");
@@ -879,16 +844,24 @@ template <typename S>
 int batch_launch(const wf_batch *b, hipStream_t st) {
   const auto *d = (const wf::BatchDesc<S> *)b->desc_dev;
   const dim3 grid((unsigned)b->ntiles), block(64 * (BATCH_ROWS + 1));
-#define WF_B_LAUNCH(JM, CO, DE, VW)                                                          \
+#define WF_B_LAUNCH_FR(JM, CO, DE, VW, FR)                                                   \
   do {                                                                                       \
     if (b->lut)                                                                              \
-      hipLaunchKernelGGL((wf::k_fused4_batch<S, JM, CO, true, DE, BATCH_ROWS, false, VW>),   \
+      hipLaunchKernelGGL((wf::k_fused4_batch<S, JM, CO, true, DE, BATCH_ROWS, false, VW, FR>), \
                          grid, block, b->lds, st, d, b->ninst, b->ntiles,                    \
                          (const int *)b->step_dev);                                          \
     else                                                                                     \
-      hipLaunchKernelGGL((wf::k_fused4_batch<S, JM, CO, false, DE, BATCH_ROWS, true, VW>),   \
-                         grid, block, 0, st, d, b->ninst, b->ntiles,                         \
+      hipLaunchKernelGGL((wf::k_fused4_batch<S, JM, CO, false, DE, BATCH_ROWS, true, VW, FR>), \
+                         grid, block, b->lds, st, d, b->ninst, b->ntiles,                    \
                          (const int *)b->step_dev);                                          \
+  } while (0)
+#define WF_B_LAUNCH(JM, CO, DE, VW)                                                          \
+  do {                                                                                       \
+    if (b->fr) {                                                                             \
+      if constexpr (VW != 1) WF_B_LAUNCH_FR(JM, CO, DE, VW, true);                           \
+    } else {                                                                                 \
+      WF_B_LAUNCH_FR(JM, CO, DE, VW, false);                                                 \
+    }                                                                                        \
   } while (0)
 #define WF_B_LAUNCH2(JM, CO, VW)                                                             \
   do {                                                                                       \
@@ -906,6 +879,7 @@ int batch_launch(const wf_batch *b, hipStream_t st) {
 #undef WF_B_LAUNCH3
 #undef WF_B_LAUNCH2
 #undef WF_B_LAUNCH
+#undef WF_B_LAUNCH_FR
   WF_LAUNCH_CHECK();
   if (b->traj_threads > 0) {
     hipLaunchKernelGGL(wf::k_traj_gather_batch<S>, dim3(nblocks(b->traj_threads, 256)),
@@ -936,10 +910,13 @@ int wf_batch_create(wf_handle *const *handles, const wf_step_args *args, int32_t
   if (!h0) return fail(WF_EINVAL, "wf_batch_create: null handle");
   const int dtype = h0->dtype;
   const int vnat = dtype == WF_F64 ? 2 : 4;
-  const int vw = cells_per_lane == 0 ? vnat : cells_per_lane;
+  const wf_step_args &a0 = args[0];
+  const int fr = a0.full_rows > 0 ? 1 : 0;
+  const int vw = (cells_per_lane == 0 || fr) ? vnat : cells_per_lane;
   if (vw != 1 && vw != vnat)
     return fail(WF_EINVAL, "wf_batch_create: cells_per_lane must be 0, 1 or %d", vnat);
-  const wf_step_args &a0 = args[0];
+  if (fr && cells_per_lane != 0 && cells_per_lane != vnat)
+    return fail(WF_EINVAL, "wf_batch_create: whole-row tiles use %d cells per lane", vnat);
   for (int i = 0; i < n; ++i) {
     const wf_handle *h = handles[i];
     const wf_step_args *a = &args[i];
@@ -960,6 +937,8 @@ int wf_batch_create(wf_handle *const *handles, const wf_step_args *args, int32_t
                   "across the batch", who);
     if (h->g.Nx % vw != 0)
       return fail(WF_EINVAL, "%s: Nx=%d not a multiple of %d cells per lane", who, h->g.Nx, vw);
+    if ((a->full_rows > 0 ? 1 : 0) != fr)
+      return fail(WF_EINVAL, "%s: all batch items use whole-row tiles, or none", who);
     if ((uint64_t)3 * (uint64_t)h->g.N * (dtype == WF_F64 ? 8 : 4) >= (1ull << 32))
       return fail(WF_ERANGE, "%s: grid too large for a batch", who);
   }
@@ -967,6 +946,7 @@ int wf_batch_create(wf_handle *const *handles, const wf_step_args *args, int32_t
   b->dtype = dtype; b->ninst = n; b->vw = vw;
   b->cond = a0.conductive; b->jm = a0.jmode; b->desc = h0->g.m1desc;
   b->lut = (a0.code && a0.mat && a0.nmat > 0 && a0.metric2d) ? 1 : 0;
+  b->fr = fr;
   const int rc = dtype == WF_F64 ? batch_build<double>(handles, args, n, vw, kz, b)
                                  : batch_build<float>(handles, args, n, vw, kz, b);
   if (rc != WF_OK) {

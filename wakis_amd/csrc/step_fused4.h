@@ -557,7 +557,9 @@ struct BatchDesc {
 // One step of every grid of a batch in one launch. The step number comes from
 // device memory (k_batch_next advances it), so a captured graph of these
 // launches replays the right steps.
-template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY, bool MK, int V>
+// FR: whole-row tiles, each grid with its own x.rows / x.qrow within the
+// workgroup of 64 * (TY + 1) threads
+template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY, bool MK, int V, bool FR>
 __global__ void __launch_bounds__(64 * (TY + 1))
 k_fused4_batch(const BatchDesc<S> *__restrict__ d, int ninst, int ntiles,
                const int *__restrict__ step_ptr) {
@@ -576,7 +578,7 @@ k_fused4_batch(const BatchDesc<S> *__restrict__ d, int ninst, int ntiles,
   x.cap_sample = b.cap_sample0 + step;
   x.skip_j = (step + 1 < b.nsteps) ? 1 : 0;
   const S *row = b.beam_table ? b.beam_table + (b.beam_row0 + step) * b.g.Nz : nullptr;
-  fused4_tile<S, JM, COND, LUT, DESC, TY, MK, V, false>(
+  fused4_tile<S, JM, COND, LUT, DESC, TY, MK, V, FR>(
       b.g, b.m, odd ? b.E2 : b.E, odd ? b.H2 : b.H, odd ? b.E : b.E2, odd ? b.H : b.H2, b.J,
       b.cf, b.dt, x, row, b.cap, tile - b.tile0);
 }

@@ -51,7 +51,7 @@ class RoutinesMixin:
         atomically; ``resume=path`` continues such a run in a new solver built
         the same way, and the result is bitwise that of the uninterrupted run.
         The fields are checked for Inf/NaN on the device at the end
-        (``fields_finite``). Runs of 2,000 steps or more on grids of 2^20
+        (``fields_finite``). Runs of 2,000 steps or more on grids of 2^22
         cells or more first time the fused kernel's workgroup shapes
         (``tune_fused``); the fields are bitwise the same for every shape.
         """
@@ -70,7 +70,7 @@ class RoutinesMixin:
                                  "and a checkpoint file name")
         ctx = self._wake_setup(wakelength, compute_plane)
         Nt = ctx["Nt"]
-        if Nt >= 2000 and self.N >= (1 << 20):  # long run: measure the kernel shapes first
+        if Nt >= 2000 and self.N >= (1 << 22):  # long run: measure the kernel shapes first
             self.tune_fused(lambda k: self._wake_args(ctx, 0, k))
         n = 0 if resume is None else self._wake_resume(ctx, resume)
         print("Running electromagnetic time-domain simulation...")

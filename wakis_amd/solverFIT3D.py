@@ -164,7 +164,8 @@ class SolverFIT3D(RoutinesMixin):
         self.fused_kz = 0  # z-planes per workgroup of the fused kernel (0: automatic)
         self.fused_variant = 0  # 0 auto, 1 scalar, 2/3/7/8: vectorised, rows per workgroup
         self.fused_cells_per_lane = 0  # x-vectorised kernel: 0 auto, 1, 2 (FP32), 16/itemsize
-        # whole-row workgroups: 0 auto, -1 never, N >= 1: N rows (tune_fused() measures)
+        # whole-row workgroups: N >= 1 rows; 0 row segments (tune_fused() measures and
+        # sets -1 = segments or N), -1 row segments
         self.fused_full_rows = 0
         self.fused_tuning = None  # {full_rows: ms per step} of the last tune_fused()
         # energy(): keep E^n and H^(n-1/2) on the exact path too (one copy per step)
@@ -747,6 +748,9 @@ class SolverFIT3D(RoutinesMixin):
         saved = [F.array.clone() for F in (self.E, self.H, self.J)]
         held = self.use_conductivity and self._sigmaE_holds()
         jclean, line = self.J.version == self._J_clean_version, self._beam_line
+        key0 = (self.E.version, self.H.version, self.J.version, self._fold_count, line)
+        mz = self._mz_ok if self._mz_key == key0 else None  # checks of the kept state
+        wr = self._wrap_ok if self._wrap_key == key0 else None
         stream = torch.cuda.current_stream(self.device)
         times = {}
         try:
@@ -767,10 +771,14 @@ class SolverFIT3D(RoutinesMixin):
             for F, X in zip((self.E, self.H, self.J), saved):
                 F.array.copy_(X)
                 F.version += 1
-            self._mz_key = self._wrap_key = self._prev_key = None
+            self._prev_key = None
             if jclean:  # J is again zero off the beam line
                 self._J_clean_version = self.J.version
             self._beam_line = line
+            # the restored state is the one checked before: carry the results over
+            key1 = (self.E.version, self.H.version, self.J.version, self._fold_count, line)
+            self._mz_key, self._mz_ok = (key1, mz) if mz is not None else (None, None)
+            self._wrap_key, self._wrap_ok = (key1, wr) if wr is not None else (None, None)
             if self.use_conductivity:
                 self._sigmaE_versions = ((self.E.version, self.J.version) if held
                                          else (-1, -1))
