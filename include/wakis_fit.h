@@ -140,6 +140,14 @@ typedef struct wf_step_args {
                              tiles with the rows per grid that fill the batch's
                              256-thread workgroup. Results are identical for every
                              choice                                                */
+  int32_t box_lo[3], box_hi[3]; /* optional (box_hi > box_lo on every axis): the
+                             x-vectorised fused kernel launches only the tiles that
+                             meet the cells [box_lo, box_hi); E, H and J outside
+                             them are then unspecified afterwards (other paths
+                             update every cell). For recomputing a region whose
+                             values depend only on the box over the steps of the
+                             call (reverse mode: the block plus one cell per step
+                             and margin). Zero: the whole grid                    */
 } wf_step_args;
 
 int wf_version(void);

@@ -224,6 +224,42 @@ def test_trajectory_and_checkpoint_modes_agree_bitwise():
     assert np.array_equal(ea, eb) and np.array_equal(sa, sb)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("segment", [3, 7])
+@pytest.mark.parametrize("full_rows", [0, 2])
+def test_checkpoint_recompute_on_the_dependence_box(dtype, segment, full_rows):
+    """Checkpoint mode recomputes each segment only on the tiles within
+    segment + 3 cells of the block (a strict part of the grid here); the
+    gradients equal the trajectory mode's bitwise."""
+    import wakis_amd as wa
+    from wakis_amd.adjoint import WakeAdjoint
+    print("adjoint recompute box", np.dtype(dtype).name, segment, full_rows, flush=True)
+    dims, region = (24, 40, 64), ((10, 14), (18, 22), (30, 34))
+    x, y, z, mask = cases.cavity_pipe_geometry(*dims)
+    ins = np.zeros(dims, dtype=bool)
+    ins[tuple(slice(a, b) for a, b in region)] = True
+    out = {}
+    for mode in ("trajectory", "checkpoint"):
+        grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"cav": mask, "ins": ins},
+                            stl_materials={"cav": "vacuum", "ins": [2.0, 1.0, 0.5]},
+                            verbose=0)
+        wake = wa.WakeSolver(q=1e-9, sigmaz=10e-3, beta=1.0, skip_cells=2, save=False,
+                             verbose=0)
+        s = wa.SolverFIT3D(grid, wake=wake, bc_low=["pec", "pec", "abc"],
+                           bc_high=["pec", "pec", "abc"], use_stl=True, bg="pec",
+                           dtype=dtype, verbose=0, use_sibc=False, tau_clamp=False)
+        s.fused_full_rows = full_rows
+        adj = WakeAdjoint(s, region, WAKELENGTH, w_loss=1.0, w_band=1e-13, mode=mode,
+                          segment=segment)
+        er, sg = _params_of(region)
+        L = adj.forward(er, sg)
+        assert adj.used_mode == mode
+        out[mode] = (L,) + tuple(g.cpu().numpy() for g in adj.backward())
+    (La, ea, sa), (Lb, eb, sb) = out["trajectory"], out["checkpoint"]
+    assert La == Lb and np.any(ea != 0)
+    assert np.array_equal(ea, eb) and np.array_equal(sa, sb)
+
+
 def test_trajectory_mode_needs_interior_region():
     import wakis_amd as wa
     from wakis_amd.adjoint import WakeAdjoint

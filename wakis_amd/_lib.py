@@ -41,7 +41,7 @@ class WfStepArgs(ctypes.Structure):
         ("traj_E", c_void_p), ("traj_H", c_void_p),
         ("traj_lo", c_int32 * 3), ("traj_hi", c_int32 * 3),
         ("traj_slots", c_int64), ("traj_slot0", c_int64), ("cells_per_lane", c_int32),
-        ("full_rows", c_int32),
+        ("full_rows", c_int32), ("box_lo", c_int32 * 3), ("box_hi", c_int32 * 3),
     ]
 
 

@@ -417,7 +417,8 @@ class WakeAdjoint:
         """Checkpoint mode for a block away from the low faces: each segment is
         recomputed from its checkpoint by one multi-step wf_step call that
         captures only the block's trajectory (as the trajectory mode's forward
-        does), then reversed from that trajectory. The solver's fields are
+        does) and updates only the block's domain of dependence over the
+        segment, then reversed from that trajectory. The solver's fields are
         restored to the end of the run afterwards."""
         s, f = self.s, self._fwd
         ctx, Nt, K = f["ctx"], f["Nt"], f["K"]
@@ -452,6 +453,13 @@ class WakeAdjoint:
             args.traj_lo[:] = tuple(lo)
             args.traj_hi[:] = tuple(hi)
             args.traj_slots, args.traj_slot0 = K + 1, 0
+            # only the block's domain of dependence over the k steps is recomputed:
+            # a cell's value moves at most one cell per step, so the tiles within
+            # k + 3 cells of the block (E box [lo, hi), H box [lo - 1, hi)) give
+            # the block's trajectory exactly
+            m = k + 3
+            args.box_lo[:] = tuple(int(v) - m for v in lo)
+            args.box_hi[:] = tuple(int(v) + m for v in hi)
             s._run(args)
             if adj_fused:
                 rJ[k].copy_(carry)

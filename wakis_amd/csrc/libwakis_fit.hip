@@ -307,6 +307,19 @@ int step_fused4(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStr
   const Fused4Fn<S> kern = pick_fused4<S, TY, V, FR>(lut, a->conductive != 0,
                                                       a->jmode == WF_J_LINE, g.m1desc != 0,
                                                       !(lut && a->zero_masked));
+  // optional box: launch only the tiles that meet it
+  const bool box = a->box_hi[0] > a->box_lo[0] && a->box_hi[1] > a->box_lo[1] &&
+                   a->box_hi[2] > a->box_lo[2];
+  x.tx0 = x.ty0 = x.tz0 = 0;
+  if (box) {
+    const int W = FR ? g.Nx : 64 * V, R = FR ? fr_rows : TY;
+    const int lx = std::max(a->box_lo[0], 0), hx = std::min(a->box_hi[0], g.Nx);
+    const int ly = std::max(a->box_lo[1], 0), hy = std::min(a->box_hi[1], g.Ny);
+    if (hx <= lx || hy <= ly || std::max(a->box_lo[2], 0) >= std::min(a->box_hi[2], g.Nz))
+      return fail(WF_EINVAL, "wf_step: box outside the grid");
+    x.tx0 = lx / W; x.ntx = (hx - 1) / W - x.tx0 + 1;
+    x.ty0 = ly / R; x.nty = (hy - 1) / R - x.ty0 + 1;
+  }
   if (a->kz > 0) {
     x.KZ = a->kz;
   } else {
@@ -314,6 +327,10 @@ int step_fused4(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStr
     x.KZ = auto_kz((int64_t)x.ntx * x.nty, g.Nz, conc);
   }
   x.ntz = (g.Nz + x.KZ - 1) / x.KZ;
+  if (box) {
+    const int lz = std::max(a->box_lo[2], 0), hz = std::min(a->box_hi[2], g.Nz);
+    x.tz0 = lz / x.KZ; x.ntz = (hz - 1) / x.KZ - x.tz0 + 1;
+  }
   const int64_t nb = (int64_t)x.ntx * x.nty * x.ntz;
   if (nb > 2147483647LL) return fail(WF_ERANGE, "wf_step: grid too large for the fused path");
   const S dt = (S)a->dt;
@@ -422,7 +439,7 @@ int step_fused(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStre
   if (a->cells_per_lane && a->variant != 1)
     return fail(WF_EINVAL, "wf_step: cells_per_lane %d needs Nx a multiple of it and every "
                 "3N array within 4 GiB", a->cells_per_lane);
-  wf::FusedAux x;
+  wf::FusedAux x{};
   x.ixs = (a->jmode == WF_J_LINE || a->beam_table || a->conductive) ? a->ixs : -1;
   x.iys = (x.ixs >= 0) ? a->iys : -1;
   x.cap_ix0 = a->cap_ix0; x.cap_iy0 = a->cap_iy0; x.cap_nx = a->cap_nx; x.cap_ny = a->cap_ny;
@@ -790,6 +807,7 @@ int batch_build(wf_handle *const *hs, const wf_step_args *as, int n, int vw, int
     x.cap_ix0 = a->cap_ix0; x.cap_iy0 = a->cap_iy0; x.cap_nx = a->cap_nx; x.cap_ny = a->cap_ny;
     x.cap_samples = a->cap_samples;
     x.cap_sample = 0;
+    x.tx0 = x.ty0 = x.tz0 = 0;
     if (b->fr) {
       // whole rows: as many as fill the batch workgroup (and 64 KiB of LDS)
       const size_t table = b->lut ? (size_t)a->nmat * 9 * sizeof(S) : 0;

@@ -30,6 +30,9 @@ struct FusedAux {
   int skip_j;
   // k_fused4 with full-row tiles: own rows per tile and lanes per row (Nx / V)
   int rows, qrow;
+  // k_fused4: first tile along x, y, z of the launched tile range (ntx, nty,
+  // ntz tiles from there); 0 for the whole grid
+  int tx0, ty0, tz0;
 };
 
 // z-face override of the face mask for components x and y at plane k:

@@ -137,9 +137,10 @@ __device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S>
     for (int t = threadIdx.x; t < cf.nmat * NM; t += nthr) sMat[t] = cf.mat[t];
   }
 
-  const int tz = tile / (x.ntx * x.nty);
-  const int rem = tile - tz * x.ntx * x.nty;
-  const int tyb = rem / x.ntx, txb = rem - tyb * x.ntx;
+  const int tzl = tile / (x.ntx * x.nty);
+  const int rem = tile - tzl * x.ntx * x.nty;
+  const int tyl = rem / x.ntx;
+  const int tz = tzl + x.tz0, tyb = tyl + x.ty0, txb = rem - tyl * x.ntx + x.tx0;
   const int i0 = txb * W, j0 = tyb * TYr, k0 = tz * x.KZ;
   const int kend = min(k0 + x.KZ, g.Nz);
 
