@@ -1,0 +1,157 @@
+"""GPU: the benchmark workloads at their full sizes, through checks that do not
+depend on the grid size (the oracle's CPU run would take hours there).
+
+* 256x256x512 cavity + pipe, beam on axis: the fused step (one launch per step,
+  32-bit offsets, ~10^5 workgroups) equals the exact general path bit for bit,
+  FP32 and FP64; a lossless FP64 run conserves the leapfrog energy.
+* 200x200x400 lossy pillbox with PML and SIBC: whole-row workgroups, row
+  segments and the general path give bitwise the same fields.
+* 128x128x256 cavity with a 32x32x24 block of per-cell eps_r and sigma: the
+  checkpoint mode (segments recomputed on the block's domain of dependence)
+  and the trajectory mode give bitwise the same gradients.
+
+The general path is pinned to the oracle bit for bit at small sizes
+(tests/test_gpu_parity.py); it shares no indexing or tiling with the fused
+kernel.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _cavity(dims, dtype, path):
+    import wakis_amd as wa
+    from wakis_amd.solids import Box, Cylinder
+    solid = Box(-25e-3, 25e-3, -25e-3, 25e-3, -25e-3, 25e-3) | Cylinder(5e-3)
+    grid = wa.GridFIT3D(-30e-3, 30e-3, -30e-3, 30e-3, -80e-3, 80e-3, *dims,
+                        stl_solids={"cavity": solid}, stl_materials={"cavity": "vacuum"},
+                        verbose=0)
+    return wa.SolverFIT3D(grid, bc_low=["pec", "pec", "abc"], bc_high=["pec", "pec", "abc"],
+                          use_stl=True, bg="pec", dtype=dtype, verbose=0, path=path)
+
+
+def _beam_steps(solvers, n, sigmaz=10e-3, beta=1.0):
+    import wakis_amd as wa
+    beams = [wa.Beam(q=1e-9, sigmaz=sigmaz, beta=beta) for _ in solvers]
+    for k in range(n):
+        for s, b in zip(solvers, beams):
+            b.update(s, (k + 0.5 * n) * s.dt)  # bunch head inside the domain
+            s.one_step()
+
+
+def _same_fields(a, b):
+    for name in ("E", "H", "J"):
+        u, v = getattr(a, name).array, getattr(b, name).array
+        assert bool((u == v).all().item()), name
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_config2_fused_equals_general_path(dtype):
+    print("256x256x512 fused vs general", np.dtype(dtype).name, flush=True)
+    dims = (256, 256, 512)
+    f, g = _cavity(dims, dtype, "fused"), _cavity(dims, dtype, "general")
+    _beam_steps([f, g], 6)
+    assert f.last_path == "fused" and g.last_path == "general"
+    assert bool(f.E.array.abs().max().item() > 0)
+    _same_fields(f, g)
+
+
+def test_config2_lossless_energy_conserved():
+    import torch
+    print("256x256x512 energy", flush=True)
+    s = _cavity((256, 256, 512), np.float64, "fused")
+    # a smooth field inside the cavity, zero near the walls and outside
+    ix = torch.arange(s.Nx, device=s.device, dtype=torch.float64)
+    iy = torch.arange(s.Ny, device=s.device, dtype=torch.float64)
+    iz = torch.arange(s.Nz, device=s.device, dtype=torch.float64)
+    bump = (torch.sin(np.pi * (iz[:, None, None] - 200) / 112).clamp(min=0)
+            * torch.sin(np.pi * (iy[None, :, None] - 40) / 176).clamp(min=0)
+            * torch.sin(np.pi * (ix[None, None, :] - 40) / 176).clamp(min=0))
+    bump[:200] = 0
+    bump[312:] = 0
+    E = torch.zeros(3 * s.N, dtype=torch.float64, device=s.device)
+    E[2 * s.N:] = bump.reshape(-1)
+    s.E.fromarray(E)
+    s.one_step()
+    s.one_step()
+    u0 = s.energy()
+    s._run(s._step_args(300))
+    u1 = s.energy()
+    assert s.last_path == "fused" and u0 > 0
+    assert abs(u1 - u0) <= 1e-9 * u0, (u0, u1)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_config3_shapes_and_general_path_agree(dtype):
+    import wakis_amd as wa
+    from wakis_amd.solids import Cylinder
+    print("200x200x400 lossy pillbox shapes", np.dtype(dtype).name, flush=True)
+    Nx, Ny, Nz = 200, 200, 400
+    x = np.linspace(-45e-3, 45e-3, Nx + 1)
+    y = np.linspace(-45e-3, 45e-3, Ny + 1)
+    z = np.linspace(-90e-3, 90e-3, Nz + 1)
+    vac = (Cylinder(40e-3, -30e-3, 30e-3) | Cylinder(10e-3)).cell_mask(x, y, z)
+    grown = vac.copy()
+    for _ in range(3):
+        gr = grown.copy()
+        gr[1:] |= grown[:-1]
+        gr[:-1] |= grown[1:]
+        gr[:, 1:] |= grown[:, :-1]
+        gr[:, :-1] |= grown[:, 1:]
+        gr[:, :, 1:] |= grown[:, :, :-1]
+        gr[:, :, :-1] |= grown[:, :, 1:]
+        grown = gr
+    wall = grown & ~vac
+
+    def solver(path, rows):
+        grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"wall": wall, "vac": vac},
+                            stl_materials={"wall": [1.0, 1.0, 1e4], "vac": "vacuum"},
+                            verbose=0)
+        s = wa.SolverFIT3D(grid, bc_low=["pec", "pec", "pml"], bc_high=["pec", "pec", "pml"],
+                           use_stl=True, bg="pec", dtype=dtype, verbose=0, n_pml=10,
+                           path=path)
+        s.fused_full_rows = rows
+        return s
+
+    runs = [solver("fused", 4), solver("fused", -1), solver("general", 0)]
+    _beam_steps(runs, 5, sigmaz=15e-3, beta=0.8)
+    assert [s.last_path for s in runs] == ["fused", "fused", "general"]
+    assert bool(runs[0].E.array.abs().max().item() > 0)
+    _same_fields(runs[0], runs[1])
+    _same_fields(runs[0], runs[2])
+
+
+def test_config4_checkpoint_and_trajectory_gradients_agree():
+    import wakis_amd as wa
+    from wakis_amd.adjoint import WakeAdjoint
+    from wakis_amd.solids import Box, Cylinder
+    print("128x128x256 adjoint modes", flush=True)
+    dims, insert = (128, 128, 256), (32, 32, 24)
+    x = np.linspace(-30e-3, 30e-3, dims[0] + 1)
+    y = np.linspace(-30e-3, 30e-3, dims[1] + 1)
+    z = np.linspace(-80e-3, 80e-3, dims[2] + 1)
+    cav = (Box(-25e-3, 25e-3, -25e-3, 25e-3, -25e-3, 25e-3) | Cylinder(5e-3)).cell_mask(x, y, z)
+    lo = [(n - m) // 2 for n, m in zip(dims, insert)]
+    region = tuple((a, a + m) for a, m in zip(lo, insert))
+    ins = np.zeros(dims, dtype=bool)
+    ins[tuple(slice(a, b) for a, b in region)] = True
+    rng = np.random.default_rng(0)
+    er, sg = rng.uniform(1.0, 10.0, insert), rng.uniform(0.0, 2.0, insert)
+    out = {}
+    for mode in ("trajectory", "checkpoint"):
+        grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"cav": cav, "ins": ins},
+                            stl_materials={"cav": "vacuum", "ins": [2.0, 1.0, 1.0]}, verbose=0)
+        wake = wa.WakeSolver(q=1e-9, sigmaz=10e-3, beta=1.0, skip_cells=10, save=False,
+                             verbose=0)
+        s = wa.SolverFIT3D(grid, wake=wake, bc_low=["pec", "pec", "abc"],
+                           bc_high=["pec", "pec", "abc"], use_stl=True, bg="pec",
+                           dtype=np.float32, verbose=0, use_sibc=False, tau_clamp=False)
+        adj = WakeAdjoint(s, region, 0.01, w_loss=1.0, w_band=1e-13, mode=mode, segment=16)
+        L = adj.forward(er, sg)
+        assert adj.used_mode == mode
+        out[mode] = (L,) + tuple(g.cpu().numpy() for g in adj.backward())
+    (La, ea, sa), (Lb, eb, sb) = out["trajectory"], out["checkpoint"]
+    assert La == Lb and np.any(ea != 0) and np.all(np.isfinite(ea))
+    assert np.array_equal(ea, eb) and np.array_equal(sa, sb)
