@@ -398,6 +398,12 @@ int step_fused(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStre
     if (!idx32v || g.Nx % vn != 0)
       return fail(WF_EINVAL, "wf_step: full-row tiles need Nx a multiple of %d and every 3N "
                   "array within 4 GiB", vn);
+    if (a->cells_per_lane != 0 && a->cells_per_lane != vn)
+      return fail(WF_EINVAL, "wf_step: full-row tiles use %d cells per lane (cells_per_lane "
+                  "%d)", vn, a->cells_per_lane);
+    if (a->variant == 1)
+      return fail(WF_EINVAL, "wf_step: full-row tiles and the scalar kernel (variant 1) "
+                  "exclude each other");
     return step_fused4<S, FR_BOUND, vn, true>(h, a, m, st, a->full_rows);
   }
   // cells per lane: vn for every row count; 1 and 2 (FP32) with the default rows

@@ -179,6 +179,8 @@ class WakeSolver:
             setattr(self, key, val)
         self.lambdas = (1 / (self.sigmaz * np.sqrt(2 * np.pi))
                         * np.exp(-(self.s ** 2) / (2 * self.sigmaz ** 2)))
+        self._save_txt("lambda.txt", self.s, self.lambdas, "Charge distribution [C/m]",
+                       x_label="s [Hz]")
 
     def calc_lambdas(self, **kwargs):
         """wakeSolver.py:783-836 (interpolated user charge distribution)."""
@@ -193,6 +195,8 @@ class WakeSolver:
             if z is None:
                 raise ValueError("Dimension error: check input dimensions")
         self.lambdas = np.interp(self.s, z, chargedist / self.q)
+        self._save_txt("lambda.txt", self.s, self.lambdas, "Charge distribution [C/m]",
+                       x_label="s [Hz]")
 
     def _dft_bins(self, signals, samples, fmax):
         ds = np.mean(self.s[1:] - self.s[:-1])
@@ -228,7 +232,9 @@ class WakeSolver:
         self.f = f
         self.Z = -WPf / lambdaf
         self.lambdaf = lambdaf
-        self._save_txt("Z.txt", self.f, self.Z, "Z [Ohm]")
+        self._save_txt("Z.txt", self.f, self.Z, "Z [Ohm]", x_label="f [Hz]")
+        self._save_txt("spectrum.txt", self.f, self.lambdaf,
+                       "Charge distribution spectrum [C/s]", x_label="f [Hz]")
 
     def calc_trans_Z(self, samples=1001, fmax=None):
         """wakeSolver.py:721-781."""
@@ -239,8 +245,8 @@ class WakeSolver:
         self.Zx = 1j * (X[1] * ds) / lambdaf
         self.Zy = 1j * (X[2] * ds) / lambdaf
         self.fx = self.fy = f
-        self._save_txt("Zx.txt", self.fx, self.Zx, "Zx [Ohm]")
-        self._save_txt("Zy.txt", self.fy, self.Zy, "Zy [Ohm]")
+        self._save_txt("Zx.txt", self.fx, self.Zx, "Zx [Ohm]", x_label="f [Hz]")
+        self._save_txt("Zy.txt", self.fy, self.Zy, "Zy [Ohm]", x_label="f [Hz]")
 
     def calc_loss_factor(self):
         """Loss factor k = -integral WP(s) lambda(s) ds in V/pC (no reference
@@ -258,7 +264,63 @@ class WakeSolver:
         self.loss_factor = float(out.item())
         return self.loss_factor
 
-    def _save_txt(self, name, a, b, label):
+    def _save_txt(self, name, a, b, label, x_label="s [m]"):
+        """The result files of wakeSolver.py:411-416, 705-718, 771-781, 827-835
+        (same names, columns and two header lines), written when save=True."""
         if self.save:
             np.savetxt(self.folder + name, np.c_[a, b],
-                       header="   s [m]" + " " * 20 + label + "\n" + "-" * 48)
+                       header="   " + x_label + " " * 20 + label + "\n" + "-" * 48)
+
+    def save_txt(self, f_name, x_data=None, y_data=None, x_name="X [-]", y_name="Y [-]"):
+        """Two-column text file with a two-line header (wakeSolver.py:1418-1471)."""
+        if f_name.endswith(".txt"):
+            f_name = f_name[:-4]
+        if x_data is not None and y_data is not None:
+            np.savetxt(f_name + ".txt", np.c_[x_data, y_data],
+                       header="   " + x_name + " " * 20 + y_name + "\n" + "-" * 48)
+        else:
+            self.log("[!] txt not saved, please provide x_data and y_data")
+
+    def read_txt(self, txt, skiprows=2, delimiter=None, usecols=None):
+        """Columns of a text file as a dict keyed by the header names ("s[m]",
+        ...; spaces and '#' dropped, split after each ']'), or by column index
+        when the header does not name them; complex columns are read as
+        complex128 (wakeSolver.py:1362-1416)."""
+        try:
+            load = np.loadtxt(txt, skiprows=skiprows, delimiter=delimiter, usecols=usecols)
+        except ValueError:
+            self.log(f"[!] Using dtype=np.complex128 to read {txt}")
+            load = np.loadtxt(txt, skiprows=skiprows, delimiter=delimiter, usecols=usecols,
+                              dtype=np.complex128)
+        load = np.atleast_2d(load)
+        try:
+            with open(txt) as f:
+                header = f.readline()
+            names = header.replace(" ", "").replace("#", "").replace("\n", "").split("]")
+            d = {names[i] + "]": load[:, i] for i in range(load.shape[1])}
+        except IndexError:
+            self.log("[!] Using integer keys since no header was found")
+            d = {i: load[:, i] for i in range(load.shape[1])}
+        return d
+
+    def load_results(self, folder=None):
+        """Read the result files of a run saved with save=True (wakeSolver.py:1473-1507):
+        lambdas, WPx, WPy, s, WP, lambdaf, Zx, Zy, f, Z."""
+        if folder is None and self.folder is None:
+            raise Exception("[!] Please provide a folder path")
+        elif folder is None:
+            folder = self.folder
+        if not folder.endswith("/"):
+            folder = folder + "/"
+        _, self.lambdas = self.read_txt(folder + "lambda.txt").values()
+        _, self.WPx = self.read_txt(folder + "WPx.txt").values()
+        _, self.WPy = self.read_txt(folder + "WPy.txt").values()
+        self.s, self.WP = self.read_txt(folder + "WP.txt").values()
+        _, self.lambdaf = self.read_txt(folder + "spectrum.txt").values()
+        _, self.Zx = self.read_txt(folder + "Zx.txt").values()
+        _, self.Zy = self.read_txt(folder + "Zy.txt").values()
+        self.f, self.Z = self.read_txt(folder + "Z.txt").values()
+        self.f = np.abs(self.f)
+        self.wakelength = self.s[-1]
+        self.folder = folder
+        self.Ez_file = folder + "Ez.h5"
