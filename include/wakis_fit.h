@@ -148,6 +148,11 @@ typedef struct wf_step_args {
                              values depend only on the box over the steps of the
                              call (reverse mode: the block plus one cell per step
                              and margin). Zero: the whole grid                    */
+  int32_t code_escape;    /* with `code`: 1 = code 255 marks cells whose coefficients
+                             are read from a, b, ieps, sigma (cells with their own
+                             parameters, e.g. a block being optimised) and the table
+                             has at most 255 rows. x-vectorised fused kernel only
+                             (not the scalar variant 1, not batches)               */
 } wf_step_args;
 
 int wf_version(void);

@@ -235,6 +235,63 @@ def test_tune_fused_restores_state_and_keeps_results(conductive, dtype):
         assert np.array_equal(got, ref), f"{name}: rel-L2 {gc.rel_l2(got, ref):.3e}"
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("conductive", [False, True])
+@pytest.mark.parametrize("rows", [0, 3])
+def test_fused_escape_codes_match_oracle(dtype, conductive, rows):
+    """A block of cells with their own eps_r (and sigma): more than 256
+    materials, so those cells get the escape code and read the coefficient
+    arrays while the rest use the table; with a beam line inside the block.
+    Bit-identical to the oracle, row segments and whole rows."""
+    from scipy.constants import epsilon_0
+    print("fused escape", np.dtype(dtype).name, conductive, rows, flush=True)
+    dims = (24, 20, 30)
+    Nx, Ny, Nz = dims
+    N = Nx * Ny * Nz
+    case = gc.random_case(PEC_ALL, dtype, conductive, seed=23 + rows, dims=dims)
+    case["fields"][2] = np.zeros_like(case["fields"][2])
+    so = gc.oracle_solver(case)
+    sa = gc.amd_solver(case, path="fused")
+    rng = np.random.default_rng(5)
+    sl = (slice(8, 14), slice(6, 12), slice(10, 18))
+    ii, jj, kk = np.meshgrid(np.arange(8, 14), np.arange(6, 12), np.arange(10, 18), indexing="ij")
+    flat = ii + Nx * jj + Nx * Ny * kk
+    for c, name in enumerate("xyz"):
+        ie = (1.0 / (rng.uniform(1.0, 5.0, ii.shape) * epsilon_0)).astype(dtype)
+        sa.ieps[sl + (name,)] = ie
+        so.ieps[c * N + flat] = ie
+        if conductive:
+            sg = rng.uniform(0.0, 2.0, ii.shape).astype(dtype)
+            sa.sigma[sl + (name,)] = sg
+            so.sigma[c * N + flat] = sg
+    sa.update_tensors()
+    so.assemble()
+    so.step_0 = False  # update_tensors ends the first-step ghost zeroing (solverFIT3D.py:414)
+    sa.fused_full_rows = rows
+    line = (10, 8)
+    vals = rng.standard_normal(Nz).astype(dtype)
+    if conductive:
+        so.J[:] = 0
+        so.one_step()
+        sa.path = "general"
+        sa.one_step()
+        sa.path = "fused"
+        sa._beam_line = line  # a beam line with no table: its J is read back
+        for _ in range(4):
+            so.one_step()
+        sa._run(sa._step_args(4))
+    else:
+        sa._write_beam_line(line[0], line[1], vals)
+        so.J[2 * N + line[0] + Nx * line[1] + Nx * Ny * np.arange(Nz)] = vals
+        for _ in range(4):
+            so.one_step()
+            sa.one_step()
+    assert sa.last_path == "fused" and sa._code_escape and sa._nmat <= 255
+    for name in ("E", "H", "J"):
+        got, ref = getattr(sa, name).to_numpy(), getattr(so, name)
+        assert np.array_equal(got, ref), f"{name}: rel-L2 {gc.rel_l2(got, ref):.3e}"
+
+
 def _zero_masked_values(case):
     """Zero E on PEC-masked edges (and H on PMC-masked faces) of a random case."""
     dims = tuple(len(a) - 1 for a in (case["x"], case["y"], case["z"]))

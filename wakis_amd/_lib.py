@@ -42,6 +42,7 @@ class WfStepArgs(ctypes.Structure):
         ("traj_lo", c_int32 * 3), ("traj_hi", c_int32 * 3),
         ("traj_slots", c_int64), ("traj_slot0", c_int64), ("cells_per_lane", c_int32),
         ("full_rows", c_int32), ("box_lo", c_int32 * 3), ("box_hi", c_int32 * 3),
+        ("code_escape", c_int32),
     ]
 
 

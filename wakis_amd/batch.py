@@ -51,6 +51,15 @@ class _Batch:
     __del__ = close
 
 
+def _without_escape(a):
+    """Batches take material codes without the escape code: such a grid reads
+    its coefficient arrays instead."""
+    if a.code and a.code_escape:
+        a.code = a.mat = a.metric2d = None
+        a.nmat = 0
+        a.code_escape = 0
+
+
 def _candidates(solvers):
     """Tile shapes to try, (cells per lane, z-planes per workgroup, whole rows):
     row segments of 1 or (when every grid allows it) 16 bytes per lane, and
@@ -92,6 +101,7 @@ def wakesolve_batch(solvers, wakelength, compute_plane="both", cells_per_lane=No
         a = s._wake_args(ctx, 0, ctx["Nt"])
         if not s._prepare_fused(a):
             raise BatchNotApplicable("a grid of the batch does not qualify for the fused path")
+        _without_escape(a)
         if a.code and not a.zero_masked:
             raise BatchNotApplicable("batched grids with material codes need zero masked fields")
         args.append(a)
@@ -212,6 +222,7 @@ def wake_adjoint_batch(adjs, eps_r, sigma=None, cells_per_lane=0, kz=0, use_grap
         a = s._wake_args(ctx, 0, Nt)
         if not s._prepare_fused(a):
             raise BatchNotApplicable("a grid of the batch does not qualify for the fused path")
+        _without_escape(a)
         tE = torch.empty((Nt + 1, 3 * adj._nE), dtype=s.E.array.dtype, device=s.device)
         tH = torch.empty((Nt + 1, 3 * adj._nH), dtype=s.E.array.dtype, device=s.device)
         lo, hi = adj._lohi()

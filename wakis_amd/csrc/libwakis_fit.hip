@@ -217,22 +217,27 @@ using Fused4Fn = void (*)(wf::GridDev, wf::MetricDev<S>, const S *, const S *, S
 // The k_fused4 instantiation for the step's J mode, conductivity, coefficient
 // form, boundary order and masking.
 template <typename S, int TY, int V, bool FR>
-Fused4Fn<S> pick_fused4(bool lut, bool cond, bool jline, bool desc, bool mk) {
-#define WF_PICK(JM, CO, LU)                                                                 \
+Fused4Fn<S> pick_fused4(bool lut, bool cond, bool jline, bool desc, bool mk, bool esc = false) {
+#define WF_PICK(JM, CO, LU, ES)                                                             \
   do {                                                                                      \
-    if (desc) return mk ? &wf::k_fused4<S, JM, CO, LU, true, TY, true, V, FR>               \
-                        : &wf::k_fused4<S, JM, CO, LU, true, TY, !LU, V, FR>;               \
-    return mk ? &wf::k_fused4<S, JM, CO, LU, false, TY, true, V, FR>                        \
-              : &wf::k_fused4<S, JM, CO, LU, false, TY, !LU, V, FR>;                        \
+    if (desc) return mk ? &wf::k_fused4<S, JM, CO, LU, true, TY, true, V, FR, ES>           \
+                        : &wf::k_fused4<S, JM, CO, LU, true, TY, !LU, V, FR, ES>;           \
+    return mk ? &wf::k_fused4<S, JM, CO, LU, false, TY, true, V, FR, ES>                    \
+              : &wf::k_fused4<S, JM, CO, LU, false, TY, !LU, V, FR, ES>;                    \
   } while (0)
-  if (lut) {
-    if (cond) WF_PICK(0, true, true);
-    if (jline) WF_PICK(1, false, true);
-    WF_PICK(0, false, true);
+  if (lut && esc) {
+    if (cond) WF_PICK(0, true, true, true);
+    if (jline) WF_PICK(1, false, true, true);
+    WF_PICK(0, false, true, true);
   }
-  if (cond) WF_PICK(0, true, false);
-  if (jline) WF_PICK(1, false, false);
-  WF_PICK(0, false, false);
+  if (lut) {
+    if (cond) WF_PICK(0, true, true, false);
+    if (jline) WF_PICK(1, false, true, false);
+    WF_PICK(0, false, true, false);
+  }
+  if (cond) WF_PICK(0, true, false, false);
+  if (jline) WF_PICK(1, false, false, false);
+  WF_PICK(0, false, false, false);
 #undef WF_PICK
 }
 
@@ -306,7 +311,8 @@ int step_fused4(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStr
   }
   const Fused4Fn<S> kern = pick_fused4<S, TY, V, FR>(lut, a->conductive != 0,
                                                       a->jmode == WF_J_LINE, g.m1desc != 0,
-                                                      !(lut && a->zero_masked));
+                                                      !(lut && a->zero_masked),
+                                                      lut && a->code_escape);
   // optional box: launch only the tiles that meet it
   const bool box = a->box_hi[0] > a->box_lo[0] && a->box_hi[1] > a->box_lo[1] &&
                    a->box_hi[2] > a->box_lo[2];
@@ -445,6 +451,9 @@ int step_fused(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStre
   if (a->cells_per_lane && a->variant != 1)
     return fail(WF_EINVAL, "wf_step: cells_per_lane %d needs Nx a multiple of it and every "
                 "3N array within 4 GiB", a->cells_per_lane);
+  if (lut && a->code_escape)
+    return fail(WF_EINVAL, "wf_step: the escape code needs the x-vectorised fused kernel "
+                "(variant 0 and every 3N array within 4 GiB)");
   wf::FusedAux x{};
   x.ixs = (a->jmode == WF_J_LINE || a->beam_table || a->conductive) ? a->ixs : -1;
   x.iys = (x.ixs >= 0) ? a->iys : -1;
@@ -570,6 +579,9 @@ int check_step_args(const wf_handle *h, const wf_step_args *a, const char *who) 
     if (a->code && (!a->mat || !a->metric2d || a->nmat < 1 || a->nmat > 256))
       return fail(WF_EINVAL, "%s: material code needs a table of 1..256 materials "
                   "and the 2-D metric tables", who);
+    if (a->code && a->code_escape && (a->nmat > 255 || !a->a || !a->b || !a->ieps || !a->sigma))
+      return fail(WF_EINVAL, "%s: escape code 255 needs a table of at most 255 materials "
+                  "and the coefficient arrays a, b, ieps, sigma", who);
     if (a->compute_f64)
       return fail(WF_EINVAL, "%s: FP64 compute on FP32 storage uses path 0", who);
   } else if (a->path != 0) {
@@ -955,6 +967,8 @@ int wf_batch_create(wf_handle *const *handles, const wf_step_args *args, int32_t
       return fail(WF_EINVAL, "%s: all batch items use material codes, or none", who);
     if (lut && !a->zero_masked)
       return fail(WF_EINVAL, "%s: batch items with material codes need zero_masked = 1", who);
+    if (lut && a->code_escape)
+      return fail(WF_EINVAL, "%s: batch items use material codes without the escape code", who);
     if (h->dtype != dtype || a->conductive != a0.conductive || a->jmode != a0.jmode ||
         h->g.m1desc != h0->g.m1desc)
       return fail(WF_EINVAL, "%s: dtype, conductivity, J mode and boundary order must match "

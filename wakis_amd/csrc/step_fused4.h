@@ -96,7 +96,8 @@ __device__ __forceinline__ int xcd_tile(int blin, int nb) {
 
 // One workgroup's z-march over tile `tile` of the grid (the whole step of that
 // tile); shared by the single-grid and the batched kernels.
-template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY, bool MK, int V, bool FR>
+template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY, bool MK, int V, bool FR,
+          bool ESC>
 __device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S> &m,
                                             const S *__restrict__ Ein, const S *__restrict__ Hin,
                                             S *__restrict__ Eout, S *__restrict__ Hout,
@@ -255,6 +256,23 @@ __device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S>
           itax = cf.itAx[(int64_t)j + (int64_t)g.Ny * k];
           ldv<V>(cf.itAy, (I)ic + (I)g.Nx * (I)k, c1y);
         }
+        if (ESC) {  // code 255: this cell's coefficients come from the arrays
+#pragma unroll
+          for (int q = 0; q < V; ++q) {
+            if (((code >> (8 * q)) & 0xffu) != 0xffu) continue;
+            const I nq = nk + (I)q;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ao[c][q] = *at(cf.a, (I)c * N + nq);
+            if (w >= 1 && body) {
+#pragma unroll
+              for (int c = 0; c < 3; ++c) {
+                bo[c][q] = *at(cf.b, (I)c * N + nq);
+                if (COND || JM == 1) ie[c][q] = *at(cf.ieps, (I)c * N + nq);
+                if (COND) sg[c][q] = *at(cf.sigma, (I)c * N + nq);
+              }
+            }
+          }
+        }
       } else {
         ldv<V>(cf.a, nk, ao[0]); ldv<V>(cf.a, N + nk, ao[1]); ldv<V>(cf.a, 2 * N + nk, ao[2]);
         if (w >= 1 && body) {
@@ -292,6 +310,7 @@ __device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S>
         if (LUT) {
           Lcode = cf.code[n];
           LA0 = cf.iAy[(int64_t)(i0 - 1) + (int64_t)g.Nx * k];
+          if (ESC && Lcode == 0xff) { LA0 = *at(cf.a, N + n); LA1 = *at(cf.a, 2 * N + n); }
         } else {
           LA0 = *at(cf.a, N + n); LA1 = *at(cf.a, 2 * N + n);
         }
@@ -363,6 +382,7 @@ __device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S>
       if (LUT) {
 #pragma unroll
         for (int q = 0; q < V; ++q) {
+          if (ESC && ((code >> (8 * q)) & 0xffu) == 0xffu) continue;  // loaded above
           const S *t = sMat + NM * (int)((code >> (8 * q)) & 0xffu);
           ao[0][q] = (S)(iax * t[0]);
           ao[1][q] = (S)(c0y[q] * t[1]);
@@ -426,7 +446,7 @@ __device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S>
         S hy = 0, hzv = 0;
         if (hasL && w >= 1 && body) {
           S lay, laz;
-          if (LUT) {
+          if (LUT && !(ESC && Lcode == 0xff)) {
             const S *t = sMat + NM * Lcode;
             lay = (S)(LA0 * t[1]);
             laz = (S)(iAzL * t[2]);
@@ -523,14 +543,18 @@ __device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S>
 }
 
 // FR: TY only bounds the workgroup (64 * (TY + 1) threads); x.rows is the tile
-template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY, bool MK, int V, bool FR>
+// ESC (with LUT): material code 255 marks cells whose coefficients are read
+// from the arrays a, b, ieps, sigma (per-cell parameters, e.g. a block of
+// cells being optimised); the table then has at most 255 rows
+template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY, bool MK, int V, bool FR,
+          bool ESC = false>
 __global__ void __launch_bounds__(64 * (TY + 1))
 k_fused4(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restrict__ Hin,
          S *__restrict__ Eout, S *__restrict__ Hout, S *__restrict__ J, Coefs<S> cf, S dt,
          FusedAux x, const S *__restrict__ beam_row, S *__restrict__ cap) {
   const int tile = xcd_tile(blockIdx.x, x.ntx * x.nty * x.ntz);
-  fused4_tile<S, JM, COND, LUT, DESC, TY, MK, V, FR>(g, m, Ein, Hin, Eout, Hout, J, cf, dt, x,
-                                                     beam_row, cap, tile);
+  fused4_tile<S, JM, COND, LUT, DESC, TY, MK, V, FR, ESC>(g, m, Ein, Hin, Eout, Hout, J, cf,
+                                                          dt, x, beam_row, cap, tile);
 }
 
 // One grid of a batch: everything a step of it needs except the step number.
@@ -579,7 +603,7 @@ k_fused4_batch(const BatchDesc<S> *__restrict__ d, int ninst, int ntiles,
   x.cap_sample = b.cap_sample0 + step;
   x.skip_j = (step + 1 < b.nsteps) ? 1 : 0;
   const S *row = b.beam_table ? b.beam_table + (b.beam_row0 + step) * b.g.Nz : nullptr;
-  fused4_tile<S, JM, COND, LUT, DESC, TY, MK, V, FR>(
+  fused4_tile<S, JM, COND, LUT, DESC, TY, MK, V, FR, false>(
       b.g, b.m, odd ? b.E2 : b.E, odd ? b.H2 : b.H, odd ? b.E : b.E2, odd ? b.H : b.H2, b.J,
       b.cf, b.dt, x, row, b.cap, tile - b.tile0);
 }

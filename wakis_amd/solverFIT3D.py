@@ -175,6 +175,11 @@ class SolverFIT3D(RoutinesMixin):
         self.skip_masks = True
         self._mz_key = self._mz_ok = None
         self.use_material_code = True  # fused path: uint8 code + coefficient table
+        # more than 256 materials: code 255 for the rarest cells, read from the
+        # arrays, when they are at most this fraction of the cells
+        self.use_code_escape = True
+        self.code_escape_max = 0.05
+        self._code_escape = False
         self._lut_key = None
         self._code = self._mat = self._metric2d = None
         self._nmat = 0
@@ -463,6 +468,7 @@ class SolverFIT3D(RoutinesMixin):
         self._lut_key = key
         self._code = self._mat = self._metric2d = None
         self._nmat = 0
+        self._code_escape = False
         if (self.ieps.version, self.imu.version) != self._fold_versions:
             return False  # materials edited without update_tensors: use the arrays
         N, (Nx, Ny, Nz), dev = self.N, self.dims, self.device
@@ -470,17 +476,38 @@ class SolverFIT3D(RoutinesMixin):
                 + [self.ieps.array[c * N:(c + 1) * N] for c in range(3)]
                 + [self.sigma.array[c * N:(c + 1) * N] for c in range(3)])
         code = torch.zeros(N, dtype=torch.int64, device=dev)
+        K = 0
         for col in cols:
             u, inv = torch.unique(col, return_inverse=True)
             if u.numel() > 256:
-                return False
+                K = 257
+                break
             code = torch.unique(code * u.numel() + inv, return_inverse=True)[1]
-        K = int(code.max().item()) + 1
-        if K > 256:
-            return False
-        rep = torch.empty(K, dtype=torch.int64, device=dev)
-        rep.scatter_(0, code, torch.arange(N, device=dev))
-        mat = torch.stack([col[rep] for col in cols], dim=1).contiguous()
+        esc = None  # cells whose coefficients come from the arrays (code 255)
+        if K <= 256:
+            K = int(code.max().item()) + 1
+        if K <= 256:
+            rep = torch.empty(K, dtype=torch.int64, device=dev)
+            rep.scatter_(0, code, torch.arange(N, device=dev))
+            mat = torch.stack([col[rep] for col in cols], dim=1).contiguous()
+        else:
+            # more than 256 materials: the 255 most frequent get codes, the rest
+            # (e.g. a block of cells with their own parameters) the escape code,
+            # if they are few enough for the per-cell reads to pay off
+            if (not self.use_code_escape or self.fused_variant == 1
+                    or 3 * N * np.dtype(self.dtype).itemsize >= 2 ** 32):
+                return False
+            u, inv, cnt = torch.unique(torch.stack(cols, dim=1), dim=0, return_inverse=True,
+                                       return_counts=True)
+            order = torch.argsort(cnt, descending=True, stable=True)
+            if int(cnt[order[255:]].sum().item()) > self.code_escape_max * N:
+                return False
+            rank = torch.empty_like(order)
+            rank[order] = torch.arange(order.numel(), device=dev)
+            code = rank[inv].clamp(max=255)
+            esc = code == 255
+            mat = u[order[:255]].contiguous()
+            K = 255
         # 2-D metric tables with the row masks folded in (iA zero on PMC-masked
         # faces, itA zero on PEC-masked edges): a mask of component c depends
         # only on the two transverse indices, like the table
@@ -501,15 +528,18 @@ class SolverFIT3D(RoutinesMixin):
         metric2d = torch.cat(iA + itA).contiguous()
         # exactness check against the folded arrays
         shp = {0: (Nz, Ny, 1), 1: (Nz, 1, Nx), 2: (1, Ny, Nx)}
-        c3 = code.view(Nz, Ny, Nx)
+        c3 = code.clamp(max=K - 1).view(Nz, Ny, Nx)
         for c in range(3):
-            a_c = iA[c].view(shp[c]) * mat[:, c][c3]
-            b_c = mat[:, 3 + c][c3] * itA[c].view(shp[c])
-            if not (torch.equal(a_c.reshape(-1), self._a[c * N:(c + 1) * N])
-                    and torch.equal(b_c.reshape(-1), self._b[c * N:(c + 1) * N])):
+            a_c = (iA[c].view(shp[c]) * mat[:, c][c3]).reshape(-1)
+            b_c = (mat[:, 3 + c][c3] * itA[c].view(shp[c])).reshape(-1)
+            a_ref, b_ref = self._a[c * N:(c + 1) * N], self._b[c * N:(c + 1) * N]
+            if esc is not None:  # escaped cells read the arrays themselves
+                a_c, b_c = torch.where(esc, a_ref, a_c), torch.where(esc, b_ref, b_c)
+            if not (torch.equal(a_c, a_ref) and torch.equal(b_c, b_ref)):
                 return False
         self._code = code.to(torch.uint8).contiguous()
         self._mat, self._nmat, self._metric2d = mat, K, metric2d
+        self._code_escape = esc is not None
         return True
 
     def _masked_zero(self):
@@ -656,10 +686,12 @@ class SolverFIT3D(RoutinesMixin):
         args.cells_per_lane = int(self.fused_cells_per_lane)
         args.full_rows = int(self.fused_full_rows)
         args.zero_masked = 1 if (self.skip_masks and self._masked_zero()) else 0
-        if self.use_material_code and self._material_table():
+        if (self.use_material_code and self._material_table()
+                and not (self._code_escape and self.fused_variant == 1)):
             args.code, args.mat, args.metric2d = (_lib.ptr(self._code), _lib.ptr(self._mat),
                                                   _lib.ptr(self._metric2d))
             args.nmat = self._nmat
+            args.code_escape = int(self._code_escape)
         if args.jmode == _lib.WF_J_FULL and not args.conductive:
             raise RuntimeError("internal: fused path without a J mode")
         if args.conductive:
