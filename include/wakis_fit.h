@@ -316,6 +316,14 @@ typedef struct wf_adjoint_args {
   const void *kappa;      /* fused only, optional: per edge -dt*ieps*sigma (coefficient
                              dtype), zero on the beam line's z edges; the conduction term
                              becomes kappa*W and ieps is read on the region only        */
+  /* fused only, optional: the forward step's material codes (wf_step_args code,
+   * mat, nmat, code_escape and metric2d with the masks folded in). The reverse
+   * kernel then recomputes a, b, kappa (or sigma) and ieps per cell as the
+   * forward folds them, reading one byte per cell; escaped cells read the
+   * arrays. Results are identical to the arrays'. */
+  const void *code, *mat;
+  int32_t nmat, code_escape;
+  const void *code_metric2d;
 } wf_adjoint_args;
 
 /* Reverse of step n of SolverFIT3D._one_step (solverFIT3D.py:482-503): maps

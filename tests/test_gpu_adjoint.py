@@ -260,6 +260,46 @@ def test_checkpoint_recompute_on_the_dependence_box(dtype, segment, full_rows):
     assert np.array_equal(ea, eb) and np.array_equal(sa, sb)
 
 
+def _block_problem(dtype, dims, region):
+    import wakis_amd as wa
+    from wakis_amd.adjoint import WakeAdjoint
+    x, y, z, mask = cases.cavity_pipe_geometry(*dims)
+    ins = np.zeros(dims, dtype=bool)
+    ins[tuple(slice(a, b) for a, b in region)] = True
+    grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"cav": mask, "ins": ins},
+                        stl_materials={"cav": "vacuum", "ins": [2.0, 1.0, 0.5]}, verbose=0)
+    wake = wa.WakeSolver(q=1e-9, sigmaz=10e-3, beta=1.0, skip_cells=2, save=False, verbose=0)
+    s = wa.SolverFIT3D(grid, wake=wake, bc_low=["pec", "pec", "abc"],
+                       bc_high=["pec", "pec", "abc"], use_stl=True, bg="pec",
+                       dtype=dtype, verbose=0, use_sibc=False, tau_clamp=False)
+    return s, WakeAdjoint(s, region, WAKELENGTH, w_loss=1.0, w_band=1e-13)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("use_kappa", [True, False])
+@pytest.mark.parametrize("escape", [False, True])
+def test_reverse_kernel_codes_equal_arrays(dtype, use_kappa, escape):
+    """The reverse kernel recomputing a, b, kappa (or sigma) and ieps from the
+    forward's material codes gives bitwise the gradients of the run that reads
+    the arrays; with more than 256 materials (an 8x8x8 block of its own
+    values) the block's cells carry the escape code and read the arrays."""
+    print("adjoint codes vs arrays", np.dtype(dtype).name, use_kappa, escape, flush=True)
+    dims, region = ((24, 40, 64), ((8, 16), (16, 24), (28, 36))) if escape else (DIMS, REGION)
+    out = {}
+    for codes in (True, False):
+        s, adj = _block_problem(dtype, dims, region)
+        adj.use_codes = codes
+        adj.use_kappa = use_kappa
+        er, sg = _params_of(region, seed=17)
+        L = adj.forward(er, sg)
+        out[codes] = (L,) + tuple(g.cpu().numpy() for g in adj.backward())
+        assert adj.used_kernel == "fused"
+        assert s._code is not None and s._code_escape == escape
+    (La, ea, sa), (Lb, eb, sb) = out[True], out[False]
+    assert La == Lb and np.any(ea != 0)
+    assert np.array_equal(ea, eb) and np.array_equal(sa, sb)
+
+
 def test_trajectory_mode_needs_interior_region():
     import wakis_amd as wa
     from wakis_amd.adjoint import WakeAdjoint

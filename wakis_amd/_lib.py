@@ -59,6 +59,8 @@ class WfAdjointArgs(ctypes.Structure):
         ("fused", c_int32), ("W2", c_void_p), ("lH2", c_void_p), ("fields_f32", c_int32),
         ("reg_W", c_void_p), ("reg_lJ", c_void_p), ("reg_slot", c_int64),
         ("reg_slots", c_int64), ("kappa", c_void_p),
+        ("code", c_void_p), ("mat", c_void_p), ("nmat", c_int32), ("code_escape", c_int32),
+        ("code_metric2d", c_void_p),
     ]
 
 

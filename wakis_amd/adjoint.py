@@ -65,6 +65,8 @@ class WakeAdjoint:
         # fused kernel: read kappa = -dt ieps sigma instead of ieps and sigma (one
         # rounding of the conduction product differs from the two-kernel path)
         self.use_kappa = True
+        # fused kernel: recompute the coefficients from the forward's material codes
+        self.use_codes = True
         self.used_kernel = None
         s = solver
         if s.wake is None:
@@ -119,6 +121,16 @@ class WakeAdjoint:
         free, _ = torch.cuda.mem_get_info(self.s.device)
         return inside and need <= free // 2
 
+    def _dep_box(self, K):
+        """Slices (z, y, x) of the cells within K + 3 cells of the block: what
+        the block's trajectory over K steps depends on (a value moves at most
+        one cell per step; the H box starts one cell lower)."""
+        m = K + 3
+        s = self.s
+        return (slice(max(self.k0 - m, 0), min(self.k1 + m, s.Nz)),
+                slice(max(self.j0 - m, 0), min(self.j1 + m, s.Ny)),
+                slice(max(self.i0 - m, 0), min(self.i1 + m, s.Nx)))
+
     def _lohi(self):
         lo = (ctypes.c_int32 * 3)(self.i0, self.j0, self.k0)
         hi = (ctypes.c_int32 * 3)(self.i1, self.j1, self.k1)
@@ -149,13 +161,22 @@ class WakeAdjoint:
                 s._run(args)
             fwd = dict(ctx=ctx, Nt=Nt, traj=(tE, tH))
         else:
-            K = self.segment or max(1, int(math.ceil(math.sqrt(max(Nt, 1)))))
+            root = int(math.ceil(math.sqrt(max(Nt, 1))))
+            interior = min(self.i0, self.j0, self.k0) >= 1
+            # interior block: each checkpoint keeps only the block's domain of
+            # dependence over one segment, so shorter segments (sqrt(Nt)/4)
+            # shrink both the recompute and the checkpoints
+            K = self.segment or (max(8, root // 4) if interior else root)
+            box = self._dep_box(K) if interior else None
             ckpt = []
             for n0 in range(0, Nt, K):
-                ckpt.append((n0, s.E.array.clone(), s.H.array.clone(), s.J.array.clone()))
+                ckpt.append((n0,) + tuple(
+                    F.array.clone() if box is None
+                    else F.array.view(3, s.Nz, s.Ny, s.Nx)[(slice(None),) + box].clone()
+                    for F in (s.E, s.H, s.J)))
                 k = min(K, Nt - n0)
                 s._run(s._wake_args(ctx, n0, k))
-            fwd = dict(ctx=ctx, ckpt=ckpt, K=K, Nt=Nt)
+            fwd = dict(ctx=ctx, ckpt=ckpt, K=K, Nt=Nt, box=box)
         torch.cuda.synchronize(s.device)
         s._wake_finish(ctx, "both")
         self._fwd = fwd
@@ -292,6 +313,12 @@ class WakeAdjoint:
             a.fused = int(self.fused_rows) or 1
             a.W2, a.lH2 = _lib.ptr(W2), _lib.ptr(lH2)
             a.fields_f32 = int(f32)
+            # the forward's material codes, when it has them: one byte per cell
+            # instead of the coefficient arrays (the same values)
+            if self.use_codes and s.use_material_code and s._material_table():
+                a.code, a.mat, a.code_metric2d = (_lib.ptr(s._code), _lib.ptr(s._mat),
+                                                  _lib.ptr(s._metric2d))
+                a.nmat, a.code_escape = s._nmat, int(s._code_escape)
             if self.use_kappa and self.conductive:
                 kap = (-float(s.dt) * s.ieps.array.to(torch.float64)
                        * s.sigma.array.to(torch.float64))
@@ -441,9 +468,10 @@ class WakeAdjoint:
             carry = torch.zeros(n3r, dtype=torch.float64, device=dev)  # lJ after the segment
             a.reg_W, a.reg_lJ, a.reg_slots = _lib.ptr(rW), _lib.ptr(rJ), K
             a.g_ieps = a.g_sigma = None
+        box = f["box"]
         for (n0, E0, H0, J0) in reversed(f["ckpt"]):
             k = min(K, Nt - n0)
-            s._load_run_state(E0, H0, J0)
+            s._load_run_box(box, E0, H0, J0)
             _lib.check(lib.wf_trajectory_gather(s._handle, _lib.ptr(s.E.array),
                                                 _lib.ptr(s.H.array), pE, pH, lo, hi, K + 1, 0,
                                                 st), "wf_trajectory_gather")
@@ -457,7 +485,7 @@ class WakeAdjoint:
             # a cell's value moves at most one cell per step, so the tiles within
             # k + 3 cells of the block (E box [lo, hi), H box [lo - 1, hi)) give
             # the block's trajectory exactly
-            m = k + 3
+            m = k + 3  # within the restored box (k <= K)
             args.box_lo[:] = tuple(int(v) - m for v in lo)
             args.box_hi[:] = tuple(int(v) + m for v in hi)
             s._run(args)

@@ -63,10 +63,22 @@ struct AdjStep {
   A *rW;
   double *rJ;
   int64_t rslot;
+  // LUT kernels: per-cell material code into mat[nmat][9] (imu, ieps, sigma per
+  // component, in LDS) and the masked 2-D metric tables of the forward step;
+  // with esc, code 255 marks cells whose coefficients come from a, b, ieps and
+  // kappa (or sigma)
+  const uint8_t *code;
+  const S *mat;
+  int nmat, esc;
+  const S *iAx, *iAy, *iAz, *itAx, *itAy, *itAz;
 };
 
-// One workgroup's z-march over tile `tile` of one grid's reverse step.
-template <typename S, typename A, int TY>
+// One workgroup's z-march over tile `tile` of one grid's reverse step. LUT:
+// the coefficients a, b, kappa (or sigma) and ieps are recomputed from the
+// material table exactly as the forward step folds them (a = fl(iA imu),
+// b = fl(ieps itA), kappa = fl((-dt ieps) sigma) in FP64, zero on the beam
+// line's z edges), reading one byte per cell instead of nine values.
+template <typename S, typename A, int TY, bool LUT>
 __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int tile) {
   const GridDev &g = p.g;
   const MetricDev<S> &m = p.m;
@@ -119,6 +131,11 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
   const I col = valid ? (I)i + (I)g.sy * (I)j : 0;
   for (int u = threadIdx.x; u < 3 * (NW + 1) * NS; u += 64 * (NW + 1))
     (&sP[0][0][0])[u] = 0.0;
+  extern __shared__ unsigned char s_dyn[];
+  S *sMat = reinterpret_cast<S *>(s_dyn);
+  if (LUT) {
+    for (int u = threadIdx.x; u < p.nmat * 9; u += 64 * (NW + 1)) sMat[u] = p.mat[u];
+  }
 
   const double tLx = valid ? (double)m.tLx[i] : 0.0, tLy = valid ? (double)m.tLy[j] : 0.0;
   const double Lx = valid ? (double)m.Lx[i] : 0.0, Ly = valid ? (double)m.Ly[j] : 0.0;
@@ -140,20 +157,59 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
   // plane p's offset of this cell, clamped into the grid (masked at use)
   const auto off = [&](int p) { return col + sz * (I)min(p, g.Nz - 1); };
   const int kstart = (k0 > 0) ? k0 - 1 : k0;
+  // LUT: material codes of planes k, k+1 (k+2 in the prefetch) and the
+  // metric those planes need; escaped cells (code 255) load the arrays instead
+  const int ii = valid ? i : 0, jj = valid ? j : 0;
+  const auto kc = [&](int p) { return min(p, g.Nz - 1); };
+  const auto esc_at = [&](int cd) { return LUT && p.esc && cd == 255; };
+  int ck = 0, ck1 = 0;
+  S bxk = 0, bxk1 = 0, byk = 0, byk1 = 0, axk = 0, ayk = 0, bz = 0, az = 0;
+  if (LUT) {
+    ck = p.code[off(kstart)];
+    ck1 = p.code[off(kstart + 1)];
+    bz = p.itAz[ii + g.Nx * jj];
+    az = p.iAz[ii + g.Nx * jj];
+    bxk = p.itAx[jj + g.Ny * kc(kstart)]; bxk1 = p.itAx[jj + g.Ny * kc(kstart + 1)];
+    byk = p.itAy[ii + g.Nx * kc(kstart)]; byk1 = p.itAy[ii + g.Nx * kc(kstart + 1)];
+    axk = p.iAx[jj + g.Ny * kc(kstart)];
+    ayk = p.iAy[ii + g.Nx * kc(kstart)];
+  }
+  const bool arr0 = !LUT || esc_at(ck), arr1 = !LUT || esc_at(ck1);
   A Wk[3], Wk1[3], lh[3];
   S bk[3], bk1[3], ac[3], ie[3], sg[3];
   {
     const I n0 = off(kstart), n1 = off(kstart + 1);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      Wk[c] = *at(Win, (I)c * N + n0); bk[c] = *at(b, (I)c * N + n0);
-      Wk1[c] = *at(Win, (I)c * N + n1); bk1[c] = *at(b, (I)c * N + n1);
+      Wk[c] = *at(Win, (I)c * N + n0);
+      bk[c] = arr0 ? *at(b, (I)c * N + n0) : S(0);
+      Wk1[c] = *at(Win, (I)c * N + n1);
+      bk1[c] = arr1 ? *at(b, (I)c * N + n1) : S(0);
       lh[c] = need_lh ? *at(lHin, (I)c * N + n0) : A(0);
-      ac[c] = need_lh ? *at(a, (I)c * N + n0) : S(0);
-      ie[c] = (need_c && (!kap || in_region(kstart))) ? *at(ieps, (I)c * N + n0) : S(0);
-      sg[c] = need_c ? *at(sgsrc, (I)c * N + n0) : S(0);
+      ac[c] = (need_lh && arr0) ? *at(a, (I)c * N + n0) : S(0);
+      ie[c] = (need_c && arr0 && (!kap || in_region(kstart))) ? *at(ieps, (I)c * N + n0) : S(0);
+      sg[c] = (need_c && arr0) ? *at(sgsrc, (I)c * N + n0) : S(0);
     }
   }
+  // coefficients of a plane from its code (or the loaded arrays when escaped)
+  const bool onl = valid && (i == x.ixs) && (j == x.iys);  // beam line: kappa_z = 0
+  const auto bval = [&](int c, int cd, S bx, S by, S arr) -> S {
+    if (!LUT || esc_at(cd)) return arr;
+    return (S)(sMat[9 * cd + 3 + c] * (c == 0 ? bx : (c == 1 ? by : bz)));
+  };
+  const auto aval = [&](int c, int cd, S ax, S ay, S arr) -> S {
+    if (!LUT || esc_at(cd)) return arr;
+    return (S)((c == 0 ? ax : (c == 1 ? ay : az)) * sMat[9 * cd + c]);
+  };
+  const auto ieval = [&](int c, int cd, S arr) -> S {
+    return (!LUT || esc_at(cd)) ? arr : sMat[9 * cd + 3 + c];
+  };
+  const auto sgval = [&](int c, int cd, S arr) -> S {  // kappa, or sigma
+    if (!LUT || esc_at(cd)) return arr;
+    if (!kap) return sMat[9 * cd + 6 + c];
+    if (c == 2 && onl) return S(0);
+    return (S)((-dt * (double)sMat[9 * cd + 3 + c]) * (double)sMat[9 * cd + 6 + c]);
+  };
   double Qpx = 0, Qpy = 0;  // Q(k-1) of this cell, x and y components
   __syncthreads();          // zeroed LDS
 
@@ -163,14 +219,25 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
     // ---- prefetch: W, b of plane k+2; lH, a, ieps, sigma of plane k+1
     A Wn2[3], lhN[3];
     S bn2[3], acN[3], ieN[3], sgN[3];
+    int cn2 = 0;
+    S bxn2 = 0, byn2 = 0, axN = 0, ayN = 0;
     {
       const I n2 = off(k + 2), n1 = off(k + 1);
+      if (LUT) {
+        cn2 = p.code[n2];
+        bxn2 = p.itAx[jj + g.Ny * kc(k + 2)];
+        byn2 = p.itAy[ii + g.Nx * kc(k + 2)];
+        axN = p.iAx[jj + g.Ny * kc(k + 1)];
+        ayN = p.iAy[ii + g.Nx * kc(k + 1)];
+      }
+      const bool arrn2 = !LUT || esc_at(cn2), arrn1 = !LUT || esc_at(ck1);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        Wn2[c] = *at(Win, (I)c * N + n2); bn2[c] = *at(b, (I)c * N + n2);
+        Wn2[c] = *at(Win, (I)c * N + n2);
+        bn2[c] = arrn2 ? *at(b, (I)c * N + n2) : S(0);
         lhN[c] = need_lh ? *at(lHin, (I)c * N + n1) : A(0);
-        acN[c] = need_lh ? *at(a, (I)c * N + n1) : S(0);
-        if (cond && need_c) {
+        acN[c] = (need_lh && arrn1) ? *at(a, (I)c * N + n1) : S(0);
+        if (cond && need_c && arrn1) {
           ieN[c] = (!kap || in_region(k + 1)) ? *at(ieps, (I)c * N + n1) : S(0);
           sgN[c] = *at(sgsrc, (I)c * N + n1);
         } else {
@@ -181,7 +248,8 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
     // ---- stage P(k)
     double Pk[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) Pk[c] = valid ? (double)bk[c] * (double)Wk[c] : 0.0;
+    for (int c = 0; c < 3; ++c)
+      Pk[c] = valid ? (double)bval(c, ck, bxk, byk, bk[c]) * (double)Wk[c] : 0.0;
     if (inrow) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) sP[c][row][slot] = Pk[c];
@@ -194,7 +262,9 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
     if (makes_q) {
       double Pk1[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) Pk1[c] = (valid && kp1) ? (double)bk1[c] * (double)Wk1[c] : 0.0;
+      for (int c = 0; c < 3; ++c)
+        Pk1[c] = (valid && kp1) ? (double)bval(c, ck1, bxk1, byk1, bk1[c]) * (double)Wk1[c]
+                                : 0.0;
       // h_terms order: x: +Py, -Py(z+1), -Pz, +Pz(y+1); y: -Px, +Px(z+1), +Pz, -Pz(x+1);
       // z: +Px, -Px(y+1), -Py, +Py(x+1)
       double s0 = 0.0;
@@ -209,7 +279,8 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
       hn[1] = keep_row(g.pmcz, 1, i, j, k, g) ? h1 + dt * (tLy * s1) : h1;
       hn[2] = keep_row(g.pmcz, 2, i, j, k, g) ? h2 + dt * (tLz * s2) : h2;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) sQ[c][row][slot] = valid ? (double)ac[c] * hn[c] : 0.0;
+      for (int c = 0; c < 3; ++c)
+        sQ[c][row][slot] = valid ? (double)aval(c, ck, axk, ayk, ac[c]) * hn[c] : 0.0;
     }
     __syncthreads();
 
@@ -241,8 +312,9 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
       if (cond) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          ljv[c] = (c == 2 && online) ? 0.0 : -dt * (double)ie[c] * wv[c];
-          wn[c] = kap ? le[c] + (double)sg[c] * wv[c] : le[c] + (double)sg[c] * ljv[c];
+          const double iec = (double)ieval(c, ck, ie[c]), sgc = (double)sgval(c, ck, sg[c]);
+          ljv[c] = (c == 2 && online) ? 0.0 : -dt * iec * wv[c];
+          wn[c] = kap ? le[c] + sgc * wv[c] : le[c] + sgc * ljv[c];
         }
       } else {
 #pragma unroll
@@ -279,12 +351,17 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
       Wk1[c] = Wn2[c]; bk1[c] = bn2[c];
       lh[c] = lhN[c]; ac[c] = acN[c]; ie[c] = ieN[c]; sg[c] = sgN[c];
     }
+    if (LUT) {
+      ck = ck1; ck1 = cn2;
+      bxk = bxk1; bxk1 = bxn2; byk = byk1; byk1 = byn2;
+      axk = axN; ayk = ayN;
+    }
   }
 }
 
-template <typename S, typename A, int TY>
+template <typename S, typename A, int TY, bool LUT = false>
 __global__ void __launch_bounds__(64 * (TY + 3)) k_adj_fused(AdjStep<S, A> p) {
-  adj_fused_tile<S, A, TY>(p, xcd_tile(blockIdx.x, p.t.ntx * p.t.nty * p.t.ntz));
+  adj_fused_tile<S, A, TY, LUT>(p, xcd_tile(blockIdx.x, p.t.ntx * p.t.nty * p.t.ntz));
 }
 
 // One grid of a reverse batch. Reverse step r of the batch is step Nt-1-r of
@@ -319,7 +396,7 @@ k_adj_fused_batch(const AdjBatchDesc<S, A> *__restrict__ d, int ninst, int ntile
   }
   p.x.cap_sample = n;
   p.rslot = n;
-  adj_fused_tile<S, A, TY>(p, tile - d[lo].tile0);
+  adj_fused_tile<S, A, TY, false>(p, tile - d[lo].tile0);
 }
 
 }  // namespace wf

@@ -326,15 +326,23 @@ int step_fused4(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStr
     x.tx0 = lx / W; x.ntx = (hx - 1) / W - x.tx0 + 1;
     x.ty0 = ly / R; x.nty = (hy - 1) / R - x.ty0 + 1;
   }
+  const int lz = box ? std::max(a->box_lo[2], 0) : 0, hz = box ? std::min(a->box_hi[2], g.Nz) : g.Nz;
   if (a->kz > 0) {
     x.KZ = a->kz;
   } else {
     const int64_t conc = (int64_t)fused_occupancy(h, (const void *)kern, threads, lds) * h->ncu;
-    x.KZ = auto_kz((int64_t)x.ntx * x.nty, g.Nz, conc);
+    if (box) {
+      // a small box has few tiles: short z chunks spread its march over the CUs
+      // (measured: recomputing 16-step segments on a 62-plane box of 11 tile
+      // rows took half the time with 2 planes per workgroup than with 8)
+      const int64_t kz = ((int64_t)(hz - lz) * x.ntx * x.nty + conc - 1) / conc;
+      x.KZ = (int)std::min<int64_t>(64, std::max<int64_t>(2, kz));
+    } else {
+      x.KZ = auto_kz((int64_t)x.ntx * x.nty, g.Nz, conc);
+    }
   }
   x.ntz = (g.Nz + x.KZ - 1) / x.KZ;
   if (box) {
-    const int lz = std::max(a->box_lo[2], 0), hz = std::min(a->box_hi[2], g.Nz);
     x.tz0 = lz / x.KZ; x.ntz = (hz - 1) / x.KZ - x.tz0 + 1;
   }
   const int64_t nb = (int64_t)x.ntx * x.nty * x.ntz;
@@ -1072,7 +1080,23 @@ wf::AdjStep<S, A> adj_step_params(const wf_handle *h, const wf_adjoint_args *a,
   p.ieps = (const S *)a->ieps; p.sigma = (const S *)a->sigma; p.kappa = (const S *)a->kappa;
   p.dt = a->dt; p.x = x; p.gcap = a->gcap; p.t = t;
   p.rW = (A *)a->reg_W; p.rJ = a->reg_lJ; p.rslot = a->reg_slot;
+  p.code = nullptr; p.mat = nullptr; p.nmat = 0; p.esc = 0;
+  p.iAx = p.iAy = p.iAz = p.itAx = p.itAy = p.itAz = nullptr;
+  if (a->code && a->mat && a->nmat > 0 && a->code_metric2d) {
+    const wf::GridDev &g = h->g;
+    const S *t = (const S *)a->code_metric2d;
+    const int64_t yz = (int64_t)g.Ny * g.Nz, xz = (int64_t)g.Nx * g.Nz, xy = (int64_t)g.Nx * g.Ny;
+    p.code = (const uint8_t *)a->code; p.mat = (const S *)a->mat;
+    p.nmat = a->nmat; p.esc = a->code_escape;
+    p.iAx = t; p.iAy = t + yz; p.iAz = t + yz + xz;
+    p.itAx = t + yz + xz + xy; p.itAy = p.itAx + yz; p.itAz = p.itAy + xz;
+  }
   return p;
+}
+
+// static LDS of k_adj_fused (P and Q planes in FP64) for TY own rows
+constexpr size_t adj_static_lds(int TY) {
+  return (size_t)(3 * (TY + 3) + 3 * (TY + 2)) * (wf::ADJ_TX + 3) * sizeof(double);
 }
 
 template <typename S, typename A, int TY>
@@ -1082,15 +1106,22 @@ int adjoint_fused(wf_handle *h, const wf_adjoint_args *a, const wf::MetricDev<S>
   wf::AdjTile t;
   t.ntx = (g.Nx + wf::ADJ_TX - 1) / wf::ADJ_TX;
   t.nty = (g.Ny + TY - 1) / TY;
-  const auto kern = &wf::k_adj_fused<S, A, TY>;
+  // material codes when the table fits beside the P/Q planes in 64 KiB of LDS
+  // (otherwise the coefficient arrays, with identical results)
+  const size_t table = (a->code && a->mat && a->nmat > 0 && a->code_metric2d)
+                           ? (size_t)a->nmat * 9 * sizeof(S) : 0;
+  const bool lut = table > 0 && adj_static_lds(TY) + table <= 65536;
+  const auto kern = lut ? &wf::k_adj_fused<S, A, TY, true> : &wf::k_adj_fused<S, A, TY, false>;
+  const size_t lds = lut ? table : 0;
   const int threads = 64 * (TY + 3);
-  const int64_t conc = (int64_t)fused_occupancy(h, (const void *)kern, threads, 0) * h->ncu;
+  const int64_t conc = (int64_t)fused_occupancy(h, (const void *)kern, threads, lds) * h->ncu;
   t.KZ = auto_kz((int64_t)t.ntx * t.nty, g.Nz, conc);
   t.ntz = (g.Nz + t.KZ - 1) / t.KZ;
   const int64_t nbf = (int64_t)t.ntx * t.nty * t.ntz;
   if (nbf > 2147483647LL) return fail(WF_ERANGE, "wf_adjoint_step: grid too large");
-  const wf::AdjStep<S, A> p = adj_step_params<S, A>(h, a, m, x, t);
-  hipLaunchKernelGGL(kern, dim3((unsigned)nbf), dim3(threads), 0, st, p);
+  wf::AdjStep<S, A> p = adj_step_params<S, A>(h, a, m, x, t);
+  if (!lut) p.code = nullptr;
+  hipLaunchKernelGGL(kern, dim3((unsigned)nbf), dim3(threads), lds, st, p);
   WF_LAUNCH_CHECK();
   return WF_OK;
 }

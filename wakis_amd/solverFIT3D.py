@@ -733,6 +733,28 @@ class SolverFIT3D(RoutinesMixin):
             self._wrap_key, self._wrap_ok = key, wr[1]
         self._prev_key = None
 
+    def _load_run_box(self, box, E, H, J):
+        """Restore E, H, J saved on a box of cells (slices of z, y, x) of a
+        state of this solver's run; the cells outside keep what they hold (a
+        recompute confined to the box's domain of dependence never reads them
+        into the cells it keeps). The run's checked invariants are carried over
+        as by _load_run_state."""
+        mz = (self._mz_key is not None, self._mz_ok)
+        wr = (self._wrap_key is not None, self._wrap_ok)
+        Nx, Ny, Nz = self.dims
+        for F, X in ((self.E, E), (self.H, H), (self.J, J)):
+            F.array.view(3, Nz, Ny, Nx)[(slice(None),) + tuple(box)].copy_(X)
+            F.version += 1
+        if self.use_conductivity:
+            self._sigmaE_versions = (self.E.version, self.J.version)
+        key = (self.E.version, self.H.version, self.J.version, self._fold_count,
+               self._beam_line)
+        if mz[0]:
+            self._mz_key, self._mz_ok = key, mz[1]
+        if wr[0]:
+            self._wrap_key, self._wrap_ok = key, wr[1]
+        self._prev_key = None
+
     def _run(self, args):
         """Run one wf_step batch on the fast or the exact path."""
         fused = self._prepare_fused(args)
