@@ -1106,12 +1106,17 @@ int adjoint_fused(wf_handle *h, const wf_adjoint_args *a, const wf::MetricDev<S>
   wf::AdjTile t;
   t.ntx = (g.Nx + wf::ADJ_TX - 1) / wf::ADJ_TX;
   t.nty = (g.Ny + TY - 1) / TY;
-  // material codes when the table fits beside the P/Q planes in 64 KiB of LDS
-  // (otherwise the coefficient arrays, with identical results)
+  // material codes for FP32 coefficients when the table fits beside the P/Q
+  // planes in 64 KiB of LDS; otherwise the coefficient arrays, with identical
+  // results (measured at 128x128x256: FP32 88 against 96 us per step; with
+  // FP64 coefficients the code path was 1.5x slower than the arrays)
   const size_t table = (a->code && a->mat && a->nmat > 0 && a->code_metric2d)
                            ? (size_t)a->nmat * 9 * sizeof(S) : 0;
-  const bool lut = table > 0 && adj_static_lds(TY) + table <= 65536;
-  const auto kern = lut ? &wf::k_adj_fused<S, A, TY, true> : &wf::k_adj_fused<S, A, TY, false>;
+  const bool lut = sizeof(S) == 4 && table > 0 && adj_static_lds(TY) + table <= 65536;
+  void (*kern)(wf::AdjStep<S, A>) = &wf::k_adj_fused<S, A, TY, false>;
+  if constexpr (sizeof(S) == 4) {
+    if (lut) kern = &wf::k_adj_fused<S, A, TY, true>;
+  }
   const size_t lds = lut ? table : 0;
   const int threads = 64 * (TY + 3);
   const int64_t conc = (int64_t)fused_occupancy(h, (const void *)kern, threads, lds) * h->ncu;

@@ -500,14 +500,17 @@ class SolverFIT3D(RoutinesMixin):
             u, inv, cnt = torch.unique(torch.stack(cols, dim=1), dim=0, return_inverse=True,
                                        return_counts=True)
             order = torch.argsort(cnt, descending=True, stable=True)
-            if int(cnt[order[255:]].sum().item()) > self.code_escape_max * N:
+            # materials of a single cell are escaped too: a short table leaves
+            # room in LDS (the FP64 reverse kernel holds it beside its planes)
+            K = max(1, min(255, int((cnt >= 2).sum().item())))
+            if int(cnt[order[K:]].sum().item()) > self.code_escape_max * N:
                 return False
             rank = torch.empty_like(order)
             rank[order] = torch.arange(order.numel(), device=dev)
-            code = rank[inv].clamp(max=255)
+            code = rank[inv].clamp(max=K)
+            code[code == K] = 255
             esc = code == 255
-            mat = u[order[:255]].contiguous()
-            K = 255
+            mat = u[order[:K]].contiguous()
         # 2-D metric tables with the row masks folded in (iA zero on PMC-masked
         # faces, itA zero on PEC-masked edges): a mask of component c depends
         # only on the two transverse indices, like the table
