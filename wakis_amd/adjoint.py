@@ -21,10 +21,13 @@ forward pass keeps that trajectory of every step (wf_step's traj_* capture,
 a few hundred KB per step for a 32x32x24 block) and the backward pass walks
 the steps back without recomputing anything ("trajectory" mode). When the
 trajectory would not fit in half of the free device memory, or the block
-touches a low face of the grid, sqrt(Nt)-spaced checkpoints of (E, H, J) are
-kept instead and each segment is recomputed from its checkpoint before it is
-reversed ("checkpoint" mode). ``wake_loss`` wraps it as a
-torch.autograd.Function.
+touches a low face of the grid, checkpoints of (E, H, J) are kept instead and
+each segment is recomputed from its checkpoint before it is reversed
+("checkpoint" mode): for a block off the low faces, segments of
+max(8, sqrt(Nt)/4) steps with checkpoints and recompute confined to the
+block's domain of dependence over a segment (a value moves at most one cell
+per step); otherwise full-field checkpoints every sqrt(Nt) steps.
+``wake_loss`` wraps it as a torch.autograd.Function.
 
 The time step must not depend on sigma: build the solver with
 ``tau_clamp=False`` (and a dt for which dt*sigma/(eps_r*eps0) stays well below
