@@ -59,7 +59,8 @@ def test_batch_refuses_grids_off_the_fast_path():
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("per_cell", [False, True])
 @pytest.mark.parametrize("use_graph", [True, False])
-def test_adjoint_batch_matches_single_problems(dtype, per_cell, use_graph):
+@pytest.mark.parametrize("full_rows", [None, False])
+def test_adjoint_batch_matches_single_problems(dtype, per_cell, use_graph, full_rows):
     """Batched forward (with the block trajectories captured in the step loop)
     and batched reverse steps give each problem's own loss and gradient, bit for
     bit. per_cell: random eps_r per block cell (the batch reads coefficient
@@ -67,7 +68,7 @@ def test_adjoint_batch_matches_single_problems(dtype, per_cell, use_graph):
     import torch
     from wakis_amd import batch
     from wakis_amd.adjoint import WakeAdjoint
-    print("adjoint batch", np.dtype(dtype).name, per_cell, use_graph, flush=True)
+    print("adjoint batch", np.dtype(dtype).name, per_cell, use_graph, full_rows, flush=True)
     specs = batch.random_box_cavities(5, n=4, cell=2.0e-3)
 
     def problems():
@@ -88,7 +89,8 @@ def test_adjoint_batch_matches_single_problems(dtype, per_cell, use_graph):
     assert len({adj.s.Nt for adj in singles}) > 1
     s0 = singles[0].s  # per-cell eps_r: escape codes (or arrays) in the single runs
     assert (s0._code is None or s0._code_escape) == per_cell
-    losses, grads = batch.wake_adjoint_batch(batched, ers, use_graph=use_graph)
+    losses, grads = batch.wake_adjoint_batch(batched, ers, use_graph=use_graph,
+                                             full_rows=full_rows)
     for (L, g), Lb, gb in zip(ref, losses, grads):
         assert Lb == L
         assert gb[1] is None

@@ -184,7 +184,8 @@ class _AdjointBatch:
     __del__ = close
 
 
-def wake_adjoint_batch(adjs, eps_r, sigma=None, cells_per_lane=0, kz=0, use_graph=True):
+def wake_adjoint_batch(adjs, eps_r, sigma=None, cells_per_lane=0, kz=0, use_graph=True,
+                       full_rows=None):
     """Loss and gradients of several ``WakeAdjoint`` problems (one solver and
     block each) computed together.
 
@@ -199,7 +200,9 @@ def wake_adjoint_batch(adjs, eps_r, sigma=None, cells_per_lane=0, kz=0, use_grap
     solvers without conductivity). Returns (losses, gradients), where
     gradients[i] = (dL/d eps_r, dL/d sigma or None). Raises
     BatchNotApplicable before anything runs if a grid cannot take the batched
-    paths.
+    paths. ``full_rows``: whole-row workgroups for the forward steps (None:
+    when every grid's rows fit one, as the forward batch tuner picks for
+    short rows).
     """
     n = len(adjs)
     if n == 0:
@@ -241,7 +244,11 @@ def wake_adjoint_batch(adjs, eps_r, sigma=None, cells_per_lane=0, kz=0, use_grap
     if any(a.code and not a.zero_masked for a in args):
         raise BatchNotApplicable("batched grids with material codes need zero masked fields")
     nmax = max(nts)
-    b = _Batch(solvers, args, cells_per_lane, kz)
+    if full_rows is None:
+        nat = _native_width(solvers[0].dtype)
+        full_rows = cells_per_lane in (0, nat) and all(
+            s.Nx % nat == 0 and s.Nx // nat <= 128 for s in solvers)
+    b = _Batch(solvers, args, cells_per_lane, kz, bool(full_rows))
     b.run(0, nmax, use_graph)
     b.close()
     losses = []
