@@ -147,3 +147,28 @@ def test_test001_golden_on_gpu():
     assert np.allclose(np.imag(wake.Z)[::20], np.imag(Zg), **tz)
     assert np.cumsum(np.abs(wake.WP))[-1] == pytest.approx(g["cumsum_abs_WP"], g["cumsum_rel"])
     assert np.cumsum(np.abs(wake.Z))[-1] == pytest.approx(g["cumsum_abs_Z"], g["cumsum_rel"])
+
+
+@pytest.mark.parametrize("dims", [(1, 4, 6), (2, 1, 5), (3, 3, 1), (4, 2, 2), (5, 7, 3),
+                                  (2, 3, 4), (2, 2, 2), (8, 8, 2), (2, 9, 70)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("conductive", [False, True])
+def test_degenerate_grids_match_oracle(dims, dtype, conductive):
+    """Grids two cells thin along an axis (the step's neighbour truncation,
+    tile edges and z-march limits at their smallest) against the oracle, on
+    whichever path the engine picks; one-cell axes are refused with a
+    ValueError (the engine needs two cells per axis)."""
+    _say("degenerate", dims, np.dtype(dtype).name, conductive)
+    case = gc.random_case(BCS[0], dtype, conductive, seed=sum(dims) + 5, dims=dims)
+    if min(dims) < 2:
+        with pytest.raises(ValueError, match="at least 2 cells"):
+            gc.amd_solver(case)
+        return
+    so = gc.oracle_solver(case)
+    sa = gc.amd_solver(case)
+    for _ in range(3):
+        so.one_step()
+        sa.one_step()
+    for name in ("E", "H", "J"):
+        got, ref = getattr(sa, name).to_numpy(), getattr(so, name)
+        assert np.array_equal(got, ref), f"{name}: rel-L2 {gc.rel_l2(got, ref):.3e}"
