@@ -474,7 +474,7 @@ class WakeAdjoint:
         box = f["box"]
         for (n0, E0, H0, J0) in reversed(f["ckpt"]):
             k = min(K, Nt - n0)
-            s._load_run_box(box, E0, H0, J0)
+            s._load_run_state(E0, H0, J0, box=box)
             _lib.check(lib.wf_trajectory_gather(s._handle, _lib.ptr(s.E.array),
                                                 _lib.ptr(s.H.array), pE, pH, lo, hi, K + 1, 0,
                                                 st), "wf_trajectory_gather")

@@ -713,40 +713,25 @@ class SolverFIT3D(RoutinesMixin):
         if self.use_conductivity:
             self._sigmaE_versions = (self.E.version, self.J.version)
 
-    def _load_run_state(self, E, H, J):
+    def _load_run_state(self, E, H, J, box=None):
         """Restore fields saved earlier in a run of this solver (checkpoints).
 
-        The fast-path invariants checked for that run (J = sigma E, zero-valued
-        masked edges and faces, dead flat-index wrap terms) hold for every state
-        the run produced, so their cached results are carried over to the
-        restored state instead of being checked again.
+        With ``box`` (slices of z, y, x), E, H, J hold only those cells and only
+        they are written; the cells outside keep what they hold (a recompute
+        confined to the box's domain of dependence never carries them into the
+        cells it keeps). The fast-path invariants checked for that run (J =
+        sigma E, zero-valued masked edges and faces, dead flat-index wrap
+        terms) hold for every state the run produced, so their cached results
+        are carried over to the restored state instead of being checked again.
         """
-        mz = (self._mz_key is not None, self._mz_ok)
-        wr = (self._wrap_key is not None, self._wrap_ok)
-        for F, X in ((self.E, E), (self.H, H), (self.J, J)):
-            F.array.copy_(X)
-            F.version += 1
-        if self.use_conductivity:
-            self._sigmaE_versions = (self.E.version, self.J.version)
-        key = (self.E.version, self.H.version, self.J.version, self._fold_count,
-               self._beam_line)
-        if mz[0]:
-            self._mz_key, self._mz_ok = key, mz[1]
-        if wr[0]:
-            self._wrap_key, self._wrap_ok = key, wr[1]
-        self._prev_key = None
-
-    def _load_run_box(self, box, E, H, J):
-        """Restore E, H, J saved on a box of cells (slices of z, y, x) of a
-        state of this solver's run; the cells outside keep what they hold (a
-        recompute confined to the box's domain of dependence never reads them
-        into the cells it keeps). The run's checked invariants are carried over
-        as by _load_run_state."""
         mz = (self._mz_key is not None, self._mz_ok)
         wr = (self._wrap_key is not None, self._wrap_ok)
         Nx, Ny, Nz = self.dims
         for F, X in ((self.E, E), (self.H, H), (self.J, J)):
-            F.array.view(3, Nz, Ny, Nx)[(slice(None),) + tuple(box)].copy_(X)
+            if box is None:
+                F.array.copy_(X)
+            else:
+                F.array.view(3, Nz, Ny, Nx)[(slice(None),) + tuple(box)].copy_(X)
             F.version += 1
         if self.use_conductivity:
             self._sigmaE_versions = (self.E.version, self.J.version)
