@@ -6,11 +6,11 @@ and ABC on z; Gaussian beam q = 1e-9 C, sigma_z = 10 mm, beta = 1 on axis,
 wakelength 0.5 m (2062 steps), skip_cells = 10, impedance up to 5 GHz.
 Fields and all wake arithmetic in FP64 through the oracle (oracle/), the
 CPU restatement of the reference's scipy path. Writes
-tests/golden/config1_fp64_oracle.npz: s, WP_3d, WPx, WPy, f, Z, Zx, Zy,
+tests/golden/cavity_48x48x128_fp64_oracle.npz: s, WP_3d, WPx, WPy, f, Z, Zx, Zy,
 lambdas, and a stride-97 sample plus exactly rounded seeded weighted sums
 of the final E, H, J.
 
-Usage: python tests/golden/make_config1_golden.py   (about two minutes)
+Usage: python tests/golden/make_cavity_48x48x128_golden.py   (about two minutes)
 """
 
 import math
@@ -27,7 +27,7 @@ from oracle import cases  # noqa: E402
 from oracle import fit_oracle as fo  # noqa: E402
 from oracle import wake_oracle as wo  # noqa: E402
 
-OUT = Path(__file__).resolve().with_name("config1_fp64_oracle.npz")
+OUT = Path(__file__).resolve().with_name("cavity_48x48x128_fp64_oracle.npz")
 RUN = dict(dims=(48, 48, 128), q=1e-9, sigmaz=10e-3, beta=1.0, wakelength=0.5,
            skip_cells=10, fmax=5e9)
 STRIDE = 97

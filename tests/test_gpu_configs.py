@@ -2,8 +2,8 @@
 
 * 48x48x128 cavity/pipe wakefield run (2062 steps, 0.5 m wake, Z up to
   5 GHz): the FP64 device run is bit-identical to the oracle's committed
-  outputs (tests/golden/config1_fp64_oracle.npz, made by
-  tests/golden/make_config1_golden.py); the FP32 device run is within 1e-4
+  outputs (tests/golden/cavity_48x48x128_fp64_oracle.npz, made by
+  tests/golden/make_cavity_48x48x128_golden.py); the FP32 device run is within 1e-4
   relative L2 of it on WP_z.
 * Lossy pillbox with PML (conductive wall shell, beta = 0.8, tau-clamped dt),
   scaled down to 40x40x80: FP32 and FP64 device runs bit-identical to the
@@ -22,10 +22,10 @@ from tests import gpu_cases as gc
 
 pytestmark = pytest.mark.gpu
 
-GOLD = Path(__file__).resolve().parent / "golden" / "config1_fp64_oracle.npz"
+GOLD = Path(__file__).resolve().parent / "golden" / "cavity_48x48x128_fp64_oracle.npz"
 
 
-def _config1(dtype):
+def _cavity_48x48x128(dtype):
     import wakis_amd as wa
     x, y, z, mask = cases.cavity_pipe_geometry(48, 48, 128)
     grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"cav": mask},
@@ -44,10 +44,10 @@ def _digest(F, seed, stride=97):
     return F[::stride], math.fsum((F * w).tolist())
 
 
-def test_config1_fp64_bit_identical_to_oracle():
-    print("config1 fp64 48x48x128, 2062 steps", flush=True)
+def test_cavity_48x48x128_fp64_bit_identical_to_oracle():
+    print("cavity 48x48x128 fp64, 2062 steps", flush=True)
     g = np.load(GOLD, allow_pickle=False)
-    s, wake = _config1(np.float64)
+    s, wake = _cavity_48x48x128(np.float64)
     assert s.Nt == int(g["Nt"]) == 2062
     assert s.dt == float(g["dt"])
     assert s.last_path == "fused"
@@ -64,10 +64,10 @@ def test_config1_fp64_bit_identical_to_oracle():
         assert dot == float(g[name + "_dot"]), name
 
 
-def test_config1_fp32_wake_within_1e4_of_fp64_oracle():
-    print("config1 fp32 48x48x128", flush=True)
+def test_cavity_48x48x128_fp32_wake_within_1e4_of_fp64_oracle():
+    print("cavity 48x48x128 fp32", flush=True)
     g = np.load(GOLD, allow_pickle=False)
-    s, wake = _config1(np.float32)
+    s, wake = _cavity_48x48x128(np.float32)
     # the FP32 run's s and f axes come from its FP32 dt: compare on the common length
     n, m = min(len(wake.s), len(g["s"])), min(len(wake.f), len(g["f"]))
     assert np.max(np.abs(wake.s[:n] - g["s"][:n])) <= 1e-6 * np.max(np.abs(g["s"]))

@@ -48,7 +48,7 @@ def _same_fields(a, b):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_config2_fused_equals_general_path(dtype):
+def test_cavity_256x256x512_fused_equals_general_path(dtype):
     print("256x256x512 fused vs general", np.dtype(dtype).name, flush=True)
     dims = (256, 256, 512)
     f, g = _cavity(dims, dtype, "fused"), _cavity(dims, dtype, "general")
@@ -58,7 +58,7 @@ def test_config2_fused_equals_general_path(dtype):
     _same_fields(f, g)
 
 
-def test_config2_lossless_energy_conserved():
+def test_cavity_256x256x512_lossless_energy_conserved():
     import torch
     print("256x256x512 energy", flush=True)
     s = _cavity((256, 256, 512), np.float64, "fused")
@@ -84,7 +84,7 @@ def test_config2_lossless_energy_conserved():
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_config3_shapes_and_general_path_agree(dtype):
+def test_lossy_pillbox_200x200x400_shapes_and_general_path_agree(dtype):
     import wakis_amd as wa
     from wakis_amd.solids import Cylinder
     print("200x200x400 lossy pillbox shapes", np.dtype(dtype).name, flush=True)
@@ -123,7 +123,7 @@ def test_config3_shapes_and_general_path_agree(dtype):
     _same_fields(runs[0], runs[2])
 
 
-def test_config4_checkpoint_and_trajectory_gradients_agree():
+def test_adjoint_128x128x256_checkpoint_and_trajectory_gradients_agree():
     import wakis_amd as wa
     from wakis_amd.adjoint import WakeAdjoint
     from wakis_amd.solids import Box, Cylinder
