@@ -22,7 +22,8 @@ Pinning status of each function (what fixes its numbers besides itself):
 * ``wake_trans_WP``           closed form (linear-in-x WP gives constant WPx)
 * ``wake_long_Z``/``trans_Z`` numpy.fft as the textbook DFT, test_001 golden Z
 * ``init_pml``                closed-form geomspace profile, test_005 invariant
-* ``emsolve`` ABC copy        brute force on tiny grids (parity unpinned against
-                              any reference test: the reference has none that
-                              asserts ABC values)
+* ``emsolve`` ABC copy        the reference's snapshot protocol (get_abc /
+                              update_abc with the n-2 rotation) run literally on
+                              numpy views of tiny grids; copied snapshots (a true
+                              two-step delay) are shown to give other fields
 """

@@ -46,6 +46,29 @@ def test_emsolve_abc_plane_wave_matches_oracle(dtype):
     assert np.any(so.E != 0)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("bc", [
+    (["abc", "abc", "abc"], ["abc", "abc", "abc"]),
+    (["abc", "periodic", "pec"], ["pec", "periodic", "abc"]),
+])
+def test_emsolve_abc_faces_random_fields_match_oracle(dtype, bc):
+    """ABC on x, y and z faces (edge and corner cells written by several faces in
+    the reference's order), random initial E and H, no source."""
+    print("emsolve abc faces", bc, np.dtype(dtype).name, flush=True)
+    x, y, z = fo.uniform_axes(-1.0, 1.0, -1.0, 1.0, 0.0, 2.0, 11, 9, 14)
+    so, sa = _pair(x, y, z, bc, dtype)
+    rng = np.random.default_rng(11)
+    so.E[:] = rng.standard_normal(so.E.shape).astype(dtype)
+    so.H[:] = rng.standard_normal(so.H.shape).astype(dtype)
+    sa.E.fromarray(so.E)
+    sa.H.fromarray(so.H)
+    fo.emsolve(so, 40)
+    sa.emsolve(40)
+    assert so.activate_abc and sa.activate_abc
+    for name in ("E", "H", "J"):
+        assert np.array_equal(getattr(sa, name).to_numpy(), getattr(so, name)), name
+
+
 def _test005_setup(Nz=200):
     x, y, z = fo.uniform_axes(-1.0, 1.0, -1.0, 1.0, 0.0, 1.0, 20, 20, Nz)
     bc = (["periodic", "periodic", "pec"], ["periodic", "periodic", "pml"])
