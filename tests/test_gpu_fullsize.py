@@ -6,6 +6,8 @@ depend on the grid size (the oracle's CPU run would take hours there).
   FP32 and FP64; a lossless FP64 run conserves the leapfrog energy.
 * 200x200x400 lossy pillbox with PML and SIBC: whole-row workgroups, row
   segments and the general path give bitwise the same fields.
+* Grids whose 3N field arrays pass 4 GiB (up to 1024x1024x700 FP32, 3N > 2^31
+  elements): the fused path's 64-bit-offset kernel equals the general path.
 * 128x128x256 cavity with a 32x32x24 block of per-cell eps_r and sigma: the
   checkpoint mode (segments recomputed on the block's domain of dependence)
   and the trajectory mode give bitwise the same gradients.
@@ -121,6 +123,47 @@ def test_lossy_pillbox_200x200x400_shapes_and_general_path_agree(dtype):
     assert bool(runs[0].E.array.abs().max().item() > 0)
     _same_fields(runs[0], runs[1])
     _same_fields(runs[0], runs[2])
+
+
+@pytest.mark.parametrize("dtype,dims", [(np.float32, (512, 512, 1536)),
+                                        (np.float64, (512, 512, 768)),
+                                        (np.float32, (1024, 1024, 700))])
+def test_field_arrays_over_4gib_fused_equals_general_path(dtype, dims):
+    """Every 3N field array is 4.83 GB (8.81 GB and 3N > 2^31 elements on the
+    1024x1024x700 grid), past the 32-bit byte offsets of the x-vectorised
+    kernel: the fused path takes its 64-bit-offset kernel and still equals the
+    general path bit for bit, from random E and H."""
+    import torch
+
+    import wakis_amd as wa
+    print("over 4 GiB fused vs general", np.dtype(dtype).name, dims, flush=True)
+    grid = wa.GridFIT3D(-30e-3, 30e-3, -30e-3, 30e-3, -80e-3, 80e-3, *dims, verbose=0)
+    # PEC on every face: the flat-index wrap terms are masked, as the fused path needs
+    s = wa.SolverFIT3D(grid, bc_low=["pec"] * 3, bc_high=["pec"] * 3, dtype=dtype, verbose=0,
+                       path="general")
+    assert 3 * s.N * np.dtype(dtype).itemsize > 2 ** 32
+    print("solver built", flush=True)
+    gen = torch.Generator(device=s.device).manual_seed(5)
+    for F in (s.E, s.H):
+        F.fromarray(torch.randn(3 * s.N, generator=gen, device=s.device,
+                                dtype=F.array.dtype))
+    s.one_step()  # first step: ghost cells zeroed
+    saved = [F.array.clone() for F in (s.E, s.H, s.J)]
+    s.path = "fused"
+    s._run(s._step_args(3))
+    assert s.last_path == "fused"
+    fused = [F.array.clone() for F in (s.E, s.H, s.J)]
+    print("fused steps done", flush=True)
+    for F, X in zip((s.E, s.H, s.J), saved):
+        F.array.copy_(X)
+        F.version += 1
+    del saved
+    s.path = "general"
+    s._run(s._step_args(3))
+    assert s.last_path == "general"
+    for F, X, name in zip((s.E, s.H, s.J), fused, "EHJ"):
+        assert bool((F.array == X).all().item()), name
+    assert bool(s.E.array.abs().max().item() > 0)
 
 
 def test_adjoint_128x128x256_checkpoint_and_trajectory_gradients_agree():
