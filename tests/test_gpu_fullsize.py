@@ -7,7 +7,7 @@ depend on the grid size (the oracle's CPU run would take hours there).
 * 200x200x400 lossy pillbox with PML and SIBC: whole-row workgroups, row
   segments and the general path give bitwise the same fields.
 * Grids whose 3N field arrays pass 4 GiB (up to 1024x1024x700 FP32, 3N > 2^31
-  elements): the fused path's 64-bit-offset kernel equals the general path.
+  elements): the fused path's 64-bit-offset kernels equal the general path.
 * 128x128x256 cavity with a 32x32x24 block of per-cell eps_r and sigma: the
   checkpoint mode (segments recomputed on the block's domain of dependence)
   and the trajectory mode give bitwise the same gradients.
@@ -125,18 +125,20 @@ def test_lossy_pillbox_200x200x400_shapes_and_general_path_agree(dtype):
     _same_fields(runs[0], runs[2])
 
 
-@pytest.mark.parametrize("dtype,dims", [(np.float32, (512, 512, 1536)),
-                                        (np.float64, (512, 512, 768)),
-                                        (np.float32, (1024, 1024, 700))])
-def test_field_arrays_over_4gib_fused_equals_general_path(dtype, dims):
+@pytest.mark.parametrize("dtype,dims,variant", [(np.float32, (512, 512, 1536), 0),
+                                                (np.float64, (512, 512, 768), 0),
+                                                (np.float32, (1024, 1024, 700), 0),
+                                                (np.float32, (512, 512, 1536), 1)])
+def test_field_arrays_over_4gib_fused_equals_general_path(dtype, dims, variant):
     """Every 3N field array is 4.83 GB (8.81 GB and 3N > 2^31 elements on the
-    1024x1024x700 grid), past the 32-bit byte offsets of the x-vectorised
-    kernel: the fused path takes its 64-bit-offset kernel and still equals the
-    general path bit for bit, from random E and H."""
+    1024x1024x700 grid), past the 32-bit byte offsets of the default fused
+    kernel: the fused path takes the same tile code with 64-bit offsets
+    (k_fused4_wide), or the one-cell-per-lane kernel (variant 1), and equals
+    the general path bit for bit, from random E and H."""
     import torch
 
     import wakis_amd as wa
-    print("over 4 GiB fused vs general", np.dtype(dtype).name, dims, flush=True)
+    print("over 4 GiB fused vs general", np.dtype(dtype).name, dims, variant, flush=True)
     grid = wa.GridFIT3D(-30e-3, 30e-3, -30e-3, 30e-3, -80e-3, 80e-3, *dims, verbose=0)
     # PEC on every face: the flat-index wrap terms are masked, as the fused path needs
     s = wa.SolverFIT3D(grid, bc_low=["pec"] * 3, bc_high=["pec"] * 3, dtype=dtype, verbose=0,
@@ -150,6 +152,7 @@ def test_field_arrays_over_4gib_fused_equals_general_path(dtype, dims):
     s.one_step()  # first step: ghost cells zeroed
     saved = [F.array.clone() for F in (s.E, s.H, s.J)]
     s.path = "fused"
+    s.fused_variant = variant
     s._run(s._step_args(3))
     assert s.last_path == "fused"
     fused = [F.array.clone() for F in (s.E, s.H, s.J)]
@@ -164,6 +167,36 @@ def test_field_arrays_over_4gib_fused_equals_general_path(dtype, dims):
     for F, X, name in zip((s.E, s.H, s.J), fused, "EHJ"):
         assert bool((F.array == X).all().item()), name
     assert bool(s.E.array.abs().max().item() > 0)
+
+
+@pytest.mark.parametrize("dtype,dims", [(np.float32, (512, 512, 1536)),
+                                        (np.float64, (512, 512, 768))])
+def test_beam_over_4gib_fused_equals_general_path(dtype, dims):
+    """A beam on the axis of a PEC box whose 3N arrays pass 4 GiB: the
+    64-bit-offset kernel with the beam-line current and no boundary masks
+    (fields zero on every masked edge) equals the general path bit for bit."""
+    import torch
+
+    import wakis_amd as wa
+    print("over 4 GiB beam fused vs general", np.dtype(dtype).name, dims, flush=True)
+    out = {}
+    for path in ("fused", "general"):
+        grid = wa.GridFIT3D(-30e-3, 30e-3, -30e-3, 30e-3, -80e-3, 80e-3, *dims, verbose=0)
+        s = wa.SolverFIT3D(grid, bc_low=["pec"] * 3, bc_high=["pec"] * 3, dtype=dtype,
+                           verbose=0, path=path)
+        assert 3 * s.N * np.dtype(dtype).itemsize > 2 ** 32
+        beam = wa.Beam(q=1e-9, sigmaz=10e-3, beta=1.0)
+        for k in range(6):
+            beam.update(s, (k + 1500) * s.dt)  # bunch inside the box
+            s.one_step()
+        assert s.last_path == path
+        out[path] = [F.array.clone() for F in (s.E, s.H, s.J)]
+        del s, grid
+        torch.cuda.empty_cache()
+        print(path, "steps done", flush=True)
+    assert bool(out["fused"][0].abs().max().item() > 0)
+    for X, Y, name in zip(out["fused"], out["general"], "EHJ"):
+        assert bool(torch.equal(X, Y)), name
 
 
 def test_adjoint_128x128x256_checkpoint_and_trajectory_gradients_agree():

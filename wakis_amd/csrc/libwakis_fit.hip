@@ -214,21 +214,31 @@ template <typename S>
 using Fused4Fn = void (*)(wf::GridDev, wf::MetricDev<S>, const S *, const S *, S *, S *, S *,
                           wf::Coefs<S>, S, wf::FusedAux, const S *, S *);
 
+// k_fused4, or k_fused4_wide (64-bit offsets) for arrays past 4 GiB
+template <typename S, int JM, bool CO, bool LU, bool DE, int TY, bool MK, int V, bool FR,
+          bool ES, bool WIDE>
+Fused4Fn<S> fused4_ptr() {
+  if constexpr (WIDE) return &wf::k_fused4_wide<S, JM, CO, LU, DE, TY, MK, V>;
+  else return &wf::k_fused4<S, JM, CO, LU, DE, TY, MK, V, FR, ES>;
+}
+
 // The k_fused4 instantiation for the step's J mode, conductivity, coefficient
 // form, boundary order and masking.
-template <typename S, int TY, int V, bool FR>
+template <typename S, int TY, int V, bool FR, bool WIDE = false>
 Fused4Fn<S> pick_fused4(bool lut, bool cond, bool jline, bool desc, bool mk, bool esc = false) {
 #define WF_PICK(JM, CO, LU, ES)                                                             \
   do {                                                                                      \
-    if (desc) return mk ? &wf::k_fused4<S, JM, CO, LU, true, TY, true, V, FR, ES>           \
-                        : &wf::k_fused4<S, JM, CO, LU, true, TY, !LU, V, FR, ES>;           \
-    return mk ? &wf::k_fused4<S, JM, CO, LU, false, TY, true, V, FR, ES>                    \
-              : &wf::k_fused4<S, JM, CO, LU, false, TY, !LU, V, FR, ES>;                    \
+    if (desc) return mk ? fused4_ptr<S, JM, CO, LU, true, TY, true, V, FR, ES, WIDE>()      \
+                        : fused4_ptr<S, JM, CO, LU, true, TY, !LU, V, FR, ES, WIDE>();      \
+    return mk ? fused4_ptr<S, JM, CO, LU, false, TY, true, V, FR, ES, WIDE>()               \
+              : fused4_ptr<S, JM, CO, LU, false, TY, !LU, V, FR, ES, WIDE>();               \
   } while (0)
-  if (lut && esc) {
-    if (cond) WF_PICK(0, true, true, true);
-    if (jline) WF_PICK(1, false, true, true);
-    WF_PICK(0, false, true, true);
+  if constexpr (!WIDE) {
+    if (lut && esc) {
+      if (cond) WF_PICK(0, true, true, true);
+      if (jline) WF_PICK(1, false, true, true);
+      WF_PICK(0, false, true, true);
+    }
   }
   if (lut) {
     if (cond) WF_PICK(0, true, true, false);
@@ -278,7 +288,7 @@ inline size_t full_rows_lds(int Nx, int V, int rows, size_t sz, size_t table) {
 constexpr int FR_BOUND = 7;            // full-row kernels: up to 64 * 8 threads
 constexpr size_t FR_LDS_MAX = 65536;   // and 64 KiB of LDS per workgroup
 
-template <typename S, int TY, int V, bool FR = false>
+template <typename S, int TY, int V, bool FR = false, bool WIDE = false>
 int step_fused4(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStream_t st,
                 int fr_rows = 0) {
   const wf::GridDev &g = h->g;
@@ -309,7 +319,7 @@ int step_fused4(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStr
     x.ntx = (g.Nx + Wc - 1) / Wc;
     x.nty = (g.Ny + TY - 1) / TY;
   }
-  const Fused4Fn<S> kern = pick_fused4<S, TY, V, FR>(lut, a->conductive != 0,
+  const Fused4Fn<S> kern = pick_fused4<S, TY, V, FR, WIDE>(lut, a->conductive != 0,
                                                       a->jmode == WF_J_LINE, g.m1desc != 0,
                                                       !(lut && a->zero_masked),
                                                       lut && a->code_escape);
@@ -456,9 +466,17 @@ int step_fused(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStre
     return fail(WF_EINVAL, "wf_step: cells_per_lane %d not built with %d rows per workgroup",
                 V, rows);
   }
+  // 3N arrays past 4 GiB: the same tile code with 64-bit offsets, default row
+  // segments (7 rows FP32: the 8-row tile has no registers left for the wider
+  // addresses), 16-byte lanes
+  if (!idx32v && a->variant == 0 && g.Nx % vn == 0 &&
+      (a->cells_per_lane == 0 || a->cells_per_lane == vn) && !(lut && a->code_escape)) {
+    if constexpr (sizeof(S) == 8) return step_fused4<S, 3, vn, false, true>(h, a, m, st);
+    else return step_fused4<S, 7, vn, false, true>(h, a, m, st);
+  }
   if (a->cells_per_lane && a->variant != 1)
     return fail(WF_EINVAL, "wf_step: cells_per_lane %d needs Nx a multiple of it and every "
-                "3N array within 4 GiB", a->cells_per_lane);
+                "3N array within 4 GiB (or %d with the default rows)", a->cells_per_lane, vn);
   if (lut && a->code_escape)
     return fail(WF_EINVAL, "wf_step: the escape code needs the x-vectorised fused kernel "
                 "(variant 0 and every 3N array within 4 GiB)");

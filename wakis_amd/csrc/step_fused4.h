@@ -22,6 +22,8 @@
 #include "fit_common.h"
 #include "step_fused.h"
 
+#include <type_traits>
+
 namespace wf {
 
 template <typename S>
@@ -73,8 +75,8 @@ __device__ __forceinline__ void stv(S *__restrict__ p, I idx, const S (&v)[V]) {
   }
 }
 // V consecutive uint8 material codes packed little-endian into one word
-template <int V>
-__device__ __forceinline__ uint32_t ldcode(const uint8_t *__restrict__ p, uint32_t idx) {
+template <int V, typename I>
+__device__ __forceinline__ uint32_t ldcode(const uint8_t *__restrict__ p, I idx) {
   if constexpr (V == 4) return *reinterpret_cast<const uint32_t *>(p + idx);
   else if constexpr (V == 2) return *reinterpret_cast<const uint16_t *>(p + idx);
   else return p[idx];
@@ -96,15 +98,17 @@ __device__ __forceinline__ int xcd_tile(int blin, int nb) {
 
 // One workgroup's z-march over tile `tile` of the grid (the whole step of that
 // tile); shared by the single-grid and the batched kernels.
+// WIDE: 64-bit element offsets (arrays past 4 GiB); otherwise 32-bit byte
+// offsets from the array base (every 3N array within 4 GiB).
 template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY, bool MK, int V, bool FR,
-          bool ESC>
+          bool ESC, bool WIDE = false>
 __device__ __forceinline__ void fused4_tile(const GridDev &g, const MetricDev<S> &m,
                                             const S *__restrict__ Ein, const S *__restrict__ Hin,
                                             S *__restrict__ Eout, S *__restrict__ Hout,
                                             S *__restrict__ J, const Coefs<S> &cf, S dt,
                                             const FusedAux &x, const S *__restrict__ beam_row,
                                             S *__restrict__ cap, int tile) {
-  using I = uint32_t;
+  using I = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
   // V cells per lane: 16 / sizeof(S) for 16-byte accesses, or 1 for narrow grids
   constexpr int OFF = V;                       // cell xi at LDS slot xi + OFF
   constexpr int NM = 9;
@@ -555,6 +559,18 @@ k_fused4(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restri
   const int tile = xcd_tile(blockIdx.x, x.ntx * x.nty * x.ntz);
   fused4_tile<S, JM, COND, LUT, DESC, TY, MK, V, FR, ESC>(g, m, Ein, Hin, Eout, Hout, J, cf,
                                                           dt, x, beam_row, cap, tile);
+}
+
+// The same step for grids whose 3N arrays pass 4 GiB: 64-bit offsets (two
+// VGPRs per address), row segments, no escape code.
+template <typename S, int JM, bool COND, bool LUT, bool DESC, int TY, bool MK, int V>
+__global__ void __launch_bounds__(64 * (TY + 1))
+k_fused4_wide(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restrict__ Hin,
+              S *__restrict__ Eout, S *__restrict__ Hout, S *__restrict__ J, Coefs<S> cf, S dt,
+              FusedAux x, const S *__restrict__ beam_row, S *__restrict__ cap) {
+  const int tile = xcd_tile(blockIdx.x, x.ntx * x.nty * x.ntz);
+  fused4_tile<S, JM, COND, LUT, DESC, TY, MK, V, false, false, true>(
+      g, m, Ein, Hin, Eout, Hout, J, cf, dt, x, beam_row, cap, tile);
 }
 
 // One grid of a batch: everything a step of it needs except the step number.
