@@ -51,7 +51,8 @@ class WakeAdjoint:
         ``mode``: "trajectory", "checkpoint" or "auto" (trajectory when it fits).
         ``kernel``: "fused" (one single-pass reverse kernel per step), "two-pass"
         or "auto" (fused when the flat-index wrap terms are dead, as for the
-        fused forward step, and the region starts at cell 1 or later in x and y).
+        fused forward step, the region starts at cell 1 or later in x and y, and
+        3N doubles fit in 4 GiB).
         ``adjoint_dtype``: storage of the adjoint fields, "float64", "float32"
         (fused kernel only; arithmetic stays FP64) or "auto" (float32 with the
         fused kernel when the solver's fields are FP32, else float64)."""
@@ -188,10 +189,14 @@ class WakeAdjoint:
     def _use_fused_kernel(self):
         if self.kernel == "two-pass":
             return False
-        ok = self.i0 >= 1 and self.j0 >= 1 and self.s._wraps_dead()
+        # the fused reverse kernel addresses its FP64-sized 3N arrays with 32-bit
+        # byte offsets; larger grids take the two-pass kernels (64-bit indices)
+        small = 3 * self.s.N * 8 < 2 ** 32
+        ok = small and self.i0 >= 1 and self.j0 >= 1 and self.s._wraps_dead()
         if self.kernel == "fused" and not ok:
             raise ValueError("WakeAdjoint: the fused reverse kernel needs dead flat-index wrap "
-                             "terms and the region at cell 1 or later in x and y")
+                             "terms, the region at cell 1 or later in x and y, and 3N "
+                             "doubles within 4 GiB")
         return ok
 
     @property
