@@ -107,7 +107,11 @@ typedef struct wf_step_args {
                              row count measured fastest: FP64 3; FP32 8, or 7 when
                              conductive or without material codes), 1 scalar kernel,
                              2, 3, 7 or 8 (FP32 only): x-vectorised kernel with
-                             that many rows per workgroup                          */
+                             that many rows per workgroup. When a 3N array passes
+                             4 GiB, 0 takes the x-vectorised kernel with 64-bit
+                             offsets (FP64 3 rows, FP32 7; Nx a multiple of
+                             16/sizeof(dtype)), and 2..8 (or another Nx) the
+                             scalar kernel                                         */
   int32_t zero_masked;    /* fused path: 1 = E is zero on every edge the PEC mask
                              zeroes and H on every face the PMC mask zeroes, no
                              source writes there, and (with `code`) metric2d has
