@@ -332,7 +332,7 @@ k_adj_e(GridDev g, MetricDev<S> m, double *__restrict__ W, double *__restrict__ 
 // indices n with it(n,k) == m of gI[l][n] * dz[k] (it as in k_wake_wp, which
 // is non-decreasing in n). One thread per (line, k): runs of equal m are
 // summed in order, so the result is deterministic.
-__global__ void k_wake_wp_adjoint(const double *__restrict__ gI, int nlines, int64_t cap_samples,
+static __global__ void k_wake_wp_adjoint(const double *__restrict__ gI, int nlines, int64_t cap_samples,
                                   int Nz, int k0, int nz, const double *__restrict__ z,
                                   const double *__restrict__ dz, const double *__restrict__ s,
                                   int64_t ns, double v, double zmin_v, double t0, double ti,
@@ -360,7 +360,7 @@ __global__ void k_wake_wp_adjoint(const double *__restrict__ gI, int nlines, int
 
 // out[n] = Re sum_f (w_re[f] + i w_im[f]) exp(-2 pi i f n / N), n < len: the
 // transpose of the kept-bin DFT, phase reduced exactly as (f*n mod N).
-__global__ void k_idft_real(const double *__restrict__ w, int64_t nbins, int64_t N, int64_t len,
+static __global__ void k_idft_real(const double *__restrict__ w, int64_t nbins, int64_t N, int64_t len,
                             double *__restrict__ out) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= len) return;

@@ -1,0 +1,8 @@
+// step_fused4 instantiations (set D) for FP32 storage.
+#include "lib_fused.h"
+
+#pragma GCC visibility push(hidden)
+namespace wff {
+WF_FUSED4_F32_D(WF_FUSED4_INSTANTIATE)
+}  // namespace wff
+#pragma GCC visibility pop

@@ -91,7 +91,7 @@ k_energy(GridDev g, MetricDev<S> m, const S *__restrict__ E, const S *__restrict
 }
 
 // one block: combine nb partial pairs (in block order) into out[0], out[1]
-__global__ void __launch_bounds__(RED_THREADS)
+static __global__ void __launch_bounds__(RED_THREADS)
 k_combine2(const double *__restrict__ part, int nb, double *__restrict__ out) {
   Acc a[2] = {{0.0, 0.0}, {0.0, 0.0}};
   for (int b = threadIdx.x; b < nb; b += RED_THREADS) {
@@ -106,7 +106,7 @@ k_combine2(const double *__restrict__ part, int nb, double *__restrict__ out) {
 }
 
 // one block: out = scale * sum_n x[n] * y[n]
-__global__ void __launch_bounds__(RED_THREADS)
+static __global__ void __launch_bounds__(RED_THREADS)
 k_dot(const double *__restrict__ x, const double *__restrict__ y, int64_t n, double scale,
       double *__restrict__ out) {
   Acc a[1] = {{0.0, 0.0}};
@@ -142,7 +142,7 @@ k_count_nonfinite(const S *__restrict__ x, int64_t n, long long *__restrict__ pa
 }
 
 // one block: out[0] = sum of nb per-block counts
-__global__ void __launch_bounds__(RED_THREADS)
+static __global__ void __launch_bounds__(RED_THREADS)
 k_sum_counts(const long long *__restrict__ part, int nb, long long *__restrict__ out) {
   __shared__ long long sh[RED_THREADS];
   long long c = 0;

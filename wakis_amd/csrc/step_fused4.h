@@ -624,7 +624,7 @@ k_fused4_batch(const BatchDesc<S> *__restrict__ d, int ninst, int ntiles,
       b.cf, b.dt, x, row, b.cap, tile - b.tile0);
 }
 
-__global__ void k_batch_next(int *step) { step[0] = step[0] + 1; }
-__global__ void k_batch_set(int *step, int v) { step[0] = v; }
+static __global__ void k_batch_next(int *step) { step[0] = step[0] + 1; }
+static __global__ void k_batch_set(int *step, int v) { step[0] = v; }
 
 }  // namespace wf

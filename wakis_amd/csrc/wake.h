@@ -111,7 +111,7 @@ __device__ inline double np_sum(const double *a, int64_t n) {
 }
 
 // WakeSolver.calc_trans_WP (wakeSolver.py:583-612) on a 3x3 window (i-major).
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_wake_trans(const double *__restrict__ WP3d, int64_t ns, double ds, double dxsum,
              double dysum, double *__restrict__ WPx, double *__restrict__ WPy) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -136,7 +136,7 @@ struct Comp {
 };
 
 // One workgroup per (signal, bin): X[f] = sum_n x[n] exp(-2 pi i f n / N).
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_dft(const double *__restrict__ x, int64_t len, int64_t N, int64_t nbins,
       double *__restrict__ out) {
   const int64_t f = blockIdx.x;
