@@ -198,15 +198,23 @@ int wf_abc_low(wf_handle *h, void *E, void *H, int32_t low_faces, void *stream);
 /* Longitudinal wake potential of every captured line with the reference's
  * line-to-line carry (WakeSolver.calc_long_WP_3d, wakeSolver.py:422-554;
  * calc_long_WP, :310-420 for one line). Output WP3d[nlines][ns] (FP64).
- * it(n,k) = trunc(((z[k]+s[n])/v - zmin_v - t0 + ti)/dt), kept if it < nt;
- * each line ends with WP /= qe12 where qe12 = q*1e12 as the caller computes it.
- * capture is [nlines][cap_samples][Nz] in `dtype`; k runs over z[0..nz) which
- * map to capture z index k + k0; dz is indexed from 0 (as the reference). */
+ * Forward (counter_moving = 0, wakeSolver.py:520-533): row k of a line,
+ *   it(n,k) = trunc(((z[k]+s[n])/v - zref_v - t0 + ti)/dt), weight dz[k],
+ *   with zref_v = zmin/v.
+ * Counter-moving (counter_moving = 1, wakeSolver.py:503-517): row nz-1-k,
+ *   it(n,k) = trunc(((z[nz-1-k]-s[n])/(-v) - zref_v - t0 + ti)/dt),
+ *   weight -dz[k], with zref_v = zmax/(-v).
+ * A term is kept if it < nt; a negative it reads column it + ncols of the
+ * reference's Ezt (ncols = len(t) >= nt columns, of which 0..nt-1 hold the
+ * samples and the rest zeros). Each line ends with WP /= qe12 where
+ * qe12 = q*1e12 as the caller computes it. capture is [nlines][cap_samples][Nz]
+ * in `dtype`; row k maps to capture z index k + k0; dz is indexed from 0 (as
+ * the reference). */
 int wf_wake_wp(const void *capture, int32_t dtype, int32_t nlines, int64_t cap_samples,
                int32_t Nz, int32_t k0, int32_t nz, const double *z, const double *dz,
-               const double *s, int64_t ns, double v, double zmin_v, double t0,
-               double ti, double dt, int64_t nt, double qe12, double *WP3d,
-               void *stream);
+               const double *s, int64_t ns, double v, double zref_v, double t0,
+               double ti, double dt, int64_t nt, int64_t ncols, int32_t counter_moving,
+               double qe12, double *WP3d, void *stream);
 
 /* Transverse wake potential (WakeSolver.calc_trans_WP, wakeSolver.py:556-629):
  * int_WP[l][n] = numpy-sum(WP3d[l][0:n]) * ds with numpy's chunked pairwise
@@ -309,7 +317,7 @@ typedef struct wf_adjoint_args {
                              the region at cell 1 or later in x and y                    */
   double *W2, *lH2;       /* fused: output buffers, 3N each                               */
   int32_t fields_f32;     /* fused only: W, lH, W2, lH2 are FP32 arrays (stored FP32,
-                             arithmetic FP64); 0: FP64 arrays                            */
+                             arithmetic FP64 unless compute_f32); 0: FP64 arrays         */
   void *reg_W;            /* fused only, optional: [slots][3][region cells] in the type of
                              W; slot reg_slot receives the region's W read by this step   */
   double *reg_lJ;         /* with reg_W: [slots][3][region cells]; slot reg_slot receives
@@ -328,6 +336,9 @@ typedef struct wf_adjoint_args {
   const void *code, *mat;
   int32_t nmat, code_escape;
   const void *code_metric2d;
+  int32_t compute_f32;    /* fused only, with fields_f32: the step's arithmetic and LDS
+                             staging in FP32 instead of FP64 (lJ_region, reg_lJ and the
+                             gradients stay FP64)                                        */
 } wf_adjoint_args;
 
 /* Reverse of step n of SolverFIT3D._one_step (solverFIT3D.py:482-503): maps
@@ -369,8 +380,8 @@ int wf_adjoint_grad(wf_handle *h, const wf_adjoint_grad_args *a, void *stream);
  * steps start later). args[i] as for wf_adjoint_step with the fused kernel and
  * the region slots (fused = 1, W/lH and W2/lH2, reg_W/reg_lJ with reg_slots >=
  * nsteps[i], g_ieps = NULL); cap_sample and reg_slot are set per step. Even r
- * reads W/lH and writes W2/lH2, odd r the reverse. dtype and fields_f32 must
- * match across the batch. The gradients are accumulated afterwards by
+ * reads W/lH and writes W2/lH2, odd r the reverse. dtype, fields_f32 and
+ * compute_f32 must match across the batch. The gradients are accumulated afterwards by
  * wf_adjoint_grad, one call per grid. */
 typedef struct wf_batch_adjoint wf_batch_adjoint;
 int wf_batch_adjoint_create(wf_handle *const *handles, const wf_adjoint_args *args,

@@ -37,18 +37,36 @@ def _line_integral(WP, Ezt, z, s, v, zmin, t0, ti, dt, dz, nt):
     return WP
 
 
-def long_WP_3d(cap, q, v, ti, wakelength, skip_cells=0, n_transverse_cells=1):
-    """``calc_long_WP_3d`` (wakeSolver.py:422-554), non counter-moving branch.
+def _line_integral_counter_moving(WP, Ezt, z, s, v, zmax, t0, ti, dt, dz, nt):
+    """In-place ``WP[n] += Ezt[-k-1, it]*(-1*dz[k])`` over k: the counter-moving
+    branch of wakeSolver.py:503-517, with
+    ``ts = (z[-k-1] - s)/(-1*v) - zmax/(-1*v) - t0 + ti``."""
+    nz = len(z)
+    for k in range(nz):
+        ts = (z[-k - 1] - s) / (-1 * v) - zmax / (-1 * v) - t0 + ti
+        it = (ts / dt).astype(np.int64)
+        valid = it < nt
+        idx = it[valid]
+        idx = np.where(idx < 0, idx + Ezt.shape[1], idx)
+        WP[valid] = WP[valid] + Ezt[-k - 1, idx] * (-1 * dz[k])
+    return WP
+
+
+def long_WP_3d(cap, q, v, ti, wakelength, skip_cells=0, n_transverse_cells=1,
+               counter_moving=False):
+    """``calc_long_WP_3d`` (wakeSolver.py:422-554), both branches.
 
     ``cap['Ez']`` has shape (nsamples, 3, 3, Nz). The running WP is zeroed
     once and carried from one transverse line to the next (wakeSolver.py:485,
-    536-537), exactly as the reference does.
+    536-537), exactly as the reference does. ``counter_moving`` integrates
+    along -z from zmax (wakeSolver.py:503-517).
     """
     t = cap["t"]
     Ez = cap["Ez"]
     nt = len(t)
     dt = t[2] - t[1]
     zf = cap["zf"]
+    zmax = np.max(zf)
     zmin = np.min(zf)
     zz = slice(skip_cells, -skip_cells) if skip_cells != 0 else np.s_[:]
     z = zf[zz]
@@ -63,29 +81,56 @@ def long_WP_3d(cap, q, v, ti, wakelength, skip_cells=0, n_transverse_cells=1):
         for j in range(-j0, j0 + 1):
             Ezt[:, :nt] = np.asarray(Ez[:nt, Ez.shape[1] // 2 + i, Ez.shape[2] // 2 + j, zz],
                                      dtype=np.float64).T
-            WP = _line_integral(WP, Ezt, z, s, v, zmin, t[0], ti, dt, cap["dz"], nt)
+            if counter_moving:
+                WP = _line_integral_counter_moving(WP, Ezt, z, s, v, zmax, t[0], ti, dt,
+                                                   cap["dz"], nt)
+            else:
+                WP = _line_integral(WP, Ezt, z, s, v, zmin, t[0], ti, dt, cap["dz"], nt)
             WP = WP / (q * 1e12)
             WP_3d[i0 + i, j0 + j, :] = WP
     return s, WP_3d[i0, j0, :].copy(), WP_3d
 
 
 def long_WP_1d(cap, q, v, ti, wakelength, skip_cells=0):
-    """``calc_long_WP`` 1-D branch (wakeSolver.py:310-420, lines 390-392)."""
+    """``calc_long_WP`` (wakeSolver.py:310-420) on a capture of Ez samples.
+
+    A line capture (nsamples, Nz) is assigned whole (lines 390-392, so
+    ``skip_cells`` gives numpy's broadcast error there); a window capture
+    (nsamples, nx, ny, Nz) reads the line [nx//2 + 1, ny//2 + 1, zz]
+    (lines 385-388: for a 3x3 window the corner line at (+1, +1), not the
+    centre). Ezt has min(len(t), nsamples) columns (allocated after that min,
+    line 382), which is where a negative time index wraps.
+    """
     t = cap["t"]
     Ez = cap["Ez"]
     nt = len(t)
-    dt = t[2] - t[1]
     zf = cap["zf"]
+    zz = slice(skip_cells, -skip_cells) if skip_cells != 0 else np.s_[:]
+    z = zf[zz]
+    nt = min(nt, Ez.shape[0])
+    Ezt = np.zeros((len(z), nt))
+    for n in range(nt):
+        if Ez.ndim == 4:
+            Ezt[:, n] = Ez[n, Ez.shape[1] // 2 + 1, Ez.shape[2] // 2 + 1, zz]
+        else:
+            Ezt[:, n] = Ez[n]  # the reference assigns the full line (no zz slicing)
+    return long_WP_from_Ezt(Ezt, t, zf, cap["dz"], q, v, ti, wakelength, skip_cells, nt)
+
+
+def long_WP_from_Ezt(Ezt, t, zf, dz, q, v, ti, wakelength, skip_cells=0, nt=None):
+    """The integral of ``calc_long_WP`` (wakeSolver.py:395-408) over an
+    (nz, ncols) array Ezt of Ez(x_test, y_test, z[zz], t) samples: terms with
+    it < nt, nt = len(t) unless given (a negative it wraps over the ncols
+    columns), then WP / (q*1e12)."""
+    nt = len(t) if nt is None else nt
+    dt = t[2] - t[1]
     zmin = np.min(zf)
     zz = slice(skip_cells, -skip_cells) if skip_cells != 0 else np.s_[:]
     z = zf[zz]
     s = np.arange(-ti * v, wakelength, dt * v)
-    nt = min(nt, Ez.shape[0])
-    Ezt = np.zeros((len(z), nt))
-    for n in range(nt):
-        Ezt[:, n] = Ez[n]  # the reference assigns the full line (no zz slicing)
     WP = np.zeros_like(s)
-    WP = _line_integral(WP, Ezt, z, s, v, zmin, t[0], ti, dt, cap["dz"], nt)
+    WP = _line_integral(WP, np.asarray(Ezt, dtype=np.float64), z, s, v, zmin, t[0], ti, dt,
+                        dz, nt)
     return s, WP / (q * 1e12)
 
 
@@ -144,14 +189,15 @@ def trans_Z(s, WPx, WPy, lambdas, v, fmax, samples=1001):
 
 
 def wake_solve(cap, q, sigmaz, beta, wakelength, skip_cells=0, fmax=None, ti=None,
-               compute_plane="both"):
+               compute_plane="both", counter_moving=False):
     """``WakeSolver.solve`` (wakeSolver.py:210-254) on an in-memory capture."""
     v = beta * c_light
     ti = wake_ti(sigmaz, beta, ti)
     fmax = v / (3.0 * sigmaz) if fmax is None else fmax
     out = {}
     if compute_plane.lower() in ("both", "transverse"):
-        s, WP, WP_3d = long_WP_3d(cap, q, v, ti, wakelength, skip_cells)
+        s, WP, WP_3d = long_WP_3d(cap, q, v, ti, wakelength, skip_cells,
+                                  counter_moving=counter_moving)
         WPx, WPy = trans_WP(s, WP_3d, cap["xf"], cap["yf"])
         lam = lambdas_analytic(s, sigmaz)
         f, Z, lambdaf = long_Z(s, WP, lam, v, fmax)

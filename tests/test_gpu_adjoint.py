@@ -82,14 +82,16 @@ def test_adjoint_matches_finite_differences(weights):
     np.testing.assert_allclose(gs, fd_sg, rtol=1e-3, atol=1e-3 * np.abs(fd_sg).max())
 
 
-def _torch_reference(er, sg, w_loss, w_band, s_dev):
+def _torch_reference(er, sg, w_loss, w_band, s_dev, dims=DIMS, region=REGION,
+                     wakelength=WAKELENGTH, ins_material=(2.0, 1.0, 0.5)):
     """The same run and loss as a plain FP64 torch-sparse leapfrog on the CPU."""
-    x, y, z, mask = cases.cavity_pipe_geometry(*DIMS)
-    ins = np.zeros(DIMS, dtype=bool)
-    (i0, i1), (j0, j1), (k0, k1) = REGION
+    x, y, z, mask = cases.cavity_pipe_geometry(*dims)
+    ins = np.zeros(dims, dtype=bool)
+    (i0, i1), (j0, j1), (k0, k1) = region
     ins[i0:i1, j0:j1, k0:k1] = True
     so = fo.OracleSolver(x, y, z, ["pec", "pec", "abc"], ["pec", "pec", "abc"], bg="pec",
-                         solids=[(mask, "vacuum"), (ins, [2.0, 1.0, 0.5])], dtype=np.float64)
+                         solids=[(mask, "vacuum"), (ins, list(ins_material))],
+                         dtype=np.float64)
     so.dt = np.float64(s_dev.dt)
     n3 = 3 * so.N
 
@@ -119,7 +121,7 @@ def _torch_reference(er, sg, w_loss, w_band, s_dev):
     w = s_dev.wake
     v = w.v
     beam = fo.OracleBeam(q=w.q, sigmaz=w.sigmaz, beta=w.beta)
-    tmax = (WAKELENGTH + w.ti * v + (so.z.max() - so.z.min())) / v
+    tmax = (wakelength + w.ti * v + (so.z.max() - so.z.min())) / v
     Nt = int(tmax / so.dt)
     E = torch.zeros(n3, dtype=torch.float64)
     H = torch.zeros_like(E)
@@ -140,7 +142,7 @@ def _torch_reference(er, sg, w_loss, w_band, s_dev):
     # WP line integrals with the carry, loss
     t = np.arange(0, Nt * so.dt, so.dt)
     dt_w = t[2] - t[1]
-    s_ax = np.arange(-w.ti * v, WAKELENGTH, dt_w * v)
+    s_ax = np.arange(-w.ti * v, wakelength, dt_w * v)
     zf = so.z[w.skip_cells:-w.skip_cells]
     dz = so.dz[:len(zf)]
     zmin = so.z.min()
@@ -359,28 +361,42 @@ def test_fused_reverse_kernel_rejects_low_face_region():
 
 
 def test_fp32_adjoint_fields_close_to_fp64():
-    """FP32 forward run: adjoint fields stored in FP32 (FP64 arithmetic) give the
-    gradient of FP64-stored adjoint fields to 1e-4 relative L2, and both are
-    within 1e-2 of the FP64 run's gradient (the FP32 forward's own error)."""
+    """FP32 forward run: adjoint fields stored in FP32, with FP64 or (the
+    default) FP32 arithmetic, give the gradient of FP64-stored adjoint fields
+    to 1e-4 relative L2, and all are within 1e-2 of the FP64 run's gradient
+    (the FP32 forward's own error)."""
     print("adjoint FP32 storage", flush=True)
     er, sg = _params(21)
     got = {}
-    for adt in ("float32", "float64"):
+    for adt, comp in (("float32", "float32"), ("float32", "float64"), ("float64", "float64")):
         s, adj = _setup(1.0, 1e-13, dtype=np.float32, adjoint_dtype=adt)
+        adj.adjoint_compute = comp
         adj.forward(er, sg)
-        got[adt] = tuple(g.cpu().numpy() for g in adj.backward())
+        got[adt, comp] = tuple(g.cpu().numpy() for g in adj.backward())
         assert adj.used_kernel == "fused" and adj.used_adjoint_dtype == adt
+        assert adj.used_adjoint_compute == comp
+    s, adj = _setup(1.0, 1e-13, dtype=np.float32)
+    adj.forward(er, sg)
+    default = tuple(g.cpu().numpy() for g in adj.backward())
+    assert adj.used_adjoint_compute == "float32"
     s, adj = _setup(1.0, 1e-13)
     adj.forward(er, sg)
     ref = tuple(g.cpu().numpy() for g in adj.backward())
     rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
     for k in range(2):
-        r32 = rel(got["float32"][k], got["float64"][k])
-        r64 = rel(got["float64"][k], ref[k])
-        print("grad", k, "fp32 storage vs fp64 storage", r32, "fp32 run vs fp64 run", r64,
-              flush=True)
-        assert r32 < 1e-4
-        assert r64 < 1e-2
+        assert np.array_equal(default[k], got["float32", "float32"][k])
+        base = got["float64", "float64"][k]
+        for key in (("float32", "float32"), ("float32", "float64")):
+            r32 = rel(got[key][k], base)
+            r64 = rel(got[key][k], ref[k])
+            print("grad", k, key, "vs fp64 storage", r32, "vs fp64 run", r64, flush=True)
+            assert r32 < 1e-4
+            assert r64 < 1e-2
+    with pytest.raises(ValueError):
+        s, adj = _setup(1.0, 1e-13, dtype=np.float32, adjoint_dtype="float64")
+        adj.adjoint_compute = "float32"
+        adj.forward(er, sg)
+        adj.backward()
 
 
 def test_lossless_adjoint_matches_finite_differences():
@@ -419,3 +435,42 @@ def test_lossless_adjoint_matches_finite_differences():
     assert np.abs(fd).max() > 0
     np.testing.assert_allclose([ge[c] for c in cells], fd, rtol=1e-3,
                                atol=1e-3 * np.abs(fd).max())
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_checkpoint_mode_with_live_wrap_terms(dtype):
+    """x faces periodic with material on them: the flat-index wrap terms are
+    live, so the run takes the exact general path, which steps every cell and
+    couples i = Nx-1 to i = 0 of the next row. A block two cells from the x-low
+    face must then get full-field checkpoints (a dependence-box restore would
+    leave stale values at the far x face that the wrap carries into the block's
+    neighbourhood); its gradients equal the trajectory mode's bitwise."""
+    import wakis_amd as wa
+    from wakis_amd.adjoint import WakeAdjoint
+    print("adjoint checkpoint with live wraps", np.dtype(dtype).name, flush=True)
+    dims, region = (16, 16, 40), ((2, 6), (6, 10), (17, 23))
+    x, y, z, _ = cases.cavity_pipe_geometry(*dims)
+    ins = np.zeros(dims, dtype=bool)
+    ins[tuple(slice(a, b) for a, b in region)] = True
+    out = {}
+    for mode in ("trajectory", "checkpoint"):
+        grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"ins": ins},
+                            stl_materials={"ins": [2.0, 1.0, 0.5]}, verbose=0)
+        wake = wa.WakeSolver(q=1e-9, sigmaz=10e-3, beta=1.0, skip_cells=2, save=False,
+                             verbose=0)
+        s = wa.SolverFIT3D(grid, wake=wake, bc_low=["periodic", "pec", "abc"],
+                           bc_high=["periodic", "pec", "abc"], use_stl=True, bg="vacuum",
+                           dtype=dtype, verbose=0, use_sibc=False, tau_clamp=False)
+        adj = WakeAdjoint(s, region, WAKELENGTH, w_loss=1.0, w_band=1e-13, mode=mode,
+                          segment=3)
+        er, sg = _params_of(region, seed=23)
+        L = adj.forward(er, sg)
+        assert adj.used_mode == mode
+        assert not s._wraps_dead() and s.last_path == "general"
+        if mode == "checkpoint":
+            assert adj._fwd["box"] is None
+        out[mode] = (L,) + tuple(g.cpu().numpy() for g in adj.backward())
+        assert adj.used_kernel == "two-pass"
+    (La, ea, sa), (Lb, eb, sb) = out["trajectory"], out["checkpoint"]
+    assert La == Lb and np.any(ea != 0) and np.any(sa != 0)
+    assert np.array_equal(ea, eb) and np.array_equal(sa, sb)

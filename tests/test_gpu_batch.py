@@ -95,3 +95,35 @@ def test_adjoint_batch_matches_single_problems(dtype, per_cell, use_graph, full_
         assert Lb == L
         assert gb[1] is None
         assert torch.equal(g[0], gb[0])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_batch_matches_the_oracle(dtype):
+    """Each grid of a batch (uneven sizes and step counts, the in-run tuner
+    choosing the tile shape) against the oracle's wakesolve of the same cavity:
+    E, H, J and WP_3d, WP_x, WP_y bit-identical, Z within 1e-12."""
+    from oracle import fit_oracle as fo
+    from oracle import wake_oracle as wo
+    from wakis_amd import batch
+    print("batch vs oracle", np.dtype(dtype).name, flush=True)
+    specs = batch.random_box_cavities(11, n=3, cell=2.0e-3)
+    solvers = [batch.cavity_solver(sp, dtype=dtype) for sp in specs]
+    wl = [0.015 + 0.01 * i for i in range(len(specs))]
+    batch.wakesolve_batch(solvers, wl, autotune=True, tune_steps=8)
+    q, sigmaz, beta = 1e-9, 5e-3, 1.0
+    ti = wo.wake_ti(sigmaz, beta)
+    for sp, s, w in zip(specs, solvers, wl):
+        so = fo.OracleSolver(sp["x"], sp["y"], sp["z"], ["pec", "pec", "abc"],
+                             ["pec", "pec", "abc"], bg="pec",
+                             solids=[(sp["vacuum"], "vacuum"), (sp["dielectric"],
+                                                                [sp["eps_r"], 1.0])],
+                             dtype=dtype)
+        cap = fo.wakesolve_fields(so, w, q, sigmaz, beta, 0.0, 0.0, 0.0, 0.0, ti)
+        ref = wo.wake_solve(cap, q, sigmaz, beta, w, skip_cells=4)
+        assert s.Nt == cap["Nt"] and s.dt == so.dt
+        for name in ("E", "H", "J"):
+            got, r = getattr(s, name).to_numpy(), getattr(so, name)
+            assert np.array_equal(got, r), f"{name}: rel-L2 {gc.rel_l2(got, r):.3e}"
+        assert np.array_equal(s.wake.WP_3d, ref["WP_3d"])
+        assert np.array_equal(s.wake.WPx, ref["WPx"]) and np.array_equal(s.wake.WPy, ref["WPy"])
+        assert np.max(np.abs(s.wake.Z - ref["Z"])) <= 1e-12 * np.max(np.abs(ref["Z"]))

@@ -138,3 +138,34 @@ def test_material_table_on_random_grid(host_only):
     sa.ieps.array[0] += 1.0  # edit without update_tensors -> arrays again
     sa.ieps.version += 1
     assert not sa._material_table()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_adjoint_kernel_choice_past_4gib(host_only, monkeypatch, dtype):
+    """Reverse-kernel choice when 3N doubles pass the fused kernel's 4 GiB of
+    32-bit offsets (the limit lowered to this grid's size): "auto" takes the
+    two-pass kernels with FP64 adjoint fields, also for FP32 solver fields;
+    kernel="fused" and FP32 adjoint fields without the fused kernel raise."""
+    from wakis_amd import adjoint as adjm
+    x, y, z, mask = cases.cavity_pipe_geometry(16, 16, 40)
+    ins = np.zeros((16, 16, 40), dtype=bool)
+    region = ((6, 10), (6, 10), (17, 23))
+    ins[tuple(slice(a, b) for a, b in region)] = True
+    grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"cav": mask, "ins": ins},
+                        stl_materials={"cav": "vacuum", "ins": [2.0, 1.0, 0.5]}, verbose=0)
+    wake = wa.WakeSolver(q=1e-9, sigmaz=10e-3, beta=1.0, save=False, verbose=0)
+    s = wa.SolverFIT3D(grid, wake=wake, bc_low=["pec", "pec", "abc"],
+                       bc_high=["pec", "pec", "abc"], use_stl=True, bg="pec", dtype=dtype,
+                       verbose=0, use_sibc=False, tau_clamp=False)
+    adj = adjm.WakeAdjoint(s, region, 0.03)
+    assert adj._use_fused_kernel()
+    assert adj._adjoint_f32(True) == (dtype == np.float32)
+    monkeypatch.setattr(adjm, "FUSED_MAX_BYTES", 3 * s.N * 8)  # this grid is one past it
+    assert not adj._use_fused_kernel()
+    assert adj._adjoint_f32(False) is False  # FP32 solver fields, FP64 adjoint fields
+    with pytest.raises(ValueError):
+        adjm.WakeAdjoint(s, region, 0.03, kernel="fused")._use_fused_kernel()
+    with pytest.raises(ValueError):
+        adjm.WakeAdjoint(s, region, 0.03, adjoint_dtype="float32")._adjoint_f32(False)
+    monkeypatch.setattr(adjm, "FUSED_MAX_BYTES", 3 * s.N * 8 + 1)
+    assert adjm.WakeAdjoint(s, region, 0.03, kernel="fused")._use_fused_kernel()

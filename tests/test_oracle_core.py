@@ -282,6 +282,101 @@ def test_beam_line_integrates_to_q():
     assert so.z[np.argmax(Jz)] == pytest.approx(0.0, abs=so.dz[0])
 
 
+@pytest.mark.parametrize("bcs", [
+    (["abc", "pec", "pec"], ["abc", "pec", "pec"]),          # x high ABC
+    (["pec", "periodic", "abc"], ["pec", "periodic", "pec"]),  # y periodic, z low ABC only
+    (["pec", "pec", "pec"], ["pec", "pec", "abc"]),          # z high ABC
+    (["periodic", "pec", "periodic"], ["periodic", "pec", "periodic"]),
+])
+def test_high_face_metric_copies_closed_form(bcs):
+    """Periodic (both faces of an axis) and ABC (high face) set the dual length
+    of the last layer along axis a to the first primal length, and the dual
+    areas of the other two components there to the first layer's primal
+    inverse areas (boundaries.py:35-70, 152-181):
+    tL_a[-1] = d_a[0], itA_b[-1] = 1/(d_a[0] d_c), everything else as the
+    grid metrics give. Non-uniform axes, so d[0] differs from d[-1]."""
+    rng = np.random.default_rng(3)
+    nodes = [np.cumsum(np.r_[0.0, rng.uniform(0.5, 1.5, n)]) for n in (5, 4, 6)]
+    dims = (5, 4, 6)
+    d = [np.diff(a) for a in nodes]
+    L, iA, tL0, itA0 = fo.grid_metrics(*nodes)
+    tL, itA, abc, _ = fo.bc_metric_edits(bcs[0], bcs[1], dims, L, iA, tL0, itA0)
+    assert abc == any(b == "abc" for b in bcs[0] + bcs[1])
+    copied = [bcs[1][a] == "abc" or (bcs[0][a] == "periodic" and bcs[1][a] == "periodic")
+              for a in range(3)]
+    exp_tL = [fo.comp(tL0, dims, c).copy() for c in range(3)]
+    exp_itA = [fo.comp(itA0, dims, c).copy() for c in range(3)]
+    for a in range(3):
+        if not copied[a]:
+            continue
+        hi = [slice(None)] * 3
+        hi[a] = -1
+        hi = tuple(hi)
+        exp_tL[a][hi] = d[a][0]
+        for b in range(3):
+            if b == a:
+                continue
+            c = 3 - a - b  # the third axis: the face of component b spans axes a and c
+            sh = [1, 1]
+            sh[c if c < a else c - 1] = dims[c]  # position of axis c in the slab at a = -1
+            area = d[a][0] * d[c]
+            exp_itA[b][hi] = np.broadcast_to((1.0 / area).reshape(sh), exp_itA[b][hi].shape)
+    for c in range(3):
+        np.testing.assert_allclose(fo.comp(tL, dims, c), exp_tL[c], rtol=1e-15, atol=0)
+        np.testing.assert_allclose(fo.comp(itA, dims, c), exp_itA[c], rtol=1e-14, atol=0)
+    # low-face ABC alone copies nothing
+    if bcs == (["pec", "periodic", "abc"], ["pec", "periodic", "pec"]):
+        assert np.array_equal(fo.comp(tL, dims, 2), fo.comp(tL0, dims, 2))
+
+
+def test_pml_eps_and_sigma_layers_closed_form():
+    """PML layers (boundaries.py:190-311) on a z-layered lossy dielectric
+    (eps_r = 1 + k, sigma = 0.15 k S/m in layer k, varying across x so that the
+    face-centre sample matters): the n low layers take the values of layer
+    n + 1 at (Nx//2, Ny//2) (layer n keeps its own), the n high layers those of
+    layer Nz - n - 1; sigma = sigma0 + max(1, sigma0) * geomspace(10, 5e-3, n)
+    on the low side and geomspace(5e-3, 10, n) on the high side."""
+    from scipy.constants import epsilon_0 as eps_0
+    Nx, Ny, Nz, n = 6, 5, 14, 3
+    x, y, z = fo.uniform_axes(0, 1, 0, 1, 0, 2, Nx, Ny, Nz)
+    solids = []
+    for k in range(Nz):
+        for i in range(Nx):
+            m = np.zeros((Nx, Ny, Nz), dtype=bool)
+            m[i, :, k] = True
+            solids.append((m, [1.0 + k + 0.25 * i, 1.0, 0.15 * k + 0.01 * i]))
+    so = fo.OracleSolver(x, y, z, ["pec", "pec", "pml"], ["pec", "pec", "pml"],
+                         solids=solids, n_pml=n)
+    ic, jc = Nx // 2, Ny // 2
+
+    def eps_r(i, k):
+        return 1.0 + k + 0.25 * i
+
+    def sig(i, k):
+        return 0.15 * k + 0.01 * i
+
+    prof_lo = np.geomspace(10.0, 5e-3, n)
+    prof_hi = np.geomspace(5e-3, 10.0, n)
+    for c in range(3):
+        ie = fo.comp(so.ieps, so.dims, c)
+        sg = fo.comp(so.sigma, so.dims, c)
+        s_lo, s_hi = sig(ic, n + 1), sig(ic, Nz - n - 1)
+        assert s_lo < 1 < s_hi  # both branches of max(1, sigma0)
+        for m in range(n):
+            np.testing.assert_allclose(ie[:, :, m], 1.0 / (eps_r(ic, n + 1) * eps_0), rtol=1e-15)
+            np.testing.assert_allclose(sg[:, :, m], s_lo + 1.0 * prof_lo[m], rtol=1e-15)
+            kh = Nz - 1 - m
+            np.testing.assert_allclose(ie[:, :, kh], 1.0 / (eps_r(ic, Nz - n - 1) * eps_0),
+                                       rtol=1e-15)
+            np.testing.assert_allclose(sg[:, :, kh], s_hi + s_hi * prof_hi[n - 1 - m],
+                                       rtol=1e-15)
+        # layer n and the layers inside keep their own values
+        for k in range(n, Nz - n):
+            for i in range(Nx):
+                np.testing.assert_allclose(ie[i, :, k], 1.0 / (eps_r(i, k) * eps_0), rtol=1e-15)
+                np.testing.assert_allclose(sg[i, :, k], sig(i, k), rtol=1e-15)
+
+
 def test_pml_profile_closed_form():
     x, y, z = fo.uniform_axes(0, 1, 0, 1, 0, 1, 6, 6, 40)
     n = 8
@@ -342,6 +437,86 @@ def test_wake_impulse_lands_at_its_retarded_position():
     lo = v * (m0 * dt - ti) + zmin - cap["zf"][k0]
     assert lo - 1e-9 <= s[hits[0]] < lo + v * dt + 1e-9
     assert WP[hits[0]] == pytest.approx(cap["dz"][k0] / (q * 1e12), rel=1e-14)
+
+
+def test_wake_counter_moving_constant_field_closed_form():
+    """Counter-moving (wakeSolver.py:503-517): a constant Ez = E0 gives
+    -E0 * sum(dz) / (q*1e12) for every s whose samples all lie in the record,
+    with the line-to-line carry; the time of row z is (zmax - z + s)/v + ti."""
+    cap = _synthetic_capture(nt=1000)
+    E0 = 3.0
+    cap["Ez"][:] = E0
+    q, v, ti = 1e-9, c_light, 2e-11
+    s, WP, WP3 = wo.long_WP_3d(cap, q, v, ti, wakelength=0.01, counter_moving=True)
+    Q = q * 1e12
+    dt = cap["t"][2] - cap["t"][1]
+    zmax = cap["zf"].max()
+    full = np.array([np.all(((zmax - cap["zf"] + sn) / v + ti) / dt < len(cap["t"]))
+                     for sn in s])
+    assert full.sum() > 10
+    S = -E0 * np.sum(cap["dz"])
+    np.testing.assert_allclose(WP3[0, 0, full], S / Q, rtol=1e-13)
+    np.testing.assert_allclose(WP3[0, 1, full], (S / Q + S) / Q, rtol=1e-13)
+
+
+def test_wake_counter_moving_impulse_position_and_weight():
+    """An impulse at (z_k, t_m) reaches the one s with
+    (zmax - z_k + s)/v + ti in [t_m, t_m+1); its weight is -dz[nz-1-k] (the
+    reference weights row -k-1 with dz[k]), distinct from -dz[k] on this
+    non-uniform axis."""
+    rng = np.random.default_rng(4)
+    Nz, nt = 40, 300
+    zn = np.cumsum(np.r_[-0.02, rng.uniform(0.5e-3, 1.5e-3, Nz)])
+    cap = _synthetic_capture(Nz=Nz, nt=nt)
+    cap["zf"], cap["dz"] = zn[:-1] + np.diff(zn) / 2, np.diff(zn)
+    k0, m0 = 11, 180
+    cap["Ez"][m0, 1, 1, k0] = 1.0
+    q, v, ti = 1e-9, c_light, 2e-11
+    s, WP, _ = wo.long_WP_3d(cap, q, v, ti, wakelength=0.05, counter_moving=True)
+    hits = np.flatnonzero(WP)
+    assert len(hits) == 1
+    dt = cap["t"][2] - cap["t"][1]
+    lo = v * (m0 * dt - ti) - cap["zf"].max() + cap["zf"][k0]
+    assert lo - 1e-9 <= s[hits[0]] < lo + v * dt + 1e-9
+    w = -cap["dz"][Nz - 1 - k0] / (q * 1e12)
+    assert WP[hits[0]] == pytest.approx(w, rel=1e-14)
+    assert abs(cap["dz"][k0] - cap["dz"][Nz - 1 - k0]) > 1e-5
+
+
+def test_long_wp_of_a_window_reads_the_plus_one_line():
+    """calc_long_WP on a 3x3 window integrates line [2, 2]
+    (Ez.shape//2 + 1, wakeSolver.py:385-388): constant fields of distinct
+    value per line give that line's closed form."""
+    cap = _synthetic_capture(nt=1000)
+    vals = np.arange(1.0, 10.0).reshape(3, 3)
+    cap["Ez"][:] = vals[None, :, :, None]
+    q, v, ti = 1e-9, c_light, 2e-11
+    s, WP = wo.long_WP_1d(cap, q, v, ti, wakelength=0.01)
+    dt = cap["t"][2] - cap["t"][1]
+    zmin = cap["zf"].min()
+    full = np.array([np.all(((cap["zf"] + sn) / v - zmin / v + ti) / dt < len(cap["t"]))
+                     for sn in s])
+    assert full.sum() > 10
+    np.testing.assert_allclose(WP[full], vals[2, 2] * np.sum(cap["dz"]) / (q * 1e12),
+                               rtol=1e-13)
+
+
+def test_long_wp_from_ezt_equals_the_line_capture():
+    """The integral over a given Ezt equals calc_long_WP over a one-line
+    capture holding the same samples, and a constant Ezt gives the closed form."""
+    rng = np.random.default_rng(2)
+    cap = _synthetic_capture(nt=400)
+    line = rng.standard_normal((400, cap["zf"].size))
+    q, v, ti = 1e-9, c_light, 2e-11
+    s1, WP1 = wo.long_WP_1d(dict(cap, Ez=line), q, v, ti, wakelength=0.02)
+    s2, WP2 = wo.long_WP_from_Ezt(line.T, cap["t"], cap["zf"], cap["dz"], q, v, ti, 0.02)
+    assert np.array_equal(s1, s2) and np.array_equal(WP1, WP2)
+    s3, WP3 = wo.long_WP_from_Ezt(np.full_like(line.T, 2.0), cap["t"], cap["zf"], cap["dz"], q,
+                                  v, ti, 0.02)
+    dt = cap["t"][2] - cap["t"][1]
+    zmin = cap["zf"].min()
+    full = np.array([np.all(((cap["zf"] + sn) / v - zmin / v + ti) / dt < 400) for sn in s3])
+    np.testing.assert_allclose(WP3[full], 2.0 * np.sum(cap["dz"]) / (q * 1e12), rtol=1e-13)
 
 
 def test_trans_wp_of_a_linear_wake_is_constant_slope():

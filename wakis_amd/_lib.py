@@ -60,7 +60,7 @@ class WfAdjointArgs(ctypes.Structure):
         ("reg_W", c_void_p), ("reg_lJ", c_void_p), ("reg_slot", c_int64),
         ("reg_slots", c_int64), ("kappa", c_void_p),
         ("code", c_void_p), ("mat", c_void_p), ("nmat", c_int32), ("code_escape", c_int32),
-        ("code_metric2d", c_void_p),
+        ("code_metric2d", c_void_p), ("compute_f32", c_int32),
     ]
 
 
@@ -88,8 +88,8 @@ EXPORTS = {
     "wf_abc_low": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
     "wf_wake_wp": (c_int32, [c_void_p, c_int32, c_int32, c_int64, c_int32, c_int32, c_int32,
                              c_void_p, c_void_p, c_void_p, c_int64, c_double, c_double,
-                             c_double, c_double, c_double, c_int64, c_double, c_void_p,
-                             c_void_p]),
+                             c_double, c_double, c_double, c_int64, c_int64, c_int32,
+                             c_double, c_void_p, c_void_p]),
     "wf_wake_trans": (c_int32, [c_void_p, c_int64, c_double, c_double, c_double, c_void_p,
                                 c_void_p, c_void_p]),
     "wf_dft": (c_int32, [c_void_p, c_int32, c_int64, c_int64, c_int64, c_void_p, c_void_p]),

@@ -23,10 +23,11 @@ the steps back without recomputing anything ("trajectory" mode). When the
 trajectory would not fit in half of the free device memory, or the block
 touches a low face of the grid, checkpoints of (E, H, J) are kept instead and
 each segment is recomputed from its checkpoint before it is reversed
-("checkpoint" mode): for a block off the low faces, segments of
-max(8, sqrt(Nt)/4) steps with checkpoints and recompute confined to the
-block's domain of dependence over a segment (a value moves at most one cell
-per step); otherwise full-field checkpoints every sqrt(Nt) steps.
+("checkpoint" mode): for a block off the low faces whose run takes the fused
+step (flat-index wrap terms shown dead), segments of max(8, sqrt(Nt)/4) steps
+with checkpoints and recompute confined to the block's domain of dependence
+over a segment (a value moves at most one cell per step); otherwise
+full-field checkpoints every sqrt(Nt) steps.
 ``wake_loss`` wraps it as a torch.autograd.Function.
 
 The time step must not depend on sigma: build the solver with
@@ -43,10 +44,14 @@ from scipy.constants import epsilon_0 as eps_0
 
 from . import _lib
 
+# The fused reverse kernel addresses 3N-double arrays with 32-bit byte offsets.
+FUSED_MAX_BYTES = 2 ** 32
+
 
 class WakeAdjoint:
     def __init__(self, solver, region, wakelength, f_band=3e9, w_loss=1.0, w_band=1.0,
-                 segment=None, mode="auto", kernel="auto", adjoint_dtype="auto"):
+                 segment=None, mode="auto", kernel="auto", adjoint_dtype="auto",
+                 adjoint_compute="auto"):
         """``region`` = ((i0, i1), (j0, j1), (k0, k1)) cells, half-open.
         ``mode``: "trajectory", "checkpoint" or "auto" (trajectory when it fits).
         ``kernel``: "fused" (one single-pass reverse kernel per step), "two-pass"
@@ -54,8 +59,17 @@ class WakeAdjoint:
         fused forward step, the region starts at cell 1 or later in x and y, and
         3N doubles fit in 4 GiB).
         ``adjoint_dtype``: storage of the adjoint fields, "float64", "float32"
-        (fused kernel only; arithmetic stays FP64) or "auto" (float32 with the
-        fused kernel when the solver's fields are FP32, else float64)."""
+        (fused kernel only) or "auto" (float32 with the fused kernel when the
+        solver's fields are FP32, else float64).
+        ``adjoint_compute``: arithmetic of the reverse step, "float64", or
+        "float32" (FP32 adjoint fields only) or "auto" (the storage type of the
+        adjoint fields); the adjoint of J on the block and the parameter
+        gradients are accumulated in FP64 either way."""
+        if adjoint_compute not in ("auto", "float32", "float64"):
+            raise ValueError("WakeAdjoint: adjoint_compute must be 'auto', 'float32' or "
+                             "'float64'")
+        self.adjoint_compute = adjoint_compute
+        self.used_adjoint_compute = None
         if adjoint_dtype not in ("auto", "float32", "float64"):
             raise ValueError("WakeAdjoint: adjoint_dtype must be 'auto', 'float32' or 'float64'")
         self.adjoint_dtype = adjoint_dtype
@@ -75,6 +89,9 @@ class WakeAdjoint:
         s = solver
         if s.wake is None:
             raise ValueError("WakeAdjoint: the solver needs a WakeSolver")
+        if s.wake.counter_moving:
+            raise NotImplementedError("WakeAdjoint: the counter-moving wake integral has no "
+                                      "reverse mode")
         # sigma is a parameter only on a conductive solver (a conductive material,
         # background or PML); otherwise the gradient is with respect to eps_r alone
         self.conductive = bool(s.use_conductivity)
@@ -167,11 +184,18 @@ class WakeAdjoint:
         else:
             root = int(math.ceil(math.sqrt(max(Nt, 1))))
             interior = min(self.i0, self.j0, self.k0) >= 1
-            # interior block: each checkpoint keeps only the block's domain of
-            # dependence over one segment, so shorter segments (sqrt(Nt)/4)
-            # shrink both the recompute and the checkpoints
-            K = self.segment or (max(8, root // 4) if interior else root)
-            box = self._dep_box(K) if interior else None
+            # interior block whose recompute runs on the fused path: each
+            # checkpoint keeps only the block's domain of dependence over one
+            # segment, so shorter segments (sqrt(Nt)/4) shrink both the
+            # recompute and the checkpoints. The fused path has shown the
+            # flat-index wrap terms dead and launches only the tiles that meet
+            # the box; the exact general path steps every cell, and its live
+            # wrap terms (x/y periodic, PMC or PML faces with material on them)
+            # carry values from the far x/y faces into the block's
+            # neighbourhood in one step, so it needs full-field checkpoints.
+            local = interior and s._choose_fused(s._wake_args(ctx, 0, 1))
+            K = self.segment or (max(8, root // 4) if local else root)
+            box = self._dep_box(K) if local else None
             ckpt = []
             for n0 in range(0, Nt, K):
                 ckpt.append((n0,) + tuple(
@@ -191,13 +215,22 @@ class WakeAdjoint:
             return False
         # the fused reverse kernel addresses its FP64-sized 3N arrays with 32-bit
         # byte offsets; larger grids take the two-pass kernels (64-bit indices)
-        small = 3 * self.s.N * 8 < 2 ** 32
+        small = 3 * self.s.N * 8 < FUSED_MAX_BYTES
         ok = small and self.i0 >= 1 and self.j0 >= 1 and self.s._wraps_dead()
         if self.kernel == "fused" and not ok:
             raise ValueError("WakeAdjoint: the fused reverse kernel needs dead flat-index wrap "
                              "terms, the region at cell 1 or later in x and y, and 3N "
                              "doubles within 4 GiB")
         return ok
+
+    def _adjoint_f32(self, adj_fused):
+        """Whether the adjoint fields are stored in FP32: asked for, or "auto"
+        with the fused reverse kernel on a solver with FP32 fields."""
+        f32 = self.adjoint_dtype == "float32" or (self.adjoint_dtype == "auto" and adj_fused
+                                                   and self.s.E.array.dtype == torch.float32)
+        if f32 and not adj_fused:
+            raise ValueError("WakeAdjoint: FP32 adjoint fields need the fused reverse kernel")
+        return f32
 
     @property
     def used_mode(self):
@@ -283,11 +316,12 @@ class WakeAdjoint:
         n3 = 3 * s.N
         adj_fused = self._use_fused_kernel()
         self.used_kernel = "fused" if adj_fused else "two-pass"
-        f32 = self.adjoint_dtype == "float32" or (self.adjoint_dtype == "auto" and adj_fused
-                                                   and s.E.array.dtype == torch.float32)
-        if f32 and not adj_fused:
-            raise ValueError("WakeAdjoint: FP32 adjoint fields need the fused reverse kernel")
+        f32 = self._adjoint_f32(adj_fused)
         self.used_adjoint_dtype = "float32" if f32 else "float64"
+        c32 = f32 and self.adjoint_compute in ("auto", "float32")
+        if self.adjoint_compute == "float32" and not f32:
+            raise ValueError("WakeAdjoint: FP32 arithmetic needs FP32 adjoint fields")
+        self.used_adjoint_compute = "float32" if c32 else "float64"
         adt = torch.float32 if f32 else torch.float64
         W = torch.zeros(n3, dtype=adt, device=dev)
         lH = torch.zeros_like(W)
@@ -321,6 +355,7 @@ class WakeAdjoint:
             a.fused = int(self.fused_rows) or 1
             a.W2, a.lH2 = _lib.ptr(W2), _lib.ptr(lH2)
             a.fields_f32 = int(f32)
+            a.compute_f32 = int(c32)
             # the forward's material codes, when it has them: one byte per cell
             # instead of the coefficient arrays (the same values)
             if self.use_codes and s.use_material_code and s._material_table():
@@ -496,6 +531,9 @@ class WakeAdjoint:
             m = k + 3  # within the restored box (k <= K)
             args.box_lo[:] = tuple(int(v) - m for v in lo)
             args.box_hi[:] = tuple(int(v) + m for v in hi)
+            if box is not None and not s._choose_fused(args):
+                raise RuntimeError("WakeAdjoint: a box checkpoint must be recomputed on the "
+                                   "fused path (the general path steps every cell)")
             s._run(args)
             if adj_fused:
                 rJ[k].copy_(carry)

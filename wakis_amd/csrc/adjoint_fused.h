@@ -1,16 +1,17 @@
 // Single-pass reverse leapfrog step (the two kernels k_adj_h and k_adj_e of
 // adjoint.h in one z-march over the grid).
 //
-// Reverse step n, P = b.W and Q = a.lH', arithmetic in FP64; W and lH are
-// stored as A (FP64, or FP32 to halve their traffic when the forward run is
-// FP32):
+// Reverse step n, P = b.W and Q = a.lH', arithmetic and LDS staging in C
+// (FP64; FP32 when the fields are stored FP32 and the caller asks for FP32
+// arithmetic); W and lH are stored as A (FP64, or FP32 to halve their traffic
+// when the forward run is FP32); lJ and the gradient slots stay FP64:
 //   lH' = lH + dt tL . (C P)            (PMC-masked rows keep lH)
 //   W'  = W - dt pec . L . (C^T Q) [+ capture gradient] [+ sigma lJ]
 //   lJ  = -dt (1 - beam line) ieps . W  (conductive; stored on the region)
 // C P uses forward differences of P (x+1, y+1, z+1), C^T Q backward
 // differences of Q (x-1, y-1, z-1); the terms are summed in the order of
 // h_terms / e_terms, so with A = double the results equal the two-kernel path
-// bit for bit. The flat-index wrap couplings are dropped: valid when their
+// bit for bit (C = double). The flat-index wrap couplings are dropped: valid when their
 // operator entries are zero or act on frozen zero fields (the condition of the
 // fused forward step), with the gradient region away from the low x and y
 // faces. Offsets are 32-bit byte offsets (3N * sizeof(double) < 4 GiB).
@@ -78,7 +79,7 @@ struct AdjStep {
 // material table exactly as the forward step folds them (a = fl(iA imu),
 // b = fl(ieps itA), kappa = fl((-dt ieps) sigma) in FP64, zero on the beam
 // line's z edges), reading one byte per cell instead of nine values.
-template <typename S, typename A, int TY, bool LUT>
+template <typename S, typename A, int TY, bool LUT, typename C>
 __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int tile) {
   const GridDev &g = p.g;
   const MetricDev<S> &m = p.m;
@@ -92,7 +93,8 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
   const S *__restrict__ ieps = p.ieps;
   const S *__restrict__ sigma = p.sigma;
   const S *__restrict__ kappa = p.kappa;
-  const double dt = p.dt;
+  const double dtd = p.dt;  // FP64 dt: the host's fold of kappa
+  const C dt = (C)dtd;
   const AdjAux &x = p.x;
   const double *__restrict__ gcap = p.gcap;
   const AdjTile &t = p.t;
@@ -103,8 +105,8 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
   using I = uint32_t;
   constexpr int NW = TY + 2;              // row waves
   constexpr int NS = ADJ_TX + 3;          // slots: 0 = i0-1, 1..64 cells, 65 = i0+64, 66 spare
-  __shared__ double sP[3][NW + 1][NS];    // row NW: zeros (above the top row)
-  __shared__ double sQ[3][NW][NS];
+  __shared__ C sP[3][NW + 1][NS];    // row NW: zeros (above the top row)
+  __shared__ C sQ[3][NW][NS];
 
   const int tz = tile / (t.ntx * t.nty);
   const int rem = tile - tz * t.ntx * t.nty;
@@ -130,15 +132,15 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
   const I N = (I)g.N, sz = (I)g.sz;
   const I col = valid ? (I)i + (I)g.sy * (I)j : 0;
   for (int u = threadIdx.x; u < 3 * (NW + 1) * NS; u += 64 * (NW + 1))
-    (&sP[0][0][0])[u] = 0.0;
+    (&sP[0][0][0])[u] = C(0);
   extern __shared__ unsigned char s_dyn[];
   S *sMat = reinterpret_cast<S *>(s_dyn);
   if (LUT) {
     for (int u = threadIdx.x; u < p.nmat * 9; u += 64 * (NW + 1)) sMat[u] = p.mat[u];
   }
 
-  const double tLx = valid ? (double)m.tLx[i] : 0.0, tLy = valid ? (double)m.tLy[j] : 0.0;
-  const double Lx = valid ? (double)m.Lx[i] : 0.0, Ly = valid ? (double)m.Ly[j] : 0.0;
+  const C tLx = valid ? (C)m.tLx[i] : C(0), tLy = valid ? (C)m.tLy[j] : C(0);
+  const C Lx = valid ? (C)m.Lx[i] : C(0), Ly = valid ? (C)m.Ly[j] : C(0);
   const bool online = own && (i == x.ixs) && (j == x.iys);
   const bool capcol = gcap && own && x.cap_sample < x.cap_samples && i >= x.cap_ix0 &&
                       i < x.cap_ix0 + x.cap_nx && j >= x.cap_iy0 && j < x.cap_iy0 + x.cap_ny;
@@ -208,9 +210,9 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
     if (!LUT || esc_at(cd)) return arr;
     if (!kap) return sMat[9 * cd + 6 + c];
     if (c == 2 && onl) return S(0);
-    return (S)((-dt * (double)sMat[9 * cd + 3 + c]) * (double)sMat[9 * cd + 6 + c]);
+    return (S)((-dtd * (double)sMat[9 * cd + 3 + c]) * (double)sMat[9 * cd + 6 + c]);
   };
-  double Qpx = 0, Qpy = 0;  // Q(k-1) of this cell, x and y components
+  C Qpx = 0, Qpy = 0;  // Q(k-1) of this cell, x and y components
   __syncthreads();          // zeroed LDS
 
   for (int k = kstart; k < kend; ++k) {
@@ -246,74 +248,73 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
       }
     }
     // ---- stage P(k)
-    double Pk[3];
+    C Pk[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-      Pk[c] = valid ? (double)bval(c, ck, bxk, byk, bk[c]) * (double)Wk[c] : 0.0;
+      Pk[c] = valid ? (C)bval(c, ck, bxk, byk, bk[c]) * (C)Wk[c] : C(0);
     if (inrow) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) sP[c][row][slot] = Pk[c];
     }
-    const double tLz = (double)m.tLz[k], Lz = (double)m.Lz[k];
+    const C tLz = (C)m.tLz[k], Lz = (C)m.Lz[k];
     __syncthreads();
 
     // ---- lH'(k) and Q(k) (row waves: all 64 columns; halo wave: column i0-1)
-    double hn[3] = {0, 0, 0};
+    C hn[3] = {0, 0, 0};
     if (makes_q) {
-      double Pk1[3];
+      C Pk1[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        Pk1[c] = (valid && kp1) ? (double)bval(c, ck1, bxk1, byk1, bk1[c]) * (double)Wk1[c]
-                                : 0.0;
+        Pk1[c] = (valid && kp1) ? (C)bval(c, ck1, bxk1, byk1, bk1[c]) * (C)Wk1[c] : C(0);
       // h_terms order: x: +Py, -Py(z+1), -Pz, +Pz(y+1); y: -Px, +Px(z+1), +Pz, -Pz(x+1);
       // z: +Px, -Px(y+1), -Py, +Py(x+1)
-      double s0 = 0.0;
+      C s0 = 0;
       s0 = s0 + Pk[1]; s0 = s0 - Pk1[1]; s0 = s0 - Pk[2]; s0 = s0 + sP[2][row + 1][slot];
-      double s1 = 0.0;
+      C s1 = 0;
       s1 = s1 - Pk[0]; s1 = s1 + Pk1[0]; s1 = s1 + Pk[2]; s1 = s1 - sP[2][row][slot + 1];
-      double s2 = 0.0;
+      C s2 = 0;
       s2 = s2 + Pk[0]; s2 = s2 - sP[0][row + 1][slot]; s2 = s2 - Pk[1];
       s2 = s2 + sP[1][row][slot + 1];
-      const double h0 = (double)lh[0], h1 = (double)lh[1], h2 = (double)lh[2];
+      const C h0 = (C)lh[0], h1 = (C)lh[1], h2 = (C)lh[2];
       hn[0] = keep_row(g.pmcz, 0, i, j, k, g) ? h0 + dt * (tLx * s0) : h0;
       hn[1] = keep_row(g.pmcz, 1, i, j, k, g) ? h1 + dt * (tLy * s1) : h1;
       hn[2] = keep_row(g.pmcz, 2, i, j, k, g) ? h2 + dt * (tLz * s2) : h2;
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        sQ[c][row][slot] = valid ? (double)aval(c, ck, axk, ayk, ac[c]) * hn[c] : 0.0;
+        sQ[c][row][slot] = valid ? (C)aval(c, ck, axk, ayk, ac[c]) * hn[c] : C(0);
     }
     __syncthreads();
 
     // ---- W'(k) of the owned cells on body planes
-    double Qx = 0.0, Qy = 0.0;
+    C Qx = 0, Qy = 0;
     if (!halo) { Qx = sQ[0][row][slot]; Qy = sQ[1][row][slot]; }
     if (own && body) {
-      const double Qz = sQ[2][row][slot];
+      const C Qz = sQ[2][row][slot];
       // e_terms order: x: +Qy(z-1), -Qy, -Qz(y-1), +Qz; y: -Qx(z-1), +Qx, +Qz(x-1), -Qz;
       // z: +Qx(y-1), -Qx, -Qy(x-1), +Qy
-      double e0 = 0.0;
+      C e0 = 0;
       e0 = e0 + Qpy; e0 = e0 - Qy; e0 = e0 - sQ[2][row - 1][slot]; e0 = e0 + Qz;
-      double e1 = 0.0;
+      C e1 = 0;
       e1 = e1 - Qpx; e1 = e1 + Qx; e1 = e1 + sQ[2][row][slot - 1]; e1 = e1 - Qz;
-      double e2 = 0.0;
+      C e2 = 0;
       e2 = e2 + sQ[0][row - 1][slot]; e2 = e2 - Qx; e2 = e2 - sQ[1][row][slot - 1]; e2 = e2 + Qy;
-      const double m0 = keep_row(g.pecz, 0, i, j, k, g) ? Lx * e0 : 0.0;
-      const double m1 = keep_row(g.pecz, 1, i, j, k, g) ? Ly * e1 : 0.0;
-      const double m2 = keep_row(g.pecz, 2, i, j, k, g) ? Lz * e2 : 0.0;
-      const double wv[3] = {(double)Wk[0], (double)Wk[1], (double)Wk[2]};
-      double le[3] = {wv[0] - dt * m0, wv[1] - dt * m1, wv[2] - dt * m2};
-      if (capcol) le[2] += gcap[(capbase + x.cap_sample) * g.Nz + k];
-      double wn[3];
+      const C m0 = keep_row(g.pecz, 0, i, j, k, g) ? Lx * e0 : C(0);
+      const C m1 = keep_row(g.pecz, 1, i, j, k, g) ? Ly * e1 : C(0);
+      const C m2 = keep_row(g.pecz, 2, i, j, k, g) ? Lz * e2 : C(0);
+      const C wv[3] = {(C)Wk[0], (C)Wk[1], (C)Wk[2]};
+      C le[3] = {wv[0] - dt * m0, wv[1] - dt * m1, wv[2] - dt * m2};
+      if (capcol) le[2] += (C)gcap[(capbase + x.cap_sample) * g.Nz + k];
+      C wn[3];
       const bool inreg = regcol && k >= x.r0[2] && k < x.r1[2];
       const int64_t tr = inreg ? (int64_t)(i - x.r0[0]) +
                                      (int64_t)rx * ((j - x.r0[1]) + (int64_t)ry * (k - x.r0[2]))
                                : 0;
-      double ljv[3] = {0, 0, 0};
+      C ljv[3] = {0, 0, 0};
       if (cond) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          const double iec = (double)ieval(c, ck, ie[c]), sgc = (double)sgval(c, ck, sg[c]);
-          ljv[c] = (c == 2 && online) ? 0.0 : -dt * iec * wv[c];
+          const C iec = (C)ieval(c, ck, ie[c]), sgc = (C)sgval(c, ck, sg[c]);
+          ljv[c] = (c == 2 && online) ? C(0) : -dt * iec * wv[c];
           wn[c] = kap ? le[c] + sgc * wv[c] : le[c] + sgc * ljv[c];
         }
       } else {
@@ -326,11 +327,11 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
             rW[base + c * nreg] = Wk[c];
-            rJ[base + c * nreg] = ljv[c];
+            rJ[base + c * nreg] = (double)ljv[c];
           }
         } else if (lJr && cond) {
 #pragma unroll
-          for (int c = 0; c < 3; ++c) lJr[c * nreg + tr] = ljv[c];
+          for (int c = 0; c < 3; ++c) lJr[c * nreg + tr] = (double)ljv[c];
         }
       }
       const I nk = col + sz * (I)k;
@@ -359,9 +360,9 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
   }
 }
 
-template <typename S, typename A, int TY, bool LUT = false>
+template <typename S, typename A, int TY, bool LUT = false, typename C = double>
 __global__ void __launch_bounds__(64 * (TY + 3)) k_adj_fused(AdjStep<S, A> p) {
-  adj_fused_tile<S, A, TY, LUT>(p, xcd_tile(blockIdx.x, p.t.ntx * p.t.nty * p.t.ntz));
+  adj_fused_tile<S, A, TY, LUT, C>(p, xcd_tile(blockIdx.x, p.t.ntx * p.t.nty * p.t.ntz));
 }
 
 // One grid of a reverse batch. Reverse step r of the batch is step Nt-1-r of
@@ -374,7 +375,7 @@ struct AdjBatchDesc {
   int tile0;
 };
 
-template <typename S, typename A, int TY>
+template <typename S, typename A, int TY, typename C = double>
 __global__ void __launch_bounds__(64 * (TY + 3))
 k_adj_fused_batch(const AdjBatchDesc<S, A> *__restrict__ d, int ninst, int ntiles,
                   const int *__restrict__ rptr) {
@@ -396,7 +397,7 @@ k_adj_fused_batch(const AdjBatchDesc<S, A> *__restrict__ d, int ninst, int ntile
   }
   p.x.cap_sample = n;
   p.rslot = n;
-  adj_fused_tile<S, A, TY, false>(p, tile - d[lo].tile0);
+  adj_fused_tile<S, A, TY, false, C>(p, tile - d[lo].tile0);
 }
 
 }  // namespace wf

@@ -52,11 +52,12 @@ wf::AdjStep<S, A> adj_step_params(const wf_handle *h, const wf_adjoint_args *a,
 }
 
 // static LDS of k_adj_fused (P and Q planes in FP64) for TY own rows
+template <typename C>
 constexpr size_t adj_static_lds(int TY) {
-  return (size_t)(3 * (TY + 3) + 3 * (TY + 2)) * (wf::ADJ_TX + 3) * sizeof(double);
+  return (size_t)(3 * (TY + 3) + 3 * (TY + 2)) * (wf::ADJ_TX + 3) * sizeof(C);
 }
 
-template <typename S, typename A, int TY>
+template <typename S, typename A, int TY, typename C>
 int adjoint_fused(wf_handle *h, const wf_adjoint_args *a, const wf::MetricDev<S> &m,
                   const wf::AdjAux &x, hipStream_t st) {
   const wf::GridDev &g = h->g;
@@ -69,10 +70,10 @@ int adjoint_fused(wf_handle *h, const wf_adjoint_args *a, const wf::MetricDev<S>
   // FP64 coefficients the code path was 1.5x slower than the arrays)
   const size_t table = (a->code && a->mat && a->nmat > 0 && a->code_metric2d)
                            ? (size_t)a->nmat * 9 * sizeof(S) : 0;
-  const bool lut = sizeof(S) == 4 && table > 0 && adj_static_lds(TY) + table <= 65536;
-  void (*kern)(wf::AdjStep<S, A>) = &wf::k_adj_fused<S, A, TY, false>;
+  const bool lut = sizeof(S) == 4 && table > 0 && adj_static_lds<C>(TY) + table <= 65536;
+  void (*kern)(wf::AdjStep<S, A>) = &wf::k_adj_fused<S, A, TY, false, C>;
   if constexpr (sizeof(S) == 4) {
-    if (lut) kern = &wf::k_adj_fused<S, A, TY, true>;
+    if (lut) kern = &wf::k_adj_fused<S, A, TY, true, C>;
   }
   const size_t lds = lut ? table : 0;
   const int threads = 64 * (TY + 3);
@@ -112,12 +113,15 @@ int adjoint_step(wf_handle *h, const wf_adjoint_args *a, const wf::MetricDev<S> 
   }
   if (a->fused) {
     const int rows = a->fused == 1 ? ADJ_ROWS : a->fused;
-    if (a->fields_f32) {
-      if (rows == 5) return adjoint_fused<S, float, 5>(h, a, m, x, st);
-      if (rows == 13) return adjoint_fused<S, float, 13>(h, a, m, x, st);
+    if (a->fields_f32 && a->compute_f32) {
+      if (rows == 5) return adjoint_fused<S, float, 5, float>(h, a, m, x, st);
+      if (rows == 13) return adjoint_fused<S, float, 13, float>(h, a, m, x, st);
+    } else if (a->fields_f32) {
+      if (rows == 5) return adjoint_fused<S, float, 5, double>(h, a, m, x, st);
+      if (rows == 13) return adjoint_fused<S, float, 13, double>(h, a, m, x, st);
     } else {
-      if (rows == 5) return adjoint_fused<S, double, 5>(h, a, m, x, st);
-      if (rows == 13) return adjoint_fused<S, double, 13>(h, a, m, x, st);
+      if (rows == 5) return adjoint_fused<S, double, 5, double>(h, a, m, x, st);
+      if (rows == 13) return adjoint_fused<S, double, 13, double>(h, a, m, x, st);
     }
     return fail(WF_EINVAL, "wf_adjoint_step: fused must be 0, 1, 5 or 13 (rows per workgroup)");
   }
@@ -153,6 +157,8 @@ int check_adjoint_args(const wf_handle *h, const wf_adjoint_args *a, const char 
       return fail(WF_EINVAL, "%s: conductive gradients need g_sigma and "
                   "lJ_region", who);
   }
+  if (a->compute_f32 && !a->fields_f32)
+    return fail(WF_EINVAL, "%s: FP32 arithmetic needs FP32 adjoint fields", who);
   if (a->fields_f32 && !a->fused)
     return fail(WF_EINVAL, "%s: FP32 adjoint fields need the fused kernel", who);
   if (a->kappa && !a->fused)
@@ -268,7 +274,7 @@ int wf_idft_real(const double *w, int64_t nbins, int64_t N, int64_t len, double 
 // Reverse mode over a batch of grids
 // ---------------------------------------------------------------------------
 struct wf_batch_adjoint {
-  int dtype = 0, f32 = 0, ninst = 0, ntiles = 0, maxsteps = 0;
+  int dtype = 0, f32 = 0, c32 = 0, ninst = 0, ntiles = 0, maxsteps = 0;
   void *desc_dev = nullptr;   // AdjBatchDesc<S, A>[ninst]
   int *step_dev = nullptr;    // reverse step read by the batch kernel
   hipStream_t cap_stream = nullptr;
@@ -311,9 +317,9 @@ int batch_adj_build(wf_handle *const *hs, const wf_adjoint_args *as, const int32
   return WF_OK;
 }
 
-template <typename S, typename A>
+template <typename S, typename A, typename C = double>
 int batch_adj_launch(const wf_batch_adjoint *b, hipStream_t st) {
-  hipLaunchKernelGGL((wf::k_adj_fused_batch<S, A, BATCH_ADJ_ROWS>), dim3((unsigned)b->ntiles),
+  hipLaunchKernelGGL((wf::k_adj_fused_batch<S, A, BATCH_ADJ_ROWS, C>), dim3((unsigned)b->ntiles),
                      dim3(64 * (BATCH_ADJ_ROWS + 3)), 0, st,
                      (const wf::AdjBatchDesc<S, A> *)b->desc_dev, b->ninst, b->ntiles,
                      (const int *)b->step_dev);
@@ -325,8 +331,12 @@ int batch_adj_launch(const wf_batch_adjoint *b, hipStream_t st) {
 
 int batch_adj_step_once(const wf_batch_adjoint *b, hipStream_t st) {
   if (b->dtype == WF_F64)
-    return b->f32 ? batch_adj_launch<double, float>(b, st) : batch_adj_launch<double, double>(b, st);
-  return b->f32 ? batch_adj_launch<float, float>(b, st) : batch_adj_launch<float, double>(b, st);
+    return b->c32 ? batch_adj_launch<double, float, float>(b, st)
+                  : (b->f32 ? batch_adj_launch<double, float>(b, st)
+                            : batch_adj_launch<double, double>(b, st));
+  return b->c32 ? batch_adj_launch<float, float, float>(b, st)
+                : (b->f32 ? batch_adj_launch<float, float>(b, st)
+                          : batch_adj_launch<float, double>(b, st));
 }
 
 }  // namespace
@@ -340,7 +350,7 @@ int wf_batch_adjoint_create(wf_handle *const *handles, const wf_adjoint_args *ar
   *out = nullptr;
   if (n < 1) return fail(WF_EINVAL, "wf_batch_adjoint_create: empty batch");
   if (!handles[0]) return fail(WF_EINVAL, "wf_batch_adjoint_create: null handle");
-  const int dtype = handles[0]->dtype, f32 = args[0].fields_f32;
+  const int dtype = handles[0]->dtype, f32 = args[0].fields_f32, c32 = args[0].compute_f32;
   for (int i = 0; i < n; ++i) {
     const wf_handle *h = handles[i];
     const wf_adjoint_args *a = &args[i];
@@ -358,11 +368,12 @@ int wf_batch_adjoint_create(wf_handle *const *handles, const wf_adjoint_args *ar
     if (a->gcap && nsteps[i] > a->cap_samples)
       return fail(WF_ERANGE, "%s: %d steps for %lld capture samples", who, (int)nsteps[i],
                   (long long)a->cap_samples);
-    if (h->dtype != dtype || a->fields_f32 != f32)
-      return fail(WF_EINVAL, "%s: dtype and fields_f32 must match across the batch", who);
+    if (h->dtype != dtype || a->fields_f32 != f32 || a->compute_f32 != c32)
+      return fail(WF_EINVAL, "%s: dtype, fields_f32 and compute_f32 must match across the "
+                  "batch", who);
   }
   wf_batch_adjoint *b = new wf_batch_adjoint();
-  b->dtype = dtype; b->f32 = f32; b->ninst = n;
+  b->dtype = dtype; b->f32 = f32; b->c32 = c32; b->ninst = n;
   int rc;
   if (dtype == WF_F64)
     rc = f32 ? batch_adj_build<double, float>(handles, args, nsteps, n, b)

@@ -324,26 +324,29 @@ int wf_abc_low(wf_handle *h, void *E, void *H, int32_t low_faces, void *stream) 
 
 int wf_wake_wp(const void *capture, int32_t dtype, int32_t nlines, int64_t cap_samples,
                int32_t Nz, int32_t k0, int32_t nz, const double *z, const double *dz,
-               const double *s, int64_t ns, double v, double zmin_v, double t0, double ti,
-               double dt, int64_t nt, double qe12, double *WP3d, void *stream) {
+               const double *s, int64_t ns, double v, double zref_v, double t0, double ti,
+               double dt, int64_t nt, int64_t ncols, int32_t counter_moving, double qe12,
+               double *WP3d, void *stream) {
   if (!capture || !z || !dz || !s || !WP3d) return fail(WF_EINVAL, "wf_wake_wp: null argument");
   if (nlines < 1 || Nz < 1 || k0 < 0 || nz < 0 || k0 + nz > Nz || ns < 0 || cap_samples < 1)
     return fail(WF_ERANGE, "wf_wake_wp: inconsistent sizes");
-  if (nt > cap_samples) return fail(WF_ERANGE, "wf_wake_wp: nt > captured samples");
+  if (nt < 0 || nt > cap_samples) return fail(WF_ERANGE, "wf_wake_wp: nt outside the captured samples");
+  if (ncols < nt) return fail(WF_ERANGE, "wf_wake_wp: ncols < nt");
+  if (dtype != WF_F64 && dtype != WF_F32) return fail(WF_EINVAL, "wf_wake_wp: bad dtype");
   if (ns == 0) return WF_OK;
   hipStream_t st = (hipStream_t)stream;
-  // the reference's Ezt has len(t) columns; python negative indices wrap there
-  const int64_t ncols = nt;
-  if (dtype == WF_F64)
-    hipLaunchKernelGGL(wf::k_wake_wp<double>, dim3(nblocks(ns, 256)), dim3(256), 0, st,
-                       (const double *)capture, nlines, cap_samples, Nz, k0, nz, z, dz, s, ns, v,
-                       zmin_v, t0, ti, dt, nt, ncols, qe12, WP3d);
-  else if (dtype == WF_F32)
-    hipLaunchKernelGGL(wf::k_wake_wp<float>, dim3(nblocks(ns, 256)), dim3(256), 0, st,
-                       (const float *)capture, nlines, cap_samples, Nz, k0, nz, z, dz, s, ns, v,
-                       zmin_v, t0, ti, dt, nt, ncols, qe12, WP3d);
-  else
-    return fail(WF_EINVAL, "wf_wake_wp: bad dtype");
+#define WF_WP_LAUNCH(S, CM)                                                                   \
+  hipLaunchKernelGGL((wf::k_wake_wp<S, CM>), dim3(nblocks(ns, 256)), dim3(256), 0, st,        \
+                     (const S *)capture, nlines, cap_samples, Nz, k0, nz, z, dz, s, ns, v,    \
+                     zref_v, t0, ti, dt, nt, ncols, qe12, WP3d)
+  if (dtype == WF_F64) {
+    if (counter_moving) WF_WP_LAUNCH(double, true);
+    else WF_WP_LAUNCH(double, false);
+  } else {
+    if (counter_moving) WF_WP_LAUNCH(float, true);
+    else WF_WP_LAUNCH(float, false);
+  }
+#undef WF_WP_LAUNCH
   WF_LAUNCH_CHECK();
   return WF_OK;
 }

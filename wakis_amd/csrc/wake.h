@@ -10,30 +10,40 @@ namespace wf {
 // (WP is zeroed once, then WP += sum_k Ez*dz and WP /= q*1e12 per line),
 // accumulating over k in increasing order: the exact operation sequence of
 // the reference's loop for this s.
-template <typename S>
+//   forward (CM false, wakeSolver.py:520-533): row k of the line,
+//     ts = (z[k] + s)/v - zref_v - t0 + ti, weight dz[k], zref_v = zmin/v;
+//   counter-moving (CM true, wakeSolver.py:503-517): row nz-1-k,
+//     ts = (z[nz-1-k] - s)/(-v) - zref_v - t0 + ti, weight (-dz[k]),
+//     zref_v = zmax/(-v).
+// it = trunc(ts/dt) is kept when it < nt; a negative it indexes from the end
+// of the reference's Ezt, which has ncols columns of which the first nt hold
+// samples (the others are zero).
+template <typename S, bool CM>
 __global__ void __launch_bounds__(256)
 k_wake_wp(const S *__restrict__ cap, int nlines, int64_t cap_samples, int Nz, int k0, int nz,
           const double *__restrict__ z, const double *__restrict__ dz,
-          const double *__restrict__ s, int64_t ns, double v, double zmin_v, double t0,
+          const double *__restrict__ s, int64_t ns, double v, double zref_v, double t0,
           double ti, double dt, int64_t nt, int64_t ncols, double qe12,
           double *__restrict__ WP3d) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= ns) return;
   const double sn = s[n];
+  const double nv = -v;
   double acc = 0.0;
   for (int l = 0; l < nlines; ++l) {
     const S *line = cap + (int64_t)l * cap_samples * Nz + k0;
     for (int k = 0; k < nz; ++k) {
-      double ts = (z[k] + sn) / v;
-      ts = ts - zmin_v;
+      const int kr = CM ? nz - 1 - k : k;  // row of Ezt
+      double ts = CM ? (z[kr] - sn) / nv : (z[kr] + sn) / v;
+      ts = ts - zref_v;
       ts = ts - t0;
       ts = ts + ti;
       const int64_t it = (int64_t)(ts / dt);
       if (it < nt) {
-        int64_t col = it < 0 ? it + ncols : it;  // python negative index
+        const int64_t col = it < 0 ? it + ncols : it;  // python negative index
         double e = 0.0;
-        if (col >= 0 && col < nt) e = (double)line[col * Nz + k];
-        acc = acc + e * dz[k];
+        if (col >= 0 && col < nt) e = (double)line[col * Nz + kr];
+        acc = acc + e * (CM ? -dz[k] : dz[k]);
       }
     }
     acc = acc / qe12;

@@ -39,8 +39,10 @@ class WakeSolver:
         else:  # wakeSolver.py:164-172 (CST convention for beta <= 1)
             self.ti = 8.548921333333334 * self.sigmaz / (np.sqrt(self.beta) * self.v)
         self.Ez_file = Ez_file
+        self.Ezt = None  # Ez(x_test, y_test, z, t) given to calc_long_WP
         self.t = None
-        self.xf = self.yf = self.zf = None
+        self.xf = self.yf = self.zf = None  # field subdomain
+        self.x = self.y = self.z = None  # full simulation domain
         self.wakelength = wakelength
         self.s = self.lambdas = self.WP = self.WP_3d = None
         self.n_transverse_cells = 1
@@ -94,41 +96,64 @@ class WakeSolver:
             self.wakelength = wakelength
         return np.arange(-self.ti * self.v, wakelength, dt * self.v), dt, zmin
 
-    def _device_wp(self, nlines):
-        if self._cap is None:
-            raise RuntimeError("no device Ez capture: run SolverFIT3D.wakesolve first "
-                               "(Ez.h5 input needs h5py, not part of this engine)")
-        if self.counter_moving:
-            raise NotImplementedError("counter_moving wake integration is not implemented")
-        if self._cap.shape[0] < nlines or self._cap_samples < 1:
-            raise IndexError("no Ez captured on the test window (it leaves the grid)")
+    def _zz(self):
+        """z rows kept by skip_cells: (first row, number of rows)."""
+        z = self.zf[slice(self.skip_cells, -self.skip_cells) if self.skip_cells != 0
+                    else np.s_[:]]
+        return (self.skip_cells if self.skip_cells != 0 else 0), len(z)
+
+    def _device_wp(self, cap, nlines, cap_samples, nt, ncols, counter_moving=False, row0=None):
+        """wf_wake_wp over `nlines` lines of the device array `cap`
+        ([nlines][cap_samples][Nz]) whose z index row0 holds the first z row kept
+        by skip_cells (default: the skip_cells offset of a full-z capture);
+        returns s and WP3d [nlines][len(s)] (FP64, on the device)."""
         s, dt, zmin = self._s_axis()
-        zz = slice(self.skip_cells, -self.skip_cells) if self.skip_cells != 0 else np.s_[:]
-        z = np.ascontiguousarray(self.zf[zz], dtype=np.float64)
-        nz = len(z)
-        k0 = self.skip_cells if self.skip_cells != 0 else 0
-        Nz = self._cap.shape[2]
-        nt = min(len(self.t), self._cap_samples)
-        dev = self._cap.device
+        k0, nz = self._zz()
+        z = np.ascontiguousarray(self.zf[k0:k0 + nz], dtype=np.float64)
+        if row0 is not None:
+            k0 = row0
+        Nz = cap.shape[-1]
+        dev = cap.device
         zt = torch.as_tensor(z, device=dev)
         dzt = torch.as_tensor(np.ascontiguousarray(self.dz[:max(nz, 1)], dtype=np.float64),
                               device=dev)
         st_ = torch.as_tensor(np.ascontiguousarray(s, dtype=np.float64), device=dev)
         WP3d = torch.zeros((nlines, len(s)), dtype=torch.float64, device=dev)
-        dtype_code = _lib.WF_F64 if self._cap_dtype == np.float64 else _lib.WF_F32
+        if counter_moving:  # wakeSolver.py:505-511: zmax/(-1*v)
+            zref_v = float(np.max(self.zf) / (-1 * self.v))
+        else:  # wakeSolver.py:524-528: zmin/v
+            zref_v = float(zmin / self.v)
+        dtype_code = _lib.WF_F64 if cap.dtype == torch.float64 else _lib.WF_F32
         _lib.check(_lib.lib().wf_wake_wp(
-            _lib.ptr(self._cap), dtype_code, nlines, self._cap.shape[1], Nz, k0, nz,
-            _lib.ptr(zt), _lib.ptr(dzt), _lib.ptr(st_), len(s), float(self.v),
-            float(zmin / self.v), float(self.t[0]), float(self.ti), float(dt), nt,
+            _lib.ptr(cap), dtype_code, nlines, cap_samples, Nz, k0, nz,
+            _lib.ptr(zt), _lib.ptr(dzt), _lib.ptr(st_), len(s), float(self.v), zref_v,
+            float(self.t[0]), float(self.ti), float(dt), nt, ncols, int(bool(counter_moving)),
             float(self.q * 1e12), _lib.ptr(WP3d), _lib.stream_ptr()), "wf_wake_wp")
         return s, WP3d
 
+    def _captured(self):
+        if self._cap is None:
+            raise RuntimeError("no device Ez capture: run SolverFIT3D.wakesolve first, or pass "
+                               "Ezt to calc_long_WP (Ez.h5 input needs h5py, not part of "
+                               "this engine)")
+        if self._cap.shape[0] < 1 or self._cap_samples < 1:
+            raise IndexError("no Ez captured on the test window (it leaves the grid)")
+        return self._cap
+
     def calc_long_WP_3d(self, **kwargs):
-        """wakeSolver.py:422-554 (3x3 lines, carry between lines)."""
+        """wakeSolver.py:422-554 (3x3 lines, carry between lines; the
+        counter-moving integral of :503-517 when counter_moving)."""
         for key, val in kwargs.items():
             setattr(self, key, val)
         i0 = self.n_transverse_cells
-        s, WP3d = self._device_wp((2 * i0 + 1) ** 2)
+        nl = (2 * i0 + 1) ** 2
+        cap = self._captured()
+        if cap.shape[0] < nl:
+            raise IndexError("no Ez captured on the test window (it leaves the grid)")
+        # Ezt has len(t) columns (allocated before nt = min(nt, len(keys)), :487-491)
+        nt = min(len(self.t), self._cap_samples)
+        s, WP3d = self._device_wp(cap, nl, cap.shape[1], nt, len(self.t),
+                                  counter_moving=self.counter_moving)
         self._WP3d_dev = WP3d
         self.s = s
         self.WP_3d = WP3d.cpu().numpy().reshape(2 * i0 + 1, 2 * i0 + 1, len(s))
@@ -136,18 +161,50 @@ class WakeSolver:
         self._save_txt("WP.txt", self.s, self.WP, "WP [V/pC]")
 
     def calc_long_WP(self, Ezt=None, **kwargs):
-        """wakeSolver.py:310-420, one captured line (compute_plane='longitudinal')."""
+        """wakeSolver.py:310-420: the wake potential of one line of Ez samples.
+
+        Without ``Ezt`` the line comes from the device capture: a single
+        captured line (compute_plane='longitudinal'), or, on a 3x3 window, the
+        line [shape//2 + 1, shape//2 + 1] that the reference reads there
+        (wakeSolver.py:385-388: the corner line at (+1, +1), not the centre).
+        ``Ezt`` is an (nz, nt) array of Ez(x_test, y_test, z, t) samples, nz the
+        number of z rows kept by skip_cells, integrated over min(len(t), nt)
+        samples (the documented input, wakeSolver.py:326-327). The reference
+        itself reads the samples from Ez.h5 even when Ezt is given (an open
+        file overwrites Ezt, :375-393; without one it fails at :376)."""
         for key, val in kwargs.items():
             setattr(self, key, val)
+        if self.zf is None:
+            self.zf = self.z
         if Ezt is not None:
-            raise NotImplementedError("calc_long_WP(Ezt=...) host input is not supported")
-        if self._cap.shape[0] != 1:
-            raise ValueError("calc_long_WP needs a single captured line")
-        if self.skip_cells != 0:  # wakeSolver.py:390-392 assigns the full line: shape error
-            raise ValueError("could not broadcast the captured line into Ezt with skip_cells")
-        s, WP3d = self._device_wp(1)
+            Ezt = np.asarray(Ezt, dtype=np.float64)
+            k0, nz = self._zz()
+            if Ezt.ndim != 2 or Ezt.shape[0] != nz or Ezt.shape[1] < 1:
+                raise ValueError(f"calc_long_WP: Ezt must have shape (nz, nt) with nz={nz} "
+                                 f"z rows, got {Ezt.shape}")
+            self.Ezt = Ezt
+            # as a one-line capture [1][ncols][nz] whose first z row is row 0
+            cap = torch.as_tensor(np.ascontiguousarray(Ezt.T)[None], device=torch.device("cuda"))
+            ncols = Ezt.shape[1]
+            s, WP3d = self._device_wp(cap, 1, ncols, min(len(self.t), ncols), ncols, row0=0)
+            WP = WP3d[0]
+        else:
+            cap = self._captured()
+            nl = cap.shape[0]
+            if nl == 1:
+                if self.skip_cells != 0:  # :390-392 assigns the full line: shape error
+                    raise ValueError("could not broadcast the captured line into Ezt with "
+                                     "skip_cells")
+                line = 0
+            else:
+                side = int(round(np.sqrt(nl)))
+                line = (side // 2 + 1) * side + (side // 2 + 1)
+            # Ezt has min(len(t), len(keys)) columns here (:381-382)
+            nt = min(len(self.t), self._cap_samples)
+            s, WP3d = self._device_wp(cap[line:line + 1], 1, cap.shape[1], nt, nt)
+            WP = WP3d[0]
         self.s = s
-        self.WP = WP3d[0].cpu().numpy()
+        self.WP = WP.cpu().numpy()
         self._save_txt("WP.txt", self.s, self.WP, "WP [V/pC]")
 
     def calc_trans_WP(self, **kwargs):
