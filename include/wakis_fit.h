@@ -258,6 +258,26 @@ int wf_loss_factor(const double *WP, const double *lam, int64_t n, double ds, do
  * handle's reduction workspace: calls on one handle must not overlap. */
 int wf_count_nonfinite(wf_handle *h, const void *x, int64_t n, int64_t *out, void *stream);
 
+/* Partial counts held by the scratch of wf_count_nonfinite_noalloc. */
+#define WF_NOALLOC_PARTIALS 1024
+
+/* wf_count_nonfinite without a handle and without any allocation. No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc), which therefore cannot create a handle for
+ * wf_count_nonfinite's workspace. x holds n values of `dtype` (WF_F32 or
+ * WF_F64); the count goes to the device int64 *out, equal to
+ * wf_count_nonfinite's. The per-workgroup partial counts live in a static
+ * device array of WF_NOALLOC_PARTIALS int64 (8 KiB) that each GPU holds until
+ * the process ends: `blocks` workgroups (0: min(ceil(n/256), 1024)), at most
+ * WF_NOALLOC_PARTIALS, more is a WF_ERANGE error. Every workgroup writes its
+ * own partial before the second kernel reads it, so nothing needs clearing.
+ * The first call on a device resolves the array's device address (make it
+ * outside graph capture); calls that overlap on one GPU share the array and
+ * race. */
+int wf_count_nonfinite_noalloc(const void *x, int32_t dtype, int64_t n, int32_t blocks,
+                               int64_t *out, void *stream);
+
 /* Batches. No reference counterpart (the reference runs one grid per process);
  * for parameter sweeps over many small grids. Each grid keeps its handle and
  * a fused-path wf_step_args (path 1, E2/H2; material codes with

@@ -97,6 +97,8 @@ EXPORTS = {
                             c_void_p, c_void_p, c_void_p]),
     "wf_loss_factor": (c_int32, [c_void_p, c_void_p, c_int64, c_double, c_void_p, c_void_p]),
     "wf_count_nonfinite": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
+    "wf_count_nonfinite_noalloc": (c_int32, [c_void_p, c_int32, c_int64, c_int32, c_void_p,
+                                             c_void_p]),
     "wf_batch_create": (c_int32, [POINTER(c_void_p), POINTER(WfStepArgs), c_int32, c_int32,
                                   c_int32, POINTER(c_void_p)]),
     "wf_batch_run": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p]),

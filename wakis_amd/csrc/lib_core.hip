@@ -1,13 +1,46 @@
 // libwakis_fit: C ABI of the MI355X FIT engine (declared in include/wakis_fit.h).
 // Handles, the exact step path, captures, wake post-processing and reductions.
 #include "lib_internal.h"
+
+#include <mutex>
 #include "monitor.h"
 #include "reduce.h"
 #include "wake.h"
 
 namespace {
 thread_local std::string g_err;
+
+// Scratch of wf_count_nonfinite_noalloc: the per-workgroup partial counts
+// (8 KiB). Each GPU the library runs on holds its own copy until the process
+// ends; every call on a GPU shares it, so calls that overlap on one GPU race.
+static __device__ __align__(256) long long scratch[WF_NOALLOC_PARTIALS];
+
+// Device address of `scratch` on each device, resolved with that device current
+// on its first call there (a host-side use of the name `scratch` is a host
+// stand-in, not the device address). Fixed-size table: no allocation.
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+long long *g_scratch_ptr[MAX_DEVICES] = {};
 }  // namespace
+
+int noalloc_scratch(long long **out) {
+  int dev = -1;
+  WF_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES)
+    return fail(WF_ERANGE, "no-allocation scratch: device %d beyond the %d-device table", dev,
+                MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(WF_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                  hipGetErrorString(e));
+    g_scratch_ptr[dev] = (long long *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return WF_OK;
+}
 
 int fail(int code, const char *fmt, ...) {
   char buf[512];
@@ -407,6 +440,36 @@ int wf_count_nonfinite(wf_handle *h, const void *x, int64_t n, int64_t *out, voi
   const int nb = (int)(want < 1 ? 1 : (want < wf::RED_BLOCKS ? want : wf::RED_BLOCKS));
   long long *part = (long long *)h->red_buf;  // RED_BLOCKS*4 doubles: room for nb counts
   if (h->dtype == WF_F64)
+    hipLaunchKernelGGL(wf::k_count_nonfinite<double>, dim3(nb), dim3(wf::RED_THREADS), 0, st,
+                       (const double *)x, n, part);
+  else
+    hipLaunchKernelGGL(wf::k_count_nonfinite<float>, dim3(nb), dim3(wf::RED_THREADS), 0, st,
+                       (const float *)x, n, part);
+  WF_LAUNCH_CHECK();
+  hipLaunchKernelGGL(wf::k_sum_counts, dim3(1), dim3(wf::RED_THREADS), 0, st,
+                     (const long long *)part, nb, (long long *)out);
+  WF_LAUNCH_CHECK();
+  return WF_OK;
+}
+
+int wf_count_nonfinite_noalloc(const void *x, int32_t dtype, int64_t n, int32_t blocks,
+                               int64_t *out, void *stream) {
+  if (!out || (!x && n > 0)) return fail(WF_EINVAL, "wf_count_nonfinite_noalloc: null argument");
+  if (n < 0) return fail(WF_ERANGE, "wf_count_nonfinite_noalloc: negative length");
+  if (dtype != WF_F64 && dtype != WF_F32)
+    return fail(WF_EINVAL, "wf_count_nonfinite_noalloc: dtype must be WF_F64 or WF_F32");
+  if (blocks < 0 || blocks > WF_NOALLOC_PARTIALS)
+    return fail(WF_ERANGE, "wf_count_nonfinite_noalloc: %d workgroups, the scratch holds the "
+                "partial counts of at most %d", (int)blocks, WF_NOALLOC_PARTIALS);
+  long long *part = nullptr;
+  const int rc = noalloc_scratch(&part);
+  if (rc != WF_OK) return rc;
+  const int64_t want = (n + wf::RED_THREADS - 1) / wf::RED_THREADS;
+  const int nb = blocks > 0 ? (int)blocks
+                            : (int)(want < 1 ? 1 : (want < WF_NOALLOC_PARTIALS ? want
+                                                                            : WF_NOALLOC_PARTIALS));
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == WF_F64)
     hipLaunchKernelGGL(wf::k_count_nonfinite<double>, dim3(nb), dim3(wf::RED_THREADS), 0, st,
                        (const double *)x, n, part);
   else

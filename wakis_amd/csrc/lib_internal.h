@@ -84,6 +84,10 @@ WF_HIDDEN const char *check_traj(const wf_handle *h, const void *tE, const void 
                                  const int32_t *lo, const int32_t *hi);
 WF_HIDDEN int check_step_args(const wf_handle *h, const wf_step_args *a, const char *who);
 
+// Device address (current device) of the static scratch of the
+// no-allocation entry points; resolved once per device.
+WF_HIDDEN int noalloc_scratch(long long **out);
+
 // The fused single-pass step (path 1) for FP32 and FP64 storage.
 WF_HIDDEN int step_fused_f32(wf_handle *h, wf_step_args *a, hipStream_t st);
 WF_HIDDEN int step_fused_f64(wf_handle *h, wf_step_args *a, hipStream_t st);
