@@ -19,6 +19,7 @@ Prints one JSON line. --gpus N>1 exits: this engine runs on one GPU.
 """
 
 import argparse
+import contextlib
 import json
 import math
 import os
@@ -194,8 +195,9 @@ def wake_run(s, wakelength, snap=None):
         if k > 0:
             s._run(s._wake_args(ctx, n0, k))
             s.nstep += k
-        if n0 == 0 and snap is not None:
-            kept = (s.E.array.to(torch.float64), s.H.array.to(torch.float64), k1)
+        if n0 == 0 and snap is not None:  # copies: the ping-pong buffers get reused
+            kept = (s.E.array.to(torch.float64, copy=True),
+                    s.H.array.to(torch.float64, copy=True), k1)
     torch.cuda.synchronize(s.device)
     t_loop = time.perf_counter() - t0
     s.fields_finite = s.check_finite(("E", "H"))
@@ -242,7 +244,8 @@ def accuracy_leg(Nx, Ny, Nz, wakelength=0.5, snap=3000):
         s = build_solver(Nx, Ny, Nz, dtype, "auto", wake=wake)
         r = wake_run(s, wakelength, snap)
         E1, H1, k1 = r.pop("kept")
-        kept[dtype] = (E1, H1, s.E.array.to(torch.float64), s.H.array.to(torch.float64))
+        kept[dtype] = (E1, H1, s.E.array.to(torch.float64, copy=True),
+                       s.H.array.to(torch.float64, copy=True))
         r["wake"] = wake
         res[dtype] = r
         log(f"accuracy {np.dtype(dtype).name}: Nt={r['Nt']} loop {r['loop_s']:.2f}s "
@@ -274,7 +277,8 @@ def pillbox_solver(dtype, dims=(200, 200, 400), sibc=True, sigma_wall=1e4, wake=
     around the vacuum in a PEC background; 90 x 90 x 180 mm on `dims` cells,
     PEC x/y, 10-layer PML on both z ends. With `sibc` (the reference's
     default) the wall is replaced by its surface cells with sigma = eps =
-    1/Z_s, Z_s = sqrt(pi fmax mu0 / sigma), fmax = 1 GHz, and dt is the CFL
+    1/Z_s, Z_s = sqrt(pi fmax mu0 / sigma), fmax = 1 GHz (the WakeSolver's fmax when
+    `wake` is given), and dt is the CFL
     step; without it the wall keeps its volume conductivity and dt is the
     reference's tau clamp (eps0 / sigma)."""
     import wakis_amd as wa
@@ -351,8 +355,8 @@ def lossy_pml(dtype, steps=300, warmup=20, dims=(200, 200, 400), keep_fields=Fal
                frac=round(s.N * b / (ms * 1e-3) / 8e12, 4),
                full_rows=s.fused_full_rows, tuning=_tuning(s))
     if keep_fields:
-        out["E"] = s.E.array.to(torch.float64)
-        out["H"] = s.H.array.to(torch.float64)
+        out["E"] = s.E.array.to(torch.float64, copy=True)
+        out["H"] = s.H.array.to(torch.float64, copy=True)
     del s, table
     torch.cuda.empty_cache()
     return out
@@ -383,7 +387,10 @@ def lossy_wake(dims=(200, 200, 400), wakelength=1.0, xsource=2e-3):
     return {"quantity": "full wakesolve, %.1f m wakelength, beta=0.8, source line %.0f mm off "
                         "axis (x index %d; test line on axis, index %d), FP32 run vs FP64 run"
                         % (wakelength, xsource * 1e3, r64["ixs_ixt"][0], r64["ixs_ixt"][1]),
-            "sibc": "use_sibc=True (the reference's default): dt = %.4g s (CFL)" % r64["dt"],
+            "sibc": "use_sibc=True (the reference's default); a solver built with a "
+                    "WakeSolver takes the wake's fmax = v/(3 sigma_z) = %.3g GHz for the "
+                    "surface impedance (solverFIT3D.py:177-178); dt = %.4g s (CFL)"
+                    % (r64["wake"].fmax / 1e9, r64["dt"]),
             "steps": {"fp32": r32["Nt"], "fp64": r64["Nt"]},
             "wake": _wake_compare(r32["wake"], r64["wake"]),
             "step_loop_s": {"fp32": round(r32["loop_s"], 3), "fp64": round(r64["loop_s"], 3)},
@@ -658,6 +665,14 @@ def cpu_config1_wakesolve():
 
 
 def main():
+    # the JSON line is the only thing on stdout: the solvers' progress prints go to stderr
+    with contextlib.redirect_stdout(sys.stderr):
+        line = run(sys.argv[1:])
+    if line is not None:
+        print(json.dumps(line, allow_nan=False), flush=True)
+
+
+def run(argv):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3000)
@@ -672,6 +687,8 @@ def main():
     ap.add_argument("--variant", type=int, default=0,
                     help="fused kernel: 0 auto, 1 scalar, 2/3/7/8 rows per workgroup")
     ap.add_argument("--kz", type=int, default=0, help="z-planes per workgroup (0 auto)")
+    ap.add_argument("--no-exact", action="store_true",
+                    help="skip the timing of the exact general path")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the FP32-vs-FP64 full wakefield run")
     ap.add_argument("--adjoint-only", action="store_true",
@@ -685,19 +702,16 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-cavity batched sweep")
     ap.add_argument("--batch", action="store_true",
                     help="only the batched sweep of 64 box cavities (one JSON line)")
-    args = ap.parse_args()
+    args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU; runs on several GPUs are not "
               "supported (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
     if args.batch:
-        print(json.dumps(batch_sweep(), allow_nan=False), flush=True)
-        return
+        return batch_sweep()
     if args.adjoint_only:
-        out = {"fp32": adjoint_cost(steps=args.adjoint_steps),
-               "fp64": adjoint_cost(np.float64, steps=args.adjoint_steps)}
-        print(json.dumps(out, allow_nan=False), flush=True)
-        return
+        return {"fp32": adjoint_cost(steps=args.adjoint_steps),
+                "fp64": adjoint_cost(np.float64, steps=args.adjoint_steps)}
     Nx, Ny, Nz = (int(v) for v in args.grid.split(","))
     import torch
     copy_gbs = hbm_copy_gbs(torch.device("cuda", 0))
@@ -711,6 +725,18 @@ def main():
         r64 = time_gpu(Nx, Ny, Nz, np.float64, args.steps, args.warmup, args.path,
                        args.variant, args.kz)
         log(f"fp64 {r64}")
+    exact = None
+    if not args.no_exact:
+        exact = {"quantity": "the exact general path (two kernels per step, flat-index wrap "
+                             "terms kept: what runs with live wrap terms, e.g. periodic or "
+                             "PMC x/y faces with material on them) on the timed workload, "
+                             "50 steps after 5",
+                 "bytes_per_cell_update_read_written": {"fp32": 96, "fp64": 192}}
+        for dt_, key in ((np.float32, "fp32"), (np.float64, "fp64")):
+            r = time_gpu(Nx, Ny, Nz, dt_, 50, 5, "general")
+            exact[key] = {"ms_per_step": round(r["ms_per_step"], 5),
+                          "gcell_s": round(r["gcell_s"], 3), "path": r["path"]}
+        log(f"exact path {exact}")
     acc = None
     if not args.no_accuracy and not args.no_fp64:
         acc = accuracy_leg(Nx, Ny, Nz)
@@ -770,6 +796,8 @@ def main():
                         "traffic": measured_traffic((Nx, Ny, Nz), "f64", r64["path"],
                                                     args.variant, args.kz)}
         line["fp32_over_fp64"] = round(r32["gcell_s"] / r64["gcell_s"], 3)
+    if exact is not None:
+        line["exact_path"] = exact
     if acc is not None:
         line["accuracy"] = acc
     if lossy is not None:
@@ -791,7 +819,7 @@ def main():
                       f"{os.cpu_count()}",
             "threadpools": _threads_info(),
             "config1_wakesolve": cfg1}
-    print(json.dumps(line, allow_nan=False), flush=True)
+    return line
 
 
 if __name__ == "__main__":

@@ -2,7 +2,10 @@
 
 128x128x256 cavity/pipe (PEC background, PEC x/y, ABC z) with a 32x32x24 block
 of cells on the axis carrying their own eps_r ~ U[1, 10] and sigma ~ U[0, 2]
-S/m (seeded), 4,000 steps at the CFL dt, L = k_loss + 1e-13 * band power:
+S/m (seeded), 4,000 steps at the CFL dt, L = k_loss + 1e-15 * band power (the
+band weight keeps |L| near 10^2: the central differences of the eps_r
+gradients, ~250 times smaller than the sigma ones, then stay well above the
+rounding noise of L):
 
 * the FP64 adjoint gradient against central finite differences (h = 1e-4) of
   the FP64 forward on 16 seeded block cells, for eps_r and for sigma: 1e-3;
@@ -23,6 +26,7 @@ pytestmark = pytest.mark.gpu
 
 DIMS4 = (128, 128, 256)
 INSERT4 = (32, 32, 24)
+W_BAND = 1e-15
 
 
 def _region(dims, insert):
@@ -45,7 +49,7 @@ def _problem(dims, insert, dtype, steps, seed=0):
                        verbose=0, use_sibc=False, tau_clamp=False)
     v = wake.v
     wl = steps * float(s.dt) * v - wake.ti * v - (s.z.max() - s.z.min())
-    adj = WakeAdjoint(s, region, wl, w_loss=1.0, w_band=1e-13)
+    adj = WakeAdjoint(s, region, wl, w_loss=1.0, w_band=W_BAND)
     rng = np.random.default_rng(seed)
     er = rng.uniform(1.0, 10.0, insert)
     sg = rng.uniform(0.0, 2.0, insert)
@@ -105,7 +109,7 @@ def test_config4_downscale_matches_torch_autograd():
     s, adj, er, sg, region, wl = _problem(dims, insert, np.float64, 1000, seed=1)
     L = adj.forward(er, sg)
     g_er, g_sg = (g.cpu().numpy() for g in adj.backward())
-    Lt, t_er, t_sg = _torch_reference(er, sg, 1.0, 1e-13, s, dims=dims, region=region,
+    Lt, t_er, t_sg = _torch_reference(er, sg, 1.0, W_BAND, s, dims=dims, region=region,
                                       wakelength=wl, ins_material=[2.0, 1.0, 1.0])
     print("L device", L, "torch", Lt, flush=True)
     assert L == pytest.approx(Lt, rel=1e-9)

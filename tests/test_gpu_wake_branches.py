@@ -40,8 +40,11 @@ def _pillbox_wake_pair(dtype, sibc, sigma_wall, xsource, xtest, wakelength=0.01,
     x, y, z, vac, wall = cases.pillbox_geometry(*dims, **PILLBOX)
     lossy = [1.0, 1.0, sigma_wall]
     bc = (["pec", "pec", "pml"], ["pec", "pec", "pml"])
+    # a solver built with a WakeSolver and the default fmax takes the wake's fmax
+    # = v/(3 sigma_z) for the surface impedance (solverFIT3D.py:177-178)
+    fmax = BETA * 299792458.0 / (3.0 * SIGMAZ)
     so = fo.OracleSolver(x, y, z, *bc, bg="pec", solids=[(wall, lossy), (vac, "vacuum")],
-                         dtype=dtype, n_pml=10, use_sibc=sibc)
+                         dtype=dtype, n_pml=10, use_sibc=sibc, fmax=fmax)
     ti = wo.wake_ti(SIGMAZ, BETA)
     cap = fo.wakesolve_fields(so, wakelength, Q, SIGMAZ, BETA, xsource, 0.0, xtest, 0.0, ti)
     ref = wo.wake_solve(cap, Q, SIGMAZ, BETA, wakelength, skip_cells=skip)
@@ -51,6 +54,9 @@ def _pillbox_wake_pair(dtype, sibc, sigma_wall, xsource, xtest, wakelength=0.01,
                          skip_cells=skip, save=False, verbose=0)
     sa = wa.SolverFIT3D(grid, wake=wake, bc_low=bc[0], bc_high=bc[1], use_stl=True, bg="pec",
                         dtype=dtype, verbose=0, n_pml=10, use_sibc=sibc)
+    assert sa.fmax == fmax == wake.fmax
+    for name in ("ieps", "imu", "sigma"):
+        assert np.array_equal(getattr(sa, name).to_numpy(), getattr(so, name)), name
     sa.wakesolve(wakelength=wakelength)
     return so, sa, cap, ref, wake
 

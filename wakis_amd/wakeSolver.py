@@ -119,6 +119,8 @@ class WakeSolver:
                               device=dev)
         st_ = torch.as_tensor(np.ascontiguousarray(s, dtype=np.float64), device=dev)
         WP3d = torch.zeros((nlines, len(s)), dtype=torch.float64, device=dev)
+        if len(s) == 0:  # wakelength below -ti*v: an empty s axis, nothing to integrate
+            return s, WP3d
         if counter_moving:  # wakeSolver.py:505-511: zmax/(-1*v)
             zref_v = float(np.max(self.zf) / (-1 * self.v))
         else:  # wakeSolver.py:524-528: zmin/v
