@@ -42,8 +42,8 @@ VALUES_PER_CELL = 18  # E r/w 3+3, H r/w 3+3, face coeffs 3, edge coeffs 3
 # for it; WRITE_SIZE matches the copy's bytes and is taken as is. Keyed by
 # (grid, dtype, path, variant, kz); other settings report null.
 MEASURED_TRAFFIC = {
-    ((256, 256, 512), "f32", "fused", 0, 0): {"fetch_B": 936_804_147, "write_B": 805_472_666},
-    ((256, 256, 512), "f64", "fused", 0, 0): {"fetch_B": 2_007_433_830, "write_B": 1_610_869_248},
+    ((256, 256, 512), "f32", "fused", 0, 0): {"fetch_B": 893_784_269, "write_B": 805_556_019},
+    ((256, 256, 512), "f64", "fused", 0, 0): {"fetch_B": 1_957_545_370, "write_B": 1_610_772_890},
 }
 
 

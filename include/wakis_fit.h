@@ -157,6 +157,19 @@ typedef struct wf_step_args {
                              parameters, e.g. a block being optimised) and the table
                              has at most 255 rows. x-vectorised fused kernel only
                              (not the scalar variant 1, not batches)               */
+  /* Optional FieldMonitor accumulation after every step of the call, what a
+   * wakesolve with a monitor does with FieldMonitor.update after each step
+   * (field_monitors.py:59-102, routines.py:447-448): with mon_acc set, the E that
+   * step s produced is added, times mon_phase[mon_row0 + s][f] (re, im), to
+   * mon_acc[c][f][n] (complex128 as (re, im) pairs, [3][mon_nf][N]), with the
+   * arithmetic of wf_field_monitor_update. mon_phase holds FP64 pairs
+   * (complex128 phases), or FP32 pairs with mon_phase_f32 = 1 (complex64).
+   * Not in batches. */
+  double *mon_acc;
+  const void *mon_phase;
+  int64_t mon_row0;
+  int32_t mon_nf;
+  int32_t mon_phase_f32;
 } wf_step_args;
 
 int wf_version(void);
@@ -329,7 +342,7 @@ typedef struct wf_adjoint_args {
   int32_t traj;           /* 1: E0, E1 are slots n, n+1 of wf_step's traj_E and H1 slot
                              n+1 of traj_H, with the region as the box (r0 >= 1); 0: full
                              3N fields                                                     */
-  int32_t fused;          /* 0: two kernels, W and lH updated in place. 1 (or 5, 13 rows
+  int32_t fused;          /* 0: two kernels, W and lH updated in place. 1 (or 5, 7, 9, 13 rows
                              per workgroup): one single-pass kernel reading W, lH and
                              writing W2, lH2 (the caller swaps
                              the pairs for the next step); drops the flat-index wrap terms,
@@ -430,10 +443,13 @@ int wf_idft_real(const double *w, int64_t nbins, int64_t N, int64_t len, double 
 /* One FieldMonitor.update (field_monitors.py:59-102): for the 3N values of E
  * (field dtype) and each of nfreq monitored frequencies,
  * acc[c][f][n] += E_c[n] * phase[f], phase given as device [nfreq][2] (re, im)
- * = exp(-2 pi i f t) evaluated by the caller; acc is device complex128 laid
- * out [3][nfreq][N][2]. */
-int wf_field_monitor_update(const void *E, int32_t dtype, int64_t N, const double *phase,
-                            int32_t nfreq, double *acc, void *stream);
+ * = exp(-2 pi i f t) evaluated by the caller: FP64 pairs (complex128), or FP32
+ * pairs with phase_f32 = 1 (complex64, what numpy evaluates for an FP32 t).
+ * The product is numpy's complex multiply in the promoted type (FP32 for FP32
+ * E with a complex64 phase, else FP64); acc is device complex128 laid out
+ * [3][nfreq][N][2]. */
+int wf_field_monitor_update(const void *E, int32_t dtype, int64_t N, const void *phase,
+                            int32_t phase_f32, int32_t nfreq, double *acc, void *stream);
 
 #ifdef __cplusplus
 }

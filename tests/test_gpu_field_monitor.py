@@ -53,34 +53,94 @@ def test_linear_accumulation_and_geometric_series():
     assert np.allclose(m.get_components()["Ex"][0].cpu().numpy(), 0.0, atol=1e-12)
 
 
-def test_monitor_in_wakesolve_bitwise_vs_numpy():
-    """wakesolve with a monitor (per-step updates) against the reference's
-    arithmetic applied to the same per-step E snapshots."""
+@pytest.mark.parametrize("dtype,dt", [(np.float32, np.float32(1.1e-12)), (np.float32, 1.1e-12),
+                                      (np.float64, np.float64(1.1e-12)),
+                                      (np.float64, np.float32(1.1e-12))])
+def test_update_bitwise_vs_numpy_promotions(dtype, dt):
+    """FieldMonitor.update on random fields against the reference's numpy
+    arithmetic (field_monitors.py:90-100) with its promotions: the phase is
+    complex64 for an FP32 dt, and E * phase is complex64 for FP32 E with a
+    complex64 phase, complex128 otherwise; the accumulators are complex128."""
     import wakis_amd as wa
     from wakis_amd import FieldMonitor
-    freqs = [0.5e9, 1.0e9, 3.0e9]
-    grid = wa.GridFIT3D(0.0, 1.0, 0.0, 1.0, 0.0, 1.0, 4, 4, 4, verbose=0)
-    wake = wa.WakeSolver(q=1e-9, sigmaz=1e-2, beta=1.0, save=False, verbose=0)
-    wake.solve = lambda *a, **k: None
-    s = wa.SolverFIT3D(grid, wake, bc_low=["pec"] * 3, bc_high=["pec"] * 3, use_stl=False,
-                       bg="pec", dtype=np.float32, verbose=0)
-    m = FieldMonitor(freqs)
-    snaps = []
-
-    def cb(solver, t):
-        snaps.append(solver.E.to_numpy().copy())
-
-    s.wakesolve(wakelength=0.1, add_space=0, use_field_monitor=True, field_monitor=m,
-                callback=cb)
-    assert m.time_index == s.Nt > 0 and m.shape == (4, 4, 4)
-    ref = {c: np.zeros((len(freqs), 4, 4, 4), dtype=np.complex128) for c in "xyz"}
-    for n, E in enumerate(snaps):
+    print("monitor update", np.dtype(dtype).name, type(dt).__name__, flush=True)
+    rng = np.random.default_rng(3)
+    Nx, Ny, Nz = 6, 5, 7
+    f = wa.Field(Nx, Ny, Nz, dtype=dtype)
+    m = FieldMonitor([0.9e9, 2.5e9])
+    ref = {c: np.zeros((2, Nx, Ny, Nz), np.complex128) for c in "xyz"}
+    for n in range(6):
+        for c in "xyz":
+            f.from_matrix(rng.standard_normal((Nx, Ny, Nz)).astype(dtype), c)
+        m.update(f, dt)
         t = n * m.dt
-        for i, fr in enumerate(freqs):
+        for i, fr in enumerate(m.frequencies):
             ph = np.exp(-2j * np.pi * fr * t)
-            for ci, c in enumerate("xyz"):
-                Ec = np.reshape(E[ci * 64:(ci + 1) * 64], (4, 4, 4), order="F")
-                ref[c][i] += Ec * ph
+            for c in "xyz":
+                ref[c][i] += f.to_matrix(c).cpu().numpy() * ph
     comps = m.get_components()
     for c in "xyz":
         assert np.array_equal(comps["E" + c].cpu().numpy(), ref[c]), c
+
+
+def _monitor_run(dtype, per_step, path="auto", snaps=None):
+    """A 16x16x32 cavity/pipe wakesolve with a 3-frequency monitor; per_step: a
+    callback forces one step per call and FieldMonitor.update after each."""
+    import wakis_amd as wa
+    from oracle import cases
+    from wakis_amd import FieldMonitor
+    x, y, z, mask = cases.cavity_pipe_geometry(16, 16, 32)
+    grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"cav": mask},
+                        stl_materials={"cav": "vacuum"}, verbose=0)
+    wake = wa.WakeSolver(q=1e-9, sigmaz=10e-3, beta=1.0, skip_cells=2, save=False, verbose=0)
+    s = wa.SolverFIT3D(grid, wake, bc_low=["pec", "pec", "abc"], bc_high=["pec", "pec", "abc"],
+                       use_stl=True, bg="pec", dtype=dtype, verbose=0, path=path)
+    m = FieldMonitor([0.7e9, 2.1e9, 4.0e9])
+    cb = None
+    if per_step:
+        def cb(solver, t):
+            if snaps is not None:
+                snaps.append(solver.E.to_numpy().copy())
+    s.wakesolve(wakelength=0.03, use_field_monitor=True, field_monitor=m, callback=cb)
+    return s, m
+
+
+def test_monitor_in_wakesolve_bitwise_vs_numpy():
+    """wakesolve with a monitor (per-step updates) against the reference's
+    arithmetic (field_monitors.py:95-100) applied to the same per-step E
+    snapshots."""
+    snaps = []
+    s, m = _monitor_run(np.float32, True, snaps=snaps)
+    assert m.time_index == s.Nt == len(snaps) > 0 and m.shape == (16, 16, 32)
+    N = 16 * 16 * 32
+    ref = {c: np.zeros((3, 16, 16, 32), dtype=np.complex128) for c in "xyz"}
+    for n, E in enumerate(snaps):
+        t = n * m.dt
+        for i, fr in enumerate(m.frequencies):
+            ph = np.exp(-2j * np.pi * fr * t)
+            for ci, c in enumerate("xyz"):
+                Ec = np.reshape(E[ci * N:(ci + 1) * N], (16, 16, 32), order="F")
+                ref[c][i] += Ec * ph
+    comps = m.get_components()
+    assert np.any(ref["z"] != 0)
+    for c in "xyz":
+        assert np.array_equal(comps["E" + c].cpu().numpy(), ref[c]), c
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("path", ["auto", "general"])
+def test_monitor_accumulated_inside_the_step_call(dtype, path):
+    """Without a callback wakesolve runs the whole loop in one step call that
+    accumulates the monitor after every step from a device table of phases
+    (no per-step host work): bitwise the per-step updates' accumulators."""
+    import torch
+    print("monitor in the step call", np.dtype(dtype).name, path, flush=True)
+    s1, m1 = _monitor_run(dtype, True, path)
+    s2, m2 = _monitor_run(dtype, False, path)
+    assert m1.time_index == m2.time_index == s1.Nt == s2.Nt
+    assert s2.last_path == ("fused" if path == "auto" else "general")
+    assert torch.equal(s1.E.array, s2.E.array)
+    for c in "xyz":
+        a, b = m1.get_components()["E" + c], m2.get_components()["E" + c]
+        assert torch.equal(a, b), c
+    assert bool((m2.get_components()["Ez"] != 0).any())

@@ -231,3 +231,62 @@ def test_adjoint_128x128x256_checkpoint_and_trajectory_gradients_agree():
     (La, ea, sa), (Lb, eb, sb) = out["trajectory"], out["checkpoint"]
     assert La == Lb and np.any(ea != 0) and np.all(np.isfinite(ea))
     assert np.array_equal(ea, eb) and np.array_equal(sa, sb)
+
+
+def test_conductive_and_box_over_4gib_fused_equals_general_path():
+    """512x512x1536 FP32 (4.83 GB per 3N array) with 10 PML layers on z: the
+    conductive instantiation of the 64-bit-offset kernel (J = sigma E, stored
+    only by the last step of a multi-step call) equals the general path bit for
+    bit from random E and H; and a multi-step call limited to a box (the
+    checkpoint recompute's launch) reproduces the full-grid run on the box
+    minus a margin of one cell per step."""
+    import torch
+
+    import wakis_amd as wa
+    dims, k = (512, 512, 1536), 3
+    print("over 4 GiB conductive and box", dims, flush=True)
+    grid = wa.GridFIT3D(-30e-3, 30e-3, -30e-3, 30e-3, -80e-3, 80e-3, *dims, verbose=0)
+    s = wa.SolverFIT3D(grid, bc_low=["pec", "pec", "pml"], bc_high=["pec", "pec", "pml"],
+                       dtype=np.float32, verbose=0, n_pml=10, path="general")
+    assert s.use_conductivity and 3 * s.N * 4 > 2 ** 32
+    gen = torch.Generator(device=s.device).manual_seed(9)
+    for F in (s.E, s.H):
+        F.fromarray(torch.randn(3 * s.N, generator=gen, device=s.device, dtype=F.array.dtype))
+    s.one_step()  # general step: ghosts zeroed, J = sigma E from here on
+    saved = [F.array.clone() for F in (s.E, s.H, s.J)]
+    print("solver built", flush=True)
+
+    def restore():
+        for F, X in zip((s.E, s.H, s.J), saved):
+            F.array.copy_(X)
+            F.version += 1
+        s._sigmaE_versions = (s.E.version, s.J.version)
+
+    s.path = "fused"
+    s._run(s._step_args(k))
+    assert s.last_path == "fused"
+    fused = [F.array.clone() for F in (s.E, s.H, s.J)]
+    restore()
+    s.path = "general"
+    s._run(s._step_args(k))
+    assert s.last_path == "general"
+    for F, X, name in zip((s.E, s.H, s.J), fused, "EHJ"):
+        assert bool(torch.equal(F.array, X)), name
+    assert bool(fused[2].abs().max().item() > 0)
+    print("conductive fused == general", flush=True)
+    # box-limited multi-step call: the cells [lo + k + 1, hi - k - 1) equal the full run
+    restore()
+    s.path = "fused"
+    a = s._step_args(k)
+    lo, hi = (100, 180, 700), (300, 330, 760)
+    a.box_lo[:] = lo
+    a.box_hi[:] = hi
+    s._run(a)
+    assert s.last_path == "fused"
+    inner = (slice(None),) + tuple(slice(l + k + 1, h - k - 1) for l, h in
+                                   zip(reversed(lo), reversed(hi)))
+    for F, X, name in zip((s.E, s.H), fused, "EH"):
+        got = F.array.view(3, dims[2], dims[1], dims[0])[inner]
+        ref = X.view(3, dims[2], dims[1], dims[0])[inner]
+        assert bool(torch.equal(got, ref)), name
+    print("box-limited call == full run inside the box", flush=True)

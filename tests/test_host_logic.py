@@ -169,3 +169,26 @@ def test_adjoint_kernel_choice_past_4gib(host_only, monkeypatch, dtype):
         adjm.WakeAdjoint(s, region, 0.03, adjoint_dtype="float32")._adjoint_f32(False)
     monkeypatch.setattr(adjm, "FUSED_MAX_BYTES", 3 * s.N * 8 + 1)
     assert adjm.WakeAdjoint(s, region, 0.03, kernel="fused")._use_fused_kernel()
+
+
+@pytest.mark.parametrize("freqs", [[0.7e9, 2.1e9, 4.0e9], np.array([1e9, 3e9], np.float32)])
+@pytest.mark.parametrize("dt", [np.float32(1.234567e-12), np.float64(1.234567e-12),
+                                1.234567e-12])
+def test_monitor_phase_table_is_the_per_update_scalar_expression(freqs, dt):
+    """FieldMonitor's vectorised phase table equals, value and dtype, the
+    reference's per-update scalar exp(-2j*pi*f*t), t = time_index*dt
+    (field_monitors.py:90-97): complex64 for an FP32 dt (or FP32 frequencies
+    with a Python float dt), complex128 otherwise."""
+    m = wa.FieldMonitor(freqs)
+    m.dt, m.time_index = dt, 5
+    ph = m._phase_values(1500)
+    for k in range(1500):
+        t = (5 + k) * m.dt
+        for i, fr in enumerate(m.frequencies):
+            sc = np.exp(-2j * np.pi * fr * t)
+            assert np.asarray(sc).dtype == ph.dtype
+            assert sc == ph[k, i], (k, i)
+    want = np.complex64 if (isinstance(dt, np.float32) or
+                            (not isinstance(dt, np.floating) and
+                             np.asarray(freqs).dtype == np.float32)) else np.complex128
+    assert ph.dtype == want

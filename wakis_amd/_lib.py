@@ -43,6 +43,8 @@ class WfStepArgs(ctypes.Structure):
         ("traj_slots", c_int64), ("traj_slot0", c_int64), ("cells_per_lane", c_int32),
         ("full_rows", c_int32), ("box_lo", c_int32 * 3), ("box_hi", c_int32 * 3),
         ("code_escape", c_int32),
+        ("mon_acc", c_void_p), ("mon_phase", c_void_p), ("mon_row0", c_int64),
+        ("mon_nf", c_int32), ("mon_phase_f32", c_int32),
     ]
 
 
@@ -117,7 +119,7 @@ EXPORTS = {
                                      c_double, c_double, c_double, c_int64, c_void_p, c_void_p]),
     "wf_idft_real": (c_int32, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p]),
     "wf_field_monitor_update": (c_int32, [c_void_p, c_int32, c_int64, c_void_p, c_int32,
-                                          c_void_p, c_void_p]),
+                                          c_int32, c_void_p, c_void_p]),
 }
 
 _lib = None

@@ -42,10 +42,13 @@ struct AdjTile {
 
 constexpr int ADJ_TX = 64;  // owned cells per tile row
 
-// is the component-c row of cell (i, j, k) kept by the zero-face bits (the
-// face_mask of the general path)?
-__device__ __forceinline__ bool keep_row(int bits, int c, int i, int j, int k, const GridDev &g) {
-  return bits == 0 || face_mask(bits, c, i, j, k, g.Nx, g.Ny, g.Nz);
+// a copy of v in a vector register, made anew where it is used: values derived
+// from it inside a loop are not hoisted out of it (and not held as lane masks
+// across it)
+__device__ __forceinline__ uint32_t vreg_copy(uint32_t v) {
+  uint32_t r;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
+  return r;
 }
 
 // Everything one reverse step of one grid reads (kernel argument by value).
@@ -133,14 +136,42 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
   const I col = valid ? (I)i + (I)g.sy * (I)j : 0;
   for (int u = threadIdx.x; u < 3 * (NW + 1) * NS; u += 64 * (NW + 1))
     (&sP[0][0][0])[u] = C(0);
-  extern __shared__ unsigned char s_dyn[];
-  S *sMat = reinterpret_cast<S *>(s_dyn);
+  // LUT: per material 12 values in three 16-byte groups, [ieps x y z, -],
+  // [imu x y z, -], [kappa (or sigma) x y z, -]; kappa is the per-cell fold
+  // (-dt ieps) sigma done once per material and component
+  extern __shared__ __align__(16) unsigned char s_adj_tab[];
+  S *sMat = reinterpret_cast<S *>(s_adj_tab);
+  const bool kap = kappa != nullptr;
   if (LUT) {
-    for (int u = threadIdx.x; u < p.nmat * 9; u += 64 * (NW + 1)) sMat[u] = p.mat[u];
+    for (int u = threadIdx.x; u < p.nmat * 12; u += 64 * (NW + 1)) {
+      const int cd = u / 12, r = u - 12 * cd, grp = r >> 2, c = r & 3;
+      const S *mt = p.mat + 9 * cd;
+      S v = S(0);
+      if (c < 3) {
+        if (grp == 0) v = mt[3 + c];
+        else if (grp == 1) v = mt[c];
+        else v = kap ? (S)((-dtd * (double)mt[3 + c]) * (double)mt[6 + c]) : mt[6 + c];
+      }
+      sMat[u] = v;
+    }
   }
+  using S4 = S __attribute__((ext_vector_type(4)));
+  const auto mgroup = [&](int cd, int grp) -> S4 {
+    return *reinterpret_cast<const S4 *>(sMat + 12 * cd + 4 * grp);
+  };
 
   const C tLx = valid ? (C)m.tLx[i] : C(0), tLy = valid ? (C)m.tLy[j] : C(0);
   const C Lx = valid ? (C)m.Lx[i] : C(0), Ly = valid ? (C)m.Ly[j] : C(0);
+  // the zero-face rows (the face_mask of the general path: PMC-masked lH rows
+  // keep lH, PEC-masked W rows drop the curl term), split into the in-plane
+  // part (per thread, one bit each: bits 0..2 the lH rows x, y, z, bits 3..5
+  // the W rows) and the z-face part (per plane)
+  const uint32_t kbits = (uint32_t)(ypart(g.pmcz, j, g.Ny) != 0) |
+                         ((uint32_t)(xpart(g.pmcz, i, g.Nx) != 0) << 1) |
+                         ((uint32_t)mz(g.pmcz, i, j, g) << 2) |
+                         ((uint32_t)(ypart(g.pecz, j, g.Ny) != 0) << 3) |
+                         ((uint32_t)(xpart(g.pecz, i, g.Nx) != 0) << 4) |
+                         ((uint32_t)mz(g.pecz, i, j, g) << 5);
   const bool online = own && (i == x.ixs) && (j == x.iys);
   const bool capcol = gcap && own && x.cap_sample < x.cap_samples && i >= x.cap_ix0 &&
                       i < x.cap_ix0 + x.cap_nx && j >= x.cap_iy0 && j < x.cap_iy0 + x.cap_ny;
@@ -152,120 +183,147 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
   const bool cond = x.conductive != 0;
   // kappa = -dt ieps sigma per edge replaces the sigma and ieps reads (ieps is
   // then read on the region only, for lJ); sg[] holds kappa in that case
-  const bool kap = kappa != nullptr;
   const S *sgsrc = kap ? kappa : sigma;
   const auto in_region = [&](int p) { return regcol && p >= x.r0[2] && p < x.r1[2]; };
 
   // plane p's offset of this cell, clamped into the grid (masked at use)
   const auto off = [&](int p) { return col + sz * (I)min(p, g.Nz - 1); };
   const int kstart = (k0 > 0) ? k0 - 1 : k0;
-  // LUT: material codes of planes k, k+1 (k+2 in the prefetch) and the
-  // metric those planes need; escaped cells (code 255) load the arrays instead
-  const int ii = valid ? i : 0, jj = valid ? j : 0;
-  const auto kc = [&](int p) { return min(p, g.Nz - 1); };
+  // LUT: material codes of planes k, k+1 (k+2 in the prefetch) and the metric
+  // those planes need (32-bit offsets into the 2-D tables); escaped cells (code
+  // 255) read the arrays where the value is used, in branches that waves
+  // without such cells skip. Without codes the arrays are prefetched.
+  const I ii = valid ? (I)i : 0, jj = valid ? (I)j : 0, nx = (I)g.Nx, ny = (I)g.Ny;
+  const auto kc = [&](int p) { return (I)min(p, g.Nz - 1); };
   const auto esc_at = [&](int cd) { return LUT && p.esc && cd == 255; };
+  const auto tab = [&](const S *t, I idx) { return *at(t, idx); };
   int ck = 0, ck1 = 0;
   S bxk = 0, bxk1 = 0, byk = 0, byk1 = 0, axk = 0, ayk = 0, bz = 0, az = 0;
   if (LUT) {
     ck = p.code[off(kstart)];
     ck1 = p.code[off(kstart + 1)];
-    bz = p.itAz[ii + g.Nx * jj];
-    az = p.iAz[ii + g.Nx * jj];
-    bxk = p.itAx[jj + g.Ny * kc(kstart)]; bxk1 = p.itAx[jj + g.Ny * kc(kstart + 1)];
-    byk = p.itAy[ii + g.Nx * kc(kstart)]; byk1 = p.itAy[ii + g.Nx * kc(kstart + 1)];
-    axk = p.iAx[jj + g.Ny * kc(kstart)];
-    ayk = p.iAy[ii + g.Nx * kc(kstart)];
+    bz = tab(p.itAz, ii + nx * jj);
+    az = tab(p.iAz, ii + nx * jj);
+    bxk = tab(p.itAx, jj + ny * kc(kstart)); bxk1 = tab(p.itAx, jj + ny * kc(kstart + 1));
+    byk = tab(p.itAy, ii + nx * kc(kstart)); byk1 = tab(p.itAy, ii + nx * kc(kstart + 1));
+    axk = tab(p.iAx, jj + ny * kc(kstart));
+    ayk = tab(p.iAy, ii + nx * kc(kstart));
   }
-  const bool arr0 = !LUT || esc_at(ck), arr1 = !LUT || esc_at(ck1);
   A Wk[3], Wk1[3], lh[3];
-  S bk[3], bk1[3], ac[3], ie[3], sg[3];
+  S bk1[3], ac[3], ie[3], sg[3];
+  C Pk[3];  // P of plane k (computed once, as P(k+1) of the plane before)
   {
     const I n0 = off(kstart), n1 = off(kstart + 1);
+    S bk[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       Wk[c] = *at(Win, (I)c * N + n0);
-      bk[c] = arr0 ? *at(b, (I)c * N + n0) : S(0);
       Wk1[c] = *at(Win, (I)c * N + n1);
-      bk1[c] = arr1 ? *at(b, (I)c * N + n1) : S(0);
       lh[c] = need_lh ? *at(lHin, (I)c * N + n0) : A(0);
-      ac[c] = (need_lh && arr0) ? *at(a, (I)c * N + n0) : S(0);
-      ie[c] = (need_c && arr0 && (!kap || in_region(kstart))) ? *at(ieps, (I)c * N + n0) : S(0);
-      sg[c] = (need_c && arr0) ? *at(sgsrc, (I)c * N + n0) : S(0);
+      if (LUT) {
+        bk[c] = bk1[c] = ac[c] = ie[c] = sg[c] = S(0);
+      } else {
+        bk[c] = *at(b, (I)c * N + n0);
+        bk1[c] = *at(b, (I)c * N + n1);
+        ac[c] = need_lh ? *at(a, (I)c * N + n0) : S(0);
+        ie[c] = (need_c && (!kap || in_region(kstart))) ? *at(ieps, (I)c * N + n0) : S(0);
+        sg[c] = need_c ? *at(sgsrc, (I)c * N + n0) : S(0);
+      }
+    }
+    if (esc_at(ck)) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) bk[c] = *at(b, (I)c * N + n0);
+    }
+    __syncthreads();  // zeroed LDS, material table
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const S bv = (!LUT || esc_at(ck))
+                       ? bk[c]
+                       : (S)(mgroup(ck, 0)[c] * (c == 0 ? bxk : (c == 1 ? byk : bz)));
+      Pk[c] = valid ? (C)bv * (C)Wk[c] : C(0);
     }
   }
-  // coefficients of a plane from its code (or the loaded arrays when escaped)
+  // coefficients of plane k from its code (escaped cells and the array path:
+  // the arrays)
   const bool onl = valid && (i == x.ixs) && (j == x.iys);  // beam line: kappa_z = 0
-  const auto bval = [&](int c, int cd, S bx, S by, S arr) -> S {
-    if (!LUT || esc_at(cd)) return arr;
-    return (S)(sMat[9 * cd + 3 + c] * (c == 0 ? bx : (c == 1 ? by : bz)));
-  };
-  const auto aval = [&](int c, int cd, S ax, S ay, S arr) -> S {
-    if (!LUT || esc_at(cd)) return arr;
-    return (S)((c == 0 ? ax : (c == 1 ? ay : az)) * sMat[9 * cd + c]);
-  };
-  const auto ieval = [&](int c, int cd, S arr) -> S {
-    return (!LUT || esc_at(cd)) ? arr : sMat[9 * cd + 3 + c];
-  };
-  const auto sgval = [&](int c, int cd, S arr) -> S {  // kappa, or sigma
-    if (!LUT || esc_at(cd)) return arr;
-    if (!kap) return sMat[9 * cd + 6 + c];
-    if (c == 2 && onl) return S(0);
-    return (S)((-dtd * (double)sMat[9 * cd + 3 + c]) * (double)sMat[9 * cd + 6 + c]);
-  };
   C Qpx = 0, Qpy = 0;  // Q(k-1) of this cell, x and y components
-  __syncthreads();          // zeroed LDS
 
   for (int k = kstart; k < kend; ++k) {
     const bool body = (k >= k0);
     const bool kp1 = (k + 1 < g.Nz);
-    // ---- prefetch: W, b of plane k+2; lH, a, ieps, sigma of plane k+1
+    const I nk = off(k), nk1 = off(k + 1);
+    // ---- this plane's loads first (escaped cells' arrays, the z metric), so
+    // that waiting for them never waits for the prefetch issued below
+    if (esc_at(ck1)) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) bk1[c] = *at(b, (I)c * N + nk1);
+    }
+    if (esc_at(ck)) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        ac[c] = *at(a, (I)c * N + nk);
+        ie[c] = (cond && need_c && (!kap || in_region(k))) ? *at(ieps, (I)c * N + nk) : S(0);
+        sg[c] = (cond && need_c) ? *at(sgsrc, (I)c * N + nk) : S(0);
+      }
+    }
+    const C tLz = (C)m.tLz[k], Lz = (C)m.Lz[k];
+    // ---- stage P(k); P(k+1) of this cell (the z+1 terms, P(k) of the next plane)
+    if (inrow) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sP[c][row][slot] = Pk[c];
+    }
+    C Pk1[3];
+    {
+      S bv[3];
+      const S4 ie1 = LUT ? mgroup(esc_at(ck1) ? 0 : ck1, 0) : S4{};
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        bv[c] = (!LUT || esc_at(ck1))
+                    ? bk1[c]
+                    : (S)(ie1[c] * (c == 0 ? bxk1 : (c == 1 ? byk1 : bz)));
+#pragma unroll
+      for (int c = 0; c < 3; ++c) Pk1[c] = (valid && kp1) ? (C)bv[c] * (C)Wk1[c] : C(0);
+    }
+    // ---- prefetch: W (and b without codes) of plane k+2; lH (and a, ieps,
+    // sigma/kappa without codes) of plane k+1; codes and metric
     A Wn2[3], lhN[3];
     S bn2[3], acN[3], ieN[3], sgN[3];
     int cn2 = 0;
     S bxn2 = 0, byn2 = 0, axN = 0, ayN = 0;
     {
-      const I n2 = off(k + 2), n1 = off(k + 1);
+      const I n2 = off(k + 2);
       if (LUT) {
         cn2 = p.code[n2];
-        bxn2 = p.itAx[jj + g.Ny * kc(k + 2)];
-        byn2 = p.itAy[ii + g.Nx * kc(k + 2)];
-        axN = p.iAx[jj + g.Ny * kc(k + 1)];
-        ayN = p.iAy[ii + g.Nx * kc(k + 1)];
+        bxn2 = tab(p.itAx, jj + ny * kc(k + 2));
+        byn2 = tab(p.itAy, ii + nx * kc(k + 2));
+        axN = tab(p.iAx, jj + ny * kc(k + 1));
+        ayN = tab(p.iAy, ii + nx * kc(k + 1));
       }
-      const bool arrn2 = !LUT || esc_at(cn2), arrn1 = !LUT || esc_at(ck1);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         Wn2[c] = *at(Win, (I)c * N + n2);
-        bn2[c] = arrn2 ? *at(b, (I)c * N + n2) : S(0);
-        lhN[c] = need_lh ? *at(lHin, (I)c * N + n1) : A(0);
-        acN[c] = (need_lh && arrn1) ? *at(a, (I)c * N + n1) : S(0);
-        if (cond && need_c && arrn1) {
-          ieN[c] = (!kap || in_region(k + 1)) ? *at(ieps, (I)c * N + n1) : S(0);
-          sgN[c] = *at(sgsrc, (I)c * N + n1);
+        lhN[c] = need_lh ? *at(lHin, (I)c * N + nk1) : A(0);
+        if (LUT) {
+          bn2[c] = acN[c] = ieN[c] = sgN[c] = S(0);
         } else {
-          ieN[c] = S(0); sgN[c] = S(0);
+          bn2[c] = *at(b, (I)c * N + n2);
+          acN[c] = need_lh ? *at(a, (I)c * N + nk1) : S(0);
+          if (cond && need_c) {
+            ieN[c] = (!kap || in_region(k + 1)) ? *at(ieps, (I)c * N + nk1) : S(0);
+            sgN[c] = *at(sgsrc, (I)c * N + nk1);
+          } else {
+            ieN[c] = S(0); sgN[c] = S(0);
+          }
         }
       }
     }
-    // ---- stage P(k)
-    C Pk[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      Pk[c] = valid ? (C)bval(c, ck, bxk, byk, bk[c]) * (C)Wk[c] : C(0);
-    if (inrow) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) sP[c][row][slot] = Pk[c];
-    }
-    const C tLz = (C)m.tLz[k], Lz = (C)m.Lz[k];
+    const uint32_t kb = vreg_copy(kbits);
+    const int zh = zface(g.pmcz, k, g.Nz), zp = zface(g.pecz, k, g.Nz);
     __syncthreads();
 
     // ---- lH'(k) and Q(k) (row waves: all 64 columns; halo wave: column i0-1)
     C hn[3] = {0, 0, 0};
     if (makes_q) {
-      C Pk1[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-        Pk1[c] = (valid && kp1) ? (C)bval(c, ck1, bxk1, byk1, bk1[c]) * (C)Wk1[c] : C(0);
       // h_terms order: x: +Py, -Py(z+1), -Pz, +Pz(y+1); y: -Px, +Px(z+1), +Pz, -Pz(x+1);
       // z: +Px, -Px(y+1), -Py, +Py(x+1)
       C s0 = 0;
@@ -276,12 +334,18 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
       s2 = s2 + Pk[0]; s2 = s2 - sP[0][row + 1][slot]; s2 = s2 - Pk[1];
       s2 = s2 + sP[1][row][slot + 1];
       const C h0 = (C)lh[0], h1 = (C)lh[1], h2 = (C)lh[2];
-      hn[0] = keep_row(g.pmcz, 0, i, j, k, g) ? h0 + dt * (tLx * s0) : h0;
-      hn[1] = keep_row(g.pmcz, 1, i, j, k, g) ? h1 + dt * (tLy * s1) : h1;
-      hn[2] = keep_row(g.pmcz, 2, i, j, k, g) ? h2 + dt * (tLz * s2) : h2;
+      hn[0] = zsel(zh, kb & 1u) ? h0 + dt * (tLx * s0) : h0;
+      hn[1] = zsel(zh, (kb >> 1) & 1u) ? h1 + dt * (tLy * s1) : h1;
+      hn[2] = ((kb >> 2) & 1u) ? h2 + dt * (tLz * s2) : h2;
+      S av[3];
+      const S4 imu = LUT ? mgroup(esc_at(ck) ? 0 : ck, 1) : S4{};
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        sQ[c][row][slot] = valid ? (C)aval(c, ck, axk, ayk, ac[c]) * hn[c] : C(0);
+        av[c] = (!LUT || esc_at(ck))
+                    ? ac[c]
+                    : (S)((c == 0 ? axk : (c == 1 ? ayk : az)) * imu[c]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sQ[c][row][slot] = valid ? (C)av[c] * hn[c] : C(0);
     }
     __syncthreads();
 
@@ -298,9 +362,9 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
       e1 = e1 - Qpx; e1 = e1 + Qx; e1 = e1 + sQ[2][row][slot - 1]; e1 = e1 - Qz;
       C e2 = 0;
       e2 = e2 + sQ[0][row - 1][slot]; e2 = e2 - Qx; e2 = e2 - sQ[1][row][slot - 1]; e2 = e2 + Qy;
-      const C m0 = keep_row(g.pecz, 0, i, j, k, g) ? Lx * e0 : C(0);
-      const C m1 = keep_row(g.pecz, 1, i, j, k, g) ? Ly * e1 : C(0);
-      const C m2 = keep_row(g.pecz, 2, i, j, k, g) ? Lz * e2 : C(0);
+      const C m0 = zsel(zp, (kb >> 3) & 1u) ? Lx * e0 : C(0);
+      const C m1 = zsel(zp, (kb >> 4) & 1u) ? Ly * e1 : C(0);
+      const C m2 = ((kb >> 5) & 1u) ? Lz * e2 : C(0);
       const C wv[3] = {(C)Wk[0], (C)Wk[1], (C)Wk[2]};
       C le[3] = {wv[0] - dt * m0, wv[1] - dt * m1, wv[2] - dt * m2};
       if (capcol) le[2] += (C)gcap[(capbase + x.cap_sample) * g.Nz + k];
@@ -311,9 +375,21 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
                                : 0;
       C ljv[3] = {0, 0, 0};
       if (cond) {
+        S iev[3], sgv[3];
+        if (!LUT || esc_at(ck)) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) { iev[c] = ie[c]; sgv[c] = sg[c]; }
+        } else {
+          const S4 ie4 = mgroup(ck, 0), sg4 = mgroup(ck, 2);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            iev[c] = ie4[c];
+            sgv[c] = (kap && c == 2 && onl) ? S(0) : sg4[c];
+          }
+        }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          const C iec = (C)ieval(c, ck, ie[c]), sgc = (C)sgval(c, ck, sg[c]);
+          const C iec = (C)iev[c], sgc = (C)sgv[c];
           ljv[c] = (c == 2 && online) ? C(0) : -dt * iec * wv[c];
           wn[c] = kap ? le[c] + sgc * wv[c] : le[c] + sgc * ljv[c];
         }
@@ -334,11 +410,15 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
           for (int c = 0; c < 3; ++c) lJr[c * nreg + tr] = (double)ljv[c];
         }
       }
-      const I nk = col + sz * (I)k;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        *atw(Wout, (I)c * N + nk) = (A)wn[c];
-        *atw(lHout, (I)c * N + nk) = (A)hn[c];
+        if constexpr (sizeof(A) == 8) {  // FP64 outputs streamed past the caches
+          __builtin_nontemporal_store((A)wn[c], atw(Wout, (I)c * N + nk));
+          __builtin_nontemporal_store((A)hn[c], atw(lHout, (I)c * N + nk));
+        } else {  // (4-byte streaming stores measured slower: 91 against 77 us)
+          *atw(Wout, (I)c * N + nk) = (A)wn[c];
+          *atw(lHout, (I)c * N + nk) = (A)hn[c];
+        }
       }
     }
     Qpx = Qx;  // Q(k) of this cell seeds the z-1 terms of plane k+1
@@ -348,7 +428,8 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
     // sQ after its own first barrier (every read of this plane's sQ is before it).
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      Wk[c] = Wk1[c]; bk[c] = bk1[c];
+      Pk[c] = Pk1[c];
+      Wk[c] = Wk1[c];
       Wk1[c] = Wn2[c]; bk1[c] = bn2[c];
       lh[c] = lhN[c]; ac[c] = acN[c]; ie[c] = ieN[c]; sg[c] = sgN[c];
     }
@@ -360,8 +441,17 @@ __device__ __forceinline__ void adj_fused_tile(const AdjStep<S, A> &p, const int
   }
 }
 
+// waves per SIMD each row count is built for. FP32 arithmetic (79 VGPRs):
+// two workgroups per CU with 7 or 9 rows (10 or 12 waves each), three with 5
+// (8 waves), one with 13 (16 waves); FP64 arithmetic: one workgroup per CU
+// (more would spill registers), and likewise with FP64 coefficients.
+constexpr int adj_waves_per_simd(int TY, bool c64) {
+  return c64 ? (TY + 3 + 3) / 4 : (TY == 5 ? 6 : TY == 7 ? 5 : TY == 9 ? 6 : 4);
+}
+
 template <typename S, typename A, int TY, bool LUT = false, typename C = double>
-__global__ void __launch_bounds__(64 * (TY + 3)) k_adj_fused(AdjStep<S, A> p) {
+__global__ void __launch_bounds__(64 * (TY + 3), adj_waves_per_simd(TY, sizeof(C) == 8 || sizeof(S) == 8))
+k_adj_fused(AdjStep<S, A> p) {
   adj_fused_tile<S, A, TY, LUT, C>(p, xcd_tile(blockIdx.x, p.t.ntx * p.t.nty * p.t.ntz));
 }
 

@@ -21,8 +21,12 @@ wf::AdjAux adj_aux(const wf_adjoint_args *a) {
   return x;
 }
 
-constexpr int ADJ_ROWS = 13;  // owned rows per workgroup of the single-pass reverse kernel
-                               // (16 waves with the halo rows and column; measured best)
+// owned rows per workgroup of the single-pass reverse kernel, measured at
+// 128x128x256 (the same gradients for every count): FP32 arithmetic 9 rows (12
+// waves with the halo rows and column, two workgroups per CU: 46.4 us per
+// step against 52.3 with 7, 53.9 with 13, 56.7 with 5); FP64 13 (161.0 us; 7
+// and 5 the same, 9 168.0)
+constexpr int ADJ_ROWS_F32 = 9, ADJ_ROWS_F64 = 13;
 
 template <typename S, typename A>
 wf::AdjStep<S, A> adj_step_params(const wf_handle *h, const wf_adjoint_args *a,
@@ -68,8 +72,9 @@ int adjoint_fused(wf_handle *h, const wf_adjoint_args *a, const wf::MetricDev<S>
   // planes in 64 KiB of LDS; otherwise the coefficient arrays, with identical
   // results (measured at 128x128x256: FP32 88 against 96 us per step; with
   // FP64 coefficients the code path was 1.5x slower than the arrays)
+  // the material table: 12 values per material (ieps, imu, kappa or sigma)
   const size_t table = (a->code && a->mat && a->nmat > 0 && a->code_metric2d)
-                           ? (size_t)a->nmat * 9 * sizeof(S) : 0;
+                           ? (size_t)a->nmat * 12 * sizeof(S) : 0;
   const bool lut = sizeof(S) == 4 && table > 0 && adj_static_lds<C>(TY) + table <= 65536;
   void (*kern)(wf::AdjStep<S, A>) = &wf::k_adj_fused<S, A, TY, false, C>;
   if constexpr (sizeof(S) == 4) {
@@ -112,18 +117,25 @@ int adjoint_step(wf_handle *h, const wf_adjoint_args *a, const wf::MetricDev<S> 
     WF_LAUNCH_CHECK();
   }
   if (a->fused) {
-    const int rows = a->fused == 1 ? ADJ_ROWS : a->fused;
+    const int rows =
+        a->fused != 1 ? a->fused : ((a->fields_f32 && a->compute_f32) ? ADJ_ROWS_F32 : ADJ_ROWS_F64);
     if (a->fields_f32 && a->compute_f32) {
       if (rows == 5) return adjoint_fused<S, float, 5, float>(h, a, m, x, st);
+      if (rows == 7) return adjoint_fused<S, float, 7, float>(h, a, m, x, st);
+      if (rows == 9) return adjoint_fused<S, float, 9, float>(h, a, m, x, st);
       if (rows == 13) return adjoint_fused<S, float, 13, float>(h, a, m, x, st);
     } else if (a->fields_f32) {
       if (rows == 5) return adjoint_fused<S, float, 5, double>(h, a, m, x, st);
+      if (rows == 7) return adjoint_fused<S, float, 7, double>(h, a, m, x, st);
+      if (rows == 9) return adjoint_fused<S, float, 9, double>(h, a, m, x, st);
       if (rows == 13) return adjoint_fused<S, float, 13, double>(h, a, m, x, st);
     } else {
       if (rows == 5) return adjoint_fused<S, double, 5, double>(h, a, m, x, st);
+      if (rows == 7) return adjoint_fused<S, double, 7, double>(h, a, m, x, st);
+      if (rows == 9) return adjoint_fused<S, double, 9, double>(h, a, m, x, st);
       if (rows == 13) return adjoint_fused<S, double, 13, double>(h, a, m, x, st);
     }
-    return fail(WF_EINVAL, "wf_adjoint_step: fused must be 0, 1, 5 or 13 (rows per workgroup)");
+    return fail(WF_EINVAL, "wf_adjoint_step: fused must be 0, 1, 5, 7, 9 or 13 (rows per workgroup)");
   }
   const unsigned nb = nblocks(g.N, 256);
   hipLaunchKernelGGL((wf::k_adj_h<S>), dim3(nb), dim3(256), 0, st, g, m, a->lH,

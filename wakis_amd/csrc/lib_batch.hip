@@ -191,6 +191,7 @@ int wf_batch_create(wf_handle *const *handles, const wf_step_args *args, int32_t
     const int rc = check_step_args(h, a, who);
     if (rc != WF_OK) return rc;
     if (a->path != 1) return fail(WF_EINVAL, "%s: batch items use the fused path (path 1)", who);
+    if (a->mon_acc) return fail(WF_EINVAL, "%s: batch items carry no field monitor", who);
     const bool lut = a->code && a->mat && a->nmat > 0 && a->metric2d;
     if (lut != (a0.code && a0.mat && a0.nmat > 0 && a0.metric2d))
       return fail(WF_EINVAL, "%s: all batch items use material codes, or none", who);

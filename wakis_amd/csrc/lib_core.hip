@@ -179,6 +179,10 @@ int step_general(wf_handle *h, const wf_step_args *a, const wf::MetricDev<S> &m,
       traj_store<S>(h, a, (const S *)a->E, (const S *)a->H, a->traj_slot0 + s + 1, st);
       WF_LAUNCH_CHECK();
     }
+    if (a->mon_acc) {
+      monitor_store<S>(h, a, (const S *)a->E, s, st);
+      WF_LAUNCH_CHECK();
+    }
   }
   return WF_OK;
 }
@@ -259,6 +263,10 @@ int check_step_args(const wf_handle *h, const wf_step_args *a, const char *who) 
                   (long long)(a->traj_slot0 + 1), (long long)(a->traj_slot0 + a->nsteps),
                   (long long)a->traj_slots);
   }
+  if (a->mon_acc && (!a->mon_phase || a->mon_nf < 1 || a->mon_row0 < 0 ||
+                     (a->mon_phase_f32 != 0 && a->mon_phase_f32 != 1)))
+    return fail(WF_EINVAL, "%s: the field monitor needs a phase table, mon_nf >= 1, "
+                "mon_row0 >= 0 and mon_phase_f32 0 or 1", who);
   if (a->path == 1) {
     if (!a->E2 || !a->H2 || a->E2 == a->E || a->H2 == a->H || a->E2 == a->H2)
       return fail(WF_EINVAL, "%s: fused path needs distinct E2/H2 buffers", who);
@@ -496,20 +504,16 @@ int wf_loss_factor(const double *WP, const double *lam, int64_t n, double ds, do
 
 extern "C" {
 
-int wf_field_monitor_update(const void *E, int32_t dtype, int64_t N, const double *phase,
-                            int32_t nfreq, double *acc, void *stream) {
+int wf_field_monitor_update(const void *E, int32_t dtype, int64_t N, const void *phase,
+                            int32_t phase_f32, int32_t nfreq, double *acc, void *stream) {
   if (!E || !phase || !acc) return fail(WF_EINVAL, "wf_field_monitor_update: null argument");
   if (N < 1 || nfreq < 0) return fail(WF_ERANGE, "wf_field_monitor_update: sizes");
   if (dtype != WF_F64 && dtype != WF_F32)
     return fail(WF_EINVAL, "wf_field_monitor_update: dtype must be WF_F64 or WF_F32");
+  if (phase_f32 != 0 && phase_f32 != 1)
+    return fail(WF_EINVAL, "wf_field_monitor_update: phase_f32 must be 0 or 1");
   if (nfreq == 0) return WF_OK;
-  const dim3 grid(nblocks(3 * N, 256));
-  if (dtype == WF_F64)
-    hipLaunchKernelGGL(wf::k_monitor<double>, grid, dim3(256), 0, (hipStream_t)stream,
-                       (const double *)E, N, phase, nfreq, acc);
-  else
-    hipLaunchKernelGGL(wf::k_monitor<float>, grid, dim3(256), 0, (hipStream_t)stream,
-                       (const float *)E, N, phase, nfreq, acc);
+  monitor_launch(E, dtype == WF_F32, N, phase, phase_f32 != 0, nfreq, acc, (hipStream_t)stream);
   WF_LAUNCH_CHECK();
   return WF_OK;
 }

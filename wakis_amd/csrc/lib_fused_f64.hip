@@ -162,6 +162,10 @@ int step_fused(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStre
       traj_store<S>(h, a, Ebuf[1 - cur], Hbuf[1 - cur], a->traj_slot0 + s + 1, st);
       WF_LAUNCH_CHECK();
     }
+    if (a->mon_acc) {
+      monitor_store<S>(h, a, Ebuf[1 - cur], s, st);
+      WF_LAUNCH_CHECK();
+    }
     cur = 1 - cur;
   }
   a->result_swapped = cur;

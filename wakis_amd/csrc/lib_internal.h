@@ -14,6 +14,7 @@
 #include "fit_common.h"
 #include "step_general.h"
 #include "adjoint.h"
+#include "monitor.h"
 
 #define WF_HIDDEN __attribute__((visibility("hidden")))
 
@@ -61,6 +62,36 @@ WF_HIDDEN void traj_store(const wf_handle *h, const wf_step_args *a, const S *E,
   const int64_t tot = 3 * (b.nE + b.nH);
   hipLaunchKernelGGL((wf::k_traj_gather<S>), dim3(nblocks(tot, 256)), dim3(256), 0, st, h->g,
                      E, H, b, (S *)a->traj_E + slot * 3 * b.nE, (S *)a->traj_H + slot * 3 * b.nH);
+}
+
+// One monitor accumulation: E (FP32 or FP64) times a row of nf phases (FP32
+// or FP64 (re, im) pairs).
+inline void monitor_launch(const void *E, bool e32, int64_t N, const void *ph, bool p32, int nf,
+                           double *acc, hipStream_t st) {
+  const dim3 grid(nblocks(3 * N, 256));
+  if (e32 && p32)
+    hipLaunchKernelGGL((wf::k_monitor<float, float>), grid, dim3(256), 0, st, (const float *)E,
+                       N, (const float *)ph, nf, acc);
+  else if (e32)
+    hipLaunchKernelGGL((wf::k_monitor<float, double>), grid, dim3(256), 0, st,
+                       (const float *)E, N, (const double *)ph, nf, acc);
+  else if (p32)
+    hipLaunchKernelGGL((wf::k_monitor<double, float>), grid, dim3(256), 0, st,
+                       (const double *)E, N, (const float *)ph, nf, acc);
+  else
+    hipLaunchKernelGGL((wf::k_monitor<double, double>), grid, dim3(256), 0, st,
+                       (const double *)E, N, (const double *)ph, nf, acc);
+}
+
+// FieldMonitor accumulation of the E that step s of the call produced
+// (wf_step_args.mon_*).
+template <typename S>
+WF_HIDDEN void monitor_store(const wf_handle *h, const wf_step_args *a, const S *E, int64_t s,
+                             hipStream_t st) {
+  const size_t pb = a->mon_phase_f32 ? sizeof(float) : sizeof(double);
+  const char *ph = (const char *)a->mon_phase + (size_t)((a->mon_row0 + s) * 2 * a->mon_nf) * pb;
+  monitor_launch(E, sizeof(S) == 4, h->g.N, ph, a->mon_phase_f32 != 0, a->mon_nf, a->mon_acc,
+                 st);
 }
 
 // Workgroups of `kern` that one CU runs at once (cached per handle).

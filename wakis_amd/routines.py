@@ -3,8 +3,9 @@
 ``wakesolve`` keeps the whole step loop on the device: the beam current is
 uploaded once as an (Nt, Nz) table, the step kernel writes the beam line and
 captures the Ez window after every E update, and the wake potential/impedance
-are computed from that capture by device kernels. There is no per-step host
-traffic unless a callback or field monitor asks for it.
+are computed from that capture by device kernels; a field monitor accumulates
+inside the same step calls. There is no per-step host traffic unless a
+callback asks for it.
 """
 
 import time
@@ -75,19 +76,23 @@ class RoutinesMixin:
         n = 0 if resume is None else self._wake_resume(ctx, resume)
         print("Running electromagnetic time-domain simulation...")
         t0 = time.time()
-        per_step = callback is not None or (use_field_monitor and field_monitor is not None)
+        monitor = field_monitor if (use_field_monitor and field_monitor is not None) else None
+        per_step = callback is not None
         step = 1 if per_step else (Nt if chunk is None else int(chunk))
         while n < Nt:
             k = min(step, Nt - n)
             if every is not None:
                 k = min(k, every - n % every)
-            self._run(self._wake_args(ctx, n, k))
+            args = self._wake_args(ctx, n, k)
+            if monitor is not None:  # accumulated after every step of the call
+                phases = monitor._step_args(args, self.E, self.dt, k)
+            self._run(args)
             self.nstep += k
-            if per_step:
-                if use_field_monitor and field_monitor is not None:
-                    field_monitor.update(self.E, self.dt)
-                if callback is not None:
-                    callback(self, n * self.dt)
+            if monitor is not None:
+                monitor.time_index += k
+                del phases
+            if callback is not None:
+                callback(self, n * self.dt)
             n += k
             if every is not None and n % every == 0 and n < Nt:
                 self._wake_checkpoint(ctx, checkpoint, n)
