@@ -631,7 +631,9 @@ def wakesolve_fields(s, wakelength, q, sigmaz, beta, xsource, ysource, xtest, yt
 
     Returns a dict with the per-step Ez window (what ``Ez.h5`` would hold),
     shape (Nt, 3, 3, Nz) or (Nt, Nz) for ``compute_plane='longitudinal'``,
-    in the field dtype, and the ``t`` axis ``np.arange(0, Nt*dt, dt)``.
+    in the field dtype, the J_z window at the same point of each step (what
+    ``Jz.h5`` would hold with ``save_J``, routines.py:441-442), and the ``t``
+    axis ``np.arange(0, Nt*dt, dt)``.
     ``nsteps`` truncates the loop (the time axis keeps the full length).
     """
     ixt = np.abs(s.x - xtest).argmin()
@@ -648,10 +650,12 @@ def wakesolve_fields(s, wakelength, q, sigmaz, beta, xsource, ysource, xtest, yt
     nrun = Nt if nsteps is None else min(Nt, nsteps)
     Ez = comp(s.E, s.dims, "z")
     samples = np.zeros((nrun,) + Ez[xx, yy, :].shape, dtype=s.dtype)
+    jsamples = np.zeros_like(samples)
     for n in range(nrun):
         beam.update(s, n * s.dt)
         samples[n] = Ez[xx, yy, :]
+        jsamples[n] = comp(s.J, s.dims, "z")[xx, yy, :]
         s.one_step()
-    return dict(Ez=samples, t=t, Nt=Nt, tmax=tmax, ixt=ixt, iyt=iyt,
+    return dict(Ez=samples, Jz=jsamples, t=t, Nt=Nt, tmax=tmax, ixt=ixt, iyt=iyt,
                 xf=s.x[xx], yf=s.y[yy], zf=s.z, dx=s.dx, dy=s.dy, dz=s.dz,
                 ixs=beam.ixs, iys=beam.iys)

@@ -1,0 +1,157 @@
+"""GPU: the reference's HDF5 files from a device wakefield run.
+
+* ``Ez.h5`` (routines.py:376-389, 437-440): the window axes, dx, dy, dz, t and
+  one dataset '#nnnnn' per step with Ez on the test window before that step,
+  written from the device capture: every sample bit-identical to the oracle's
+  Ez window at the same point of its loop, in the field dtype.
+* ``Jz.h5`` with ``save_J`` (routines.py:391-401, 441-442): J_z on the window,
+  bit-identical to the oracle's J (beam line, sigma*E on a conductive wall and
+  in the PML, zero elsewhere).
+* A WakeSolver given only the Ez.h5 file (wakeSolver.py:422-554 read it with
+  h5py) reproduces the run's WP, WP_3d, WP_x, WP_y and Z bitwise.
+* save_state / load_state / read_state in HDF5 (solverFIT3D.py:886-993).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cases
+from oracle import fit_oracle as fo
+from oracle import wake_oracle as wo
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _run(kind, dtype, plane, tmp_path):
+    import wakis_amd as wa
+    if kind == "pillbox":  # conductive wall and PML: J = sigma*E off the beam line
+        x, y, z, vac, wall = cases.pillbox_geometry(25, 25, 48, r_cav=20e-3, half_len=15e-3,
+                                                    r_pipe=5e-3, wall_cells=3, half_x=25e-3,
+                                                    half_z=45e-3)
+        solids = [(wall, [1.0, 1.0, 10.0]), (vac, "vacuum")]
+        names = {"wall": (wall, [1.0, 1.0, 10.0]), "vac": (vac, "vacuum")}
+        bc = (["pec", "pec", "pml"], ["pec", "pec", "pml"])
+        q, sigmaz, beta, xsource, wakelength, skip = 1e-9, 15e-3, 0.8, 2e-3, 0.01, 3
+        kw = dict(n_pml=10)
+    else:
+        x, y, z, mask = cases.cavity_pipe_geometry(16, 16, 32)
+        names = {"cav": (mask, "vacuum")}
+        solids = [(mask, "vacuum")]
+        bc = (["pec", "pec", "abc"], ["pec", "pec", "abc"])
+        # a single captured line takes no skipped cells (wakeSolver.py:390-392)
+        q, sigmaz, beta, xsource, wakelength = 1e-9, 10e-3, 1.0, 0.0, 0.04
+        skip = 0 if plane == "longitudinal" else 2
+        kw = {}
+    so = fo.OracleSolver(x, y, z, *bc, bg="pec", solids=solids, dtype=dtype, **kw)
+    cap = fo.wakesolve_fields(so, wakelength, q, sigmaz, beta, xsource, 0.0, 0.0, 0.0,
+                              wo.wake_ti(sigmaz, beta), compute_plane=plane)
+    grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={k: m for k, (m, _) in names.items()},
+                        stl_materials={k: v for k, (_, v) in names.items()}, verbose=0)
+    wake = wa.WakeSolver(q=q, sigmaz=sigmaz, beta=beta, xsource=xsource, skip_cells=skip,
+                         save=True, results_folder=str(tmp_path / "results"), verbose=0)
+    sa = wa.SolverFIT3D(grid, wake=wake, bc_low=bc[0], bc_high=bc[1], use_stl=True, bg="pec",
+                        dtype=dtype, verbose=0, use_sibc=False, **kw)
+    sa.wakesolve(wakelength=wakelength, compute_plane=plane, save_J=True)
+    assert sa.use_conductivity == (kind == "pillbox")
+    return so, sa, cap, wake, dict(q=q, sigmaz=sigmaz, beta=beta, xsource=xsource, skip=skip,
+                                   wakelength=wakelength)
+
+
+@pytest.mark.parametrize("kind,dtype,plane", [
+    ("pillbox", np.float64, "both"), ("pillbox", np.float32, "both"),
+    ("cavity", np.float64, "longitudinal"), ("cavity", np.float32, "both")])
+def test_wakesolve_h5_files_match_oracle(kind, dtype, plane, tmp_path, monkeypatch):
+    from wakis_amd import h5io
+    _say("Ez.h5/Jz.h5", kind, np.dtype(dtype).name, plane)
+    monkeypatch.chdir(tmp_path)  # Jz.h5 goes to the working directory, as in the reference
+    so, sa, cap, wake, p = _run(kind, dtype, plane, tmp_path)
+    Nt = cap["Nt"]
+    assert sa.Nt == Nt and wake.Ez_file == str(tmp_path / "results") + "/Ez.h5"
+    for fname, want in ((wake.Ez_file, cap["Ez"]), (str(tmp_path / "Jz.h5"), cap["Jz"])):
+        with h5io.File(fname, "r") as hf:
+            keys = hf.keys()
+            samples = sorted(k for k in keys if k.startswith("#"))
+            assert samples == ["#" + str(n).zfill(5) for n in range(Nt)], fname
+            assert set(keys) - set(samples) == {"x", "y", "z", "dx", "dy", "dz", "t"}
+            assert np.array_equal(np.array(hf["t"]), cap["t"])
+            assert np.array_equal(np.array(hf["x"]), cap["xf"])
+            assert np.array_equal(np.array(hf["y"]), cap["yf"])
+            assert np.array_equal(np.array(hf["z"]), cap["zf"])
+            assert np.array_equal(np.array(hf["dz"]), cap["dz"])
+            for n in range(Nt):
+                a = hf[samples[n]][:]
+                assert a.dtype == np.dtype(dtype) and a.shape == want[n].shape, (fname, n)
+                assert np.array_equal(a, want[n]), f"{fname} sample {n} differs"
+    if kind == "pillbox":  # the J file holds sigma*E off the beam line
+        assert np.any(cap["Jz"][:, 0, 1, :] != 0)
+
+    # the WakeSolver from the file alone gives the run's results bitwise
+    import wakis_amd as wa
+    w2 = wa.WakeSolver(q=p["q"], sigmaz=p["sigmaz"], beta=p["beta"], xsource=p["xsource"],
+                       skip_cells=p["skip"], wakelength=p["wakelength"], Ez_file=wake.Ez_file,
+                       save=False, verbose=0)
+    w2.solve(compute_plane=plane)
+    assert np.array_equal(w2.s, wake.s) and np.array_equal(w2.WP, wake.WP)
+    assert np.array_equal(w2.Z, wake.Z) and np.array_equal(w2.f, wake.f)
+    if plane == "both":
+        assert np.array_equal(w2.WP_3d, wake.WP_3d)
+        assert np.array_equal(w2.WPx, wake.WPx) and np.array_equal(w2.WPy, wake.WPy)
+        assert np.array_equal(w2.Zx, wake.Zx) and np.array_equal(w2.Zy, wake.Zy)
+
+
+def test_wakesolve_without_save_writes_no_file(tmp_path, monkeypatch):
+    import wakis_amd as wa
+    monkeypatch.chdir(tmp_path)
+    x, y, z, mask = cases.cavity_pipe_geometry(16, 16, 32)
+    grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"cav": mask},
+                        stl_materials={"cav": "vacuum"}, verbose=0)
+    wake = wa.WakeSolver(q=1e-9, sigmaz=10e-3, beta=1.0, skip_cells=2, save=False, verbose=0)
+    sa = wa.SolverFIT3D(grid, wake=wake, bc_low=["pec", "pec", "abc"],
+                        bc_high=["pec", "pec", "abc"], use_stl=True, bg="pec", verbose=0)
+    sa.wakesolve(wakelength=0.02)
+    assert not (tmp_path / "results").exists() and wake.WP is not None
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_state_file_round_trip(dtype, tmp_path):
+    from wakis_amd import h5io
+    import wakis_amd as wa
+    _say("state file", np.dtype(dtype).name)
+    x, y, z, mask = cases.cavity_pipe_geometry(12, 10, 16)
+    grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"cav": mask},
+                        stl_materials={"cav": "vacuum"}, verbose=0)
+
+    def solver():
+        return wa.SolverFIT3D(grid, bc_low=["pec", "pec", "abc"], bc_high=["pec", "pec", "abc"],
+                              use_stl=True, bg="pec", dtype=dtype, verbose=0)
+
+    rng = np.random.default_rng(3)
+    a = solver()
+    for F in (a.E, a.H, a.J):
+        F.fromarray(rng.standard_normal(3 * a.N).astype(dtype))
+    path = str(tmp_path / "solver_state.h5")
+    a.save_state(path)
+    with a.read_state(path) as st:  # the reference's keys, flat 3N arrays in the field dtype
+        assert sorted(st.keys()) == ["E", "H", "J"]
+        for name in ("E", "H", "J"):
+            assert st[name].shape == (3 * a.N,) and st[name].dtype == np.dtype(dtype)
+    b = solver()
+    b.load_state(path)
+    for name in ("E", "H", "J"):
+        assert np.array_equal(getattr(b, name).to_numpy(), getattr(a, name).to_numpy()), name
+    # both continue identically
+    for s in (a, b):
+        for _ in range(5):
+            s.one_step()
+    assert np.array_equal(a.E.to_numpy(), b.E.to_numpy())
+    # close=False hands back the open file
+    f = a.save_state(str(tmp_path / "s2.h5"), close=False)
+    assert sorted(f.keys()) == ["E", "H", "J"]
+    f.close()
+    assert h5io.is_hdf5(str(tmp_path / "s2.h5"))
+    torch.cuda.synchronize()

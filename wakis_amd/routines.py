@@ -8,6 +8,7 @@ inside the same step calls. There is no per-step host traffic unless a
 callback asks for it.
 """
 
+import os
 import time
 
 import numpy as np
@@ -60,9 +61,8 @@ class RoutinesMixin:
             self.wake = wake
         if add_space is not None and self.wake is not None:
             self.wake.skip_cells = add_space
-        if plot or save_J:
-            raise NotImplementedError("wakesolve: plotting and Jz.h5 saving are not part of "
-                                      "this engine")
+        if plot:
+            raise NotImplementedError("wakesolve: plotting is not part of this engine")
         every = None
         if checkpoint_every is not None:
             every = int(checkpoint_every)
@@ -73,6 +73,10 @@ class RoutinesMixin:
         Nt = ctx["Nt"]
         if Nt >= 2000 and self.N >= (1 << 22):  # long run: measure the kernel shapes first
             self.tune_fused(lambda k: self._wake_args(ctx, 0, k))
+        if save_J:  # J_z on the window before the run, for Jz.h5
+            ix0, iy0, nx, ny = ctx["win"]
+            ctx["J0"] = self.J.to_matrix("z")[ix0:ix0 + nx, iy0:iy0 + ny, :].reshape(
+                nx * ny, self.Nz).cpu().numpy()
         n = 0 if resume is None else self._wake_resume(ctx, resume)
         print("Running electromagnetic time-domain simulation...")
         t0 = time.time()
@@ -101,6 +105,8 @@ class RoutinesMixin:
         self.fields_finite = self.check_finite(("E", "H"))
         if not self.fields_finite:
             print("[!] Warning: the fields hold Inf/NaN values after the time stepping")
+        if save_J:
+            self._write_Jz_file(ctx, compute_plane)
         self._wake_finish(ctx, compute_plane)
 
     def _wake_state_key(self, ctx):
@@ -122,6 +128,8 @@ class RoutinesMixin:
                      j_beam_only=np.bool_(self.J.version == self._J_clean_version),
                      j_sigma_e=np.bool_(self.use_conductivity and self._sigmaE_holds()),
                      beam_line=np.asarray(bl, dtype=np.int64), **self._wake_state_key(ctx))
+        if "J0" in ctx:
+            state["wake_J0"] = ctx["J0"]
         tmp = f"{path}.part"
         with open(tmp, "wb") as f:
             np.savez(f, **state)
@@ -151,6 +159,10 @@ class RoutinesMixin:
                 self._beam_line = None if bl[0] < 0 else bl
             if bool(st["j_sigma_e"]):
                 self._sigmaE_versions = (self.E.version, self.J.version)
+            if "J0" in ctx:
+                if "wake_J0" not in st:
+                    raise ValueError(f"wakesolve: {path} holds no Jz window for save_J")
+                ctx["J0"] = st["wake_J0"]
         return n
 
     def _wake_setup(self, wakelength, compute_plane):
@@ -215,8 +227,77 @@ class RoutinesMixin:
         a.cap_samples, a.cap_sample0 = Nt, n + 1
         return a
 
+    def _write_Ez_file(self, ctx, compute_plane):
+        """Ez.h5 of the run (routines.py:376-389, 437-440): the test window's
+        axes x, y, z (scalars x[ixt], y[iyt] for a longitudinal run), dx, dy,
+        dz, the time axis t, and for every step n < Nt the dataset '#%05d' of
+        Ez on the window before step n (shape (Nz,) or (3, 3, Nz)) in the
+        field dtype, written from the device capture after the run."""
+        from . import h5io
+        ix0, iy0, nx, ny = ctx["win"]
+        longit = compute_plane.lower() == "longitudinal"
+        path = self.wake.Ez_file
+        folder = os.path.dirname(path)
+        if folder:
+            os.makedirs(folder, exist_ok=True)
+        with h5io.File(path, "w") as hf:
+            if longit:
+                hf["x"], hf["y"] = self.x[self.ixt], self.y[self.iyt]
+            else:
+                hf["x"] = self.x[self.ixt - 1:self.ixt + 2]
+                hf["y"] = self.y[self.iyt - 1:self.iyt + 2]
+            hf["z"] = self.z
+            hf["dx"], hf["dy"], hf["dz"] = self.grid.dx, self.grid.dy, self.grid.dz
+            hf["t"] = ctx["t_axis"]
+            if nx * ny == 0:  # the window leaves the grid: nothing was captured
+                print("[!] Warning: the Ez test window leaves the grid; Ez.h5 holds no samples")
+                return
+            cap = ctx["cap"].cpu().numpy()  # [lines][Nt][Nz], lines x-major
+            for n in range(ctx["Nt"]):
+                a = cap[:, n, :]
+                hf["#" + str(n).zfill(5)] = a[0] if longit else a.reshape(nx, ny, self.Nz)
+
+    def _write_Jz_file(self, ctx, compute_plane):
+        """Jz.h5 (routines.py:391-401, 441-442): like Ez.h5, with J_z on the
+        window before each step n. That J is the beam's current of step n on
+        the beam line (Beam.update runs before the save) and, elsewhere, what
+        the previous step left: sigma * E (the step's J update) when the run is
+        conductive, else J as it was before the run; sample 0 is the J before
+        the run. It is formed here from the Ez capture, the beam table and
+        the sigma array, with the step's single rounding of sigma * E."""
+        from . import h5io
+        ix0, iy0, nx, ny = ctx["win"]
+        longit = compute_plane.lower() == "longitudinal"
+        with h5io.File("Jz.h5", "w") as hf:
+            if longit:
+                hf["x"], hf["y"] = self.x[self.ixt], self.y[self.iyt]
+            else:
+                hf["x"] = self.x[self.ixt - 1:self.ixt + 2]
+                hf["y"] = self.y[self.iyt - 1:self.iyt + 2]
+            hf["z"] = self.z
+            hf["dx"], hf["dy"], hf["dz"] = self.grid.dx, self.grid.dy, self.grid.dz
+            hf["t"] = ctx["t_axis"]
+            if nx * ny == 0:
+                return
+            cap = ctx["cap"].cpu().numpy()
+            J0 = ctx["J0"]
+            sig = self.sigma.to_matrix("z")[ix0:ix0 + nx, iy0:iy0 + ny, :].reshape(
+                nx * ny, self.Nz).cpu().numpy().astype(cap.dtype)
+            beam = ctx["beam"]
+            bi, bj = int(beam.ixs) - ix0, int(beam.iys) - iy0
+            bline = bi * ny + bj if (0 <= bi < nx and 0 <= bj < ny) else None
+            table = ctx["table"].cpu().numpy() if bline is not None else None
+            for n in range(ctx["Nt"]):
+                a = (sig * cap[:, n, :]) if (n > 0 and self.use_conductivity) else J0.copy()
+                if bline is not None:
+                    a[bline] = table[n]
+                hf["#" + str(n).zfill(5)] = a[0] if longit else a.reshape(nx, ny, self.Nz)
+
     def _wake_finish(self, ctx, compute_plane):
-        """Hand the capture to the WakeSolver and run its post-processing."""
+        """Hand the capture to the WakeSolver (and to Ez.h5 when the wake
+        results are saved or an Ez_file was named), then post-process."""
+        if self.wake.save or getattr(self.wake, "_Ez_file_given", False):
+            self._write_Ez_file(ctx, compute_plane)
         ix0, iy0, nx, ny = ctx["win"]
         self.wake._set_capture(cap=ctx["cap"], t=ctx["t_axis"], xf=self.x[ix0:ix0 + nx],
                                yf=self.y[iy0:iy0 + ny], zf=self.z, dx=self.grid.dx,

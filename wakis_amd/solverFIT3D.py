@@ -887,16 +887,43 @@ class SolverFIT3D(RoutinesMixin):
         self._J_clean_version = self.J.version
         self._beam_line = None
 
-    def save_state(self, filename="solver_state.npz", close=True):
-        """Dynamic state E, H, J (solverFIT3D.py:886-934) as an .npz file."""
-        np.savez(filename, H=self.H.to_numpy(), E=self.E.to_numpy(), J=self.J.to_numpy())
+    def save_state(self, filename="solver_state.h5", close=True):
+        """Dynamic state H, E, J as flat 3N datasets of an HDF5 file
+        (solverFIT3D.py:886-934), in the field dtype. ``close=False`` returns
+        the open file, as the reference does. A name ending in ``.npz`` writes
+        the same three arrays with numpy instead."""
+        H, E, J = self.H.to_numpy(), self.E.to_numpy(), self.J.to_numpy()
+        if str(filename).endswith(".npz"):
+            np.savez(filename, H=H, E=E, J=J)
+            return None
+        from . import h5io
+        state = h5io.File(filename, "w")
+        state.create_dataset("H", data=H)
+        state.create_dataset("E", data=E)
+        state.create_dataset("J", data=J)
+        if close:
+            state.close()
+            return None
+        return state
 
-    def load_state(self, filename="solver_state.npz"):
-        """solverFIT3D.py:936-980 (single device)."""
-        with np.load(filename, allow_pickle=False) as st:
-            self.E.fromarray(st["E"])
-            self.H.fromarray(st["H"])
-            self.J.fromarray(st["J"])
+    def load_state(self, filename="solver_state.h5"):
+        """solverFIT3D.py:936-980 (single device): E, H, J from a file written
+        by save_state (HDF5, the reference's files included, or .npz)."""
+        from . import h5io
+        if h5io.is_hdf5(filename):
+            with h5io.File(filename, "r") as st:
+                E, H, J = st["E"][:], st["H"][:], st["J"][:]
+        else:
+            with np.load(filename, allow_pickle=False) as st:
+                E, H, J = st["E"], st["H"], st["J"]
+        self.E.fromarray(E)
+        self.H.fromarray(H)
+        self.J.fromarray(J)
+
+    def read_state(self, filename="solver_state.h5"):
+        """The state file opened read-only, not loaded (solverFIT3D.py:982-993)."""
+        from . import h5io
+        return h5io.File(filename, "r")
 
     def synchronize(self):
         torch.cuda.synchronize(self.device)

@@ -38,7 +38,11 @@ class WakeSolver:
             self.ti = ti
         else:  # wakeSolver.py:164-172 (CST convention for beta <= 1)
             self.ti = 8.548921333333334 * self.sigmaz / (np.sqrt(self.beta) * self.v)
+        # Ez.h5 is written by wakesolve when results are saved or a file name is
+        # given; the default name is the reference's (wakeSolver.py:202-203)
+        self._Ez_file_given = Ez_file is not None
         self.Ez_file = Ez_file
+        self.Ez_hf = None
         self.Ezt = None  # Ez(x_test, y_test, z, t) given to calc_long_WP
         self.t = None
         self.xf = self.yf = self.zf = None  # field subdomain
@@ -52,6 +56,8 @@ class WakeSolver:
         self.folder = results_folder if results_folder.endswith("/") else results_folder + "/"
         if self.save:
             os.makedirs(self.folder, exist_ok=True)
+        if self.Ez_file is None:
+            self.Ez_file = self.folder + "Ez.h5"
         self._cap = None
 
     def log(self, msg):
@@ -61,6 +67,7 @@ class WakeSolver:
     # capture handed over by SolverFIT3D.wakesolve
     def _set_capture(self, cap, t, xf, yf, zf, dx, dy, dz, dtype, nsamples):
         self._cap = cap
+        self._cap_from_file = False
         self._cap_dtype = np.dtype(dtype)
         self._cap_samples = nsamples
         self.t, self.xf, self.yf, self.zf = t, xf, yf, zf
@@ -133,11 +140,81 @@ class WakeSolver:
             float(self.q * 1e12), _lib.ptr(WP3d), _lib.stream_ptr()), "wf_wake_wp")
         return s, WP3d
 
+    # ------------------------------------------------------------- Ez.h5
+    def read_Ez(self, filename=None, return_value=False):
+        """Open an Ez.h5 file and take its axes (wakeSolver.py:1313-1360):
+        xf, yf, zf, dx, dy, dz and t from the datasets of those names. With
+        ``return_value`` the open file is returned (and kept as ``Ez_hf``);
+        otherwise it is closed again."""
+        from . import h5io
+        if filename is None:
+            filename = self.Ez_file
+        hf = h5io.File(filename, "r")
+        self.log(f"Reading h5 file {filename}")
+        self.log("Size of the h5 file: " + str(round((os.path.getsize(filename) / 10**9), 2))
+                 + " Gb")
+        self.Ez_hf = hf
+        self.Ez_file = filename
+        keys = set(hf.keys())
+        for key, attr in (("x", "xf"), ("y", "yf"), ("z", "zf"), ("dx", "dx"), ("dy", "dy"),
+                          ("dz", "dz"), ("t", "t")):
+            if key in keys:
+                setattr(self, attr, np.array(hf[key]))
+        if return_value:
+            return hf
+        hf.close()
+        self.Ez_hf = None
+
+    def _capture_from_file(self):
+        """The Ez samples of an Ez.h5 file as a device capture [lines][samples][Nz].
+
+        The samples are the datasets whose names start with '#', in sorted name
+        order (wakeSolver.py:486-491). A 3-D sample (x, y, z) gives the
+        (2n+1)^2 lines around its centre [shape//2 + i, shape//2 + j]
+        (:496-497); a 1-D sample is one line (:384-393)."""
+        self.read_Ez(return_value=True)
+        hf = self.Ez_hf
+        try:
+            keys = sorted(k for k in hf.keys() if isinstance(k, str) and k.startswith("#"))
+            if not keys:
+                raise IndexError(f"no Ez samples ('#' datasets) in {self.Ez_file}")
+            first = hf[keys[0]]
+            i0 = self.n_transverse_cells
+            if first.ndim == 3:
+                cx, cy = first.shape[0] // 2, first.shape[1] // 2
+                if (cx - i0 < 0 or cy - i0 < 0 or cx + i0 >= first.shape[0]
+                        or cy + i0 >= first.shape[1]):
+                    raise IndexError(f"Ez samples of shape {first.shape} hold no "
+                                     f"{2 * i0 + 1}x{2 * i0 + 1} window")
+                sel = (slice(cx - i0, cx + i0 + 1), slice(cy - i0, cy + i0 + 1))
+                lines = (2 * i0 + 1) ** 2
+            elif first.ndim == 1:
+                sel, lines = None, 1
+            else:
+                raise IndexError(f"Ez samples of shape {first.shape}: 1-D or 3-D expected")
+            nz = first.shape[-1]
+            host = np.empty((lines, len(keys), nz), dtype=first.dtype)
+            for n, key in enumerate(keys):
+                a = hf[key][:]
+                host[:, n, :] = a.reshape(1, nz) if sel is None else a[sel].reshape(lines, nz)
+        finally:
+            hf.close()
+            self.Ez_hf = None
+        if self.zf is None:
+            self.zf = self.z
+        self._cap = torch.as_tensor(host, device=torch.device("cuda"))
+        self._cap_dtype = host.dtype
+        self._cap_samples = len(keys)
+        self._cap_from_file = True
+        return self._cap
+
     def _captured(self):
         if self._cap is None:
-            raise RuntimeError("no device Ez capture: run SolverFIT3D.wakesolve first, or pass "
-                               "Ezt to calc_long_WP (Ez.h5 input needs h5py, not part of "
-                               "this engine)")
+            if self.Ez_file is not None and os.path.exists(self.Ez_file):
+                return self._capture_from_file()
+            raise RuntimeError("no Ez samples: run SolverFIT3D.wakesolve first, give an Ez.h5 "
+                               f"file (Ez_file, not found: {self.Ez_file}), or pass Ezt to "
+                               "calc_long_WP")
         if self._cap.shape[0] < 1 or self._cap_samples < 1:
             raise IndexError("no Ez captured on the test window (it leaves the grid)")
         return self._cap
