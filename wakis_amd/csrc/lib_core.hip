@@ -152,21 +152,25 @@ template <typename S, typename Cm>
 int step_general(wf_handle *h, const wf_step_args *a, const wf::MetricDev<S> &m,
                  hipStream_t st) {
   const wf::GridDev &g = h->g;
-  const unsigned nb = nblocks(g.N, 256);
+  if ((g.Ny + wf::EX_TY - 1) / wf::EX_TY > 65535)
+    return fail(WF_ERANGE, "wf_step: Ny = %d is too large for the exact path", g.Ny);
+  const dim3 blk(wf::EX_TX, wf::EX_TY);
+  const dim3 grd((unsigned)((g.Nx + wf::EX_TX - 1) / wf::EX_TX),
+                 (unsigned)((g.Ny + wf::EX_TY - 1) / wf::EX_TY), (unsigned)std::min(g.Nz, 65535));
   wf::EStepAux x;
   x.jmode = a->jmode; x.ixs = a->ixs; x.iys = a->iys;
   x.cap_ix0 = a->cap_ix0; x.cap_iy0 = a->cap_iy0; x.cap_nx = a->cap_nx; x.cap_ny = a->cap_ny;
   x.cap_samples = a->cap_samples;
   const Cm dt = (Cm)a->dt;
   for (int s = 0; s < a->nsteps; ++s) {
-    hipLaunchKernelGGL((wf::k_h_general<S, Cm>), dim3(nb), dim3(256), 0, st, g, m, (S *)a->H,
+    hipLaunchKernelGGL((wf::k_h_exact<S, Cm>), grd, blk, 0, st, g, m, (S *)a->H,
                        (const S *)a->E, (const S *)a->a, dt);
     WF_LAUNCH_CHECK();
     const S *row = a->beam_table ? (const S *)a->beam_table + (a->beam_row0 + s) * g.Nz : nullptr;
     x.cap_sample = a->cap_sample0 + s;
     S *cap = (S *)a->capture;
 #define WF_E_LAUNCH(JM, CO)                                                                  \
-  hipLaunchKernelGGL((wf::k_e_general<S, Cm, JM, CO>), dim3(nb), dim3(256), 0, st, g, m,    \
+  hipLaunchKernelGGL((wf::k_e_exact<S, Cm, JM, CO>), grd, blk, 0, st, g, m,                \
                      (S *)a->E, (const S *)a->H, (S *)a->J, (const S *)a->b,                 \
                      (const S *)a->ieps, (const S *)a->sigma, dt, x, row, cap)
     if (a->conductive) WF_E_LAUNCH(2, true);

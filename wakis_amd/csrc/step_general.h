@@ -67,32 +67,100 @@ __device__ __forceinline__ void e_terms(int comp, const Cell &c0, const GridDev 
   }
 }
 
+// Launch geometry of the exact kernels: 64 x 4 cells of one z plane per
+// workgroup, the grid's z dimension striding over the planes (so no cell
+// index is divided out of a flat index).
+constexpr int EX_TX = 64, EX_TY = 4;
+
+// One term of a row: its sign s, the neighbour value v (0 when the column is
+// absent or masked), the metric weight w = fl(coef * L) of its column, and
+// whether the column exists in the CSR row at all. A term whose weight is an
+// exact zero is skipped, as the stored-entry loop of the reference skips it.
 template <typename S, typename Cm>
-__global__ void __launch_bounds__(256)
-k_h_general(GridDev g, MetricDev<S> m, S *__restrict__ H, const S *__restrict__ E,
-            const S *__restrict__ a, Cm dt) {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= g.N) return;
-  const Cell c0 = cell_of(n, g);
-#pragma unroll
-  for (int comp = 0; comp < 3; ++comp) {
-    Term t[4];
-    h_terms(comp, c0, g, t);
-    const S ar = a[comp * g.N + n];
-    Cm acc = Cm(0);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const Term &tt = t[g.m1desc ? 3 - q : q];
-      if (!tt.ok) continue;
-      if (!face_mask(g.pecz, tt.comp, tt.c.i, tt.c.j, tt.c.k, g.Nx, g.Ny, g.Nz)) continue;
-      const S mm = ar * len_of(m.Lx, m.Ly, m.Lz, tt.comp, tt.c);
-      if (mm == S(0)) continue;
-      const Cm w = dt * (Cm)mm;
-      const Cm p = w * (Cm)E[tt.comp * g.N + flat(tt.c, g)];
-      acc = (tt.sign > 0) ? acc + p : acc - p;
+__device__ __forceinline__ void add_h_term(Cm &acc, Cm dt, S w, S v, bool ok, int sign) {
+  if (!ok || w == S(0)) return;
+  const Cm p = (dt * (Cm)w) * (Cm)v;
+  acc = (sign > 0) ? acc + p : acc - p;
+}
+template <typename S>
+__device__ __forceinline__ void add_e_term(S &acc, S w, S v, bool ok, int sign) {
+  if (!ok || w == S(0)) return;
+  const S p = w * v;
+  acc = (sign > 0) ? acc + p : acc - p;
+}
+
+// H sweep of the exact path: every row of M1 with the reference operator's
+// flat-index neighbours (n+1, n+sy, n+sz; the x and y ones wrap into the next
+// row/plane, all are dropped past the array end), the PEC column mask, terms
+// in CSR column order (ascending, or descending when m1desc).
+template <typename S, typename Cm>
+__global__ void __launch_bounds__(EX_TX * EX_TY)
+k_h_exact(GridDev g, MetricDev<S> m, S *__restrict__ H, const S *__restrict__ E,
+          const S *__restrict__ a, Cm dt) {
+  const int i = blockIdx.x * EX_TX + threadIdx.x, j = blockIdx.y * EX_TY + threadIdx.y;
+  if (i >= g.Nx || j >= g.Ny) return;
+  const int64_t N = g.N;
+  // x+ neighbour cell (flat n+1) and y+ neighbour cell (flat n+sy) of row j
+  const bool xin = i < g.Nx - 1, yin = j < g.Ny - 1;
+  const int xi = xin ? i + 1 : 0, xj = xin ? j : (yin ? j + 1 : 0);
+  const S Lxi = m.Lx[i], Lyj = m.Ly[j], Lyx = m.Ly[xj];
+  for (int k = blockIdx.z; k < g.Nz; k += gridDim.z) {
+    const int64_t n = (int64_t)i + g.sy * j + g.sz * k;
+    const int xk = (xin || yin) ? k : k + 1, yk = yin ? k : k + 1;
+    const bool xok = xk < g.Nz, yok = yk < g.Nz, zok = k + 1 < g.Nz;
+    const int yj = yin ? j + 1 : 0;
+    const S Lzk = m.Lz[k];
+    const S Lzx = xok ? m.Lz[xk] : S(0), Lzy = yok ? m.Lz[yk] : S(0);
+    // neighbour values with the PEC column mask (a masked column's term is
+    // not stored; its value is taken as 0 and its weight test skipped below)
+    const int pz = g.pecz;
+    const bool mx0 = face_mask(pz, 0, i, j, k, g.Nx, g.Ny, g.Nz);
+    const bool my0 = face_mask(pz, 1, i, j, k, g.Nx, g.Ny, g.Nz);
+    const bool mz0 = face_mask(pz, 2, i, j, k, g.Nx, g.Ny, g.Nz);
+    const bool mxz = zok && face_mask(pz, 0, i, j, k + 1, g.Nx, g.Ny, g.Nz);
+    const bool myz = zok && face_mask(pz, 1, i, j, k + 1, g.Nx, g.Ny, g.Nz);
+    const bool mxy = yok && face_mask(pz, 0, i, yj, yk, g.Nx, g.Ny, g.Nz);
+    const bool mzy = yok && face_mask(pz, 2, i, yj, yk, g.Nx, g.Ny, g.Nz);
+    const bool myx = xok && face_mask(pz, 1, xi, xj, xk, g.Nx, g.Ny, g.Nz);
+    const bool mzx = xok && face_mask(pz, 2, xi, xj, xk, g.Nx, g.Ny, g.Nz);
+    const S ex0 = E[n], ey0 = E[N + n], ez0 = E[2 * N + n];
+    const S exz = mxz ? E[n + g.sz] : S(0), eyz = myz ? E[N + n + g.sz] : S(0);
+    const S exy = mxy ? E[n + g.sy] : S(0), ezy = mzy ? E[2 * N + n + g.sy] : S(0);
+    const S eyx = myx ? E[N + n + 1] : S(0), ezx = mzx ? E[2 * N + n + 1] : S(0);
+    const S a0 = a[n], a1 = a[N + n], a2 = a[2 * N + n];
+    Cm acc0 = Cm(0), acc1 = Cm(0), acc2 = Cm(0);
+    // rows (h_terms order): x: +Ey, -Ey(z+), -Ez, +Ez(y+); y: -Ex, +Ex(z+), +Ez,
+    // -Ez(x+); z: +Ex, -Ex(y+), -Ey, +Ey(x+)
+    if (!g.m1desc) {
+      add_h_term<S, Cm>(acc0, dt, (S)(a0 * Lyj), ey0, my0, +1);
+      add_h_term<S, Cm>(acc0, dt, (S)(a0 * Lyj), eyz, myz, -1);
+      add_h_term<S, Cm>(acc0, dt, (S)(a0 * Lzk), ez0, mz0, -1);
+      add_h_term<S, Cm>(acc0, dt, (S)(a0 * Lzy), ezy, mzy, +1);
+      add_h_term<S, Cm>(acc1, dt, (S)(a1 * Lxi), ex0, mx0, -1);
+      add_h_term<S, Cm>(acc1, dt, (S)(a1 * Lxi), exz, mxz, +1);
+      add_h_term<S, Cm>(acc1, dt, (S)(a1 * Lzk), ez0, mz0, +1);
+      add_h_term<S, Cm>(acc1, dt, (S)(a1 * Lzx), ezx, mzx, -1);
+      add_h_term<S, Cm>(acc2, dt, (S)(a2 * Lxi), ex0, mx0, +1);
+      add_h_term<S, Cm>(acc2, dt, (S)(a2 * Lxi), exy, mxy, -1);
+      add_h_term<S, Cm>(acc2, dt, (S)(a2 * Lyj), ey0, my0, -1);
+      add_h_term<S, Cm>(acc2, dt, (S)(a2 * Lyx), eyx, myx, +1);
+    } else {
+      add_h_term<S, Cm>(acc0, dt, (S)(a0 * Lzy), ezy, mzy, +1);
+      add_h_term<S, Cm>(acc0, dt, (S)(a0 * Lzk), ez0, mz0, -1);
+      add_h_term<S, Cm>(acc0, dt, (S)(a0 * Lyj), eyz, myz, -1);
+      add_h_term<S, Cm>(acc0, dt, (S)(a0 * Lyj), ey0, my0, +1);
+      add_h_term<S, Cm>(acc1, dt, (S)(a1 * Lzx), ezx, mzx, -1);
+      add_h_term<S, Cm>(acc1, dt, (S)(a1 * Lzk), ez0, mz0, +1);
+      add_h_term<S, Cm>(acc1, dt, (S)(a1 * Lxi), exz, mxz, +1);
+      add_h_term<S, Cm>(acc1, dt, (S)(a1 * Lxi), ex0, mx0, -1);
+      add_h_term<S, Cm>(acc2, dt, (S)(a2 * Lyx), eyx, myx, +1);
+      add_h_term<S, Cm>(acc2, dt, (S)(a2 * Lyj), ey0, my0, -1);
+      add_h_term<S, Cm>(acc2, dt, (S)(a2 * Lxi), exy, mxy, -1);
+      add_h_term<S, Cm>(acc2, dt, (S)(a2 * Lxi), ex0, mx0, +1);
     }
-    const int64_t r = comp * g.N + n;
-    H[r] = (S)((Cm)H[r] - acc);
+    H[n] = (S)((Cm)H[n] - acc0);
+    H[N + n] = (S)((Cm)H[N + n] - acc1);
+    H[2 * N + n] = (S)((Cm)H[2 * N + n] - acc2);
   }
 }
 
@@ -102,54 +170,89 @@ struct EStepAux {
   int64_t cap_samples, cap_sample;
 };
 
+// E sweep of the exact path: M2 rows with the flat-index backward neighbours
+// (n-1, n-sy, n-sz, wrapping into the previous row/plane), the PMC column
+// mask, ascending order; then the J term, the conduction update of J, the
+// beam line and the Ez window capture.
 template <typename S, typename Cm, int JM, bool COND>
-__global__ void __launch_bounds__(256)
-k_e_general(GridDev g, MetricDev<S> m, S *__restrict__ E, const S *__restrict__ H,
-            S *__restrict__ J, const S *__restrict__ b, const S *__restrict__ ieps,
-            const S *__restrict__ sigma, Cm dt, EStepAux x, const S *__restrict__ beam_row,
-            S *__restrict__ cap) {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= g.N) return;
-  const Cell c0 = cell_of(n, g);
+__global__ void __launch_bounds__(EX_TX * EX_TY)
+k_e_exact(GridDev g, MetricDev<S> m, S *__restrict__ E, const S *__restrict__ H,
+          S *__restrict__ J, const S *__restrict__ b, const S *__restrict__ ieps,
+          const S *__restrict__ sigma, Cm dt, EStepAux x, const S *__restrict__ beam_row,
+          S *__restrict__ cap) {
+  const int i = blockIdx.x * EX_TX + threadIdx.x, j = blockIdx.y * EX_TY + threadIdx.y;
+  if (i >= g.Nx || j >= g.Ny) return;
+  const int64_t N = g.N;
+  const bool xin = i > 0, yin = j > 0;
+  const int xi = xin ? i - 1 : g.Nx - 1, xj = xin ? j : (yin ? j - 1 : g.Ny - 1);
+  const int yj = yin ? j - 1 : g.Ny - 1;
+  const S tLxi = m.tLx[i], tLyj = m.tLy[j], tLxx = m.tLx[xi], tLyx = m.tLy[xj];
+  const bool online = (i == x.ixs) && (j == x.iys);
+  const int li = i - x.cap_ix0, lj = j - x.cap_iy0;
+  const bool incap = cap && li >= 0 && li < x.cap_nx && lj >= 0 && lj < x.cap_ny &&
+                     x.cap_sample < x.cap_samples;
+  for (int k = blockIdx.z; k < g.Nz; k += gridDim.z) {
+    const int64_t n = (int64_t)i + g.sy * j + g.sz * k;
+    const int xk = (xin || yin) ? k : k - 1, yk = yin ? k : k - 1;
+    const bool xok = xk >= 0, yok = yk >= 0, zok = k > 0;
+    const S tLzk = m.tLz[k];
+    const S tLzy = yok ? m.tLz[yk] : S(0);
+    const int hz = g.pmcz;
+    const bool mx0 = face_mask(hz, 0, i, j, k, g.Nx, g.Ny, g.Nz);
+    const bool my0 = face_mask(hz, 1, i, j, k, g.Nx, g.Ny, g.Nz);
+    const bool mz0 = face_mask(hz, 2, i, j, k, g.Nx, g.Ny, g.Nz);
+    const bool mxz = zok && face_mask(hz, 0, i, j, k - 1, g.Nx, g.Ny, g.Nz);
+    const bool myz = zok && face_mask(hz, 1, i, j, k - 1, g.Nx, g.Ny, g.Nz);
+    const bool mxy = yok && face_mask(hz, 0, i, yj, yk, g.Nx, g.Ny, g.Nz);
+    const bool mzy = yok && face_mask(hz, 2, i, yj, yk, g.Nx, g.Ny, g.Nz);
+    const bool myx = xok && face_mask(hz, 1, xi, xj, xk, g.Nx, g.Ny, g.Nz);
+    const bool mzx = xok && face_mask(hz, 2, xi, xj, xk, g.Nx, g.Ny, g.Nz);
+    const S hx0 = H[n], hy0 = H[N + n], hz0 = H[2 * N + n];
+    const S hxz = mxz ? H[n - g.sz] : S(0), hyz = myz ? H[N + n - g.sz] : S(0);
+    const S hxy = mxy ? H[n - g.sy] : S(0), hzy = mzy ? H[2 * N + n - g.sy] : S(0);
+    const S hyx = myx ? H[N + n - 1] : S(0), hzx = mzx ? H[2 * N + n - 1] : S(0);
+    S acc[3];
+    {
+      const S b0 = b[n], b1 = b[N + n], b2 = b[2 * N + n];
+      acc[0] = acc[1] = acc[2] = S(0);
+      // rows (e_terms order): x: +Hy(z-), -Hy, -Hz(y-), +Hz; y: -Hx(z-), +Hx,
+      // +Hz(x-), -Hz; z: +Hx(y-), -Hx, -Hy(x-), +Hy
+      add_e_term<S>(acc[0], (S)(b0 * tLyj), hyz, myz, +1);
+      add_e_term<S>(acc[0], (S)(b0 * tLyj), hy0, my0, -1);
+      add_e_term<S>(acc[0], (S)(b0 * tLzy), hzy, mzy, -1);
+      add_e_term<S>(acc[0], (S)(b0 * tLzk), hz0, mz0, +1);
+      add_e_term<S>(acc[1], (S)(b1 * tLxi), hxz, mxz, -1);
+      add_e_term<S>(acc[1], (S)(b1 * tLxi), hx0, mx0, +1);
+      add_e_term<S>(acc[1], (S)(b1 * (xok ? m.tLz[xk] : S(0))), hzx, mzx, +1);
+      add_e_term<S>(acc[1], (S)(b1 * tLzk), hz0, mz0, -1);
+      add_e_term<S>(acc[2], (S)(b2 * tLxi), hxy, mxy, +1);
+      add_e_term<S>(acc[2], (S)(b2 * tLxi), hx0, mx0, -1);
+      add_e_term<S>(acc[2], (S)(b2 * tLyx), hyx, myx, -1);
+      add_e_term<S>(acc[2], (S)(b2 * tLyj), hy0, my0, +1);
+    }
 #pragma unroll
-  for (int comp = 0; comp < 3; ++comp) {
-    Term t[4];
-    e_terms(comp, c0, g, t);
-    const int64_t r = comp * g.N + n;
-    const S br = b[r];
-    S acc = S(0);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const Term &tt = t[q];
-      if (!tt.ok) continue;
-      if (!face_mask(g.pmcz, tt.comp, tt.c.i, tt.c.j, tt.c.k, g.Nx, g.Ny, g.Nz)) continue;
-      const S mm = br * len_of(m.tLx, m.tLy, m.tLz, tt.comp, tt.c);
-      if (mm == S(0)) continue;
-      const S p = mm * H[tt.comp * g.N + flat(tt.c, g)];
-      acc = (tt.sign > 0) ? acc + p : acc - p;
-    }
-    const bool online = (comp == 2) && (c0.i == x.ixs) && (c0.j == x.iys);
-    S d = acc;
-    S jv = S(0);
-    if (JM == 2) {
-      jv = (online && beam_row) ? beam_row[c0.k] : J[r];
-      d = acc - ieps[r] * jv;
-    } else if (JM == 1 && online) {
-      jv = beam_row ? beam_row[c0.k] : J[r];
-      d = acc - ieps[r] * jv;
-    }
-    const S en = (S)((Cm)E[r] + dt * (Cm)d);
-    E[r] = en;
-    if (COND) {
-      J[r] = sigma[r] * en;
-    } else if (online && beam_row) {
-      J[r] = jv;
-    }
-    if (cap && comp == 2) {
-      const int li = c0.i - x.cap_ix0, lj = c0.j - x.cap_iy0;
-      if (li >= 0 && li < x.cap_nx && lj >= 0 && lj < x.cap_ny && x.cap_sample < x.cap_samples) {
+    for (int comp = 0; comp < 3; ++comp) {
+      const int64_t r = comp * N + n;
+      const bool line = (comp == 2) && online;
+      S d = acc[comp];
+      S jv = S(0);
+      if (JM == 2) {
+        jv = (line && beam_row) ? beam_row[k] : J[r];
+        d = d - ieps[r] * jv;
+      } else if (JM == 1 && line) {
+        jv = beam_row ? beam_row[k] : J[r];
+        d = d - ieps[r] * jv;
+      }
+      const S en = (S)((Cm)E[r] + dt * (Cm)d);
+      E[r] = en;
+      if (COND) {
+        J[r] = sigma[r] * en;
+      } else if (line && beam_row) {
+        J[r] = jv;
+      }
+      if (comp == 2 && incap) {
         const int64_t l = (int64_t)li * x.cap_ny + lj;
-        cap[(l * x.cap_samples + x.cap_sample) * g.Nz + c0.k] = en;
+        cap[(l * x.cap_samples + x.cap_sample) * g.Nz + k] = en;
       }
     }
   }
