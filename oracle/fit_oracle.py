@@ -613,11 +613,17 @@ def abc_copy_low_faces(s):
                 V[tuple(sl0)] = V[tuple(sl1)]
 
 
-def emsolve(s, Nt, source=None, callback=None):
-    """Time loop of ``emsolve`` without file output (routines.py:183-225)."""
+def emsolve(s, Nt, source=None, callback=None, record=None):
+    """Time loop of ``emsolve`` without file output (routines.py:183-225).
+
+    ``record(s, n)``, if given, is called where the reference saves the fields
+    of step n: after the source update, before the step (routines.py:193-204).
+    """
     for n in range(Nt):
         if source is not None:
             source.update(s, n * s.dt)
+        if record is not None:
+            record(s, n)
         s.one_step()
         if s.activate_abc:
             abc_copy_low_faces(s)

@@ -155,3 +155,56 @@ def test_state_file_round_trip(dtype, tmp_path):
     f.close()
     assert h5io.is_hdf5(str(tmp_path / "s2.h5"))
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_emsolve_save_matches_oracle(dtype, tmp_path, monkeypatch):
+    """emsolve(save=True): the field files of routines.py:132-156, 193-204 with
+    every saved component bit-identical to the oracle's fields at the save
+    point (after the source update, before the step); |E| within one rounding
+    of numpy's sqrt(x^2 + y^2 + z^2)."""
+    from scipy.constants import c as c_light
+
+    import wakis_amd as wa
+    from wakis_amd import h5io
+    _say("emsolve save", np.dtype(dtype).name)
+    monkeypatch.chdir(tmp_path)
+    x, y, z = fo.uniform_axes(-1.0, 1.0, -1.0, 1.0, 0.0, 1.0, 12, 10, 40)
+    bc = (["periodic", "periodic", "abc"], ["periodic", "periodic", "abc"])
+    so = fo.OracleSolver(x, y, z, bc[0], bc[1], bg=[1.0, 1.0], dtype=dtype)
+    grid = wa.GridFIT3D(x=x, y=y, z=z, verbose=0)
+    sa = wa.SolverFIT3D(grid, bc_low=bc[0], bc_high=bc[1], use_stl=False, bg=[1.0, 1.0],
+                        dtype=dtype, verbose=0)
+    f = 15 / (1.0 / c_light)
+    Nt, every = 50, 3
+    sub = (slice(2, 9), slice(1, 8), slice(0, 30))
+    want = {}
+
+    def record(s, n):
+        if n % every == 0:
+            E = {d: fo.comp(s.E, s.dims, d)[sub] for d in "xyz"}
+            want[n] = dict(Ez=E["z"].copy(), Hy=fo.comp(s.H, s.dims, "y")[sub].copy(),
+                           Ex=E["x"].copy(),
+                           EAbs=np.sqrt(E["x"] ** 2 + E["y"] ** 2 + E["z"] ** 2))
+
+    fo.emsolve(so, Nt, source=fo.OraclePlaneWave(zs=5, f=f, nodes=2, amplitude=10.0),
+               record=record)
+    sa.emsolve(Nt, source=wa.PlaneWave(zs=5, f=f, nodes=2, amplitude=10.0), save=True,
+               fields=["E", "Hy"], components=["z", "x", "Abs"], save_every=every,
+               subdomain=list(sub))
+    for name in ("E", "H", "J"):
+        assert np.array_equal(getattr(sa, name).to_numpy(), getattr(so, name)), name
+    for fname in ("Ez", "Ex", "EAbs", "Hy"):
+        with h5io.File(fname + ".h5", "r") as hf:
+            keys = sorted(k for k in hf.keys() if k.startswith("#"))
+            assert keys == ["#" + str(n).zfill(5) for n in range(0, Nt, every)], fname
+            assert np.array_equal(np.array(hf["t"]), np.arange(0, Nt * sa.dt, every * sa.dt))
+            assert np.array_equal(np.array(hf["z"]), so.z)  # the solver's cell-centre axis
+            for n in range(0, Nt, every):
+                a, r = hf["#" + str(n).zfill(5)][:], want[n][fname]
+                assert a.shape == r.shape and a.dtype == np.dtype(dtype), (fname, n)
+                if fname == "EAbs":
+                    assert np.allclose(a, r, rtol=4 * np.finfo(dtype).eps, atol=0), n
+                else:
+                    assert np.array_equal(a, r), (fname, n)
+    assert np.any(want[Nt - 1 - (Nt - 1) % every]["Ez"] != 0)

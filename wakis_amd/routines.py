@@ -22,17 +22,63 @@ class RoutinesMixin:
     def emsolve(self, Nt, source=None, callback=None, save=False, fields=["E"],
                 components=["Abs"], save_every=1, subdomain=None, plot=False, plot_every=1,
                 use_etd=False, plot3d=False, **kwargs):
-        """Time loop with host sources and the CPU-semantics ABC (routines.py:17-229)."""
-        if save or plot or plot3d:
-            raise NotImplementedError("emsolve: HDF5 saving and plotting are not part of "
-                                      "this engine")
+        """Time loop with host sources and the CPU-semantics ABC (routines.py:17-229).
+
+        ``save=True`` writes one HDF5 file per saved field, named field +
+        component + '.h5' ('Ex.h5', 'EAbs.h5', ...; a name of two or more
+        letters in ``fields`` carries its own component), with the axes x, y,
+        z, dx, dy, dz, the time axis of the saved steps and, every
+        ``save_every`` steps, the dataset '#nnnnn' of F[xx, yy, zz, component]
+        over ``subdomain`` after the source update of step n (routines.py:
+        132-156, 193-204). (The reference's own set-up loop iterates over the
+        file names instead of the files and fails before the first step; this
+        is the layout its loop body writes.)"""
+        if plot or plot3d:
+            raise NotImplementedError("emsolve: plotting is not part of this engine")
         self.Nt = Nt
         if source is not None:
             self.source = source
+        hfs = self._emsolve_files(Nt, fields, components, save_every) if save else {}
+        if subdomain is not None:
+            xx, yy, zz = subdomain
+        else:
+            xx, yy, zz = slice(0, self.Nx), slice(0, self.Ny), slice(0, self.Nz)
         low_abc = sum(1 << a for a in range(3) if self.bc_low[a].lower() == "abc")
+        try:
+            self._emsolve_loop(Nt, source, callback, hfs, save_every, (xx, yy, zz), low_abc)
+        finally:
+            for hf in hfs.values():
+                hf.close()
+
+    def _emsolve_files(self, Nt, fields, components, save_every):
+        from . import h5io
+        names = []
+        for field in fields:
+            if len(field) == 1:
+                names += [field + component for component in components]
+            else:
+                names.append(field)
+        hfs = {}
+        for name in names:
+            if name[0] not in ("E", "H", "J") or name[1:] not in ("x", "y", "z", "Abs"):
+                raise AttributeError(f"Component {name} not valid. Input must have a field "
+                                     "[\"E\", \"H\", \"J\"] and a component [\"x\", \"y\", "
+                                     "\"z\", \"Abs\"]")
+            hf = h5io.File(name + ".h5", "w")
+            hfs[name] = hf
+            hf["x"], hf["y"], hf["z"] = self.x, self.y, self.z
+            hf["dx"], hf["dy"], hf["dz"] = self.grid.dx, self.grid.dy, self.grid.dz
+            hf["t"] = np.arange(0, Nt * self.dt, save_every * self.dt)
+        return hfs
+
+    def _emsolve_loop(self, Nt, source, callback, hfs, save_every, sub, low_abc):
+        xx, yy, zz = sub
         for n in range(Nt):
             if source is not None:
                 source.update(self, n * self.dt)
+            if hfs and n % save_every == 0:
+                for name, hf in hfs.items():
+                    hf["#" + str(n).zfill(5)] = getattr(self, name[0])[xx, yy, zz, name[1:]]
             self.one_step()
             if self.activate_abc and low_abc:
                 _lib.check(_lib.lib().wf_abc_low(self._handle, _lib.ptr(self.E.array),
