@@ -426,13 +426,16 @@ int wf_batch_adjoint_run(wf_batch_adjoint *b, int32_t r0, int32_t nr, int32_t us
 int wf_batch_adjoint_destroy(wf_batch_adjoint *b);
 
 /* Transpose of wf_wake_wp's line integral without the line-to-line carry:
- * gcap[l][m][k0+k] += sum over s indices n with it(n,k) == m of gI[l][n]*dz[k]
+ * gcap[l][m][k0+kr] += sum over s indices n with it(n,k) == m of gI[l][n]*w_k
  * (gI: gradient w.r.t. the raw line sums; the carry and the 1/qe12 scaling
- * are the caller's). Arguments as wf_wake_wp. */
+ * are the caller's). kr, it and w_k as in wf_wake_wp: kr = k and w_k = dz[k],
+ * or with counter_moving kr = nz-1-k and w_k = -dz[k]; zref_v is zmin/v, or
+ * zmax/(-v) with counter_moving. Other arguments as wf_wake_wp. No
+ * counterpart in the reference (its wake post-processing has no gradient). */
 int wf_wake_wp_adjoint(const double *gI, int32_t nlines, int64_t cap_samples, int32_t Nz,
                        int32_t k0, int32_t nz, const double *z, const double *dz, const double *s,
-                       int64_t ns, double v, double zmin_v, double t0, double ti, double dt,
-                       int64_t nt, double *gcap, void *stream);
+                       int64_t ns, double v, double zref_v, double t0, double ti, double dt,
+                       int64_t nt, int32_t counter_moving, double *gcap, void *stream);
 
 /* Transpose of wf_dft for one signal: out[n] = Re sum_f w[f] exp(-2 pi i f n/N)
  * for n < len, w given as [nbins][2] (re, im); exact phase reduction,

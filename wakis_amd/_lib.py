@@ -116,7 +116,8 @@ EXPORTS = {
                                        c_void_p]),
     "wf_wake_wp_adjoint": (c_int32, [c_void_p, c_int32, c_int64, c_int32, c_int32, c_int32,
                                      c_void_p, c_void_p, c_void_p, c_int64, c_double, c_double,
-                                     c_double, c_double, c_double, c_int64, c_void_p, c_void_p]),
+                                     c_double, c_double, c_double, c_int64, c_int32, c_void_p,
+                                     c_void_p]),
     "wf_idft_real": (c_int32, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p]),
     "wf_field_monitor_update": (c_int32, [c_void_p, c_int32, c_int64, c_void_p, c_int32,
                                           c_int32, c_void_p, c_void_p]),

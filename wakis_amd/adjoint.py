@@ -89,9 +89,6 @@ class WakeAdjoint:
         s = solver
         if s.wake is None:
             raise ValueError("WakeAdjoint: the solver needs a WakeSolver")
-        if s.wake.counter_moving:
-            raise NotImplementedError("WakeAdjoint: the counter-moving wake integral has no "
-                                      "reverse mode")
         # sigma is a parameter only on a conductive solver (a conductive material,
         # background or PML); otherwise the gradient is with respect to eps_r alone
         self.conductive = bool(s.use_conductivity)
@@ -297,10 +294,15 @@ class WakeAdjoint:
         gcap = torch.zeros(cap.shape, dtype=torch.float64, device=dev)
         t = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=dev)
         gIt, zt, dzt, st_ = t(gI), t(z), t(w.dz[:max(nz, 1)]), t(s_ax)
+        if w.counter_moving:  # reference time zmax/(-v) (wakeSolver.py:505-511)
+            zref_v = float(np.max(w.zf) / (-1 * w.v))
+        else:  # zmin/v (wakeSolver.py:524-528)
+            zref_v = float(zmin / w.v)
         _lib.check(_lib.lib().wf_wake_wp_adjoint(
             _lib.ptr(gIt), nl, cap.shape[1], cap.shape[2], k0, nz, _lib.ptr(zt), _lib.ptr(dzt),
-            _lib.ptr(st_), len(s_ax), float(w.v), float(zmin / w.v), float(w.t[0]),
-            float(w.ti), float(dt), nt, _lib.ptr(gcap), _lib.stream_ptr()), "wf_wake_wp_adjoint")
+            _lib.ptr(st_), len(s_ax), float(w.v), zref_v, float(w.t[0]), float(w.ti), float(dt),
+            nt, int(bool(w.counter_moving)), _lib.ptr(gcap), _lib.stream_ptr()),
+            "wf_wake_wp_adjoint")
         return gcap
 
     def _reverse_setup(self):

@@ -328,34 +328,42 @@ k_adj_e(GridDev g, MetricDev<S> m, double *__restrict__ W, double *__restrict__ 
   }
 }
 
-// Capture gradient of the line integrals: gcap[l][m][k0+k] = sum over the s
-// indices n with it(n,k) == m of gI[l][n] * dz[k] (it as in k_wake_wp, which
-// is non-decreasing in n). One thread per (line, k): runs of equal m are
-// summed in order, so the result is deterministic.
-static __global__ void k_wake_wp_adjoint(const double *__restrict__ gI, int nlines, int64_t cap_samples,
+// Capture gradient of the line integrals: gcap[l][m][k0+kr] = sum over the s
+// indices n with it(n,k) == m of gI[l][n] * w_k, with the row kr, the time
+// index it and the weight w_k of k_wake_wp<CM> (forward: kr = k, w_k = dz[k];
+// counter-moving: kr = nz-1-k, w_k = -dz[k]). In both cases it is
+// non-decreasing in n (ts grows with s). One thread per (line, k): runs of
+// equal m are summed in order, so the result is deterministic.
+template <bool CM>
+__global__ void k_wake_wp_adjoint(const double *__restrict__ gI, int nlines, int64_t cap_samples,
                                   int Nz, int k0, int nz, const double *__restrict__ z,
                                   const double *__restrict__ dz, const double *__restrict__ s,
-                                  int64_t ns, double v, double zmin_v, double t0, double ti,
+                                  int64_t ns, double v, double zref_v, double t0, double ti,
                                   double dt, int64_t nt, double *__restrict__ gcap) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)nlines * nz) return;
   const int l = (int)(t / nz), k = (int)(t - (int64_t)l * nz);
-  const double zk = z[k], dzk = dz[k];
+  const int kr = CM ? nz - 1 - k : k;
+  const double zk = z[kr], wk = CM ? -dz[k] : dz[k];
+  const double nv = -v;
   int64_t cur = -1;
   double acc = 0.0;
   for (int64_t n = 0; n < ns; ++n) {
-    const double ts = (zk + s[n]) / v - zmin_v - t0 + ti;
+    double ts = CM ? (zk - s[n]) / nv : (zk + s[n]) / v;
+    ts = ts - zref_v;
+    ts = ts - t0;
+    ts = ts + ti;
     int64_t it = (int64_t)(ts / dt);
     if (it >= nt) break;
     if (it < 0) it += nt;  // python negative index (not reached for t0 = 0)
     if (it != cur) {
-      if (cur >= 0) gcap[((int64_t)l * cap_samples + cur) * Nz + k0 + k] += acc;
+      if (cur >= 0) gcap[((int64_t)l * cap_samples + cur) * Nz + k0 + kr] += acc;
       cur = it;
       acc = 0.0;
     }
-    acc += gI[(int64_t)l * ns + n] * dzk;
+    acc += gI[(int64_t)l * ns + n] * wk;
   }
-  if (cur >= 0) gcap[((int64_t)l * cap_samples + cur) * Nz + k0 + k] += acc;
+  if (cur >= 0) gcap[((int64_t)l * cap_samples + cur) * Nz + k0 + kr] += acc;
 }
 
 // out[n] = Re sum_f (w_re[f] + i w_im[f]) exp(-2 pi i f n / N), n < len: the

@@ -255,16 +255,21 @@ int wf_adjoint_grad(wf_handle *h, const wf_adjoint_grad_args *a, void *stream) {
 
 int wf_wake_wp_adjoint(const double *gI, int32_t nlines, int64_t cap_samples, int32_t Nz,
                        int32_t k0, int32_t nz, const double *z, const double *dz, const double *s,
-                       int64_t ns, double v, double zmin_v, double t0, double ti, double dt,
-                       int64_t nt, double *gcap, void *stream) {
+                       int64_t ns, double v, double zref_v, double t0, double ti, double dt,
+                       int64_t nt, int32_t counter_moving, double *gcap, void *stream) {
   if (!gI || !z || !dz || !s || !gcap) return fail(WF_EINVAL, "wf_wake_wp_adjoint: null argument");
   if (nlines < 1 || nz < 0 || k0 < 0 || k0 + nz > Nz || nt > cap_samples || dt <= 0)
     return fail(WF_ERANGE, "wf_wake_wp_adjoint: inconsistent sizes");
   const int64_t tot = (int64_t)nlines * nz;
   if (tot == 0) return WF_OK;
-  hipLaunchKernelGGL(wf::k_wake_wp_adjoint, dim3(nblocks(tot, 64)), dim3(64), 0,
-                     (hipStream_t)stream, gI, nlines, cap_samples, Nz, k0, nz, z, dz, s, ns, v,
-                     zmin_v, t0, ti, dt, nt, gcap);
+  if (counter_moving)
+    hipLaunchKernelGGL(wf::k_wake_wp_adjoint<true>, dim3(nblocks(tot, 64)), dim3(64), 0,
+                       (hipStream_t)stream, gI, nlines, cap_samples, Nz, k0, nz, z, dz, s, ns, v,
+                       zref_v, t0, ti, dt, nt, gcap);
+  else
+    hipLaunchKernelGGL(wf::k_wake_wp_adjoint<false>, dim3(nblocks(tot, 64)), dim3(64), 0,
+                       (hipStream_t)stream, gI, nlines, cap_samples, Nz, k0, nz, z, dz, s, ns, v,
+                       zref_v, t0, ti, dt, nt, gcap);
   WF_LAUNCH_CHECK();
   return WF_OK;
 }
