@@ -89,6 +89,14 @@ def test_wakesolve_h5_files_match_oracle(kind, dtype, plane, tmp_path, monkeypat
                 assert np.array_equal(a, want[n]), f"{fname} sample {n} differs"
     if kind == "pillbox":  # the J file holds sigma*E off the beam line
         assert np.any(cap["Jz"][:, 0, 1, :] != 0)
+    # the run log of routines.py:478-482 beside the results
+    from wakis_amd.logger import Logger
+    logs = Logger.read_logs(str(tmp_path / "results" / "wakis.log"))
+    want = ["use_gpu", "use_mpi", "background", "bc_low", "bc_high"]
+    want += (["n_pml"] if kind == "pillbox" else []) + ["dt", "solverInitializationTime"]
+    assert list(logs["solver"]) == want
+    assert logs["wakeSolver"]["wakelength"] == p["wakelength"]
+    assert "simulationTime" in logs["wakeSolver"] and logs["grid"]["Nz"] == sa.Nz
 
     # the WakeSolver from the file alone gives the run's results bitwise
     import wakis_amd as wa

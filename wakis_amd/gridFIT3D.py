@@ -8,8 +8,11 @@ engine; solids are given as ``wakis_amd.solids`` objects and marked with the
 reference's "any corner inside" rule.
 """
 
+import time
+
 import numpy as np
 
+from .logger import Logger
 from .materials import material_lib
 from .solids import Solid
 
@@ -19,6 +22,8 @@ class GridFIT3D:
                  Nx=None, Ny=None, Nz=None, x=None, y=None, z=None, use_mpi=False,
                  stl_solids=None, stl_materials=None, stl_colors=None, stl_tol=1e-3,
                  verbose=1, **kwargs):
+        t0 = time.time()
+        self.logger = Logger()
         if use_mpi:
             raise NotImplementedError("wakis_amd runs the full domain on one GPU (use_mpi=False)")
         unsupported = {k for k, v in kwargs.items()
@@ -77,6 +82,22 @@ class GridFIT3D:
             if isinstance(mat, str):
                 lib = material_lib[mat.lower()]
                 self.stl_materials[key] = list(lib)
+        # the reference's grid log keys (gridFIT3D.py:181-208, 255-256); solids
+        # are named by their description or mask shape instead of an STL path
+        self.use_mesh_refinement = False
+        self.update_logger(["Nx", "Ny", "Nz", "dx", "dy", "dz"])
+        self.update_logger(["xmin", "xmax", "ymin", "ymax", "zmin", "zmax"])
+        self.logger.grid["stl_solids"] = {
+            k: (f"cell mask {tuple(v.shape)}, {int(np.count_nonzero(v))} cells"
+                if isinstance(v, np.ndarray) else str(v)) for k, v in self.stl_solids.items()}
+        self.update_logger(["stl_materials", "use_mesh_refinement"])
+        self.gridInitializationTime = time.time() - t0
+        self.update_logger(["gridInitializationTime"])
+
+    def update_logger(self, attrs):
+        """Copy grid attributes into ``logger.grid`` (gridFIT3D.py:1001-1011)."""
+        for atr in attrs:
+            self.logger.grid[atr] = getattr(self, atr)
 
     def _mark_cells_in_surface(self, key):
         """Keep only the surface cells of solid ``key`` (gridFIT3D.py:518-533), for the

@@ -154,6 +154,12 @@ class RoutinesMixin:
         if save_J:
             self._write_Jz_file(ctx, compute_plane)
         self._wake_finish(ctx, compute_plane)
+        # run log (routines.py:478-482); wakis.log is written with the saved results
+        self.logger.wakeSolver = self.wake.logger.wakeSolver
+        self.logger.wakeSolver["wakelength"] = wakelength
+        self.logger.wakeSolver["simulationTime"] = time.time() - t0
+        if self.wake.save:
+            self.logger.save_logs()
 
     def _wake_state_key(self, ctx):
         return dict(wake_Nt=np.int64(ctx["Nt"]), wake_dt=np.float64(self.dt),

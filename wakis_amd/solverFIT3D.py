@@ -25,6 +25,7 @@ from scipy.constants import mu_0
 
 from . import _lib
 from .field import Field, torch_dtype
+from .logger import Logger
 from .materials import material_lib
 from .routines import RoutinesMixin
 
@@ -72,6 +73,7 @@ class SolverFIT3D(RoutinesMixin):
                  use_sibc=True, fmax=1e9, dtype=np.float64, n_pml=10, bg=[1.0, 1.0],
                  verbose=1, path="auto", tau_clamp=True):
         t0 = time.time()
+        self.logger = Logger()
         if use_mpi:
             raise NotImplementedError("wakis_amd runs the full domain on one GPU (use_mpi=False)")
         if use_conductors:
@@ -189,6 +191,13 @@ class SolverFIT3D(RoutinesMixin):
         self._sigmaE_versions = (self.E.version, self.J.version)  # J == sigma*E (all zero)
         self.last_path = None
         self.solverInitializationTime = time.time() - t0
+        # the reference's solver log keys (solverFIT3D.py:149, 171-176, 234, 281, 299, 350-354)
+        self.update_logger(["use_gpu", "use_mpi", "grid", "background", "bc_low", "bc_high"])
+        if self.activate_pml:
+            self.update_logger(["n_pml"])
+        self.update_logger(["dt", "solverInitializationTime"])
+        if self.wake is not None:
+            self.logger.wakeSolver = self.wake.logger.wakeSolver
         if verbose:
             print(f"Total solver initialization time: {self.solverInitializationTime} s")
 
@@ -880,6 +889,15 @@ class SolverFIT3D(RoutinesMixin):
         self.one_step()
 
     # ------------------------------------------------------- state helpers
+    def update_logger(self, attrs):
+        """Copy solver attributes into ``logger.solver``; 'grid' takes the
+        grid's log (solverFIT3D.py:1015-1029)."""
+        for atr in attrs:
+            if atr == "grid":
+                self.logger.grid = self.grid.logger.grid
+            else:
+                self.logger.solver[atr] = getattr(self, atr)
+
     def reset_fields(self):
         """solverFIT3D.py:995-1004."""
         for F in (self.E, self.H, self.J):

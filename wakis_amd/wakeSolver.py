@@ -17,6 +17,7 @@ import torch
 from scipy.constants import c as c_light
 
 from . import _lib
+from .logger import Logger
 
 
 class WakeSolver:
@@ -59,6 +60,15 @@ class WakeSolver:
         if self.Ez_file is None:
             self.Ez_file = self.folder + "Ez.h5"
         self._cap = None
+        self.logger = Logger()
+        self.assign_logs()
+
+    def assign_logs(self):
+        """The wake parameters into ``logger.wakeSolver`` (wakeSolver.py:1709-1723)."""
+        for key in ("ti", "q", "sigmaz", "beta", "xsource", "ysource", "xtest", "ytest",
+                    "chargedist", "skip_cells"):
+            self.logger.wakeSolver[key] = getattr(self, key)
+        self.logger.wakeSolver["results_folder"] = self.folder
 
     def log(self, msg):
         if self.verbose:
