@@ -157,6 +157,8 @@ def test_state_file_round_trip(dtype, tmp_path):
         for _ in range(5):
             s.one_step()
     assert np.array_equal(a.E.to_numpy(), b.E.to_numpy())
+    text = str(a.E)  # the three components one after another (field.py:475-486)
+    assert text.startswith("x:\n") and "\ny:\n" in text and "\nz:\n" in text
     # close=False hands back the open file
     f = a.save_state(str(tmp_path / "s2.h5"), close=False)
     assert sorted(f.keys()) == ["E", "H", "J"]

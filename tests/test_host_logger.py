@@ -47,3 +47,10 @@ def test_save_logs_layout_and_round_trip(tmp_path):
     empty.wakeSolver["results_folder"] = str(tmp_path / "e")
     empty.save_logs()
     assert Logger.read_logs(str(tmp_path / "e" / "wakis.log"))["grid"] is None
+
+
+def test_wake_solver_copy_is_shallow():
+    w = wa.WakeSolver(q=3e-9, sigmaz=4e-3, save=False, verbose=0)
+    c = w.copy()
+    assert c is not w and type(c) is wa.WakeSolver
+    assert c.__dict__ == w.__dict__ and c.logger is w.logger

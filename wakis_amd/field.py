@@ -213,5 +213,11 @@ class Field:
             a = torch.sqrt(self.field_x ** 2 + self.field_y ** 2 + self.field_z ** 2)
         return a.cpu().numpy()
 
+    def __str__(self):
+        """The three components as host arrays, one after another (field.py:475-486)."""
+        return ("x:\n" + str(self.to_matrix("x").cpu().numpy()) + "\n" + "y:\n"
+                + str(self.to_matrix("y").cpu().numpy()) + "\n" + "z:\n"
+                + str(self.to_matrix("z").cpu().numpy()))
+
     def __repr__(self):
         return f"Field({self.Nx}x{self.Ny}x{self.Nz}, {np.dtype(self.dtype).name}, {self.device})"

@@ -70,6 +70,13 @@ class WakeSolver:
             self.logger.wakeSolver[key] = getattr(self, key)
         self.logger.wakeSolver["results_folder"] = self.folder
 
+    def copy(self):
+        """A shallow copy: a new WakeSolver sharing this one's attribute values
+        (wakeSolver.py:1509-1520)."""
+        obj = type(self).__new__(type(self))
+        obj.__dict__.update(self.__dict__)
+        return obj
+
     def log(self, msg):
         if self.verbose:
             print(msg)
