@@ -170,6 +170,16 @@ typedef struct wf_step_args {
   int64_t mon_row0;
   int32_t mon_nf;
   int32_t mon_phase_f32;
+  /* Fused path: 1 = advance two steps per launch (temporal blocking) where
+   * the two-step kernel applies: FP32, material codes with zero_masked, no
+   * conduction, J a beam line fed from beam_table, Nx <= 256 and a
+   * multiple of 4, row
+   * segments (variant 0, full_rows <= 0, cells_per_lane 0 or 4), no box, no
+   * trajectory, no monitor; otherwise one step per launch. An odd step
+   * count ends with one single step. Results are identical to one step per
+   * launch. 0 = one step per launch; 4 or 8: two steps per launch with that
+   * many rows per workgroup (1 = the default, 8). */
+  int32_t two_step;
 } wf_step_args;
 
 int wf_version(void);

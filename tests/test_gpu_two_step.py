@@ -1,0 +1,120 @@
+"""GPU: two leapfrog steps per launch (k_fused4_t2) against one step per launch.
+
+The two-step kernel forms every value with the single-step kernel's
+expressions, so E, H, J and the captured Ez window must be bit-identical to
+one step per launch (which the other tests tie bitwise to the oracle), for
+both row counts, z chunks that split the grid unevenly, rows that do not fill
+the last tile, odd and even step counts, and Nx below and at the 256-cell
+row. The wakefield run of the cavity is also compared with the oracle
+directly.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cases
+from oracle import fit_oracle as fo
+from oracle import wake_oracle as wo
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _cavity_solver(dims, two_step, kz=0):
+    import wakis_amd as wa
+    from wakis_amd.solids import Box, Cylinder
+    solid = Box(-25e-3, 25e-3, -25e-3, 25e-3, -25e-3, 25e-3) | Cylinder(5e-3)
+    grid = wa.GridFIT3D(-30e-3, 30e-3, -30e-3, 30e-3, -80e-3, 80e-3, *dims,
+                        stl_solids={"cavity": solid}, stl_materials={"cavity": "vacuum"},
+                        verbose=0)
+    s = wa.SolverFIT3D(grid, bc_low=["pec", "pec", "abc"], bc_high=["pec", "pec", "abc"],
+                       use_stl=True, bg="pec", dtype=np.float32, verbose=0)
+    s.fused_two_step = two_step
+    s.fused_kz = kz
+    return s
+
+
+def _beam_run(dims, two_step, chunks, kz=0):
+    """Steps with the beam table and a 3x3 Ez capture, in calls of `chunks` steps."""
+    import wakis_amd as wa
+    from wakis_amd import _lib
+    s = _cavity_solver(dims, two_step, kz)
+    tot = sum(chunks)
+    beam = wa.Beam(q=1e-9, sigmaz=10e-3, beta=1.0)
+    dev = s.device
+    table = torch.as_tensor(beam.table(s, tot), device=dev).contiguous()
+    cap = torch.zeros((9, tot + 1, s.Nz), dtype=table.dtype, device=dev)
+    ixt, iyt = int(np.abs(s.x).argmin()), int(np.abs(s.y).argmin())
+    s._set_ghosts_to_0()
+    s.step_0 = False
+    s._beam_line = (int(beam.ixs), int(beam.iys))
+    n0 = 0
+    for k in chunks:
+        a = s._step_args(k)
+        a.ixs, a.iys = int(beam.ixs), int(beam.iys)
+        a.jmode = _lib.WF_J_LINE
+        a.beam_table, a.beam_row0, a.beam_rows = _lib.ptr(table), n0, tot
+        a.capture = _lib.ptr(cap)
+        a.cap_ix0, a.cap_iy0, a.cap_nx, a.cap_ny = ixt - 1, iyt - 1, 3, 3
+        a.cap_samples, a.cap_sample0 = tot + 1, n0 + 1
+        s._run(a)
+        assert s.last_path == "fused"
+        n0 += k
+    torch.cuda.synchronize()
+    return (s.E.to_numpy(), s.H.to_numpy(), s.J.to_numpy(), cap.cpu().numpy())
+
+
+@pytest.mark.parametrize("dims,chunks,kz", [
+    ((256, 24, 64), (12,), 0),
+    ((256, 24, 100), (5, 8), 3),
+    ((64, 40, 48), (11,), 0),
+    ((20, 13, 30), (7, 6), 4),
+    ((36, 9, 7), (9,), 2),
+    ((256, 256, 40), (10,), 0),
+])
+@pytest.mark.parametrize("two_step", [4, 8])
+def test_two_step_equals_single_steps(dims, chunks, kz, two_step):
+    _say("two-step", dims, chunks, kz, two_step)
+    ref = _beam_run(dims, 0, chunks, kz)
+    got = _beam_run(dims, two_step, chunks, kz)
+    assert np.any(ref[0] != 0) and np.any(ref[3] != 0)
+    for name, r, g in zip(("E", "H", "J", "capture"), ref, got):
+        assert np.array_equal(g, r), f"{name} differs ({np.count_nonzero(g != r)} values)"
+
+
+def test_two_step_applies_only_where_built():
+    """Runs outside the kernel's case (FP64, coefficient arrays, wide rows) fall
+    back to one step per launch with the same results."""
+    _say("two-step fallback")
+    ref = _beam_run((260, 12, 20), 0, (6,))
+    got = _beam_run((260, 12, 20), 1, (6,))  # Nx > 256: single steps
+    for r, g in zip(ref, got):
+        assert np.array_equal(g, r)
+
+
+@pytest.mark.parametrize("two_step", [4, 8])
+def test_two_step_wakesolve_matches_oracle(two_step):
+    import wakis_amd as wa
+    _say("two-step wakesolve vs oracle", two_step)
+    x, y, z, mask = cases.cavity_pipe_geometry(20, 20, 40)
+    so = fo.OracleSolver(x, y, z, ["pec", "pec", "abc"], ["pec", "pec", "abc"], bg="pec",
+                         solids=[(mask, "vacuum")], dtype=np.float32)
+    q, sigmaz, beta, wl = 1e-9, 10e-3, 1.0, 0.05
+    cap = fo.wakesolve_fields(so, wl, q, sigmaz, beta, 0.0, 0.0, 0.0, 0.0,
+                              wo.wake_ti(sigmaz, beta))
+    ref = wo.wake_solve(cap, q, sigmaz, beta, wl, skip_cells=2)
+    grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"cav": mask},
+                        stl_materials={"cav": "vacuum"}, verbose=0)
+    wake = wa.WakeSolver(q=q, sigmaz=sigmaz, beta=beta, skip_cells=2, save=False, verbose=0)
+    sa = wa.SolverFIT3D(grid, wake=wake, bc_low=["pec", "pec", "abc"],
+                        bc_high=["pec", "pec", "abc"], use_stl=True, bg="pec",
+                        dtype=np.float32, verbose=0)
+    sa.fused_two_step = two_step
+    sa.wakesolve(wakelength=wl)
+    for name in ("E", "H", "J"):
+        assert np.array_equal(getattr(sa, name).to_numpy(), getattr(so, name)), name
+    assert np.array_equal(wake.WP_3d, ref["WP_3d"])

@@ -44,7 +44,7 @@ class WfStepArgs(ctypes.Structure):
         ("full_rows", c_int32), ("box_lo", c_int32 * 3), ("box_hi", c_int32 * 3),
         ("code_escape", c_int32),
         ("mon_acc", c_void_p), ("mon_phase", c_void_p), ("mon_row0", c_int64),
-        ("mon_nf", c_int32), ("mon_phase_f32", c_int32),
+        ("mon_nf", c_int32), ("mon_phase_f32", c_int32), ("two_step", c_int32),
     ]
 
 

@@ -19,10 +19,11 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libwakis_fit.so"
 SOURCES = ["lib_fused_f32_a.hip", "lib_fused_f32_b.hip", "lib_fused_f32_c.hip",
            "lib_fused_f32_d.hip", "lib_fused_f64_a.hip", "lib_fused_f64_b.hip",
-           "lib_fused_f32.hip", "lib_fused_f64.hip", "lib_batch.hip", "lib_adjoint.hip",
-           "lib_core.hip"]
+           "lib_fused_f32.hip", "lib_fused_f64.hip", "lib_fused_t2.hip", "lib_batch.hip",
+           "lib_adjoint.hip", "lib_core.hip"]
 HEADERS = ["lib_internal.h", "lib_fused.h", "fit_common.h", "step_general.h", "step_fused.h",
-           "step_fused4.h", "wake.h", "reduce.h", "adjoint.h", "adjoint_fused.h", "monitor.h"]
+           "step_fused4.h", "step_fused_t2.h", "wake.h", "reduce.h", "adjoint.h",
+           "adjoint_fused.h", "monitor.h"]
 
 # -ffp-contract=off keeps every multiply and add separately rounded, as numpy
 # and scipy's csr_matvec do on the host; -packed-fp32-ops keeps the compiler

@@ -237,6 +237,8 @@ const char *check_traj(const wf_handle *h, const void *tE, const void *tH, const
 int check_step_args(const wf_handle *h, const wf_step_args *a, const char *who) {
   if (!a->E || !a->H || !a->a || !a->b) return fail(WF_EINVAL, "%s: null field/coefficient", who);
   if (a->nsteps < 0) return fail(WF_EINVAL, "%s: nsteps < 0", who);
+  if (a->two_step != 0 && a->two_step != 1 && a->two_step != 4 && a->two_step != 8)
+    return fail(WF_EINVAL, "%s: two_step must be 0, 1, 4 or 8", who);
   if (a->jmode < WF_J_NONE || a->jmode > WF_J_FULL) return fail(WF_EINVAL, "%s: bad jmode", who);
   if (a->conductive && (!a->sigma || !a->J || a->jmode != WF_J_FULL))
     return fail(WF_EINVAL, "%s: conductive step needs J, sigma and jmode WF_J_FULL", who);

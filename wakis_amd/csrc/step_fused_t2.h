@@ -1,0 +1,349 @@
+// Two leapfrog steps per launch (temporal blocking) for the common wakefield
+// step: material codes, no conduction, masks folded into the coefficients
+// (zero_masked), J the beam line fed from the beam table (JM = 1), whole rows
+// of Nx <= 64*V cells per wave (V = 16 / sizeof(S) cells per lane).
+//
+// A workgroup owns TY rows j0 .. j0+TY-1 and a chunk of z-planes and marches
+// up in z like k_fused4, but carries two time levels, the second one plane
+// behind the first: at plane k it computes step n+1 (level 1: H1(k), E1(k))
+// and step n+2 at plane k-1 (level 2: H2(k-1), E2(k-1)), and stores only
+// level 2. Level 2 on the own rows needs level-1 values on rows j0-1 .. j0+TY
+// (E1) and j0-2 .. j0+TY (H1), so the workgroup has TY + 4 waves, wave w on
+// row j0-2+w (the last one only stages E0 of row j0+TY+1); halo values are
+// recomputed, never exchanged. Every value is formed by the same expression
+// as in k_fused4 (h_row, e_sum, the coefficient recomputation from the
+// codes), so both levels are bit-identical to two single steps.
+//
+// Per plane, with one barrier between phases:
+//   P1 loads (H0(k), codes of plane k, E0(k+1)); stage E0(k)
+//   P2 H1(k) on waves 0..TY+2; stage H1(k)
+//   P3 E1(k) on waves 1..TY+2; H2(k-1) on waves 1..TY+1 (reads E1(k-1)
+//      staged in the previous P4); stage H2(k-1)
+//   P4 E2(k-1) on waves 2..TY+1, stores; stage E1(k)
+// E and H are read once and written once per two steps.
+#pragma once
+#include "step_fused4.h"
+
+namespace wf {
+
+template <typename S, int JM, bool DESC, int TY>
+__global__ void __launch_bounds__(64 * (TY + 4))
+k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restrict__ Hin,
+            S *__restrict__ Eout, S *__restrict__ Hout, S *__restrict__ J, Coefs<S> cf, S dt,
+            FusedAux x, const S *__restrict__ beam1, const S *__restrict__ beam2,
+            S *__restrict__ cap) {
+  using I = uint32_t;  // byte offsets: every 3N array within 4 GiB
+  constexpr int V = 16 / sizeof(S);
+  constexpr int W = 64 * V;
+  constexpr int OFF = V;  // cell i at LDS slot i + OFF; slots OFF-1 and OFF+W stay 0
+  constexpr int RW = W + 2 * V;
+  constexpr int NM = 9;
+  constexpr int RE0 = TY + 4, RH1 = TY + 3, RE1 = TY + 2, RH2 = TY + 1;
+  constexpr int NT = 64 * (TY + 4);
+  __shared__ S sE0_[3 * RE0 * RW];  // E0(k), rows j0-2 .. j0+TY+1
+  __shared__ S sH1_[3 * RH1 * RW];  // H1(k), rows j0-2 .. j0+TY
+  __shared__ S sE1_[3 * RE1 * RW];  // E1(k-1) until P4 stages E1(k), rows j0-1 .. j0+TY
+  __shared__ S sH2_[3 * RH2 * RW];  // H2(k-1), rows j0-1 .. j0+TY-1
+  extern __shared__ unsigned char s_dyn[];
+  S *sMat = reinterpret_cast<S *>(s_dyn);
+  const auto sE0 = [&](int c, int r) { return sE0_ + (c * RE0 + r) * RW; };
+  const auto sH1 = [&](int c, int r) { return sH1_ + (c * RH1 + r) * RW; };
+  const auto sE1 = [&](int c, int r) { return sE1_ + (c * RE1 + r) * RW; };
+  const auto sH2 = [&](int c, int r) { return sH2_ + (c * RH2 + r) * RW; };
+
+  for (int t = threadIdx.x; t < cf.nmat * NM; t += NT) sMat[t] = cf.mat[t];
+  // the x-halo slots (cells -1 and W) are zero for the whole launch
+  for (int t = threadIdx.x; t < 3 * (RE0 + RH1 + RE1 + RH2) * 2; t += NT) {
+    int r = t >> 1;  // component row over the four buffers
+    const int slot = (t & 1) ? OFF + W : OFF - 1;
+    S *base;
+    if (r < 3 * RE0) {
+      base = sE0_;
+    } else if ((r -= 3 * RE0) < 3 * RH1) {
+      base = sH1_;
+    } else if ((r -= 3 * RH1) < 3 * RE1) {
+      base = sE1_;
+    } else {
+      r -= 3 * RE1;
+      base = sH2_;
+    }
+    base[r * RW + slot] = S(0);
+  }
+
+  const int tile = xcd_tile(blockIdx.x, x.nty * x.ntz);
+  const int tz = tile / x.nty, ty = tile - tz * x.nty;
+  const int j0 = ty * TY, k0 = tz * x.KZ;
+  const int kend = min(k0 + x.KZ, g.Nz);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = j0 - 2 + w;
+  const bool rowok = (j >= 0) && (j < g.Ny);
+  const int xi = V * lane;
+  const bool lact = rowok && (xi < g.Nx);
+  const I N = (I)g.N, sz = (I)g.sz;
+  const I col = lact ? (I)xi + (I)g.sy * (I)j : (I)0;
+  const bool doH1 = w <= TY + 2, doE1 = w >= 1 && w <= TY + 2;
+  const bool doH2 = w >= 1 && w <= TY + 1, own = w >= 2 && w <= TY + 1;
+
+  S Lx[V], tLx[V], iAz[V], itAz[V];
+#pragma unroll
+  for (int q = 0; q < V; ++q) {
+    Lx[q] = lact ? m.Lx[xi + q] : S(0);
+    tLx[q] = lact ? m.tLx[xi + q] : S(0);
+    iAz[q] = S(0);
+    itAz[q] = S(0);
+  }
+  if (lact) {
+    const I o = (I)xi + (I)g.Nx * (I)j;
+    ldv<V>(cf.iAz, o, iAz);
+    ldv<V>(cf.itAz, o, itAz);
+  }
+  const S Ly = rowok ? m.Ly[j] : S(0), tLy = rowok ? m.tLy[j] : S(0);
+  const int qbeam = (lact && JM == 1 && j == x.iys && x.ixs >= xi && x.ixs < xi + V) ? x.ixs - xi
+                                                                                    : -1;
+  // Ez window capture: own rows only, one sample per level
+  int capq0 = V, capq1 = 0;
+  int64_t capb1 = 0, capb2 = 0;
+  if (cap && own && lact && j >= x.cap_iy0 && j < x.cap_iy0 + x.cap_ny) {
+    const int lo = max(xi, x.cap_ix0), hi = min(xi + V, x.cap_ix0 + x.cap_nx);
+    if (lo < hi) {
+      capq0 = lo - xi; capq1 = hi - xi;
+      capb1 = ((int64_t)(j - x.cap_iy0) * x.cap_samples + x.cap_sample) * g.Nz;
+      capb2 = capb1 + g.Nz;
+    }
+  }
+  const bool cap1 = x.cap_sample < x.cap_samples, cap2 = x.cap_sample + 1 < x.cap_samples;
+  const int64_t capl = (int64_t)x.cap_ny * x.cap_samples * g.Nz;  // stride of one x line
+
+  const int kstart = max(k0 - 2, 0);
+  const int kE1 = max(k0 - 1, 0);  // first plane with E1 (and H2)
+  S Ek[3][V], Ek1[3][V];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int q = 0; q < V; ++q) { Ek[c][q] = S(0); Ek1[c][q] = S(0); }
+  if (lact) {
+    const I n = col + sz * (I)kstart;
+    ldv<V>(Ein, n, Ek[0]); ldv<V>(Ein, N + n, Ek[1]); ldv<V>(Ein, 2 * N + n, Ek[2]);
+    if (kstart + 1 < g.Nz) {
+      ldv<V>(Ein, n + sz, Ek1[0]); ldv<V>(Ein, N + n + sz, Ek1[1]);
+      ldv<V>(Ein, 2 * N + n + sz, Ek1[2]);
+    }
+  }
+  // carried between planes: H1(k-1), H2(k-2) (x, y), the codes and metric
+  // values of plane k-1 (level 2's coefficients)
+  S H1p[3][V], H2p[2][V];
+#pragma unroll
+  for (int q = 0; q < V; ++q) {
+    H1p[0][q] = H1p[1][q] = H1p[2][q] = S(0);
+    H2p[0][q] = H2p[1][q] = S(0);
+  }
+  uint32_t codep = 0;
+  S iaxp = 0, itaxp = 0, c0yp[V], c1yp[V];
+#pragma unroll
+  for (int q = 0; q < V; ++q) { c0yp[q] = S(0); c1yp[q] = S(0); }
+  __syncthreads();
+
+  for (int k = kstart; k <= kend; ++k) {
+    const bool pl1 = k < g.Nz;                // level-1 plane exists
+    const int p = k - 1;                      // level-2 plane
+    const bool l2h = p >= kE1 && p < kend;    // H2(p) needed
+    const bool l2e = p >= k0 && p < kend;     // E2(p) stored
+    const I nk = col + sz * (I)k;
+    // ---- P1: loads of plane k and E0(k+1), stage E0(k). E0 is fetched one plane
+    // ahead only: a deeper prefetch costs the registers that the two levels
+    // need (measured at 256x256x512: 0.249 ms per step against 0.291)
+    S Ho[3][V], c0y[V], c1y[V];
+    S iax = 0, itax = 0;
+    uint32_t code = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < V; ++q) Ho[c][q] = S(0);
+#pragma unroll
+    for (int q = 0; q < V; ++q) { c0y[q] = S(0); c1y[q] = S(0); }
+    if (lact && pl1 && doH1) {
+      ldv<V>(Hin, nk, Ho[0]); ldv<V>(Hin, N + nk, Ho[1]); ldv<V>(Hin, 2 * N + nk, Ho[2]);
+      code = ldcode<V>(cf.code, nk);
+      iax = cf.iAx[(int64_t)j + (int64_t)g.Ny * k];
+      itax = cf.itAx[(int64_t)j + (int64_t)g.Ny * k];
+      ldv<V>(cf.iAy, (I)xi + (I)g.Nx * (I)k, c0y);
+      ldv<V>(cf.itAy, (I)xi + (I)g.Nx * (I)k, c1y);
+    }
+    if (lact && k + 1 < g.Nz && k > kstart) {  // (kstart + 1 is loaded before the loop)
+      const I n1 = nk + sz;
+      ldv<V>(Ein, n1, Ek1[0]); ldv<V>(Ein, N + n1, Ek1[1]); ldv<V>(Ein, 2 * N + n1, Ek1[2]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < V; ++q) sE0(c, w)[OFF + xi + q] = (lact && pl1) ? Ek[c][q] : S(0);
+    const S Lz = pl1 ? m.Lz[k] : S(0), tLz = pl1 ? m.tLz[k] : S(0);
+    const S Lzp = (p >= 0) ? m.Lz[p] : S(0), tLzp = (p >= 0) ? m.tLz[p] : S(0);
+    __syncthreads();
+
+    // ---- P2: H1(k)
+    S H1[3][V];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < V; ++q) H1[c][q] = S(0);
+    if (lact && pl1 && doH1) {
+      S EX[V + 1], EY[V + 1], EZ[V + 1];
+#pragma unroll
+      for (int q = 0; q < V + 1; ++q) {
+        EX[q] = sE0(0, w)[OFF + xi + q]; EY[q] = sE0(1, w)[OFF + xi + q];
+        EZ[q] = sE0(2, w)[OFF + xi + q];
+      }
+#pragma unroll
+      for (int q = 0; q < V; ++q) {
+        const S *t = sMat + NM * (int)((code >> (8 * q)) & 0xffu);
+        const S a0 = (S)(iax * t[0]), a1 = (S)(c0y[q] * t[1]), a2 = (S)(iAz[q] * t[2]);
+        H1[0][q] = h_row<S, S, DESC>(Ho[0][q], a0, dt, Ly, Lz, EY[q], Ek1[1][q], EZ[q],
+                                     sE0(2, w + 1)[OFF + xi + q], +1, -1);
+        H1[1][q] = h_row<S, S, DESC>(Ho[1][q], a1, dt, Lx[q], Lz, EX[q], Ek1[0][q], EZ[q],
+                                     EZ[q + 1], -1, +1);
+        H1[2][q] = h_row<S, S, DESC>(Ho[2][q], a2, dt, Lx[q], Ly, EX[q],
+                                     sE0(0, w + 1)[OFF + xi + q], EY[q], EY[q + 1], +1, -1);
+      }
+    }
+    if (doH1) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int q = 0; q < V; ++q) sH1(c, w)[OFF + xi + q] = H1[c][q];
+    }
+    __syncthreads();
+
+    // ---- P3: E1(k), then H2(k-1)
+    S E1[3][V];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < V; ++q) E1[c][q] = S(0);
+    if (lact && pl1 && doE1 && k >= kE1) {
+      S HX[V + 1], HY[V + 1], HZ[V + 1];  // [0] = -x neighbour
+#pragma unroll
+      for (int q = 0; q < V + 1; ++q) {
+        HX[q] = sH1(0, w)[OFF + xi + q - 1];
+        HY[q] = sH1(1, w)[OFF + xi + q - 1];
+        HZ[q] = sH1(2, w)[OFF + xi + q - 1];
+      }
+#pragma unroll
+      for (int q = 0; q < V; ++q) {
+        const S *t = sMat + NM * (int)((code >> (8 * q)) & 0xffu);
+        const S b0 = (S)(t[3] * itax), b1 = (S)(t[4] * c1y[q]), b2 = (S)(t[5] * itAz[q]);
+        const S d0 = e_sum<S>(b0, tLy, tLz, H1p[1][q], HY[q + 1], sH1(2, w - 1)[OFF + xi + q],
+                              HZ[q + 1], +1, -1);
+        const S d1 = e_sum<S>(b1, tLx[q], tLz, H1p[0][q], HX[q + 1], HZ[q], HZ[q + 1], -1, +1);
+        S d2 = e_sum<S>(b2, tLx[q], tLy, sH1(0, w - 1)[OFF + xi + q], HX[q + 1], HY[q],
+                        HY[q + 1], +1, -1);
+        if (JM == 1 && q == qbeam) d2 = d2 - t[5] * beam1[k];
+        E1[0][q] = (S)(sE0(0, w)[OFF + xi + q] + dt * d0);  // E0(k) as staged
+        E1[1][q] = (S)(sE0(1, w)[OFF + xi + q] + dt * d1);
+        E1[2][q] = (S)(sE0(2, w)[OFF + xi + q] + dt * d2);
+        if (cap1 && k >= k0 && k < kend && q >= capq0 && q < capq1)
+          cap[capb1 + (int64_t)(q + xi - x.cap_ix0) * capl + k] = E1[2][q];
+      }
+    }
+    S H2[3][V];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < V; ++q) H2[c][q] = S(0);
+    if (lact && doH2 && l2h) {
+      // E1(p): own row from sE1 row w-1 (x neighbour in slot q = V), row j+1 from row w
+      S EX[V + 1], EY[V + 1], EZ[V + 1];
+#pragma unroll
+      for (int q = 0; q < V + 1; ++q) {
+        EX[q] = sE1(0, w - 1)[OFF + xi + q]; EY[q] = sE1(1, w - 1)[OFF + xi + q];
+        EZ[q] = sE1(2, w - 1)[OFF + xi + q];
+      }
+#pragma unroll
+      for (int q = 0; q < V; ++q) {
+        const S *t = sMat + NM * (int)((codep >> (8 * q)) & 0xffu);
+        const S a0 = (S)(iaxp * t[0]), a1 = (S)(c0yp[q] * t[1]), a2 = (S)(iAz[q] * t[2]);
+        H2[0][q] = h_row<S, S, DESC>(H1p[0][q], a0, dt, Ly, Lzp, EY[q], E1[1][q], EZ[q],
+                                     sE1(2, w)[OFF + xi + q], +1, -1);
+        H2[1][q] = h_row<S, S, DESC>(H1p[1][q], a1, dt, Lx[q], Lzp, EX[q], E1[0][q], EZ[q],
+                                     EZ[q + 1], -1, +1);
+        H2[2][q] = h_row<S, S, DESC>(H1p[2][q], a2, dt, Lx[q], Ly, EX[q],
+                                     sE1(0, w)[OFF + xi + q], EY[q], EY[q + 1], +1, -1);
+      }
+    }
+    if (doH2) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int q = 0; q < V; ++q) sH2(c, w - 1)[OFF + xi + q] = H2[c][q];
+    }
+    // H2(k-1) of the own rows is final: stored now; its x, y components are
+    // kept for E2(k) of the next plane
+    S H2c[2][V];
+#pragma unroll
+    for (int q = 0; q < V; ++q) { H2c[0][q] = H2[0][q]; H2c[1][q] = H2[1][q]; }
+    if (lact && own && l2e) {
+      const I np = col + sz * (I)p;
+      stv_nt<V>(Hout, np, H2[0]); stv_nt<V>(Hout, N + np, H2[1]);
+      stv_nt<V>(Hout, 2 * N + np, H2[2]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < V; ++q) H1p[c][q] = H1[c][q];
+    __syncthreads();
+
+    // ---- P4: E2(k-1) on the own rows, stores; stage E1(k)
+    if (lact && own && l2e) {
+      S HX[V + 1], HY[V + 1], HZ[V + 1];
+#pragma unroll
+      for (int q = 0; q < V + 1; ++q) {
+        HX[q] = sH2(0, w - 1)[OFF + xi + q - 1];
+        HY[q] = sH2(1, w - 1)[OFF + xi + q - 1];
+        HZ[q] = sH2(2, w - 1)[OFF + xi + q - 1];
+      }
+      const I np = col + sz * (I)p;
+      S E2[3][V];
+#pragma unroll
+      for (int q = 0; q < V; ++q) {
+        const S *t = sMat + NM * (int)((codep >> (8 * q)) & 0xffu);
+        const S b0 = (S)(t[3] * itaxp), b1 = (S)(t[4] * c1yp[q]), b2 = (S)(t[5] * itAz[q]);
+        const S d0 = e_sum<S>(b0, tLy, tLzp, H2p[1][q], HY[q + 1], sH2(2, w - 2)[OFF + xi + q],
+                              HZ[q + 1], +1, -1);
+        const S d1 = e_sum<S>(b1, tLx[q], tLzp, H2p[0][q], HX[q + 1], HZ[q], HZ[q + 1], -1, +1);
+        S d2 = e_sum<S>(b2, tLx[q], tLy, sH2(0, w - 2)[OFF + xi + q], HX[q + 1], HY[q],
+                        HY[q + 1], +1, -1);
+        S jline = 0;
+        if (JM == 1 && q == qbeam) {
+          jline = beam2[p];
+          d2 = d2 - t[5] * jline;
+        }
+        E2[0][q] = (S)(sE1(0, w - 1)[OFF + xi + q] + dt * d0);
+        E2[1][q] = (S)(sE1(1, w - 1)[OFF + xi + q] + dt * d1);
+        E2[2][q] = (S)(sE1(2, w - 1)[OFF + xi + q] + dt * d2);
+        if (JM == 1 && q == qbeam) *atw(J, 2 * N + np + (I)q) = jline;
+        if (cap2 && q >= capq0 && q < capq1)
+          cap[capb2 + (int64_t)(q + xi - x.cap_ix0) * capl + p] = E2[2][q];
+      }
+      stv_nt<V>(Eout, np, E2[0]); stv_nt<V>(Eout, N + np, E2[1]);
+      stv_nt<V>(Eout, 2 * N + np, E2[2]);
+    }
+#pragma unroll
+    for (int q = 0; q < V; ++q) { H2p[0][q] = H2c[0][q]; H2p[1][q] = H2c[1][q]; }
+    if (doE1) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int q = 0; q < V; ++q) sE1(c, w - 1)[OFF + xi + q] = E1[c][q];
+    }
+    // level 2 of the next plane uses this plane's coefficients
+    codep = code; iaxp = iax; itaxp = itax;
+#pragma unroll
+    for (int q = 0; q < V; ++q) { c0yp[q] = c0y[q]; c1yp[q] = c1y[q]; }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < V; ++q) { Ek[c][q] = Ek1[c][q]; Ek1[c][q] = S(0); }
+  }
+}
+
+}  // namespace wf

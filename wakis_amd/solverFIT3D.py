@@ -169,6 +169,10 @@ class SolverFIT3D(RoutinesMixin):
         # whole-row workgroups: N >= 1 rows; 0 row segments (tune_fused() measures and
         # sets -1 = segments or N), -1 row segments
         self.fused_full_rows = 0
+        # two leapfrog steps per launch where that kernel applies (the wakefield
+        # step with material codes, FP32, Nx <= 256): 0 off, 1 on (8 rows per
+        # workgroup), 4 or 8 rows; bit-identical to one step per launch
+        self.fused_two_step = 1
         self.fused_tuning = None  # {full_rows: ms per step} of the last tune_fused()
         # energy(): keep E^n and H^(n-1/2) on the exact path too (one copy per step)
         self.keep_previous_fields = False
@@ -697,6 +701,7 @@ class SolverFIT3D(RoutinesMixin):
         args.variant = int(self.fused_variant)
         args.cells_per_lane = int(self.fused_cells_per_lane)
         args.full_rows = int(self.fused_full_rows)
+        args.two_step = int(self.fused_two_step)
         args.zero_masked = 1 if (self.skip_masks and self._masked_zero()) else 0
         if (self.use_material_code and self._material_table()
                 and not (self._code_escape and self.fused_variant == 1)):
@@ -781,6 +786,10 @@ class SolverFIT3D(RoutinesMixin):
         make = make_args if make_args is not None else self._step_args
         if not self._choose_fused(make(steps)):
             return None
+        # with two steps per launch, warm up and time whole launches (even counts)
+        warm = 2 if self.fused_two_step else 1
+        if self.fused_two_step:
+            steps += steps % 2
         size = np.dtype(self.dtype).itemsize
         vn = 16 // size
         if self.Nx % vn:
@@ -808,7 +817,7 @@ class SolverFIT3D(RoutinesMixin):
             for c in cands:
                 self.fused_full_rows = c
                 try:
-                    self._run(make(1))
+                    self._run(make(warm))
                 except ValueError:
                     continue
                 e0 = torch.cuda.Event(enable_timing=True)
