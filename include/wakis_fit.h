@@ -180,6 +180,7 @@ typedef struct wf_step_args {
    * launch. 0 = one step per launch; 4 or 8: two steps per launch with that
    * many rows per workgroup (1 = the default, 8). */
   int32_t two_step;
+  int32_t two_step_used;  /* out: 1 if the call ran two steps per launch            */
 } wf_step_args;
 
 int wf_version(void);

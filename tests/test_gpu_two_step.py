@@ -118,3 +118,32 @@ def test_two_step_wakesolve_matches_oracle(two_step):
     for name in ("E", "H", "J"):
         assert np.array_equal(getattr(sa, name).to_numpy(), getattr(so, name)), name
     assert np.array_equal(wake.WP_3d, ref["WP_3d"])
+
+
+def test_energy_after_two_step_launches():
+    """energy() needs the fields before the last step: after an odd count the
+    last launch is a single step and the energy equals that of single steps;
+    after an even count (the last launch a pair) it is refused."""
+    import wakis_amd as wa
+    from wakis_amd import _lib
+    _say("two-step energy")
+    out = {}
+    for two_step, chunks in ((0, (7,)), (8, (7,)), (8, (6,))):
+        s = _cavity_solver((64, 40, 48), two_step)
+        beam = wa.Beam(q=1e-9, sigmaz=10e-3, beta=1.0)
+        table = torch.as_tensor(beam.table(s, 16), device=s.device).contiguous()
+        s._set_ghosts_to_0()
+        s.step_0 = False
+        s._beam_line = (int(beam.ixs), int(beam.iys))
+        a = s._step_args(chunks[0])
+        a.ixs, a.iys = int(beam.ixs), int(beam.iys)
+        a.jmode = _lib.WF_J_LINE
+        a.beam_table, a.beam_row0, a.beam_rows = _lib.ptr(table), 0, 16
+        s._run(a)
+        assert s.last_two_step == (two_step != 0)
+        if chunks[0] % 2 == 0 and two_step:
+            with pytest.raises(RuntimeError):
+                s.energy()
+        else:
+            out[two_step] = s.energy()
+    assert out[0] == out[8] and out[0] > 0

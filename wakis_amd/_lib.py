@@ -45,6 +45,7 @@ class WfStepArgs(ctypes.Structure):
         ("code_escape", c_int32),
         ("mon_acc", c_void_p), ("mon_phase", c_void_p), ("mon_row0", c_int64),
         ("mon_nf", c_int32), ("mon_phase_f32", c_int32), ("two_step", c_int32),
+        ("two_step_used", c_int32),
     ]
 
 

@@ -305,6 +305,7 @@ int wf_step(wf_handle *h, wf_step_args *a, void *stream) {
   const int rc = check_step_args(h, a, "wf_step");
   if (rc != WF_OK) return rc;
   a->result_swapped = 0;
+  a->two_step_used = 0;
   hipStream_t st = (hipStream_t)stream;
   if (a->path == 1) {
     if (h->dtype == WF_F64) return step_fused_f64(h, a, st);

@@ -194,6 +194,7 @@ class SolverFIT3D(RoutinesMixin):
         self._J_clean_version = self.J.version
         self._sigmaE_versions = (self.E.version, self.J.version)  # J == sigma*E (all zero)
         self.last_path = None
+        self.last_two_step = False  # the last step call ran two steps per launch
         self.solverInitializationTime = time.time() - t0
         # the reference's solver log keys (solverFIT3D.py:149, 171-176, 234, 281, 299, 350-354)
         self.update_logger(["use_gpu", "use_mpi", "grid", "background", "bc_low", "bc_high"])
@@ -715,15 +716,18 @@ class SolverFIT3D(RoutinesMixin):
             args.jmode = _lib.WF_J_FULL
         return True
 
-    def _finish_run(self, fused, swapped, kept=False):
-        """Bookkeeping after a batch: buffer swap and the invariants it keeps."""
+    def _finish_run(self, fused, swapped, kept=False, pair_last=False):
+        """Bookkeeping after a batch: buffer swap and the invariants it keeps.
+        ``pair_last``: the batch ended with a two-step launch, whose inputs in
+        the spare buffers are the fields two steps back."""
         self.last_path = "fused" if fused else "general"
         if fused and swapped:
             self.E.array, self._E2 = self._E2, self.E.array
             self.H.array, self._H2 = self._H2, self.H.array
         # the spare buffers now hold E^n, H^(n-1/2) of the last step (fused: the
         # ping-pong inputs; exact path: the copies made before the step)
-        self._prev_key = (self.E.version, self.H.version) if (fused or kept) else None
+        self._prev_key = ((self.E.version, self.H.version)
+                          if ((fused and not pair_last) or kept) else None)
         if self.use_conductivity:
             self._sigmaE_versions = (self.E.version, self.J.version)
 
@@ -769,7 +773,10 @@ class SolverFIT3D(RoutinesMixin):
             self._H2.copy_(self.H.array)
         _lib.check(_lib.lib().wf_step(self._handle, ctypes.byref(args), _lib.stream_ptr()),
                    "wf_step")
-        self._finish_run(fused, bool(args.result_swapped), keep)
+        # two steps per launch with an even count: the last launch was a pair
+        self.last_two_step = bool(args.two_step_used)
+        self._finish_run(fused, bool(args.result_swapped), keep,
+                         pair_last=self.last_two_step and args.nsteps % 2 == 0)
 
     def tune_fused(self, make_args=None, steps=3):
         """Choose the fused kernel's workgroup shape by timing it (no reference
