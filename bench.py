@@ -341,9 +341,11 @@ def pillbox_solver(dtype, dims=(200, 200, 400), sibc=True, sigma_wall=1e4, wake=
     grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"wall": wall, "vac": vac},
                         stl_materials={"wall": [1.0, 1.0, sigma_wall], "vac": "vacuum"},
                         verbose=0)
-    return wa.SolverFIT3D(grid, wake=wake, bc_low=["pec", "pec", "pml"],
-                          bc_high=["pec", "pec", "pml"], use_stl=True, bg="pec", dtype=dtype,
-                          verbose=0, n_pml=10, use_sibc=sibc)
+    s = wa.SolverFIT3D(grid, wake=wake, bc_low=["pec", "pec", "pml"],
+                       bc_high=["pec", "pec", "pml"], use_stl=True, bg="pec", dtype=dtype,
+                       verbose=0, n_pml=10, use_sibc=sibc)
+    s.fused_two_step = TWO_STEP[0]
+    return s
 
 
 def lossy_pml(dtype, steps=300, warmup=20, dims=(200, 200, 400), keep_fields=False, sibc=True):

@@ -171,8 +171,8 @@ typedef struct wf_step_args {
   int32_t mon_nf;
   int32_t mon_phase_f32;
   /* Fused path: 1 = advance two steps per launch (temporal blocking) where
-   * the two-step kernel applies: FP32, material codes with zero_masked, no
-   * conduction, J a beam line fed from beam_table, Nx <= 256 and a
+   * the two-step kernel applies: FP32, material codes with zero_masked, the
+   * beam line fed from beam_table (with or without conduction), Nx <= 256 and a
    * multiple of 4, row
    * segments (variant 0, full_rows <= 0, cells_per_lane 0 or 4), no box, no
    * trajectory, no monitor; otherwise one step per launch. An odd step

@@ -38,11 +38,29 @@ def _cavity_solver(dims, two_step, kz=0):
     return s
 
 
-def _beam_run(dims, two_step, chunks, kz=0):
+def _pillbox_solver(dims, two_step, kz=0):
+    """The lossy pillbox: a 3-cell conductive wall (sigma 10 S/m), 10 PML layers
+    on z; conduction everywhere (J = sigma E)."""
+    import wakis_amd as wa
+    x, y, z, vac, wall = cases.pillbox_geometry(*dims, r_cav=20e-3, half_len=15e-3,
+                                                r_pipe=5e-3, wall_cells=3, half_x=25e-3,
+                                                half_z=45e-3)
+    grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"wall": wall, "vac": vac},
+                        stl_materials={"wall": [1.0, 1.0, 10.0], "vac": "vacuum"}, verbose=0)
+    s = wa.SolverFIT3D(grid, bc_low=["pec", "pec", "pml"], bc_high=["pec", "pec", "pml"],
+                       use_stl=True, bg="pec", dtype=np.float32, verbose=0, n_pml=10,
+                       use_sibc=False)
+    assert s.use_conductivity
+    s.fused_two_step = two_step
+    s.fused_kz = kz
+    return s
+
+
+def _beam_run(dims, two_step, chunks, kz=0, make=_cavity_solver):
     """Steps with the beam table and a 3x3 Ez capture, in calls of `chunks` steps."""
     import wakis_amd as wa
     from wakis_amd import _lib
-    s = _cavity_solver(dims, two_step, kz)
+    s = make(dims, two_step, kz)
     tot = sum(chunks)
     beam = wa.Beam(q=1e-9, sigmaz=10e-3, beta=1.0)
     dev = s.device
@@ -56,13 +74,15 @@ def _beam_run(dims, two_step, chunks, kz=0):
     for k in chunks:
         a = s._step_args(k)
         a.ixs, a.iys = int(beam.ixs), int(beam.iys)
-        a.jmode = _lib.WF_J_LINE
+        if a.jmode == _lib.WF_J_NONE:
+            a.jmode = _lib.WF_J_LINE
         a.beam_table, a.beam_row0, a.beam_rows = _lib.ptr(table), n0, tot
         a.capture = _lib.ptr(cap)
         a.cap_ix0, a.cap_iy0, a.cap_nx, a.cap_ny = ixt - 1, iyt - 1, 3, 3
         a.cap_samples, a.cap_sample0 = tot + 1, n0 + 1
         s._run(a)
         assert s.last_path == "fused"
+        assert s.last_two_step == (two_step != 0 and k >= 2 and s.Nx <= 256)
         n0 += k
     torch.cuda.synchronize()
     return (s.E.to_numpy(), s.H.to_numpy(), s.J.to_numpy(), cap.cpu().numpy())
@@ -82,6 +102,23 @@ def test_two_step_equals_single_steps(dims, chunks, kz, two_step):
     ref = _beam_run(dims, 0, chunks, kz)
     got = _beam_run(dims, two_step, chunks, kz)
     assert np.any(ref[0] != 0) and np.any(ref[3] != 0)
+    for name, r, g in zip(("E", "H", "J", "capture"), ref, got):
+        assert np.array_equal(g, r), f"{name} differs ({np.count_nonzero(g != r)} values)"
+
+
+@pytest.mark.parametrize("dims,chunks,kz", [
+    ((40, 40, 80), (12,), 0),
+    ((200, 24, 60), (5, 8), 4),
+    ((100, 100, 50), (7,), 0),
+])
+@pytest.mark.parametrize("two_step", [4, 8])
+def test_two_step_conductive_equals_single_steps(dims, chunks, kz, two_step):
+    """With conduction (wall and PML): J = sigma E recomputed from E, stored by
+    the last launch of each call."""
+    _say("two-step conductive", dims, chunks, kz, two_step)
+    ref = _beam_run(dims, 0, chunks, kz, make=_pillbox_solver)
+    got = _beam_run(dims, two_step, chunks, kz, make=_pillbox_solver)
+    assert np.any(ref[0] != 0) and np.any(ref[2] != 0)
     for name, r, g in zip(("E", "H", "J", "capture"), ref, got):
         assert np.array_equal(g, r), f"{name} differs ({np.count_nonzero(g != r)} values)"
 

@@ -10,10 +10,12 @@ int fused_t2_applies(const wf_handle *h, const wf_step_args *a) {
   const bool lut = a->code && a->mat && a->nmat > 0 && a->metric2d;
   const bool box = a->box_hi[0] > a->box_lo[0] && a->box_hi[1] > a->box_lo[1] &&
                    a->box_hi[2] > a->box_lo[2];
-  // the wakefield step: a beam line fed from the beam table
-  const bool jok = a->jmode == WF_J_LINE && a->beam_table;
+  // the wakefield step: a beam line fed from the beam table (with conduction,
+  // J = sigma E elsewhere)
+  const bool jok = a->beam_table && (a->conductive ? a->jmode == WF_J_FULL
+                                                   : a->jmode == WF_J_LINE);
   return h->dtype == WF_F32 && a->two_step != 0 && a->nsteps >= 2 && lut && a->zero_masked &&
-         !a->code_escape && !a->conductive && !a->compute_f64 && jok && g.Nx <= 256 &&
+         !a->code_escape && !a->compute_f64 && jok && g.Nx <= 256 &&
          g.Nx % 4 == 0 &&
          (uint64_t)3 * (uint64_t)g.N * sizeof(float) < (1ull << 32) && a->variant == 0 &&
          a->full_rows <= 0 && (a->cells_per_lane == 0 || a->cells_per_lane == 4) && !box &&
@@ -28,8 +30,9 @@ using T2Fn = void (*)(wf::GridDev, wf::MetricDev<S>, const S *, const S *, S *, 
                       wf::Coefs<S>, S, wf::FusedAux, const S *, const S *, S *);
 
 template <typename S, int TY>
-T2Fn<S> pick_t2(bool desc) {
-  return desc ? &wf::k_fused4_t2<S, 1, true, TY> : &wf::k_fused4_t2<S, 1, false, TY>;
+T2Fn<S> pick_t2(bool cond, bool desc) {
+  if (cond) return desc ? &wf::k_fused4_t2<S, true, true, TY> : &wf::k_fused4_t2<S, true, false, TY>;
+  return desc ? &wf::k_fused4_t2<S, false, true, TY> : &wf::k_fused4_t2<S, false, false, TY>;
 }
 
 template <typename S, int TY>
@@ -46,7 +49,7 @@ int launch_t2(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStrea
   x.qrow = 64;
   const size_t lds = (size_t)a->nmat * 9 * sizeof(S);
   const int threads = 64 * (TY + 4);
-  const T2Fn<S> kern = pick_t2<S, TY>(g.m1desc != 0);
+  const T2Fn<S> kern = pick_t2<S, TY>(a->conductive != 0, g.m1desc != 0);
   if (a->kz > 0) {
     x.KZ = a->kz;
   } else {
@@ -79,6 +82,7 @@ int launch_t2(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStrea
     const S *row1 = table ? table + (a->beam_row0 + s) * g.Nz : nullptr;
     const S *row2 = table ? table + (a->beam_row0 + s + 1) * g.Nz : nullptr;
     x.cap_sample = a->cap_sample0 + s;
+    x.skip_j = (s + 2 < a->nsteps) ? 1 : 0;  // J = sigma E stored by the last launch
     hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(threads), lds, st, g, m, Ebuf[cur],
                        Hbuf[cur], Ebuf[1 - cur], Hbuf[1 - cur], (S *)a->J, cf, dt, x, row1, row2,
                        (S *)a->capture);

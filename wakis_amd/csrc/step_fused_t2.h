@@ -1,7 +1,8 @@
-// Two leapfrog steps per launch (temporal blocking) for the common wakefield
-// step: material codes, no conduction, masks folded into the coefficients
-// (zero_masked), J the beam line fed from the beam table (JM = 1), whole rows
-// of Nx <= 64*V cells per wave (V = 16 / sizeof(S) cells per lane).
+// Two leapfrog steps per launch (temporal blocking) for the wakefield step:
+// material codes, masks folded into the coefficients (zero_masked), the beam
+// line fed from the beam table, with or without conduction (COND: J = sigma E
+// everywhere, recomputed from E as in k_fused4 and stored by the last launch
+// of a call), whole rows of Nx <= 64*V cells per wave (V = 16 / sizeof(S)).
 //
 // A workgroup owns TY rows j0 .. j0+TY-1 and a chunk of z-planes and marches
 // up in z like k_fused4, but carries two time levels, the second one plane
@@ -26,7 +27,7 @@
 
 namespace wf {
 
-template <typename S, int JM, bool DESC, int TY>
+template <typename S, bool COND, bool DESC, int TY>
 __global__ void __launch_bounds__(64 * (TY + 4))
 k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restrict__ Hin,
             S *__restrict__ Eout, S *__restrict__ Hout, S *__restrict__ J, Coefs<S> cf, S dt,
@@ -98,8 +99,7 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
     ldv<V>(cf.itAz, o, itAz);
   }
   const S Ly = rowok ? m.Ly[j] : S(0), tLy = rowok ? m.tLy[j] : S(0);
-  const int qbeam = (lact && JM == 1 && j == x.iys && x.ixs >= xi && x.ixs < xi + V) ? x.ixs - xi
-                                                                                    : -1;
+  const int qbeam = (lact && j == x.iys && x.ixs >= xi && x.ixs < xi + V) ? x.ixs - xi : -1;
   // Ez window capture: own rows only, one sample per level
   int capq0 = V, capq1 = 0;
   int64_t capb1 = 0, capb2 = 0;
@@ -232,15 +232,23 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
       for (int q = 0; q < V; ++q) {
         const S *t = sMat + NM * (int)((code >> (8 * q)) & 0xffu);
         const S b0 = (S)(t[3] * itax), b1 = (S)(t[4] * c1y[q]), b2 = (S)(t[5] * itAz[q]);
-        const S d0 = e_sum<S>(b0, tLy, tLz, H1p[1][q], HY[q + 1], sH1(2, w - 1)[OFF + xi + q],
-                              HZ[q + 1], +1, -1);
-        const S d1 = e_sum<S>(b1, tLx[q], tLz, H1p[0][q], HX[q + 1], HZ[q], HZ[q + 1], -1, +1);
+        S d0 = e_sum<S>(b0, tLy, tLz, H1p[1][q], HY[q + 1], sH1(2, w - 1)[OFF + xi + q],
+                        HZ[q + 1], +1, -1);
+        S d1 = e_sum<S>(b1, tLx[q], tLz, H1p[0][q], HX[q + 1], HZ[q], HZ[q + 1], -1, +1);
         S d2 = e_sum<S>(b2, tLx[q], tLy, sH1(0, w - 1)[OFF + xi + q], HX[q + 1], HY[q],
                         HY[q + 1], +1, -1);
-        if (JM == 1 && q == qbeam) d2 = d2 - t[5] * beam1[k];
-        E1[0][q] = (S)(sE0(0, w)[OFF + xi + q] + dt * d0);  // E0(k) as staged
-        E1[1][q] = (S)(sE0(1, w)[OFF + xi + q] + dt * d1);
-        E1[2][q] = (S)(sE0(2, w)[OFF + xi + q] + dt * d2);
+        const S e0 = sE0(0, w)[OFF + xi + q], e1 = sE0(1, w)[OFF + xi + q];
+        const S e2 = sE0(2, w)[OFF + xi + q];  // E0(k) as staged
+        if (COND) {  // J = sigma E0 (the beam's current on its line)
+          d0 = d0 - t[3] * (S)(t[6] * e0);
+          d1 = d1 - t[4] * (S)(t[7] * e1);
+          d2 = d2 - t[5] * ((q == qbeam) ? beam1[k] : (S)(t[8] * e2));
+        } else if (q == qbeam) {
+          d2 = d2 - t[5] * beam1[k];
+        }
+        E1[0][q] = (S)(e0 + dt * d0);
+        E1[1][q] = (S)(e1 + dt * d1);
+        E1[2][q] = (S)(e2 + dt * d2);
         if (cap1 && k >= k0 && k < kend && q >= capq0 && q < capq1)
           cap[capb1 + (int64_t)(q + xi - x.cap_ix0) * capl + k] = E1[2][q];
       }
@@ -302,30 +310,41 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
         HZ[q] = sH2(2, w - 1)[OFF + xi + q - 1];
       }
       const I np = col + sz * (I)p;
-      S E2[3][V];
+      S E2[3][V], Jn[3][V];
 #pragma unroll
       for (int q = 0; q < V; ++q) {
         const S *t = sMat + NM * (int)((codep >> (8 * q)) & 0xffu);
         const S b0 = (S)(t[3] * itaxp), b1 = (S)(t[4] * c1yp[q]), b2 = (S)(t[5] * itAz[q]);
-        const S d0 = e_sum<S>(b0, tLy, tLzp, H2p[1][q], HY[q + 1], sH2(2, w - 2)[OFF + xi + q],
-                              HZ[q + 1], +1, -1);
-        const S d1 = e_sum<S>(b1, tLx[q], tLzp, H2p[0][q], HX[q + 1], HZ[q], HZ[q + 1], -1, +1);
+        S d0 = e_sum<S>(b0, tLy, tLzp, H2p[1][q], HY[q + 1], sH2(2, w - 2)[OFF + xi + q],
+                        HZ[q + 1], +1, -1);
+        S d1 = e_sum<S>(b1, tLx[q], tLzp, H2p[0][q], HX[q + 1], HZ[q], HZ[q + 1], -1, +1);
         S d2 = e_sum<S>(b2, tLx[q], tLy, sH2(0, w - 2)[OFF + xi + q], HX[q + 1], HY[q],
                         HY[q + 1], +1, -1);
-        S jline = 0;
-        if (JM == 1 && q == qbeam) {
-          jline = beam2[p];
-          d2 = d2 - t[5] * jline;
+        const S e0 = sE1(0, w - 1)[OFF + xi + q], e1 = sE1(1, w - 1)[OFF + xi + q];
+        const S e2 = sE1(2, w - 1)[OFF + xi + q];  // E1(k-1)
+        if (COND) {
+          d0 = d0 - t[3] * (S)(t[6] * e0);
+          d1 = d1 - t[4] * (S)(t[7] * e1);
+          d2 = d2 - t[5] * ((q == qbeam) ? beam2[p] : (S)(t[8] * e2));
+        } else if (q == qbeam) {
+          d2 = d2 - t[5] * beam2[p];
         }
-        E2[0][q] = (S)(sE1(0, w - 1)[OFF + xi + q] + dt * d0);
-        E2[1][q] = (S)(sE1(1, w - 1)[OFF + xi + q] + dt * d1);
-        E2[2][q] = (S)(sE1(2, w - 1)[OFF + xi + q] + dt * d2);
-        if (JM == 1 && q == qbeam) *atw(J, 2 * N + np + (I)q) = jline;
+        E2[0][q] = (S)(e0 + dt * d0);
+        E2[1][q] = (S)(e1 + dt * d1);
+        E2[2][q] = (S)(e2 + dt * d2);
+        if (!COND && q == qbeam) *atw(J, 2 * N + np + (I)q) = beam2[p];
+        if (COND && !x.skip_j) {  // J = sigma E2 by the call's last launch
+#pragma unroll
+          for (int c = 0; c < 3; ++c) Jn[c][q] = (S)(t[6 + c] * E2[c][q]);
+        }
         if (cap2 && q >= capq0 && q < capq1)
           cap[capb2 + (int64_t)(q + xi - x.cap_ix0) * capl + p] = E2[2][q];
       }
       stv_nt<V>(Eout, np, E2[0]); stv_nt<V>(Eout, N + np, E2[1]);
       stv_nt<V>(Eout, 2 * N + np, E2[2]);
+      if (COND && !x.skip_j) {
+        stv_nt<V>(J, np, Jn[0]); stv_nt<V>(J, N + np, Jn[1]); stv_nt<V>(J, 2 * N + np, Jn[2]);
+      }
     }
 #pragma unroll
     for (int q = 0; q < V; ++q) { H2p[0][q] = H2c[0][q]; H2p[1][q] = H2c[1][q]; }
