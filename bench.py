@@ -37,13 +37,14 @@ VALUES_PER_CELL = 18  # E r/w 3+3, H r/w 3+3, face coeffs 3, edge coeffs 3
 
 # HBM bytes per launch of the step kernel on the default workload, measured
 # with rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over this script
-# (profiles/k_fused4_cavity_pipe_256x256x512.md): FETCH_SIZE is scaled by 2.0,
+# (profiles/k_fused4_t2_cavity_pipe_256x256x512.md for FP32, two steps per
+# launch; profiles/k_fused4_cavity_pipe_256x256x512.md for FP64): FETCH_SIZE is scaled by 2.0,
 # the ratio of the bytes a 1 GiB device copy reads to what the counter reports
 # for it; WRITE_SIZE matches the copy's bytes and is taken as is. Keyed by
 # (grid, dtype, path, variant, kz); other settings report null.
 MEASURED_TRAFFIC = {
     # FP32: one launch of the two-step kernel (k_fused4_t2) = two steps
-    ((256, 256, 512), "f32", "fused", 0, 0): {"fetch_B": 935_396_147, "write_B": 830_889_370,
+    ((256, 256, 512), "f32", "fused", 0, 0): {"fetch_B": 894_291_968, "write_B": 818_300_723,
                                              "steps_per_launch": 2},
     ((256, 256, 512), "f64", "fused", 0, 0): {"fetch_B": 1_957_545_370, "write_B": 1_610_772_890,
                                              "steps_per_launch": 1},
