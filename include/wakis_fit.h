@@ -178,7 +178,8 @@ typedef struct wf_step_args {
    * trajectory, no monitor; otherwise one step per launch. An odd step
    * count ends with one single step. Results are identical to one step per
    * launch. 0 = one step per launch; 4 or 8: two steps per launch with that
-   * many rows per workgroup (1 = the default, 8). */
+   * many rows per workgroup; 1 = 8 rows on grids of 2^20 cells or more (on
+   * smaller ones one step per launch is faster). */
   int32_t two_step;
   int32_t two_step_used;  /* out: 1 if the call ran two steps per launch            */
 } wf_step_args;

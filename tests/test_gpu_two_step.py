@@ -56,12 +56,11 @@ def _pillbox_solver(dims, two_step, kz=0):
     return s
 
 
-def _beam_run(dims, two_step, chunks, kz=0, make=_cavity_solver):
-    """Steps with the beam table and a 3x3 Ez capture, in calls of `chunks` steps."""
+def _beam_setup(s, tot):
+    """Beam-table steps with a 3x3 Ez capture: returns make(n0, k), the
+    arguments of steps n0 .. n0+k-1, and the capture buffer."""
     import wakis_amd as wa
     from wakis_amd import _lib
-    s = make(dims, two_step, kz)
-    tot = sum(chunks)
     beam = wa.Beam(q=1e-9, sigmaz=10e-3, beta=1.0)
     dev = s.device
     table = torch.as_tensor(beam.table(s, tot), device=dev).contiguous()
@@ -70,8 +69,8 @@ def _beam_run(dims, two_step, chunks, kz=0, make=_cavity_solver):
     s._set_ghosts_to_0()
     s.step_0 = False
     s._beam_line = (int(beam.ixs), int(beam.iys))
-    n0 = 0
-    for k in chunks:
+
+    def make(n0, k):
         a = s._step_args(k)
         a.ixs, a.iys = int(beam.ixs), int(beam.iys)
         if a.jmode == _lib.WF_J_NONE:
@@ -80,12 +79,26 @@ def _beam_run(dims, two_step, chunks, kz=0, make=_cavity_solver):
         a.capture = _lib.ptr(cap)
         a.cap_ix0, a.cap_iy0, a.cap_nx, a.cap_ny = ixt - 1, iyt - 1, 3, 3
         a.cap_samples, a.cap_sample0 = tot + 1, n0 + 1
-        s._run(a)
+        return a
+    return make, cap
+
+
+def _beam_run(dims, two_step, chunks, kz=0, make=_cavity_solver, tune=False):
+    """Steps with the beam table and a 3x3 Ez capture, in calls of `chunks` steps."""
+    s = make(dims, two_step, kz)
+    args, cap = _beam_setup(s, sum(chunks))
+    if tune:
+        s.tune_fused(lambda k: args(0, k), steps=4)
+    n0 = 0
+    for k in chunks:
+        s._run(args(n0, k))
         assert s.last_path == "fused"
-        assert s.last_two_step == (two_step != 0 and k >= 2 and s.Nx <= 256)
+        if not tune:  # the default (1) applies from 2^20 cells
+            big = two_step != 1 or s.N >= 1 << 20
+            assert s.last_two_step == (two_step != 0 and big and k >= 2 and s.Nx <= 256)
         n0 += k
     torch.cuda.synchronize()
-    return (s.E.to_numpy(), s.H.to_numpy(), s.J.to_numpy(), cap.cpu().numpy())
+    return (s.E.to_numpy(), s.H.to_numpy(), s.J.to_numpy(), cap.cpu().numpy()), s
 
 
 @pytest.mark.parametrize("dims,chunks,kz", [
@@ -99,8 +112,8 @@ def _beam_run(dims, two_step, chunks, kz=0, make=_cavity_solver):
 @pytest.mark.parametrize("two_step", [4, 8])
 def test_two_step_equals_single_steps(dims, chunks, kz, two_step):
     _say("two-step", dims, chunks, kz, two_step)
-    ref = _beam_run(dims, 0, chunks, kz)
-    got = _beam_run(dims, two_step, chunks, kz)
+    ref = _beam_run(dims, 0, chunks, kz)[0]
+    got = _beam_run(dims, two_step, chunks, kz)[0]
     assert np.any(ref[0] != 0) and np.any(ref[3] != 0)
     for name, r, g in zip(("E", "H", "J", "capture"), ref, got):
         assert np.array_equal(g, r), f"{name} differs ({np.count_nonzero(g != r)} values)"
@@ -116,8 +129,8 @@ def test_two_step_conductive_equals_single_steps(dims, chunks, kz, two_step):
     """With conduction (wall and PML): J = sigma E recomputed from E, stored by
     the last launch of each call."""
     _say("two-step conductive", dims, chunks, kz, two_step)
-    ref = _beam_run(dims, 0, chunks, kz, make=_pillbox_solver)
-    got = _beam_run(dims, two_step, chunks, kz, make=_pillbox_solver)
+    ref = _beam_run(dims, 0, chunks, kz, make=_pillbox_solver)[0]
+    got = _beam_run(dims, two_step, chunks, kz, make=_pillbox_solver)[0]
     assert np.any(ref[0] != 0) and np.any(ref[2] != 0)
     for name, r, g in zip(("E", "H", "J", "capture"), ref, got):
         assert np.array_equal(g, r), f"{name} differs ({np.count_nonzero(g != r)} values)"
@@ -127,10 +140,40 @@ def test_two_step_applies_only_where_built():
     """Runs outside the kernel's case (FP64, coefficient arrays, wide rows) fall
     back to one step per launch with the same results."""
     _say("two-step fallback")
-    ref = _beam_run((260, 12, 20), 0, (6,))
-    got = _beam_run((260, 12, 20), 1, (6,))  # Nx > 256: single steps
+    ref = _beam_run((260, 12, 20), 0, (6,))[0]
+    got = _beam_run((260, 12, 20), 8, (6,))[0]  # Nx > 256: single steps
     for r, g in zip(ref, got):
         assert np.array_equal(g, r)
+
+
+def test_two_step_default_from_2_20_cells():
+    """The default (1) runs two steps per launch from 2^20 cells on, one step
+    per launch below; the results are the same."""
+    _say("two-step default by size")
+    for dims, two in (((64, 40, 48), False), ((256, 64, 64), True)):
+        ref = _beam_run(dims, 0, (6,))[0]
+        got, s = _beam_run(dims, 1, (6,))
+        assert s.last_two_step == two
+        for r, g in zip(ref, got):
+            assert np.array_equal(g, r)
+
+
+def test_tune_fused_times_two_step_launches():
+    """tune_fused() times two steps per launch (-2) beside the one-step shapes,
+    sets fused_two_step to 0 when one step of row segments wins, restores the
+    fields, and the run that follows equals one step per launch bitwise."""
+    _say("two-step tuner")
+    dims, chunks = (256, 64, 64), (5, 6)
+    ref = _beam_run(dims, 0, chunks)[0]
+    got, s = _beam_run(dims, 1, chunks, tune=True)
+    t = s.fused_tuning
+    assert -2 in t and -1 in t and len(t) >= 4
+    best = min(t, key=t.get)
+    assert s.fused_full_rows == max(best, -1)
+    assert s.fused_two_step == (0 if best == -1 else 1)
+    assert s.last_two_step == (best == -2)
+    for name, r, g in zip(("E", "H", "J", "capture"), ref, got):
+        assert np.array_equal(g, r), f"{name} differs ({np.count_nonzero(g != r)} values)"
 
 
 @pytest.mark.parametrize("two_step", [4, 8])

@@ -171,9 +171,11 @@ class SolverFIT3D(RoutinesMixin):
         self.fused_full_rows = 0
         # two leapfrog steps per launch where that kernel applies (the wakefield
         # step with material codes, FP32, Nx <= 256): 0 off, 1 on (8 rows per
-        # workgroup), 4 or 8 rows; bit-identical to one step per launch
+        # workgroup), 4 or 8 rows; bit-identical to one step per launch.
+        # tune_fused() sets 0 when one step per launch measures faster
         self.fused_two_step = 1
-        self.fused_tuning = None  # {full_rows: ms per step} of the last tune_fused()
+        # {full_rows: ms per step} of the last tune_fused(); -2: two steps per launch
+        self.fused_tuning = None
         # energy(): keep E^n and H^(n-1/2) on the exact path too (one copy per step)
         self.keep_previous_fields = False
         self._prev_key = None
@@ -780,13 +782,17 @@ class SolverFIT3D(RoutinesMixin):
 
     def tune_fused(self, make_args=None, steps=3):
         """Choose the fused kernel's workgroup shape by timing it (no reference
-        counterpart). Candidates: the 64-lane row segments, and whole-row
-        workgroups with the most rows for each wave count that fits. Each runs
-        one untimed and `steps` timed steps of ``make_args(k)`` (default: plain
-        steps); E, H and J are restored afterwards, and every shape gives
-        bitwise the same fields. Sets ``fused_full_rows`` to the fastest and
-        returns it (None when the fused path would not run or the choice was
-        fixed by the caller)."""
+        counterpart). Candidates: the 64-lane row segments, whole-row
+        workgroups with the most rows for each wave count that fits and, with
+        ``fused_two_step`` on and where that kernel applies, two steps per
+        launch (key -2 of ``fused_tuning``). Each runs one untimed and `steps`
+        timed steps of ``make_args(k)`` (default: plain steps; two and an even
+        count with two steps per launch on); E, H and J are restored
+        afterwards, and every shape gives bitwise the same fields. Sets
+        ``fused_full_rows`` to the fastest (-1 for both kinds of row segments),
+        ``fused_two_step`` to 0 when one step per launch of row segments beat
+        two, and returns ``fused_full_rows`` (None when the fused path would
+        not run or the choice was fixed by the caller)."""
         if (self.fused_full_rows != 0 or self.fused_variant != 0 or self.fused_cells_per_lane
                 or self.path == "general"):
             return None
@@ -811,7 +817,8 @@ class SolverFIT3D(RoutinesMixin):
             by_waves[thr // 64] = T
         if not by_waves:
             return None
-        cands = [-1] + sorted(by_waves.values())
+        two = int(self.fused_two_step)
+        cands = ([-2] if two else []) + [-1] + sorted(by_waves.values())
         saved = [F.array.clone() for F in (self.E, self.H, self.J)]
         held = self.use_conductivity and self._sigmaE_holds()
         jclean, line = self.J.version == self._J_clean_version, self._beam_line
@@ -822,11 +829,14 @@ class SolverFIT3D(RoutinesMixin):
         times = {}
         try:
             for c in cands:
-                self.fused_full_rows = c
+                self.fused_full_rows = max(c, -1)
+                self.fused_two_step = two if c == -2 else 0
                 try:
                     self._run(make(warm))
                 except ValueError:
                     continue
+                if c == -2 and not self.last_two_step:
+                    continue  # two steps per launch do not apply: -1 times these shapes
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
                 e0.record(stream)
@@ -835,6 +845,7 @@ class SolverFIT3D(RoutinesMixin):
                 e1.synchronize()
                 times[c] = e0.elapsed_time(e1) / steps
         finally:
+            self.fused_two_step = two
             for F, X in zip((self.E, self.H, self.J), saved):
                 F.array.copy_(X)
                 F.version += 1
@@ -850,9 +861,11 @@ class SolverFIT3D(RoutinesMixin):
                 self._sigmaE_versions = ((self.E.version, self.J.version) if held
                                          else (-1, -1))
         best = min(times, key=times.get) if times else 0
-        self.fused_full_rows = best
+        if best == -1 and -2 in times:
+            self.fused_two_step = 0  # one step per launch measured faster
+        self.fused_full_rows = max(best, -1)
         self.fused_tuning = times
-        return best
+        return self.fused_full_rows
 
     def energy(self, parts=False):
         """Discrete FIT energy of the last step, in joules (no reference
