@@ -36,8 +36,12 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
   using I = uint32_t;  // byte offsets: every 3N array within 4 GiB
   constexpr int V = 16 / sizeof(S);
   constexpr int W = 64 * V;
-  constexpr int OFF = V;  // cell i at LDS slot i + OFF; slots OFF-1 and OFF+W stay 0
-  constexpr int RW = W + 2 * V;
+  // LDS rows are q-major: cell V*lane+q at slot q*QS + lane + 1, so that one
+  // value per lane is 64 consecutive words (no bank conflicts for the
+  // single-word reads of the x neighbours and of cells one at a time). Slot 0
+  // of block V-1 is cell -1 and slot QS-1 of block 0 is cell W: both stay 0.
+  constexpr int QS = 64 + 2;
+  constexpr int RW = V * QS;
   constexpr int NM = 9;
   constexpr int RE0 = TY + 4, RH1 = TY + 3, RE1 = TY + 2, RH2 = TY + 1;
   constexpr int NT = 64 * (TY + 4);
@@ -56,7 +60,7 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
   // the x-halo slots (cells -1 and W) are zero for the whole launch
   for (int t = threadIdx.x; t < 3 * (RE0 + RH1 + RE1 + RH2) * 2; t += NT) {
     int r = t >> 1;  // component row over the four buffers
-    const int slot = (t & 1) ? OFF + W : OFF - 1;
+    const int slot = (t & 1) ? QS - 1 : (V - 1) * QS;
     S *base;
     if (r < 3 * RE0) {
       base = sE0_;
@@ -79,6 +83,10 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
   const int j = j0 - 2 + w;
   const bool rowok = (j >= 0) && (j < g.Ny);
   const int xi = V * lane;
+  // LDS slot of cell xi + q, -1 <= q <= V (q is a constant after unrolling)
+  const auto atx = [&](int q) {
+    return q < 0 ? (V - 1) * QS + lane : q >= V ? lane + 2 : q * QS + lane + 1;
+  };
   const bool lact = rowok && (xi < g.Nx);
   const I N = (I)g.N, sz = (I)g.sz;
   const I col = lact ? (I)xi + (I)g.sy * (I)j : (I)0;
@@ -176,7 +184,7 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int q = 0; q < V; ++q) sE0(c, w)[OFF + xi + q] = (lact && pl1) ? Ek[c][q] : S(0);
+      for (int q = 0; q < V; ++q) sE0(c, w)[atx(q)] = (lact && pl1) ? Ek[c][q] : S(0);
     const S Lz = pl1 ? m.Lz[k] : S(0), tLz = pl1 ? m.tLz[k] : S(0);
     const S Lzp = (p >= 0) ? m.Lz[p] : S(0), tLzp = (p >= 0) ? m.tLz[p] : S(0);
     __syncthreads();
@@ -191,26 +199,26 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
       S EX[V + 1], EY[V + 1], EZ[V + 1];
 #pragma unroll
       for (int q = 0; q < V + 1; ++q) {
-        EX[q] = sE0(0, w)[OFF + xi + q]; EY[q] = sE0(1, w)[OFF + xi + q];
-        EZ[q] = sE0(2, w)[OFF + xi + q];
+        EX[q] = sE0(0, w)[atx(q)]; EY[q] = sE0(1, w)[atx(q)];
+        EZ[q] = sE0(2, w)[atx(q)];
       }
 #pragma unroll
       for (int q = 0; q < V; ++q) {
         const S *t = sMat + NM * (int)((code >> (8 * q)) & 0xffu);
         const S a0 = (S)(iax * t[0]), a1 = (S)(c0y[q] * t[1]), a2 = (S)(iAz[q] * t[2]);
         H1[0][q] = h_row<S, S, DESC>(Ho[0][q], a0, dt, Ly, Lz, EY[q], Ek1[1][q], EZ[q],
-                                     sE0(2, w + 1)[OFF + xi + q], +1, -1);
+                                     sE0(2, w + 1)[atx(q)], +1, -1);
         H1[1][q] = h_row<S, S, DESC>(Ho[1][q], a1, dt, Lx[q], Lz, EX[q], Ek1[0][q], EZ[q],
                                      EZ[q + 1], -1, +1);
         H1[2][q] = h_row<S, S, DESC>(Ho[2][q], a2, dt, Lx[q], Ly, EX[q],
-                                     sE0(0, w + 1)[OFF + xi + q], EY[q], EY[q + 1], +1, -1);
+                                     sE0(0, w + 1)[atx(q)], EY[q], EY[q + 1], +1, -1);
       }
     }
     if (doH1) {
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int q = 0; q < V; ++q) sH1(c, w)[OFF + xi + q] = H1[c][q];
+        for (int q = 0; q < V; ++q) sH1(c, w)[atx(q)] = H1[c][q];
     }
     __syncthreads();
 
@@ -224,21 +232,21 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
       S HX[V + 1], HY[V + 1], HZ[V + 1];  // [0] = -x neighbour
 #pragma unroll
       for (int q = 0; q < V + 1; ++q) {
-        HX[q] = sH1(0, w)[OFF + xi + q - 1];
-        HY[q] = sH1(1, w)[OFF + xi + q - 1];
-        HZ[q] = sH1(2, w)[OFF + xi + q - 1];
+        HX[q] = sH1(0, w)[atx(q - 1)];
+        HY[q] = sH1(1, w)[atx(q - 1)];
+        HZ[q] = sH1(2, w)[atx(q - 1)];
       }
 #pragma unroll
       for (int q = 0; q < V; ++q) {
         const S *t = sMat + NM * (int)((code >> (8 * q)) & 0xffu);
         const S b0 = (S)(t[3] * itax), b1 = (S)(t[4] * c1y[q]), b2 = (S)(t[5] * itAz[q]);
-        S d0 = e_sum<S>(b0, tLy, tLz, H1p[1][q], HY[q + 1], sH1(2, w - 1)[OFF + xi + q],
+        S d0 = e_sum<S>(b0, tLy, tLz, H1p[1][q], HY[q + 1], sH1(2, w - 1)[atx(q)],
                         HZ[q + 1], +1, -1);
         S d1 = e_sum<S>(b1, tLx[q], tLz, H1p[0][q], HX[q + 1], HZ[q], HZ[q + 1], -1, +1);
-        S d2 = e_sum<S>(b2, tLx[q], tLy, sH1(0, w - 1)[OFF + xi + q], HX[q + 1], HY[q],
+        S d2 = e_sum<S>(b2, tLx[q], tLy, sH1(0, w - 1)[atx(q)], HX[q + 1], HY[q],
                         HY[q + 1], +1, -1);
-        const S e0 = sE0(0, w)[OFF + xi + q], e1 = sE0(1, w)[OFF + xi + q];
-        const S e2 = sE0(2, w)[OFF + xi + q];  // E0(k) as staged
+        const S e0 = sE0(0, w)[atx(q)], e1 = sE0(1, w)[atx(q)];
+        const S e2 = sE0(2, w)[atx(q)];  // E0(k) as staged
         if (COND) {  // J = sigma E0 (the beam's current on its line)
           d0 = d0 - t[3] * (S)(t[6] * e0);
           d1 = d1 - t[4] * (S)(t[7] * e1);
@@ -263,26 +271,26 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
       S EX[V + 1], EY[V + 1], EZ[V + 1];
 #pragma unroll
       for (int q = 0; q < V + 1; ++q) {
-        EX[q] = sE1(0, w - 1)[OFF + xi + q]; EY[q] = sE1(1, w - 1)[OFF + xi + q];
-        EZ[q] = sE1(2, w - 1)[OFF + xi + q];
+        EX[q] = sE1(0, w - 1)[atx(q)]; EY[q] = sE1(1, w - 1)[atx(q)];
+        EZ[q] = sE1(2, w - 1)[atx(q)];
       }
 #pragma unroll
       for (int q = 0; q < V; ++q) {
         const S *t = sMat + NM * (int)((codep >> (8 * q)) & 0xffu);
         const S a0 = (S)(iaxp * t[0]), a1 = (S)(c0yp[q] * t[1]), a2 = (S)(iAz[q] * t[2]);
         H2[0][q] = h_row<S, S, DESC>(H1p[0][q], a0, dt, Ly, Lzp, EY[q], E1[1][q], EZ[q],
-                                     sE1(2, w)[OFF + xi + q], +1, -1);
+                                     sE1(2, w)[atx(q)], +1, -1);
         H2[1][q] = h_row<S, S, DESC>(H1p[1][q], a1, dt, Lx[q], Lzp, EX[q], E1[0][q], EZ[q],
                                      EZ[q + 1], -1, +1);
         H2[2][q] = h_row<S, S, DESC>(H1p[2][q], a2, dt, Lx[q], Ly, EX[q],
-                                     sE1(0, w)[OFF + xi + q], EY[q], EY[q + 1], +1, -1);
+                                     sE1(0, w)[atx(q)], EY[q], EY[q + 1], +1, -1);
       }
     }
     if (doH2) {
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int q = 0; q < V; ++q) sH2(c, w - 1)[OFF + xi + q] = H2[c][q];
+        for (int q = 0; q < V; ++q) sH2(c, w - 1)[atx(q)] = H2[c][q];
     }
     // H2(k-1) of the own rows is final: stored now; its x, y components are
     // kept for E2(k) of the next plane
@@ -305,9 +313,9 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
       S HX[V + 1], HY[V + 1], HZ[V + 1];
 #pragma unroll
       for (int q = 0; q < V + 1; ++q) {
-        HX[q] = sH2(0, w - 1)[OFF + xi + q - 1];
-        HY[q] = sH2(1, w - 1)[OFF + xi + q - 1];
-        HZ[q] = sH2(2, w - 1)[OFF + xi + q - 1];
+        HX[q] = sH2(0, w - 1)[atx(q - 1)];
+        HY[q] = sH2(1, w - 1)[atx(q - 1)];
+        HZ[q] = sH2(2, w - 1)[atx(q - 1)];
       }
       const I np = col + sz * (I)p;
       S E2[3][V], Jn[3][V];
@@ -315,13 +323,13 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
       for (int q = 0; q < V; ++q) {
         const S *t = sMat + NM * (int)((codep >> (8 * q)) & 0xffu);
         const S b0 = (S)(t[3] * itaxp), b1 = (S)(t[4] * c1yp[q]), b2 = (S)(t[5] * itAz[q]);
-        S d0 = e_sum<S>(b0, tLy, tLzp, H2p[1][q], HY[q + 1], sH2(2, w - 2)[OFF + xi + q],
+        S d0 = e_sum<S>(b0, tLy, tLzp, H2p[1][q], HY[q + 1], sH2(2, w - 2)[atx(q)],
                         HZ[q + 1], +1, -1);
         S d1 = e_sum<S>(b1, tLx[q], tLzp, H2p[0][q], HX[q + 1], HZ[q], HZ[q + 1], -1, +1);
-        S d2 = e_sum<S>(b2, tLx[q], tLy, sH2(0, w - 2)[OFF + xi + q], HX[q + 1], HY[q],
+        S d2 = e_sum<S>(b2, tLx[q], tLy, sH2(0, w - 2)[atx(q)], HX[q + 1], HY[q],
                         HY[q + 1], +1, -1);
-        const S e0 = sE1(0, w - 1)[OFF + xi + q], e1 = sE1(1, w - 1)[OFF + xi + q];
-        const S e2 = sE1(2, w - 1)[OFF + xi + q];  // E1(k-1)
+        const S e0 = sE1(0, w - 1)[atx(q)], e1 = sE1(1, w - 1)[atx(q)];
+        const S e2 = sE1(2, w - 1)[atx(q)];  // E1(k-1)
         if (COND) {
           d0 = d0 - t[3] * (S)(t[6] * e0);
           d1 = d1 - t[4] * (S)(t[7] * e1);
@@ -352,7 +360,7 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int q = 0; q < V; ++q) sE1(c, w - 1)[OFF + xi + q] = E1[c][q];
+        for (int q = 0; q < V; ++q) sE1(c, w - 1)[atx(q)] = E1[c][q];
     }
     // level 2 of the next plane uses this plane's coefficients
     codep = code; iaxp = iax; itaxp = itax;
