@@ -51,9 +51,21 @@ int launch_t2(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStrea
   x.nty = (g.Ny + TY - 1) / TY;
   x.rows = TY;
   x.qrow = 64;
-  const size_t lds = (size_t)a->nmat * 9 * sizeof(S);
   const int threads = 64 * (TY + 4);
   const T2Fn<S> kern = pick_t2<S, TY>(a->conductive != 0, g.m1desc != 0);
+  // dynamic LDS: the material table, then per plane of a chunk (KZ + 3 planes)
+  // Lz, tLz, two beam values and iAx, itAx of the TY + 4 rows
+  hipFuncAttributes fa{};
+  int dev = 0, lds_dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipFuncGetAttributes(&fa, (const void *)kern) != hipSuccess ||
+      hipDeviceGetAttribute(&lds_dev, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    return fail(WF_EHIP, "wf_step: two-step kernel attributes");
+  const size_t mat = (size_t)a->nmat * 9 * sizeof(S);
+  const size_t per_plane = (size_t)(4 + 2 * (TY + 4)) * sizeof(S);
+  const int64_t room = (int64_t)lds_dev - (int64_t)fa.sharedSizeBytes - (int64_t)mat;
+  const int kz_lds = (int)(room / (int64_t)per_plane) - 3;
+  if (kz_lds < 8) return fail(WF_ERANGE, "wf_step: too many materials for the two-step kernel");
+  size_t lds = mat + per_plane * (size_t)(std::min(64, kz_lds) + 3);  // occupancy query
   if (a->kz > 0) {
     x.KZ = a->kz;
   } else {
@@ -64,6 +76,8 @@ int launch_t2(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStrea
     const int64_t kz1 = ((int64_t)g.Nz * x.nty + conc - 1) / std::max<int64_t>(conc, 1);
     x.KZ = (kz1 <= 64) ? (int)std::max<int64_t>(kz1, 8) : auto_kz(x.nty, g.Nz, conc);
   }
+  x.KZ = std::min(x.KZ, kz_lds);
+  lds = mat + per_plane * (size_t)(x.KZ + 3);
   x.ntz = (g.Nz + x.KZ - 1) / x.KZ;
   const int64_t nb = (int64_t)x.nty * x.ntz;
   if (nb > 2147483647LL) return fail(WF_ERANGE, "wf_step: grid too large for the fused path");

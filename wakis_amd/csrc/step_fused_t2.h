@@ -15,17 +15,30 @@
 // as in k_fused4 (h_row, e_sum, the coefficient recomputation from the
 // codes), so both levels are bit-identical to two single steps.
 //
-// Per plane, with one barrier between phases:
-//   P1 loads (H0(k), codes of plane k, E0(k+1)); stage E0(k)
-//   P2 H1(k) on waves 0..TY+2; stage H1(k)
-//   P3 E1(k) on waves 1..TY+2; H2(k-1) on waves 1..TY+1 (reads E1(k-1)
-//      staged in the previous P4); stage H2(k-1)
-//   P4 E2(k-1) on waves 2..TY+1, stores; stage E1(k)
-// E and H are read once and written once per two steps.
+// Per plane, with one barrier after each phase:
+//   P1 H1(k) on waves 0..TY+2; stage H1(k)
+//   P2 E1(k) on waves 1..TY+2; H2(k-1) on waves 1..TY+1 (reads E1(k-1)
+//      staged in the previous P3); stage H2(k-1)
+//   P3 stage E0(k+1); issue the loads of E0(k+2), of the codes of plane k+1
+//      and (waves 0, 1) of its iAy / itAy row; E2(k-1) on waves 2..TY+1,
+//      stores; stage E1(k) and the iAy / itAy row; issue the loads of H0(k+1)
+// Loads are in flight during the E2 update before their use, no register is
+// copied from a load before that use, and the per-plane scalars (Lz, tLz, the
+// beam, iAx, itAx) are staged in LDS once per launch, so the waves rarely wait
+// for memory. LDS rows are q-major, so a single-word read takes consecutive
+// words in consecutive lanes (no bank conflicts). E and H are read once and
+// written once per two steps.
 #pragma once
 #include "step_fused4.h"
 
 namespace wf {
+
+// A read of read-only data through the constant address space: with a uniform
+// index it compiles to a scalar load (SGPR result, scalar cache).
+template <typename T>
+__device__ __forceinline__ T ldc(const T *p, int64_t i) {
+  return ((const __attribute__((address_space(4))) T *)p)[i];
+}
 
 template <typename S, bool COND, bool DESC, int TY>
 __global__ void __launch_bounds__(64 * (TY + 4))
@@ -47,14 +60,24 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
   constexpr int NT = 64 * (TY + 4);
   __shared__ S sE0_[3 * RE0 * RW];  // E0(k), rows j0-2 .. j0+TY+1
   __shared__ S sH1_[3 * RH1 * RW];  // H1(k), rows j0-2 .. j0+TY
-  __shared__ S sE1_[3 * RE1 * RW];  // E1(k-1) until P4 stages E1(k), rows j0-1 .. j0+TY
+  __shared__ S sE1_[3 * RE1 * RW];  // E1(k-1) until P3 stages E1(k), rows j0-1 .. j0+TY
   __shared__ S sH2_[3 * RH2 * RW];  // H2(k-1), rows j0-1 .. j0+TY-1
+  // iAy and itAy of planes k-1, k, k+1 (slot plane % 3), lane-major, written
+  // by waves 0 and 1
+  __shared__ __attribute__((aligned(16))) S sCy_[3 * 2 * W];
   extern __shared__ unsigned char s_dyn[];
   S *sMat = reinterpret_cast<S *>(s_dyn);
   const auto sE0 = [&](int c, int r) { return sE0_ + (c * RE0 + r) * RW; };
   const auto sH1 = [&](int c, int r) { return sH1_ + (c * RH1 + r) * RW; };
   const auto sE1 = [&](int c, int r) { return sE1_ + (c * RE1 + r) * RW; };
   const auto sH2 = [&](int c, int r) { return sH2_ + (c * RH2 + r) * RW; };
+  const auto sCy = [&](int kk, int which) { return sCy_ + (((kk + 3) % 3) * 2 + which) * W; };
+  typedef S vec __attribute__((ext_vector_type(V)));
+  const auto ldsv = [&](const S *p, S (&o)[V]) {
+    const vec v = *reinterpret_cast<const vec *>(p);
+#pragma unroll
+    for (int q = 0; q < V; ++q) o[q] = v[q];
+  };
 
   for (int t = threadIdx.x; t < cf.nmat * NM; t += NT) sMat[t] = cf.mat[t];
   // the x-halo slots (cells -1 and W) are zero for the whole launch
@@ -79,7 +102,8 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
   const int tz = tile / x.nty, ty = tile - tz * x.nty;
   const int j0 = ty * TY, k0 = tz * x.KZ;
   const int kend = min(k0 + x.KZ, g.Nz);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // the wave index is uniform: row flags, row metrics and branches are scalar
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int j = j0 - 2 + w;
   const bool rowok = (j >= 0) && (j < g.Ny);
   const int xi = V * lane;
@@ -106,7 +130,7 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
     ldv<V>(cf.iAz, o, iAz);
     ldv<V>(cf.itAz, o, itAz);
   }
-  const S Ly = rowok ? m.Ly[j] : S(0), tLy = rowok ? m.tLy[j] : S(0);
+  const S Ly = rowok ? ldc(m.Ly, j) : S(0), tLy = rowok ? ldc(m.tLy, j) : S(0);
   const int qbeam = (lact && j == x.iys && x.ixs >= xi && x.ixs < xi + V) ? x.ixs - xi : -1;
   // Ez window capture: own rows only, one sample per level
   int capq0 = V, capq1 = 0;
@@ -124,6 +148,36 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
 
   const int kstart = max(k0 - 2, 0);
   const int kE1 = max(k0 - 1, 0);  // first plane with E1 (and H2)
+  // per-plane values of planes kstart .. kend, staged once (no scalar memory
+  // load in the z-march: its out-of-order return would make every LDS wait
+  // also wait for it): Lz, tLz, the beam of both levels, and iAx, itAx of the
+  // workgroup's rows
+  const int NP = kend - kstart + 1;
+  S *sPl = sMat + cf.nmat * NM;     // [4][NP]
+  S *sRow = sPl + 4 * NP;           // [2][TY+4][NP]
+  for (int t = threadIdx.x; t < 4 * NP; t += NT) {
+    const int which = t / NP, kk = kstart + t - which * NP;
+    S v = S(0);
+    if (kk < g.Nz) {
+      if (which == 0) v = m.Lz[kk];
+      else if (which == 1) v = m.tLz[kk];
+      else if (which == 2) v = beam1 ? beam1[kk] : S(0);
+      else v = beam2 ? beam2[kk] : S(0);
+    }
+    sPl[t] = v;
+  }
+  for (int t = threadIdx.x; t < 2 * (TY + 4) * NP; t += NT) {
+    const int which = t / ((TY + 4) * NP), rem = t - which * (TY + 4) * NP;
+    const int r = rem / NP, kk = kstart + rem - r * NP, jj = j0 - 2 + r;
+    S v = S(0);
+    if (jj >= 0 && jj < g.Ny && kk < g.Nz)
+      v = (which ? cf.itAx : cf.iAx)[(int64_t)jj + (int64_t)g.Ny * kk];
+    sRow[t] = v;
+  }
+  const auto plv = [&](int which, int kk) { return sPl[which * NP + kk - kstart]; };
+  const auto rowv = [&](int which, int kk) {
+    return sRow[(which * (TY + 4) + w) * NP + kk - kstart];
+  };
   S Ek[3][V], Ek1[3][V];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
@@ -146,9 +200,51 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
     H2p[0][q] = H2p[1][q] = S(0);
   }
   uint32_t codep = 0;
-  S iaxp = 0, itaxp = 0, c0yp[V], c1yp[V];
+  // the plane in hand, loaded one plane ahead: H0, codes, row metrics, the beam
+  // values of both levels; waves 0 and 1 also carry iAy / itAy for sCy
+  struct Plane {
+    S h[3][V], cy[V];
+    uint32_t code;
+  };
+  // iAy / itAy of plane kk (issued early: few registers)
+  const auto load_plane_a = [&](int kk, Plane &P) {
 #pragma unroll
-  for (int q = 0; q < V; ++q) { c0yp[q] = S(0); c1yp[q] = S(0); }
+    for (int q = 0; q < V; ++q) P.cy[q] = S(0);
+    if (kk < g.Nz && w < 2 && xi < g.Nx)
+      ldv<V>(w == 0 ? cf.iAy : cf.itAy, (I)xi + (I)g.Nx * (I)kk, P.cy);
+  };
+  // the codes of plane kk
+  const auto load_code = [&](int kk, Plane &P) {
+    P.code = (kk < g.Nz && lact && doH1) ? ldcode<V>(cf.code, col + sz * (I)kk) : 0u;
+  };
+  // H0 of plane kk (issued last: its registers live until H1)
+  const auto load_plane_b = [&](int kk, Plane &P) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < V; ++q) P.h[c][q] = S(0);
+    if (kk < g.Nz && lact && doH1) {
+      const I n = col + sz * (I)kk;
+      ldv<V>(Hin, n, P.h[0]); ldv<V>(Hin, N + n, P.h[1]); ldv<V>(Hin, 2 * N + n, P.h[2]);
+    }
+  };
+  const auto put_ring = [&](int kk, const Plane &P) {
+    if (w < 2 && kk <= g.Nz) {
+      vec v;
+#pragma unroll
+      for (int q = 0; q < V; ++q) v[q] = P.cy[q];
+      *reinterpret_cast<vec *>(sCy(kk, w) + xi) = v;
+    }
+  };
+  Plane P;
+  load_plane_a(kstart, P);
+  load_code(kstart, P);
+  load_plane_b(kstart, P);
+  put_ring(kstart, P);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int q = 0; q < V; ++q) sE0(c, w)[atx(q)] = lact ? Ek[c][q] : S(0);
   __syncthreads();
 
   for (int k = kstart; k <= kend; ++k) {
@@ -156,46 +252,20 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
     const int p = k - 1;                      // level-2 plane
     const bool l2h = p >= kE1 && p < kend;    // H2(p) needed
     const bool l2e = p >= k0 && p < kend;     // E2(p) stored
-    const I nk = col + sz * (I)k;
-    // ---- P1: loads of plane k and E0(k+1), stage E0(k). E0 is fetched one plane
-    // ahead only: a deeper prefetch costs the registers that the two levels
-    // need (measured at 256x256x512: 0.249 ms per step against 0.291)
-    S Ho[3][V], c0y[V], c1y[V];
-    S iax = 0, itax = 0;
-    uint32_t code = 0;
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int q = 0; q < V; ++q) Ho[c][q] = S(0);
-#pragma unroll
-    for (int q = 0; q < V; ++q) { c0y[q] = S(0); c1y[q] = S(0); }
-    if (lact && pl1 && doH1) {
-      ldv<V>(Hin, nk, Ho[0]); ldv<V>(Hin, N + nk, Ho[1]); ldv<V>(Hin, 2 * N + nk, Ho[2]);
-      code = ldcode<V>(cf.code, nk);
-      iax = cf.iAx[(int64_t)j + (int64_t)g.Ny * k];
-      itax = cf.itAx[(int64_t)j + (int64_t)g.Ny * k];
-      ldv<V>(cf.iAy, (I)xi + (I)g.Nx * (I)k, c0y);
-      ldv<V>(cf.itAy, (I)xi + (I)g.Nx * (I)k, c1y);
-    }
-    if (lact && k + 1 < g.Nz && k > kstart) {  // (kstart + 1 is loaded before the loop)
-      const I n1 = nk + sz;
-      ldv<V>(Ein, n1, Ek1[0]); ldv<V>(Ein, N + n1, Ek1[1]); ldv<V>(Ein, 2 * N + n1, Ek1[2]);
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int q = 0; q < V; ++q) sE0(c, w)[atx(q)] = (lact && pl1) ? Ek[c][q] : S(0);
-    const S Lz = pl1 ? m.Lz[k] : S(0), tLz = pl1 ? m.tLz[k] : S(0);
-    const S Lzp = (p >= 0) ? m.Lz[p] : S(0), tLzp = (p >= 0) ? m.tLz[p] : S(0);
-    __syncthreads();
+    const uint32_t code = P.code;
+    const S iax = rowv(0, k), itax = rowv(1, k), Lz = plv(0, k), tLz = plv(1, k);
+    const int pp = max(p, kstart);  // level 2 reads only for p >= kE1 >= kstart
+    const S iaxp = rowv(0, pp), itaxp = rowv(1, pp), Lzp = plv(0, pp), tLzp = plv(1, pp);
 
-    // ---- P2: H1(k)
+    // ---- P1: H1(k)
     S H1[3][V];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int q = 0; q < V; ++q) H1[c][q] = S(0);
     if (lact && pl1 && doH1) {
+      S c0y[V];
+      ldsv(sCy(k, 0) + xi, c0y);
       S EX[V + 1], EY[V + 1], EZ[V + 1];
 #pragma unroll
       for (int q = 0; q < V + 1; ++q) {
@@ -206,11 +276,11 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
       for (int q = 0; q < V; ++q) {
         const S *t = sMat + NM * (int)((code >> (8 * q)) & 0xffu);
         const S a0 = (S)(iax * t[0]), a1 = (S)(c0y[q] * t[1]), a2 = (S)(iAz[q] * t[2]);
-        H1[0][q] = h_row<S, S, DESC>(Ho[0][q], a0, dt, Ly, Lz, EY[q], Ek1[1][q], EZ[q],
+        H1[0][q] = h_row<S, S, DESC>(P.h[0][q], a0, dt, Ly, Lz, EY[q], Ek1[1][q], EZ[q],
                                      sE0(2, w + 1)[atx(q)], +1, -1);
-        H1[1][q] = h_row<S, S, DESC>(Ho[1][q], a1, dt, Lx[q], Lz, EX[q], Ek1[0][q], EZ[q],
+        H1[1][q] = h_row<S, S, DESC>(P.h[1][q], a1, dt, Lx[q], Lz, EX[q], Ek1[0][q], EZ[q],
                                      EZ[q + 1], -1, +1);
-        H1[2][q] = h_row<S, S, DESC>(Ho[2][q], a2, dt, Lx[q], Ly, EX[q],
+        H1[2][q] = h_row<S, S, DESC>(P.h[2][q], a2, dt, Lx[q], Ly, EX[q],
                                      sE0(0, w + 1)[atx(q)], EY[q], EY[q + 1], +1, -1);
       }
     }
@@ -222,13 +292,15 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
     }
     __syncthreads();
 
-    // ---- P3: E1(k), then H2(k-1)
+    // ---- P2: E1(k), then H2(k-1)
     S E1[3][V];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int q = 0; q < V; ++q) E1[c][q] = S(0);
     if (lact && pl1 && doE1 && k >= kE1) {
+      S c1y[V];
+      ldsv(sCy(k, 1) + xi, c1y);
       S HX[V + 1], HY[V + 1], HZ[V + 1];  // [0] = -x neighbour
 #pragma unroll
       for (int q = 0; q < V + 1; ++q) {
@@ -250,9 +322,9 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
         if (COND) {  // J = sigma E0 (the beam's current on its line)
           d0 = d0 - t[3] * (S)(t[6] * e0);
           d1 = d1 - t[4] * (S)(t[7] * e1);
-          d2 = d2 - t[5] * ((q == qbeam) ? beam1[k] : (S)(t[8] * e2));
+          d2 = d2 - t[5] * ((q == qbeam) ? plv(2, k) : (S)(t[8] * e2));
         } else if (q == qbeam) {
-          d2 = d2 - t[5] * beam1[k];
+          d2 = d2 - t[5] * plv(2, k);
         }
         E1[0][q] = (S)(e0 + dt * d0);
         E1[1][q] = (S)(e1 + dt * d1);
@@ -267,6 +339,8 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
 #pragma unroll
       for (int q = 0; q < V; ++q) H2[c][q] = S(0);
     if (lact && doH2 && l2h) {
+      S c0yp[V];
+      ldsv(sCy(p, 0) + xi, c0yp);
       // E1(p): own row from sE1 row w-1 (x neighbour in slot q = V), row j+1 from row w
       S EX[V + 1], EY[V + 1], EZ[V + 1];
 #pragma unroll
@@ -308,8 +382,27 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
       for (int q = 0; q < V; ++q) H1p[c][q] = H1[c][q];
     __syncthreads();
 
-    // ---- P4: E2(k-1) on the own rows, stores; stage E1(k)
+    // ---- P3: stage E0(k+1) (sE0 is free once E1(k) is done); loads of plane
+    // k+1 and of E0(k+2); E2(k-1) on the own rows, stores; stage E1(k)
+    const bool more = k < kend;
+    const S bv2 = plv(3, pp);  // the beam at plane p, level 2
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < V; ++q) sE0(c, w)[atx(q)] = Ek1[c][q];
+    load_plane_a(more ? k + 1 : g.Nz + 1, P);
+    load_code(more ? k + 1 : g.Nz, P);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < V; ++q) Ek1[c][q] = S(0);
+    if (lact && more && k + 2 < g.Nz) {
+      const I n2 = col + sz * (I)(k + 2);
+      ldv<V>(Ein, n2, Ek1[0]); ldv<V>(Ein, N + n2, Ek1[1]); ldv<V>(Ein, 2 * N + n2, Ek1[2]);
+    }
     if (lact && own && l2e) {
+      S c1yp[V];
+      ldsv(sCy(p, 1) + xi, c1yp);
       S HX[V + 1], HY[V + 1], HZ[V + 1];
 #pragma unroll
       for (int q = 0; q < V + 1; ++q) {
@@ -333,14 +426,14 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
         if (COND) {
           d0 = d0 - t[3] * (S)(t[6] * e0);
           d1 = d1 - t[4] * (S)(t[7] * e1);
-          d2 = d2 - t[5] * ((q == qbeam) ? beam2[p] : (S)(t[8] * e2));
+          d2 = d2 - t[5] * ((q == qbeam) ? bv2 : (S)(t[8] * e2));
         } else if (q == qbeam) {
-          d2 = d2 - t[5] * beam2[p];
+          d2 = d2 - t[5] * bv2;
         }
         E2[0][q] = (S)(e0 + dt * d0);
         E2[1][q] = (S)(e1 + dt * d1);
         E2[2][q] = (S)(e2 + dt * d2);
-        if (!COND && q == qbeam) *atw(J, 2 * N + np + (I)q) = beam2[p];
+        if (!COND && q == qbeam) *atw(J, 2 * N + np + (I)q) = bv2;
         if (COND && !x.skip_j) {  // J = sigma E2 by the call's last launch
 #pragma unroll
           for (int c = 0; c < 3; ++c) Jn[c][q] = (S)(t[6 + c] * E2[c][q]);
@@ -362,14 +455,11 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
 #pragma unroll
         for (int q = 0; q < V; ++q) sE1(c, w - 1)[atx(q)] = E1[c][q];
     }
+    put_ring(k + 1, P);
+    load_plane_b(more ? k + 1 : g.Nz, P);
     // level 2 of the next plane uses this plane's coefficients
-    codep = code; iaxp = iax; itaxp = itax;
-#pragma unroll
-    for (int q = 0; q < V; ++q) { c0yp[q] = c0y[q]; c1yp[q] = c1y[q]; }
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int q = 0; q < V; ++q) { Ek[c][q] = Ek1[c][q]; Ek1[c][q] = S(0); }
+    codep = code;
+    __syncthreads();
   }
 }
 
