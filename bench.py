@@ -44,7 +44,7 @@ VALUES_PER_CELL = 18  # E r/w 3+3, H r/w 3+3, face coeffs 3, edge coeffs 3
 # (grid, dtype, path, variant, kz); other settings report null.
 MEASURED_TRAFFIC = {
     # FP32: one launch of the two-step kernel (k_fused4_t2) = two steps
-    ((256, 256, 512), "f32", "fused", 0, 0): {"fetch_B": 894_291_968, "write_B": 818_300_723,
+    ((256, 256, 512), "f32", "fused", 0, 0): {"fetch_B": 889_475_301, "write_B": 805_666_202,
                                              "steps_per_launch": 2},
     ((256, 256, 512), "f64", "fused", 0, 0): {"fetch_B": 1_957_545_370, "write_B": 1_610_772_890,
                                              "steps_per_launch": 1},
