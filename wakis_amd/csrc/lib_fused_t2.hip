@@ -72,8 +72,12 @@ int launch_t2(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStrea
     // each z chunk recomputes two warm-up planes, so the fewest chunks that
     // still fill the machine in one round: at 256x256x512, 64 planes (256
     // workgroups, 0.2444 ms per step) against 32 (two rounds, 0.2477)
+    // (as many z chunks per row tile as fit beside the other tiles in one
+    // round: at 160x160x320, 20 tiles x 12 chunks = 240 workgroups, where
+    // chunks of ceil(Nz * tiles / CUs) planes made 260, a second round for 4)
     const int64_t conc = (int64_t)fused_occupancy(h, (const void *)kern, threads, lds) * h->ncu;
-    const int64_t kz1 = ((int64_t)g.Nz * x.nty + conc - 1) / std::max<int64_t>(conc, 1);
+    const int64_t ntz1 = conc / std::max<int64_t>(x.nty, 1);
+    const int64_t kz1 = ntz1 > 0 ? ((int64_t)g.Nz + ntz1 - 1) / ntz1 : (int64_t)g.Nz;
     x.KZ = (kz1 <= 64) ? (int)std::max<int64_t>(kz1, 8) : auto_kz(x.nty, g.Nz, conc);
   }
   x.KZ = std::min(x.KZ, kz_lds);
