@@ -108,6 +108,7 @@ def _beam_run(dims, two_step, chunks, kz=0, make=_cavity_solver, tune=False):
     ((20, 13, 30), (7, 6), 4),
     ((36, 9, 7), (9,), 2),
     ((256, 256, 40), (10,), 0),
+    ((64, 16, 40), (9,), 5),  # the 3x3 capture (rows 7..9) straddles two row tiles
 ])
 @pytest.mark.parametrize("two_step", [4, 8])
 def test_two_step_equals_single_steps(dims, chunks, kz, two_step):
@@ -123,6 +124,7 @@ def test_two_step_equals_single_steps(dims, chunks, kz, two_step):
     ((40, 40, 80), (12,), 0),
     ((200, 24, 60), (5, 8), 4),
     ((100, 100, 50), (7,), 0),
+    ((256, 20, 70), (6, 5), 9),  # full 256-cell rows, a partial row tile, uneven chunks
 ])
 @pytest.mark.parametrize("two_step", [4, 8])
 def test_two_step_conductive_equals_single_steps(dims, chunks, kz, two_step):
