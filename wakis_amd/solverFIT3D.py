@@ -65,6 +65,29 @@ def face_mask_kji(comp, zbits, Nx, Ny, Nz, device):
     return ~zero
 
 
+def material_codes(cols, kmax=256):
+    """Dense material codes of N cells from per-cell value columns (no
+    reference counterpart): the code of a cell is the rank of its tuple of
+    column values in lexicographic order among the distinct tuples. Returns
+    (code int64 tensor, number of codes), or (None, kmax + 1) when a column
+    has more than kmax distinct values. Per-column ranks are combined in mixed
+    radix and ranked again only when the radix product would pass 2^62, and
+    once at the end.
+    """
+    code = torch.zeros(cols[0].numel(), dtype=torch.int64, device=cols[0].device)
+    span = 1
+    for col in cols:
+        u, inv = torch.unique(col, return_inverse=True)
+        if u.numel() > kmax:
+            return None, kmax + 1
+        if span * u.numel() > 2 ** 62:
+            code = torch.unique(code, return_inverse=True)[1]
+            span = int(code.max().item()) + 1
+        code, span = code * u.numel() + inv, span * u.numel()
+    code = torch.unique(code, return_inverse=True)[1]
+    return code, int(code.max().item()) + 1
+
+
 class SolverFIT3D(RoutinesMixin):
     def __init__(self, grid, wake=None, cfln=0.5, dt=None,
                  bc_low=["Periodic", "Periodic", "Periodic"],
@@ -491,17 +514,8 @@ class SolverFIT3D(RoutinesMixin):
         cols = ([self.imu.array[c * N:(c + 1) * N] for c in range(3)]
                 + [self.ieps.array[c * N:(c + 1) * N] for c in range(3)]
                 + [self.sigma.array[c * N:(c + 1) * N] for c in range(3)])
-        code = torch.zeros(N, dtype=torch.int64, device=dev)
-        K = 0
-        for col in cols:
-            u, inv = torch.unique(col, return_inverse=True)
-            if u.numel() > 256:
-                K = 257
-                break
-            code = torch.unique(code * u.numel() + inv, return_inverse=True)[1]
+        code, K = material_codes(cols)
         esc = None  # cells whose coefficients come from the arrays (code 255)
-        if K <= 256:
-            K = int(code.max().item()) + 1
         if K <= 256:
             rep = torch.empty(K, dtype=torch.int64, device=dev)
             rep.scatter_(0, code, torch.arange(N, device=dev))
