@@ -42,6 +42,21 @@ def test_codes_with_forced_compaction():
     assert K == k == n and np.array_equal(code.numpy(), ref)
 
 
+def test_codes_with_single_valued_columns():
+    """Columns with one value (imu, or sigma of a lossless grid) are skipped."""
+    rng = np.random.default_rng(11)
+    n = 2500
+    cols = [torch.full((n,), 1.25), torch.as_tensor(rng.integers(0, 5, n) * 0.5),
+            torch.zeros(n), torch.as_tensor(rng.integers(0, 3, n) * 2.0), torch.full((n,), -7.0),
+            torch.as_tensor(rng.integers(0, 4, n) * 1.0), torch.zeros(n), torch.zeros(n),
+            torch.as_tensor(rng.integers(0, 2, n) * 3.0)]
+    ref, k = _brute(cols)
+    code, K = material_codes(cols)
+    assert K == k and np.array_equal(code.numpy(), ref)
+    code, K = material_codes([torch.full((9,), 2.0)] * 9)
+    assert K == 1 and not code.any()
+
+
 def test_column_over_the_limit():
     cols = [torch.arange(300, dtype=torch.float32)] + [torch.zeros(300)] * 8
     code, K = material_codes(cols)

@@ -72,11 +72,15 @@ def material_codes(cols, kmax=256):
     (code int64 tensor, number of codes), or (None, kmax + 1) when a column
     has more than kmax distinct values. Per-column ranks are combined in mixed
     radix and ranked again only when the radix product would pass 2^62, and
-    once at the end.
+    once at the end; a column with one value (rank 0 everywhere) is skipped,
+    found for all columns with one device sync.
     """
     code = torch.zeros(cols[0].numel(), dtype=torch.int64, device=cols[0].device)
     span = 1
-    for col in cols:
+    single = torch.stack([(c == c[:1]).all() for c in cols]).tolist()
+    for col, one in zip(cols, single):
+        if one:
+            continue
         u, inv = torch.unique(col, return_inverse=True)
         if u.numel() > kmax:
             return None, kmax + 1
