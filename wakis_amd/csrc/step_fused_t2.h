@@ -191,8 +191,8 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
       ldv<V>(Ein, 2 * N + n + sz, Ek1[2]);
     }
   }
-  // carried between planes: H1(k-1), H2(k-2) (x, y), the codes and metric
-  // values of plane k-1 (level 2's coefficients)
+  // carried between planes: H1(k-1), H2(k-2) (x, y) and the codes of plane
+  // k-1 (level 2's coefficients; its metric values come from the tables)
   S H1p[3][V], H2p[2][V];
 #pragma unroll
   for (int q = 0; q < V; ++q) {
@@ -200,8 +200,8 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
     H2p[0][q] = H2p[1][q] = S(0);
   }
   uint32_t codep = 0;
-  // the plane in hand, loaded one plane ahead: H0, codes, row metrics, the beam
-  // values of both levels; waves 0 and 1 also carry iAy / itAy for sCy
+  // the plane in hand, loaded one plane ahead: H0 and the codes; waves 0 and 1
+  // also carry its iAy / itAy row on the way to sCy
   struct Plane {
     S h[3][V], cy[V];
     uint32_t code;
