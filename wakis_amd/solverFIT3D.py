@@ -47,22 +47,19 @@ def face_mask_kji(comp, zbits, Nx, Ny, Nz, device):
 
     Same rule as the kernel's face_mask: for component d the two faces of the
     other axes are assigned in x, y, z order and the last assignment wins
-    (boundaries.py:90-103).
+    (boundaries.py:90-103). A cell is masked when the bit of its face is set
+    in ``zbits``; the faces are written as planes in that order, so a later
+    face overrides an earlier one on their common edge.
     """
-    i = torch.arange(Nx, device=device).view(1, 1, Nx)
-    j = torch.arange(Ny, device=device).view(1, Ny, 1)
-    k = torch.arange(Nz, device=device).view(Nz, 1, 1)
-    f = torch.full((Nz, Ny, Nx), -1, dtype=torch.int8, device=device)
-    order = {0: [(j, 0, 1), (j, Ny - 1, 4), (k, 0, 2), (k, Nz - 1, 5)],
-             1: [(i, 0, 0), (i, Nx - 1, 3), (k, 0, 2), (k, Nz - 1, 5)],
-             2: [(i, 0, 0), (i, Nx - 1, 3), (j, 0, 1), (j, Ny - 1, 4)]}[comp]
-    for idx, val, face in order:
-        f = torch.where(idx == val, torch.tensor(face, dtype=torch.int8, device=device), f)
-    zero = torch.zeros_like(f, dtype=torch.bool)
-    for face in range(6):
-        if (zbits >> face) & 1:
-            zero |= f == face
-    return ~zero
+    keep = torch.ones((Nz, Ny, Nx), dtype=torch.bool, device=device)
+    if zbits == 0:
+        return keep
+    planes = {0: [(1, 0, 1), (1, Ny - 1, 4), (0, 0, 2), (0, Nz - 1, 5)],
+              1: [(2, 0, 0), (2, Nx - 1, 3), (0, 0, 2), (0, Nz - 1, 5)],
+              2: [(2, 0, 0), (2, Nx - 1, 3), (1, 0, 1), (1, Ny - 1, 4)]}[comp]
+    for axis, val, face in planes:
+        keep.select(axis, val).fill_(not (zbits >> face) & 1)
+    return keep
 
 
 def material_codes(cols, kmax=256):
@@ -621,87 +618,72 @@ class SolverFIT3D(RoutinesMixin):
                self._beam_line)
         if key == self._wrap_key:
             return self._wrap_ok
-        Nx, Ny, Nz, N = self.Nx, self.Ny, self.Nz, self.N
+        self._wrap_key, self._wrap_ok = key, self._wraps_dead_now()
+        return self._wrap_ok
+
+    def _wraps_dead_now(self):
+        """The test of ``_wraps_dead`` on the boundary planes. Each wrap term
+        pairs a row with the column one flat index away (+-1 across a row end,
+        +-Nx across the last row of a plane), so rows and columns are two views
+        of the same boundary plane shifted by one element."""
+        Nx, Ny, Nz = self.Nx, self.Ny, self.Nz
         dev = self.device
-        ar = torch.arange
-        a, b = self._a, self._b
-        E, H, J, ie = self.E.array, self.H.array, self.J.array, self.ieps.array
+        a3 = self._a.view(3, Nz, Ny, Nx)
+        b3 = self._b.view(3, Nz, Ny, Nx)
+        E3 = self.E.array.view(3, Nz, Ny, Nx)
+        H3 = self.H.array.view(3, Nz, Ny, Nx)
+        J3 = self.J.array.view(3, Nz, Ny, Nx)
+        ie3 = self.ieps.array.view(3, Nz, Ny, Nx)
         bl = self._beam_line
+        beam = None
+        if bl is not None:
+            beam = torch.zeros((Nz, Ny, Nx), dtype=torch.bool, device=dev)
+            beam[:, bl[1], bl[0]] = True
 
-        def lin(i, j, k):
-            return i + Nx * j + Nx * Ny * k
-
-        def fmask(zbits, comp, i, j, k):
+        def keep(zbits, c):  # columns not masked by the face masks
             if zbits == 0:
-                return torch.ones_like(i, dtype=torch.bool)
-            f = torch.full_like(i, -1)
-            order = {0: [(j, 0, 1), (j, Ny - 1, 4), (k, 0, 2), (k, Nz - 1, 5)],
-                     1: [(i, 0, 0), (i, Nx - 1, 3), (k, 0, 2), (k, Nz - 1, 5)],
-                     2: [(i, 0, 0), (i, Nx - 1, 3), (j, 0, 1), (j, Ny - 1, 4)]}[comp]
-            for idx, val, face in order:
-                f = torch.where(idx == val, torch.full_like(f, face), f)
-            zero = torch.zeros_like(i, dtype=torch.bool)
-            for face in range(6):
-                if (zbits >> face) & 1:
-                    zero |= f == face
-            return ~zero
+                return torch.ones((Nz, Ny, Nx), dtype=torch.bool, device=dev)
+            return face_mask_kji(c, zbits, Nx, Ny, Nz, dev)
 
-        def frozen_E(c, i, j, k):
-            t = c * N + lin(i, j, k)
-            fz = (b[t] == 0) & (E[t] == 0) & ((ie[t] == 0) | (J[t] == 0))
-            if bl is not None and c == 2:
-                fz &= ~((i == bl[0]) & (j == bl[1]))
+        def frozen_E(c, sl):  # E column frozen at zero (no coefficient, no source)
+            fz = (b3[c][sl] == 0) & (E3[c][sl] == 0) & ((ie3[c][sl] == 0) | (J3[c][sl] == 0))
+            if beam is not None and c == 2:
+                fz = fz & ~beam[sl]
             return fz
 
-        def frozen_H(c, i, j, k):
-            t = c * N + lin(i, j, k)
-            return (a[t] == 0) & (H[t] == 0)
+        def frozen_H(c, sl):
+            return (a3[c][sl] == 0) & (H3[c][sl] == 0)
 
-        bad = torch.zeros((), dtype=torch.bool, device=dev)
-        # (1) H_y, H_z rows at i=Nx-1 use E_z, E_y at (0, j+1, k) / (0, 0, k+1)
-        jj, kk = torch.meshgrid(ar(Ny, device=dev), ar(Nz, device=dev), indexing="ij")
-        jj, kk = jj.reshape(-1), kk.reshape(-1)
-        valid = ~((jj == Ny - 1) & (kk == Nz - 1))
-        tj = torch.where(jj < Ny - 1, jj + 1, torch.zeros_like(jj))
-        tk = torch.where(jj < Ny - 1, kk, kk + 1).clamp(max=Nz - 1)
-        ti = torch.zeros_like(jj)
-        rn = lin(torch.full_like(jj, Nx - 1), jj, kk)
+        tL = [torch.as_tensor(self._tL1[0], device=dev).view(1, 1, Nx).expand(Nz, Ny, Nx),
+              torch.as_tensor(self._tL1[1], device=dev).view(1, Ny, 1).expand(Nz, Ny, Nx),
+              torch.as_tensor(self._tL1[2], device=dev).view(Nz, 1, 1).expand(Nz, Ny, Nx)]
+        X0 = (slice(None), slice(None), 0)        # plane i = 0, flattened in (k, j)
+        X1 = (slice(None), slice(None), Nx - 1)   # plane i = Nx-1
+        Ylo = (slice(1, Nz), 0, slice(None))          # row j = 0 of planes 1..Nz-1
+        Yhi = (slice(0, Nz - 1), Ny - 1, slice(None))  # row j = Ny-1 of planes 0..Nz-2
+        terms = []
+        # (1) H_y, H_z rows at i = Nx-1 take E_z, E_y at flat index + 1
+        #     ((0, j+1, k), or (0, 0, k+1) after the last row of a plane)
         for rc, cc in ((1, 2), (2, 1)):
-            w = valid & (a[rc * N + rn] != 0) & fmask(self.pec_zero, cc, ti, tj, tk)
-            bad |= (w & ~frozen_E(cc, ti, tj, tk)).any()
-        # (2) H_x, H_z rows at j=Ny-1 use E_z, E_x at (i, 0, k+1)
-        ii, kk = torch.meshgrid(ar(Nx, device=dev), ar(Nz - 1, device=dev), indexing="ij")
-        ii, kk = ii.reshape(-1), kk.reshape(-1)
-        rn = lin(ii, torch.full_like(ii, Ny - 1), kk)
-        tj, tk = torch.zeros_like(ii), kk + 1
+            w = (a3[rc][X1] != 0).reshape(-1)[:-1] & keep(self.pec_zero, cc)[X0].reshape(-1)[1:]
+            terms.append(w & ~frozen_E(cc, X0).reshape(-1)[1:])
+        # (2) H_x, H_z rows at j = Ny-1 take E_z, E_x at flat index + Nx: (i, 0, k+1)
         for rc, cc in ((0, 2), (2, 0)):
-            w = (a[rc * N + rn] != 0) & fmask(self.pec_zero, cc, ii, tj, tk)
-            bad |= (w & ~frozen_E(cc, ii, tj, tk)).any()
-        # (3) E_y, E_z rows at i=0 use H_z, H_y at (Nx-1, j-1, k) / (Nx-1, Ny-1, k-1)
-        jj, kk = torch.meshgrid(ar(Ny, device=dev), ar(Nz, device=dev), indexing="ij")
-        jj, kk = jj.reshape(-1), kk.reshape(-1)
-        valid = ~((jj == 0) & (kk == 0))
-        sj = torch.where(jj > 0, jj - 1, torch.full_like(jj, Ny - 1))
-        sk = torch.where(jj > 0, kk, kk - 1).clamp(min=0)
-        si = torch.full_like(jj, Nx - 1)
-        rn = lin(torch.zeros_like(jj), jj, kk)
-        tLz = torch.as_tensor(self._tL1[2], device=dev)
-        tLy = torch.as_tensor(self._tL1[1], device=dev)
-        tLx = torch.as_tensor(self._tL1[0], device=dev)
-        for rc, cc, tl in ((1, 2, tLz[sk]), (2, 1, tLy[sj])):
-            w = valid & (b[rc * N + rn] != 0) & (tl != 0) & fmask(self.pmc_zero, cc, si, sj, sk)
-            bad |= (w & ~frozen_H(cc, si, sj, sk)).any()
-        # (4) E_x, E_z rows at j=0 use H_z, H_x at (i, Ny-1, k-1)
-        ii, kk = torch.meshgrid(ar(Nx, device=dev), ar(1, Nz, device=dev), indexing="ij")
-        ii, kk = ii.reshape(-1), kk.reshape(-1)
-        rn = lin(ii, torch.zeros_like(ii), kk)
-        sj, sk = torch.full_like(ii, Ny - 1), kk - 1
-        for rc, cc, tl in ((0, 2, tLz[sk]), (2, 0, tLx[ii])):
-            w = (b[rc * N + rn] != 0) & (tl != 0) & fmask(self.pmc_zero, cc, ii, sj, sk)
-            bad |= (w & ~frozen_H(cc, ii, sj, sk)).any()
-        ok = not bool(bad.item())
-        self._wrap_key, self._wrap_ok = key, ok
-        return ok
+            w = (a3[rc][Yhi] != 0) & keep(self.pec_zero, cc)[Ylo]
+            terms.append((w & ~frozen_E(cc, Ylo)).reshape(-1))
+        # (3) E_y, E_z rows at i = 0 take H_z, H_y at flat index - 1 (weight tLz, tLy
+        #     of the column)
+        for rc, cc, ax in ((1, 2, 2), (2, 1, 1)):
+            w = ((b3[rc][X0] != 0).reshape(-1)[1:] & (tL[ax][X1] != 0).reshape(-1)[:-1]
+                 & keep(self.pmc_zero, cc)[X1].reshape(-1)[:-1])
+            terms.append(w & ~frozen_H(cc, X1).reshape(-1)[:-1])
+        # (4) E_x, E_z rows at j = 0 take H_z, H_x at flat index - Nx: (i, Ny-1, k-1)
+        #     (weight tLz, tLx of the column)
+        for rc, cc, ax in ((0, 2, 2), (2, 0, 0)):
+            w = (b3[rc][Ylo] != 0) & (tL[ax][Yhi] != 0) & keep(self.pmc_zero, cc)[Yhi]
+            terms.append((w & ~frozen_H(cc, Yhi)).reshape(-1))
+        bad = torch.stack([t.any() for t in terms]).any()
+        return not bool(bad.item())
 
     def _prepare_fused(self, args):
         """Choose the path for a batch and fill the fused-path fields of args.
