@@ -1,4 +1,5 @@
-"""GPU: two leapfrog steps per launch (k_fused4_t2) against one step per launch.
+"""GPU: two leapfrog steps per launch (k_fused4_t2) against one step per launch,
+FP32 (8 or 4 rows per workgroup) and FP64 (2 rows, two waves per row).
 
 The two-step kernel forms every value with the single-step kernel's
 expressions, so E, H, J and the captured Ez window must be bit-identical to
@@ -24,7 +25,7 @@ def _say(*a):
     print(*a, flush=True)
 
 
-def _cavity_solver(dims, two_step, kz=0):
+def _cavity_solver(dims, two_step, kz=0, dtype=np.float32):
     import wakis_amd as wa
     from wakis_amd.solids import Box, Cylinder
     solid = Box(-25e-3, 25e-3, -25e-3, 25e-3, -25e-3, 25e-3) | Cylinder(5e-3)
@@ -32,13 +33,13 @@ def _cavity_solver(dims, two_step, kz=0):
                         stl_solids={"cavity": solid}, stl_materials={"cavity": "vacuum"},
                         verbose=0)
     s = wa.SolverFIT3D(grid, bc_low=["pec", "pec", "abc"], bc_high=["pec", "pec", "abc"],
-                       use_stl=True, bg="pec", dtype=np.float32, verbose=0)
+                       use_stl=True, bg="pec", dtype=dtype, verbose=0)
     s.fused_two_step = two_step
     s.fused_kz = kz
     return s
 
 
-def _pillbox_solver(dims, two_step, kz=0):
+def _pillbox_solver(dims, two_step, kz=0, dtype=np.float32):
     """The lossy pillbox: a 3-cell conductive wall (sigma 10 S/m), 10 PML layers
     on z; conduction everywhere (J = sigma E)."""
     import wakis_amd as wa
@@ -48,7 +49,7 @@ def _pillbox_solver(dims, two_step, kz=0):
     grid = wa.GridFIT3D(x=x, y=y, z=z, stl_solids={"wall": wall, "vac": vac},
                         stl_materials={"wall": [1.0, 1.0, 10.0], "vac": "vacuum"}, verbose=0)
     s = wa.SolverFIT3D(grid, bc_low=["pec", "pec", "pml"], bc_high=["pec", "pec", "pml"],
-                       use_stl=True, bg="pec", dtype=np.float32, verbose=0, n_pml=10,
+                       use_stl=True, bg="pec", dtype=dtype, verbose=0, n_pml=10,
                        use_sibc=False)
     assert s.use_conductivity
     s.fused_two_step = two_step
@@ -83,9 +84,10 @@ def _beam_setup(s, tot):
     return make, cap
 
 
-def _beam_run(dims, two_step, chunks, kz=0, make=_cavity_solver, tune=False):
+def _beam_run(dims, two_step, chunks, kz=0, make=_cavity_solver, tune=False,
+              dtype=np.float32):
     """Steps with the beam table and a 3x3 Ez capture, in calls of `chunks` steps."""
-    s = make(dims, two_step, kz)
+    s = make(dims, two_step, kz, dtype)
     args, cap = _beam_setup(s, sum(chunks))
     if tune:
         s.tune_fused(lambda k: args(0, k), steps=4)
@@ -110,11 +112,12 @@ def _beam_run(dims, two_step, chunks, kz=0, make=_cavity_solver, tune=False):
     ((256, 256, 40), (10,), 0),
     ((64, 16, 40), (9,), 5),  # the 3x3 capture (rows 7..9) straddles two row tiles
 ])
-@pytest.mark.parametrize("two_step", [4, 8])
-def test_two_step_equals_single_steps(dims, chunks, kz, two_step):
-    _say("two-step", dims, chunks, kz, two_step)
-    ref = _beam_run(dims, 0, chunks, kz)[0]
-    got = _beam_run(dims, two_step, chunks, kz)[0]
+@pytest.mark.parametrize("dtype,two_step", [(np.float32, 4), (np.float32, 8),
+                                            (np.float64, 8)])
+def test_two_step_equals_single_steps(dims, chunks, kz, dtype, two_step):
+    _say("two-step", dims, chunks, kz, np.dtype(dtype).name, two_step)
+    ref = _beam_run(dims, 0, chunks, kz, dtype=dtype)[0]
+    got = _beam_run(dims, two_step, chunks, kz, dtype=dtype)[0]
     assert np.any(ref[0] != 0) and np.any(ref[3] != 0)
     for name, r, g in zip(("E", "H", "J", "capture"), ref, got):
         assert np.array_equal(g, r), f"{name} differs ({np.count_nonzero(g != r)} values)"
@@ -126,13 +129,14 @@ def test_two_step_equals_single_steps(dims, chunks, kz, two_step):
     ((100, 100, 50), (7,), 0),
     ((256, 20, 70), (6, 5), 9),  # full 256-cell rows, a partial row tile, uneven chunks
 ])
-@pytest.mark.parametrize("two_step", [4, 8])
-def test_two_step_conductive_equals_single_steps(dims, chunks, kz, two_step):
+@pytest.mark.parametrize("dtype,two_step", [(np.float32, 4), (np.float32, 8),
+                                            (np.float64, 8)])
+def test_two_step_conductive_equals_single_steps(dims, chunks, kz, dtype, two_step):
     """With conduction (wall and PML): J = sigma E recomputed from E, stored by
     the last launch of each call."""
-    _say("two-step conductive", dims, chunks, kz, two_step)
-    ref = _beam_run(dims, 0, chunks, kz, make=_pillbox_solver)[0]
-    got = _beam_run(dims, two_step, chunks, kz, make=_pillbox_solver)[0]
+    _say("two-step conductive", dims, chunks, kz, np.dtype(dtype).name, two_step)
+    ref = _beam_run(dims, 0, chunks, kz, make=_pillbox_solver, dtype=dtype)[0]
+    got = _beam_run(dims, two_step, chunks, kz, make=_pillbox_solver, dtype=dtype)[0]
     assert np.any(ref[0] != 0) and np.any(ref[2] != 0)
     for name, r, g in zip(("E", "H", "J", "capture"), ref, got):
         assert np.array_equal(g, r), f"{name} differs ({np.count_nonzero(g != r)} values)"
@@ -178,13 +182,14 @@ def test_tune_fused_times_two_step_launches():
         assert np.array_equal(g, r), f"{name} differs ({np.count_nonzero(g != r)} values)"
 
 
-@pytest.mark.parametrize("two_step", [4, 8])
-def test_two_step_wakesolve_matches_oracle(two_step):
+@pytest.mark.parametrize("dtype,two_step", [(np.float32, 4), (np.float32, 8),
+                                            (np.float64, 8)])
+def test_two_step_wakesolve_matches_oracle(dtype, two_step):
     import wakis_amd as wa
-    _say("two-step wakesolve vs oracle", two_step)
+    _say("two-step wakesolve vs oracle", np.dtype(dtype).name, two_step)
     x, y, z, mask = cases.cavity_pipe_geometry(20, 20, 40)
     so = fo.OracleSolver(x, y, z, ["pec", "pec", "abc"], ["pec", "pec", "abc"], bg="pec",
-                         solids=[(mask, "vacuum")], dtype=np.float32)
+                         solids=[(mask, "vacuum")], dtype=dtype)
     q, sigmaz, beta, wl = 1e-9, 10e-3, 1.0, 0.05
     cap = fo.wakesolve_fields(so, wl, q, sigmaz, beta, 0.0, 0.0, 0.0, 0.0,
                               wo.wake_ti(sigmaz, beta))
@@ -194,7 +199,7 @@ def test_two_step_wakesolve_matches_oracle(two_step):
     wake = wa.WakeSolver(q=q, sigmaz=sigmaz, beta=beta, skip_cells=2, save=False, verbose=0)
     sa = wa.SolverFIT3D(grid, wake=wake, bc_low=["pec", "pec", "abc"],
                         bc_high=["pec", "pec", "abc"], use_stl=True, bg="pec",
-                        dtype=np.float32, verbose=0)
+                        dtype=dtype, verbose=0)
     sa.fused_two_step = two_step
     sa.wakesolve(wakelength=wl)
     for name in ("E", "H", "J"):

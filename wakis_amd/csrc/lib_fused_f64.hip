@@ -28,6 +28,7 @@ inline int auto_cells_per_lane(int Nx, int vn) {
 template <typename S, typename Cm>
 int step_fused(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStream_t st) {
   const wf::GridDev &g = h->g;
+  if (a->two_step && fused_t2_applies(h, a)) return step_fused_t2_f64(h, a, st);
   // x-vectorised kernel: Nx a multiple of the lane width (16 bytes) and every
   // 3N array within 4 GiB
   const bool idx32v = (uint64_t)3 * (uint64_t)g.N * sizeof(S) < (1ull << 32);

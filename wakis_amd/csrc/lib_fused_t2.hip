@@ -1,5 +1,5 @@
-// Two leapfrog steps per launch (k_fused4_t2) for FP32 storage: the
-// applicability rule and the launcher.
+// Two leapfrog steps per launch (k_fused4_t2): the applicability rule and the
+// launchers (FP32: 8 or 4 rows per workgroup; FP64: 2 rows, two waves each).
 #include "lib_fused.h"
 #include "step_fused_t2.h"
 
@@ -18,11 +18,12 @@ int fused_t2_applies(const wf_handle *h, const wf_step_args *a) {
   // launch leave the machine idle and one step per launch is faster (64^3:
   // 28.5 against 17.1 us per step; 96x96x160: 32.7 against 37.1)
   const bool big = a->two_step != 1 || g.N >= (int64_t)1 << 20;
-  return h->dtype == WF_F32 && a->two_step != 0 && big && a->nsteps >= 2 && lut && a->zero_masked &&
-         !a->code_escape && !a->compute_f64 && jok && g.Nx <= 256 &&
-         g.Nx % 4 == 0 &&
-         (uint64_t)3 * (uint64_t)g.N * sizeof(float) < (1ull << 32) && a->variant == 0 &&
-         a->full_rows <= 0 && (a->cells_per_lane == 0 || a->cells_per_lane == 4) && !box &&
+  const int elem = h->dtype == WF_F64 ? 8 : 4, vn = 16 / elem;  // cells per lane
+  return (h->dtype == WF_F32 || h->dtype == WF_F64) && a->two_step != 0 && big &&
+         a->nsteps >= 2 && lut && a->zero_masked && !a->code_escape && !a->compute_f64 && jok &&
+         g.Nx <= 256 && g.Nx % vn == 0 &&
+         (uint64_t)3 * (uint64_t)g.N * (uint64_t)elem < (1ull << 32) && a->variant == 0 &&
+         a->full_rows <= 0 && (a->cells_per_lane == 0 || a->cells_per_lane == vn) && !box &&
          !a->traj_E && !a->mon_acc;
 }
 
@@ -51,7 +52,7 @@ int launch_t2(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStrea
   x.nty = (g.Ny + TY - 1) / TY;
   x.rows = TY;
   x.qrow = 64;
-  const int threads = 64 * (TY + 4);
+  const int threads = 64 * (TY + 4) * (int)(sizeof(S) / 4);
   const T2Fn<S> kern = pick_t2<S, TY>(a->conductive != 0, g.m1desc != 0);
   // dynamic LDS: the material table, then per plane of a chunk (KZ + 3 planes)
   // Lz, tLz, two beam values and iAx, itAx of the TY + 4 rows
@@ -121,7 +122,7 @@ int launch_t2(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStrea
       b.E = a->E2; b.E2 = a->E;
       b.H = a->H2; b.H2 = a->H;
     }
-    const int rc = step_fused_f32(h, &b, st);
+    const int rc = sizeof(S) == 4 ? step_fused_f32(h, &b, st) : step_fused_f64(h, &b, st);
     if (rc != WF_OK) return rc;
     if (b.result_swapped) cur = 1 - cur;
   }
@@ -138,6 +139,12 @@ int launch_t2(wf_handle *h, wf_step_args *a, const wf::MetricDev<S> &m, hipStrea
 int step_fused_t2_f32(wf_handle *h, wf_step_args *a, hipStream_t st) {
   if (a->two_step == 4) return wff::launch_t2<float, 4>(h, a, h->mf, st);
   return wff::launch_t2<float, 8>(h, a, h->mf, st);
+}
+
+// FP64: 2 rows per workgroup (12 waves, two per row: the 2 KB rows of six
+// staged rows fill 112 KB of LDS)
+int step_fused_t2_f64(wf_handle *h, wf_step_args *a, hipStream_t st) {
+  return wff::launch_t2<double, 2>(h, a, h->md, st);
 }
 
 #pragma GCC visibility pop

@@ -124,4 +124,5 @@ WF_HIDDEN int step_fused_f32(wf_handle *h, wf_step_args *a, hipStream_t st);
 // two steps per launch (lib_fused_t2.hip): 1 if this call qualifies
 WF_HIDDEN int fused_t2_applies(const wf_handle *h, const wf_step_args *a);
 WF_HIDDEN int step_fused_t2_f32(wf_handle *h, wf_step_args *a, hipStream_t st);
+WF_HIDDEN int step_fused_t2_f64(wf_handle *h, wf_step_args *a, hipStream_t st);
 WF_HIDDEN int step_fused_f64(wf_handle *h, wf_step_args *a, hipStream_t st);

@@ -2,15 +2,16 @@
 // material codes, masks folded into the coefficients (zero_masked), the beam
 // line fed from the beam table, with or without conduction (COND: J = sigma E
 // everywhere, recomputed from E as in k_fused4 and stored by the last launch
-// of a call), whole rows of Nx <= 64*V cells per wave (V = 16 / sizeof(S)).
+// of a call), rows of Nx <= 256 cells: one wave per row with FP32 (V = 4
+// cells per lane), two waves per row with FP64 (V = 2).
 //
 // A workgroup owns TY rows j0 .. j0+TY-1 and a chunk of z-planes and marches
 // up in z like k_fused4, but carries two time levels, the second one plane
 // behind the first: at plane k it computes step n+1 (level 1: H1(k), E1(k))
 // and step n+2 at plane k-1 (level 2: H2(k-1), E2(k-1)), and stores only
 // level 2. Level 2 on the own rows needs level-1 values on rows j0-1 .. j0+TY
-// (E1) and j0-2 .. j0+TY (H1), so the workgroup has TY + 4 waves, wave w on
-// row j0-2+w (the last one only stages E0 of row j0+TY+1); halo values are
+// (E1) and j0-2 .. j0+TY (H1), so the workgroup has TY + 4 rows of waves,
+// row w = j0-2+w (the last one only stages E0 of row j0+TY+1); halo values are
 // recomputed, never exchanged. Every value is formed by the same expression
 // as in k_fused4 (h_row, e_sum, the coefficient recomputation from the
 // codes), so both levels are bit-identical to two single steps.
@@ -41,23 +42,26 @@ __device__ __forceinline__ T ldc(const T *p, int64_t i) {
 }
 
 template <typename S, bool COND, bool DESC, int TY>
-__global__ void __launch_bounds__(64 * (TY + 4))
+__global__ void __launch_bounds__(64 * (TY + 4) * (int)(sizeof(S) / 4))
 k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__restrict__ Hin,
             S *__restrict__ Eout, S *__restrict__ Hout, S *__restrict__ J, Coefs<S> cf, S dt,
             FusedAux x, const S *__restrict__ beam1, const S *__restrict__ beam2,
             S *__restrict__ cap) {
   using I = uint32_t;  // byte offsets: every 3N array within 4 GiB
   constexpr int V = 16 / sizeof(S);
-  constexpr int W = 64 * V;
-  // LDS rows are q-major: cell V*lane+q at slot q*QS + lane + 1, so that one
-  // value per lane is 64 consecutive words (no bank conflicts for the
-  // single-word reads of the x neighbours and of cells one at a time). Slot 0
-  // of block V-1 is cell -1 and slot QS-1 of block 0 is cell W: both stay 0.
-  constexpr int QS = 64 + 2;
+  constexpr int WPR = sizeof(S) / 4;  // waves per row: 1 FP32, 2 FP64
+  constexpr int LR = 64 * WPR;        // lanes per row
+  constexpr int W = LR * V;           // cells per row (256)
+  // LDS rows are q-major: the cell V*ln+q of row lane ln at slot q*QS + ln + 1,
+  // so that one value per lane is LR consecutive words (no bank conflicts for
+  // the single-word reads of the x neighbours and of cells one at a time).
+  // Slot 0 of block V-1 is cell -1 and slot QS-1 of block 0 is cell W: both
+  // stay 0.
+  constexpr int QS = LR + 2;
   constexpr int RW = V * QS;
   constexpr int NM = 9;
   constexpr int RE0 = TY + 4, RH1 = TY + 3, RE1 = TY + 2, RH2 = TY + 1;
-  constexpr int NT = 64 * (TY + 4);
+  constexpr int NT = 64 * (TY + 4) * WPR;
   __shared__ S sE0_[3 * RE0 * RW];  // E0(k), rows j0-2 .. j0+TY+1
   __shared__ S sH1_[3 * RH1 * RW];  // H1(k), rows j0-2 .. j0+TY
   __shared__ S sE1_[3 * RE1 * RW];  // E1(k-1) until P3 stages E1(k), rows j0-1 .. j0+TY
@@ -102,14 +106,16 @@ k_fused4_t2(GridDev g, MetricDev<S> m, const S *__restrict__ Ein, const S *__res
   const int tz = tile / x.nty, ty = tile - tz * x.nty;
   const int j0 = ty * TY, k0 = tz * x.KZ;
   const int kend = min(k0 + x.KZ, g.Nz);
-  // the wave index is uniform: row flags, row metrics and branches are scalar
-  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // the row of a wave is uniform: row flags, row metrics and branches are scalar
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int w = wave / WPR;                              // row of the workgroup
+  const int ln = (wave - w * WPR) * 64 + (threadIdx.x & 63);  // lane in the row
   const int j = j0 - 2 + w;
   const bool rowok = (j >= 0) && (j < g.Ny);
-  const int xi = V * lane;
+  const int xi = V * ln;
   // LDS slot of cell xi + q, -1 <= q <= V (q is a constant after unrolling)
   const auto atx = [&](int q) {
-    return q < 0 ? (V - 1) * QS + lane : q >= V ? lane + 2 : q * QS + lane + 1;
+    return q < 0 ? (V - 1) * QS + ln : q >= V ? ln + 2 : q * QS + ln + 1;
   };
   const bool lact = rowok && (xi < g.Nx);
   const I N = (I)g.N, sz = (I)g.sz;
