@@ -43,11 +43,11 @@ VALUES_PER_CELL = 18  # E r/w 3+3, H r/w 3+3, face coeffs 3, edge coeffs 3
 # for it; WRITE_SIZE matches the copy's bytes and is taken as is. Keyed by
 # (grid, dtype, path, variant, kz); other settings report null.
 MEASURED_TRAFFIC = {
-    # FP32: one launch of the two-step kernel (k_fused4_t2) = two steps
-    ((256, 256, 512), "f32", "fused", 0, 0): {"fetch_B": 889_475_301, "write_B": 805_666_202,
+    # one launch of the two-step kernel (k_fused4_t2) = two steps
+    ((256, 256, 512), "f32", "fused", 0, 0): {"fetch_B": 888_720_998, "write_B": 805_661_184,
                                              "steps_per_launch": 2},
-    ((256, 256, 512), "f64", "fused", 0, 0): {"fetch_B": 1_957_545_370, "write_B": 1_610_772_890,
-                                             "steps_per_launch": 1},
+    ((256, 256, 512), "f64", "fused", 0, 0): {"fetch_B": 1_814_107_750, "write_B": 1_610_804_429,
+                                             "steps_per_launch": 2},
 }
 
 
@@ -834,8 +834,13 @@ def run(argv):
                         "achieved_GBs": round(r64["achieved_GBs"], 1),
                         "frac": round(r64["achieved_GBs"] / HBM_PEAK_GBS, 4),
                         "traffic": measured_traffic((Nx, Ny, Nz), "f64", r64["path"],
-                                                    args.variant, args.kz),
-                        "steps_per_launch": 2 if r64.get("two_step") else 1}
+                                                    args.variant, args.kz,
+                                                    2 if r64.get("two_step") else 1),
+                        "steps_per_launch": 2 if r64.get("two_step") else 1,
+                        "kernel": "k_fused4_t2" if r64.get("two_step") else "k_fused4"}
+        t64, spl64 = line["fp64"]["traffic"], line["fp64"]["steps_per_launch"]
+        if t64:  # the real HBM rate (traffic per launch over the launch time)
+            line["fp64"]["actual_GBs"] = round(t64 / (r64["ms_per_step"] * spl64 * 1e-3) / 1e9, 1)
         line["fp32_over_fp64"] = round(r32["gcell_s"] / r64["gcell_s"], 3)
     if exact is not None:
         line["exact_path"] = exact

@@ -171,15 +171,17 @@ typedef struct wf_step_args {
   int32_t mon_nf;
   int32_t mon_phase_f32;
   /* Fused path: 1 = advance two steps per launch (temporal blocking) where
-   * the two-step kernel applies: FP32, material codes with zero_masked, the
-   * beam line fed from beam_table (with or without conduction), Nx <= 256 and a
-   * multiple of 4, row
-   * segments (variant 0, full_rows <= 0, cells_per_lane 0 or 4), no box, no
-   * trajectory, no monitor; otherwise one step per launch. An odd step
-   * count ends with one single step. Results are identical to one step per
-   * launch. 0 = one step per launch; 4 or 8: two steps per launch with that
-   * many rows per workgroup; 1 = 8 rows on grids of 2^20 cells or more (on
-   * smaller ones one step per launch is faster). */
+   * the two-step kernel applies: FP32 or FP64 storage (not compute_f64),
+   * material codes with zero_masked and no escape code, the beam line fed
+   * from beam_table (with or without conduction), Nx <= 256 and a multiple of
+   * the cells per lane (4 FP32, 2 FP64), row segments (variant 0, full_rows
+   * <= 0, cells_per_lane 0 or the cells per lane), no box, no trajectory, no
+   * monitor; otherwise one step per launch. An odd step count ends with one
+   * single step. Results are identical to one step per launch. 0 = one step
+   * per launch; 4 or 8: two steps per launch with that many rows per
+   * workgroup (FP64 always 2 rows, two waves per row); 1 = 8 rows (FP64 2) on
+   * grids of 2^20 cells or more (on smaller ones one step per launch is
+   * faster). */
   int32_t two_step;
   int32_t two_step_used;  /* out: 1 if the call ran two steps per launch            */
 } wf_step_args;

@@ -194,8 +194,9 @@ class SolverFIT3D(RoutinesMixin):
         # sets -1 = segments or N), -1 row segments
         self.fused_full_rows = 0
         # two leapfrog steps per launch where that kernel applies (the wakefield
-        # step with material codes, FP32, Nx <= 256): 0 off, 1 on (8 rows per
-        # workgroup), 4 or 8 rows; bit-identical to one step per launch.
+        # step with material codes, FP32 or FP64, Nx <= 256): 0 off, 1 on (8 rows
+        # per workgroup, FP64 2), 4 or 8 rows (FP32); bit-identical to one step
+        # per launch.
         # tune_fused() sets 0 when one step per launch measures faster
         self.fused_two_step = 1
         # {full_rows: ms per step} of the last tune_fused(); -2: two steps per launch
