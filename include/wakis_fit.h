@@ -180,8 +180,8 @@ typedef struct wf_step_args {
    * single step. Results are identical to one step per launch. 0 = one step
    * per launch; 4 or 8: two steps per launch with that many rows per
    * workgroup (FP64 always 2 rows, two waves per row); 1 = 8 rows (FP64 2) on
-   * grids of 2^20 cells or more (on smaller ones one step per launch is
-   * faster). */
+   * grids of 2^20 cells or more, FP64 also with Nx >= 224 (otherwise one step
+   * per launch is as fast or faster). */
   int32_t two_step;
   int32_t two_step_used;  /* out: 1 if the call ran two steps per launch            */
 } wf_step_args;

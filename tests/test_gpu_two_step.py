@@ -95,8 +95,9 @@ def _beam_run(dims, two_step, chunks, kz=0, make=_cavity_solver, tune=False,
     for k in chunks:
         s._run(args(n0, k))
         assert s.last_path == "fused"
-        if not tune:  # the default (1) applies from 2^20 cells
-            big = two_step != 1 or s.N >= 1 << 20
+        if not tune:  # the default (1) applies from 2^20 cells (FP64: and Nx >= 224)
+            wide = np.dtype(s.dtype) == np.float32 or s.Nx >= 224
+            big = two_step != 1 or (s.N >= 1 << 20 and wide)
             assert s.last_two_step == (two_step != 0 and big and k >= 2 and s.Nx <= 256)
         n0 += k
     torch.cuda.synchronize()
@@ -153,12 +154,15 @@ def test_two_step_applies_only_where_built():
 
 
 def test_two_step_default_from_2_20_cells():
-    """The default (1) runs two steps per launch from 2^20 cells on, one step
-    per launch below; the results are the same."""
+    """The default (1) runs two steps per launch from 2^20 cells on (FP64 also
+    from 224 cells per row), one step per launch below; the results are the
+    same."""
     _say("two-step default by size")
-    for dims, two in (((64, 40, 48), False), ((256, 64, 64), True)):
-        ref = _beam_run(dims, 0, (6,))[0]
-        got, s = _beam_run(dims, 1, (6,))
+    for dims, dtype, two in (((64, 40, 48), np.float32, False), ((256, 64, 64), np.float32, True),
+                             ((160, 80, 82), np.float64, False),
+                             ((256, 64, 64), np.float64, True)):
+        ref = _beam_run(dims, 0, (6,), dtype=dtype)[0]
+        got, s = _beam_run(dims, 1, (6,), dtype=dtype)
         assert s.last_two_step == two
         for r, g in zip(ref, got):
             assert np.array_equal(g, r)

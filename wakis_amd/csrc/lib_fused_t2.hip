@@ -16,8 +16,13 @@ int fused_t2_applies(const wf_handle *h, const wf_step_args *a) {
                                                    : a->jmode == WF_J_LINE);
   // the default (1) starts at 2^20 cells: below that the few workgroups of a
   // launch leave the machine idle and one step per launch is faster (64^3:
-  // 28.5 against 17.1 us per step; 96x96x160: 32.7 against 37.1)
-  const bool big = a->two_step != 1 || g.N >= (int64_t)1 << 20;
+  // 28.5 against 17.1 us per step; 96x96x160: 32.7 against 37.1). FP64 rows
+  // take two waves, the second one idle up to 128 cells: there the default
+  // needs rows of 224 cells or more (256x256x128: 134 against 223 us per
+  // step; 160x160x320 and the 200-cell lossy pillbox: one step per launch is
+  // as fast or faster)
+  const bool big = a->two_step != 1 ||
+                   (g.N >= (int64_t)1 << 20 && (h->dtype == WF_F32 || g.Nx >= 224));
   const int elem = h->dtype == WF_F64 ? 8 : 4, vn = 16 / elem;  // cells per lane
   return (h->dtype == WF_F32 || h->dtype == WF_F64) && a->two_step != 0 && big &&
          a->nsteps >= 2 && lut && a->zero_masked && !a->code_escape && !a->compute_f64 && jok &&
